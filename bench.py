@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""BD steps/s of the MI355X Brownian-dynamics step at N = 2000 RPY + block
+Lanczos (s = 4, tol 1e-6), mixed precision (FP32 mobility storage, FP64
+accumulation and Krylov small problem) and FP64.
+
+One "step" is one EM step of the bds_run loop (bd_solver_m.f90:167-238):
+every s = 4 steps the RPY mobility is assembled and the block-Lanczos B.xi
+computed; every step forces, the drift D.F and the update run.  The timed
+region covers a whole number of s-step blocks; the Gaussian vectors come
+from the device Philox generator (synthetic, resident in HBM).
+
+Prints one JSON line (see --help).  --gpus > 1 is refused: this build runs on
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_BEADS = 2000
+S_BLOCK = 4
+HBM_PEAK_GBS = 8000.0
+
+
+def build_system(precision, seed=2000, reorth=False):
+    from brownpak_amd import BdStep
+    from brownpak_amd.configs import chain_bonds, fene_wca_params, random_walk_chain
+    crd = random_walk_chain(N_BEADS, seed)
+    bond, vdw = fene_wca_params()
+    g = BdStep(N_BEADS, lhdia=True, mob_fctr="LANC", nts_mobsam=S_BLOCK, lanc_mxitr=50,
+               lanc_tol=1e-6, tim_stp=1e-3, precision=precision, lanc_reorth=reorth)
+    g.set_box(0, (0.0, 0.0, 0.0))
+    g.ia_setup(chain_bonds(N_BEADS), [2], [bond], vdw_styles=[1], vdw_params=[vdw],
+               excluded_atoms=0, mth_ptgen="DIR", rcutoff=vdw[2])
+    g.set_coordinates(crd)
+    g.set_rng(12345)
+    return g, crd
+
+
+def time_path(precision, steps, warmup):
+    import torch
+    g, _ = build_system(precision)
+    g.bds_run(warmup)
+    g.sync()
+    g.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.bds_run(steps)
+    g.sync()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kms, launches, tile_bytes = g.prof_get()
+    info = g.runinfo()
+    g.prof_enable(False)
+    g.close()
+    return dict(seconds=dt, steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps,
+                kernel_ms=kms, launches=launches, tile_bytes=tile_bytes,
+                lanczos_iters=info["total_lanczos_iters"], mob_updates=info["n_mob_updates"])
+
+
+def displacement_error(nstates=2):
+    """Relative error of one EM displacement, mixed vs FP64, same state and
+    the same injected xi (FP64 reorthogonalised Lanczos on both sides)."""
+    errs = []
+    n = 3 * N_BEADS
+    for k in range(nstates):
+        xi = np.random.default_rng(100 + k).standard_normal(n * S_BLOCK)
+        disp = {}
+        for prec in ("fp64", "fp32"):
+            g, crd = build_system(prec, seed=2000 + k, reorth=True)
+            c0 = g.get_coordinates()
+            c0 = c0 - c0.mean(0)
+            g.bds_run(1, xi)
+            disp[prec] = g.get_coordinates() - c0
+            g.close()
+        errs.append(float(np.linalg.norm(disp["fp32"] - disp["fp64"]) / np.linalg.norm(disp["fp64"])))
+    return errs
+
+
+def cpu_baseline(budget_s=20.0):
+    """The CPU oracle (C FP64 restatement of the reference algorithm, one
+    thread) on a bounded sample of the same workload: one mobility update
+    block of s = 4 steps (RPY, block Lanczos, 4 x forces + dsymv + update),
+    repeated while the budget lasts."""
+    from oracle import oracle as O
+    from brownpak_amd.configs import chain_bonds, fene_wca_params, random_walk_chain
+    crd = random_walk_chain(N_BEADS, 2000)
+    bond, vdw = fene_wca_params()
+    so = O.System(N_BEADS, chain_bonds(N_BEADS), [2], [bond], vdw_styles=[1], vdw_params=[vdw],
+                  excluded_atoms=0, mth_ptgen=0, rcutoff=vdw[2])
+    n = 3 * N_BEADS
+    rng = np.random.default_rng(7)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        xi = rng.standard_normal(n * S_BLOCK)
+        crd, _, _ = O.bd_run(so, crd, S_BLOCK, xi, mob_fctr=O.LANC, nts_mobsam=S_BLOCK,
+                             lanc_mxitr=50, lanc_tol=1e-6)
+        steps += S_BLOCK
+        if time.perf_counter() - t0 > budget_s * 0.5:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=steps / dt, unit="steps/s", cores=1, kind="port",
+                sample=f"{steps} EM steps ({steps // S_BLOCK} mobility updates) of N=2000 RPY + "
+                       f"block Lanczos s=4 tol 1e-6, FP64, oracle/bdoracle.c at 1 thread, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+    steps = max(S_BLOCK, (args.steps // S_BLOCK) * S_BLOCK)
+    warmup = max(S_BLOCK, (args.warmup // S_BLOCK) * S_BLOCK)
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    mixed = time_path("fp32", steps, warmup)
+    print(f"# mixed: {mixed['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
+    fp64 = time_path("fp64", steps, warmup)
+    print(f"# fp64: {fp64['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
+    errs = [] if args.no_error_check else displacement_error()
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+
+    n = 3 * N_BEADS
+    # algorithmic bytes of one tile-product launch: the lower triangle of D
+    # (n(n+1)/2 entries) plus reading and writing the n x s block
+    avg_launch_s = mixed["kernel_ms"] / max(mixed["launches"], 1) * 1e-3
+    launches_per_step = mixed["launches"] / steps
+    # launches mix s = 4 (Lanczos) and s = 1 (drift) products; weight bytes by count
+    n_lz = mixed["lanczos_iters"]
+    n_dr = steps
+    elem = 4
+    alg_bytes = ((n_lz + n_dr) * elem * n * (n + 1) / 2 + (n_lz * 2 * n * 4 + n_dr * 2 * n) * 8) / max(n_lz + n_dr, 1)
+    achieved = alg_bytes / avg_launch_s / 1e9
+    out = {
+        "metric": "BD steps/sec at N=2000 RPY+Lanczos, mixed vs FP64 at <=1e-6 rel err; HBM GB/s",
+        "value": round(mixed["steps_per_s"], 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(mixed["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 mobility storage / f64 accumulate + Krylov",
+        "data": "synthetic (seeded random-walk chain, device Philox noise)",
+        "config": {"workload": "single chain N=2000, FENE/WCA, RPY, block Lanczos s=4 tol 1e-6 "
+                               "(lanc_mxitr 50), dt 1e-3, EM",
+                   "nbeads": N_BEADS, "block": S_BLOCK, "parallelism": "single GPU"},
+        "fp64_steps_per_s": round(fp64["steps_per_s"], 2),
+        "fp64_ms_per_step": round(fp64["ms_per_step"], 4),
+        "mixed_over_fp64": round(mixed["steps_per_s"] / fp64["steps_per_s"], 3),
+        "mixed_rel_disp_err": errs,
+        "lanczos_iters_per_update": round(mixed["lanczos_iters"] / max(mixed["mob_updates"], 1), 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "symm_tiles_kernel<float,S>",
+                     "avg_launch_us": round(avg_launch_s * 1e6, 2),
+                     "launches_per_step": round(launches_per_step, 2),
+                     "fp64_avg_launch_us": round(fp64["kernel_ms"] / max(fp64["launches"], 1) * 1e3, 2)},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["vs_cpu_fp64"] = round(fp64["steps_per_s"] / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,727 @@
+// bd_forces.hip -- bond and pair forces over a device neighbour list.
+//
+// Replaces ia_calc_forces (interaction_m.f90:84-147) with
+// ia_add_bond_forces (219-272) / ia_get_bond_force (ia_bond_m.f90:85-362),
+// ia_add_vdw_forces (interaction_m.f90:151-215) / vdw_lj (ia_vdw_m.f90:155-
+// 180), and the pair-table builders of pairtab_m.fpp (DIR 152-190, VER
+// 194-252) with exclusions from connectivity_m.f90:43-317.
+//
+// The reference accumulates +f on i and -f on j pair by pair.  Here every
+// bead gathers its own terms (each pair is evaluated from both ends), so no
+// atomics are needed and the sum order is fixed.  The pair predicate
+// r^2 < rc^2 (or (rc+skin)^2 for VER) uses the reference's expression
+// ((x*x + y*y) + z*z) with separately rounded operations and the nint
+// minimum image, so the device pair set is bit-identical to a scalar FP64
+// evaluation of the reference's loop.
+#include <algorithm>
+
+#include "bd_internal.h"
+
+namespace bd {
+
+constexpr int NBR_CAP0 = 64;        // initial neighbours per bead (full list)
+constexpr int FB = 256;             // threads per force block
+
+__device__ inline double img(double r, double L)
+{
+    // r - L*nint(r/L), nint = round half away from zero, no contraction
+    return __dsub_rn(r, __dmul_rn(L, round(__ddiv_rn(r, L))));
+}
+
+__device__ inline double rsq3(double x, double y, double z)
+{
+    return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+}
+
+struct BoxArg {
+    int imcon;
+    double L[3];
+};
+
+__device__ inline void pair_vec(const double *crd, int i, int j, const BoxArg &bx, double &x, double &y, double &z)
+{
+    x = __dsub_rn(crd[3 * j], crd[3 * i]);
+    y = __dsub_rn(crd[3 * j + 1], crd[3 * i + 1]);
+    z = __dsub_rn(crd[3 * j + 2], crd[3 * i + 2]);
+    if (bx.imcon) {
+        x = img(x, bx.L[0]);
+        y = img(y, bx.L[1]);
+        z = img(z, bx.L[2]);
+    }
+}
+
+// ia_get_bond_force (ia_bond_m.f90:85-362); returns 1 on overstretch.
+__device__ inline int bond_force(double r, int sty, const double *p, double &enrg, double &frc)
+{
+    enrg = 0.0;
+    frc = 0.0;
+    switch (sty) {
+    case 1:
+        enrg = 0.5 * p[0] * (r - p[1]) * (r - p[1]);
+        frc = p[0] * (r - p[1]);
+        return 0;
+    case 2: {
+        double k = p[0], rmax = p[1], r0 = p[2], rmaxsq = p[3];
+        double extn = r - r0, extnsq = extn * extn;
+        if (r >= rmax + r0) return 1;
+        enrg = -0.5 * k * rmaxsq * log(1.0 - extnsq / rmaxsq);
+        frc = k * extn / (1.0 - extnsq / rmaxsq);
+        return 0;
+    }
+    case 3: {
+        double k = p[0], rmax = p[1], eps = p[2], sigma = p[3], rmaxsq = p[4], rcut = p[5];
+        double rsq = r * r;
+        if (r >= rmax) return 1;
+        if (r >= rcut) {
+            enrg = -0.5 * k * rmaxsq * log(1.0 - rsq / rmaxsq);
+            frc = k * r / (1.0 - rsq / rmaxsq);
+        } else {
+            double sir = sigma / r, sir2 = sir * sir, sir6 = sir2 * sir2 * sir2, sir12 = sir6 * sir6;
+            enrg = -0.5 * k * rmaxsq * log(1.0 - rsq / rmaxsq) + 4 * eps * (sir12 - sir6) + eps;
+            frc = k * r / (1.0 - rsq / rmaxsq) - 24 * eps * (2 * sir12 - sir6) / r;
+        }
+        return 0;
+    }
+    case 4: {
+        double lp = p[0], rmax = p[1];
+        if (r >= rmax) return 1;
+        double rt = r / rmax, rrt = 1.0 / (1.0 - rt);
+        enrg = (-0.5 * rt * rt + 0.25 * rrt * rrt + 0.25 * rt) * (rmax / lp);
+        frc = (-rt - 0.25 * rrt * rrt + 0.25) / lp;
+        return 0;
+    }
+    default:
+        return 0;
+    }
+}
+
+// vdw_lj (ia_vdw_m.f90:155-180), truncated and force-shifted 12-6.
+__device__ inline void lj_force(double r, const double *p, double &enrg, double &frc)
+{
+    enrg = 0.0;
+    frc = 0.0;
+    if (r < p[2]) {
+        double sir = p[1] / r, sir2 = sir * sir, sir6 = sir2 * sir2 * sir2, sir12 = sir6 * sir6;
+        enrg = 4 * p[0] * (sir12 - sir6) - p[3] - (r - p[2]) * p[4];
+        frc = -24 * p[0] * (2 * sir12 - sir6) / r - p[4];
+    }
+}
+
+// ------------------------------------------------------------ energy sums
+// acc layout: 0 e_bond, 1 e_vdw, 2..10 stress(3x3 col-major), 11 bndlen sum,
+// 12 bndlen min, 13 bndlen max
+constexpr int NACC = 14;
+
+__device__ void block_acc_store(double (&a)[NACC], double *out)
+{
+    __shared__ double sh[FB / 64][NACC];
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    for (int e = 0; e < NACC; e++) {
+        double v = a[e];
+        for (int off = 32; off > 0; off >>= 1) {
+            double o = __shfl_xor(v, off, 64);
+            v = (e == 12) ? fmin(v, o) : (e == 13) ? fmax(v, o) : v + o;
+        }
+        if (lane == 0) sh[wv][e] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NACC) {
+        int e = threadIdx.x;
+        double v = sh[0][e];
+        for (int w = 1; w < FB / 64; w++) v = (e == 12) ? fmin(v, sh[w][e]) : (e == 13) ? fmax(v, sh[w][e]) : v + sh[w][e];
+        out[blockIdx.x * NACC + e] = v;
+    }
+}
+
+// ------------------------------------------------------------ forces
+// Per-bead gather of bond and pair terms.  Energies/stress/bond lengths are
+// counted once per term: bonds from their first bead, pairs from the lower
+// index.
+__global__ __launch_bounds__(FB) void force_gather_kernel(
+    const double *__restrict__ crd, int nb, BoxArg bx, const int *__restrict__ bond_ptr,
+    const int *__restrict__ bond_idx, const int *__restrict__ bond_ij, const int *__restrict__ bond_type,
+    const int *__restrict__ bstyles, const double *__restrict__ bparams, int use_pairs,
+    const int *__restrict__ nbr_cnt, const int *__restrict__ nbr_idx, int nbr_cap,
+    const int *__restrict__ atype, int natype, const int *__restrict__ ptype, const int *__restrict__ vstyles,
+    const double *__restrict__ vparams, double *__restrict__ frc, int want_energy, double *__restrict__ epart,
+    DevStatus *st)
+{
+    const int i = blockIdx.x * FB + threadIdx.x;
+    double acc[NACC];
+    for (int e = 0; e < NACC; e++) acc[e] = 0.0;
+    acc[12] = 1.0e300;
+    acc[13] = 0.0;
+    if (i < nb) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int k = bond_ptr[i]; k < bond_ptr[i + 1]; k++) {
+            int b = bond_idx[k];
+            int bi = bond_ij[2 * b], bj = bond_ij[2 * b + 1];
+            double x, y, z;
+            pair_vec(crd, bi, bj, bx, x, y, z);
+            double r = sqrt(x * x + y * y + z * z);
+            int t = bond_type[b];
+            double en, f;
+            if (bond_force(r, bstyles[t], bparams + t * MXPARAM, en, f)) atomicExch(&st->ierr, 1);
+            double gx = f * x / r, gy = f * y / r, gz = f * z / r;
+            double sg = (bi == i) ? 1.0 : -1.0;
+            if (bi == i && bj == i) sg = 0.0;  // degenerate self bond
+            fx += sg * gx; fy += sg * gy; fz += sg * gz;
+            if (want_energy && bi == i) {
+                acc[0] += en;
+                double rv[3] = {x, y, z}, fv[3] = {gx, gy, gz};
+                for (int bb = 0; bb < 3; bb++)
+                    for (int a = 0; a < 3; a++) acc[2 + a + 3 * bb] -= rv[a] * fv[bb];
+                acc[11] += r;
+                acc[12] = fmin(acc[12], r);
+                acc[13] = fmax(acc[13], r);
+            }
+        }
+        if (use_pairs) {
+            const int ati = atype[i];
+            const int cnt = min(nbr_cnt[i], nbr_cap);
+            for (int k = 0; k < cnt; k++) {
+                int j = nbr_idx[(long)k * nb + i];
+                int t = ptype[ati * natype + atype[j]];
+                if (vstyles[t] != 1) continue;
+                double x, y, z;
+                pair_vec(crd, i, j, bx, x, y, z);
+                double r = sqrt(x * x + y * y + z * z);
+                double en, f;
+                lj_force(r, vparams + t * MXPARAM, en, f);
+                double gx = f * x / r, gy = f * y / r, gz = f * z / r;
+                fx += gx; fy += gy; fz += gz;
+                if (want_energy && i < j) {
+                    acc[1] += en;
+                    double rv[3] = {x, y, z}, fv[3] = {gx, gy, gz};
+                    for (int bb = 0; bb < 3; bb++)
+                        for (int a = 0; a < 3; a++) acc[2 + a + 3 * bb] -= rv[a] * fv[bb];
+                }
+            }
+        }
+        frc[3 * i] = fx;
+        frc[3 * i + 1] = fy;
+        frc[3 * i + 2] = fz;
+    }
+    if (want_energy) block_acc_store(acc, epart);
+}
+
+__global__ void energy_final_kernel(const double *__restrict__ epart, int nblk, int nbonds, int imcon, double vol,
+                                    double *__restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double a[NACC];
+    for (int e = 0; e < NACC; e++) a[e] = epart[e];
+    for (int b = 1; b < nblk; b++)
+        for (int e = 0; e < NACC; e++) {
+            double v = epart[b * NACC + e];
+            a[e] = (e == 12) ? fmin(a[e], v) : (e == 13) ? fmax(a[e], v) : a[e] + v;
+        }
+    out[0] = a[0];
+    out[1] = a[1];
+    out[2] = a[0] + a[1];
+    for (int k = 0; k < 9; k++) out[3 + k] = imcon ? a[2 + k] / vol : a[2 + k];
+    out[12] = nbonds > 0 ? a[11] / nbonds : 0.0;
+    out[13] = nbonds > 0 ? a[12] : 0.0;
+    out[14] = a[13];
+}
+
+// ------------------------------------------------------------ pair lists
+__device__ inline bool excluded(const int *ex_ptr, const int *ex_col, int i, int j)
+{
+    for (int k = ex_ptr[i]; k < ex_ptr[i + 1]; k++)
+        if (ex_col[k] == j) return true;
+    return false;
+}
+
+// Verlet trigger (pairtab_m.fpp:211-222): rebuild iff first call or
+// 4 max|dr|^2 >= skin^2.  One block.
+__global__ __launch_bounds__(1024) void verlet_check_kernel(const double *__restrict__ crd,
+                                                            const double *__restrict__ save, int nb, BoxArg bx,
+                                                            double tskin_sq, int first, DevStatus *st)
+{
+    __shared__ double sh[16];
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        double dx = __dsub_rn(crd[3 * i], save[3 * i]);
+        double dy = __dsub_rn(crd[3 * i + 1], save[3 * i + 1]);
+        double dz = __dsub_rn(crd[3 * i + 2], save[3 * i + 2]);
+        if (bx.imcon) { dx = img(dx, bx.L[0]); dy = img(dy, bx.L[1]); dz = img(dz, bx.L[2]); }
+        mx = fmax(mx, rsq3(dx, dy, dz));
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / 64); w++) m = fmax(m, sh[w]);
+        st->rebuild = (first || !(4.0 * m < tskin_sq)) ? 1 : 0;
+    }
+}
+
+// All-pairs neighbour build: one wavefront per bead scans every j != i.
+// Keeps j with r^2 < rc2 that is not excluded.  Entries are in ascending j.
+__global__ __launch_bounds__(256) void nbr_n2_kernel(const double *__restrict__ crd, int nb, BoxArg bx, double rc2,
+                                                     const int *__restrict__ ex_ptr, const int *__restrict__ ex_col,
+                                                     int *__restrict__ nbr_cnt, int *__restrict__ nbr_idx, int cap,
+                                                     int check_flag, double *__restrict__ save, DevStatus *st)
+{
+    if (check_flag && !st->rebuild) return;
+    const int lane = threadIdx.x % 64;
+    const int i = blockIdx.x * 4 + threadIdx.x / 64;
+    if (i >= nb) return;
+    int cnt = 0;
+    for (int j0 = 0; j0 < nb; j0 += 64) {
+        int j = j0 + lane;
+        bool keep = false;
+        if (j < nb && j != i) {
+            double x, y, z;
+            pair_vec(crd, i, j, bx, x, y, z);
+            if (rsq3(x, y, z) < rc2) keep = !excluded(ex_ptr, ex_col, i, j);
+        }
+        unsigned long long m = __ballot(keep);
+        if (keep) {
+            int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (pos < cap) nbr_idx[(long)pos * nb + i] = j;
+        }
+        cnt += __popcll(m);
+    }
+    if (lane == 0) {
+        nbr_cnt[i] = cnt;
+        if (cnt > cap) atomicExch(&st->overflow, 1);
+    }
+    if (save && lane < 3) save[3 * i + lane] = crd[3 * i + lane];
+}
+
+// ---- cell-list accelerated Verlet build (periodic boxes with >= 3 cells)
+__global__ void cell_assign_kernel(const double *__restrict__ crd, int nb, BoxArg bx, int ncx, int ncy, int ncz,
+                                   int *__restrict__ bead_cell, int *__restrict__ cell_cnt, int check_flag,
+                                   const DevStatus *st)
+{
+    if (check_flag && !st->rebuild) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    int nc[3] = {ncx, ncy, ncz}, ic[3];
+    for (int d = 0; d < 3; d++) {
+        double L = bx.L[d];
+        double x = crd[3 * i + d];
+        double f = x / L;
+        f -= floor(f);
+        int c = (int)(f * nc[d]);
+        if (c >= nc[d]) c = nc[d] - 1;
+        if (c < 0) c = 0;
+        ic[d] = c;
+    }
+    int cell = (ic[2] * ncy + ic[1]) * ncx + ic[0];
+    bead_cell[i] = cell;
+    atomicAdd(&cell_cnt[cell], 1);
+}
+
+// exclusive scan of cell counts (single block, any size) + reset fill counters
+__global__ __launch_bounds__(1024) void cell_scan_kernel(int *__restrict__ cell_cnt, int *__restrict__ cell_ptr, int ncell,
+                                                         int check_flag, const DevStatus *st)
+{
+    if (check_flag && !st->rebuild) return;
+    __shared__ int sums[1024];
+    const int t = threadIdx.x;
+    const int per = (ncell + 1023) / 1024;
+    int b = t * per, e = min(ncell, b + per);
+    int s = 0;
+    for (int k = b; k < e; k++) s += cell_cnt[k];
+    sums[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = (t >= off) ? sums[t - off] : 0;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    int run = sums[t] - s;
+    for (int k = b; k < e; k++) {
+        cell_ptr[k] = run;
+        run += cell_cnt[k];
+        cell_cnt[k] = 0;  // reused as fill counter
+    }
+    if (t == 1023) cell_ptr[ncell] = sums[1023];
+}
+
+__global__ void cell_fill_kernel(const int *__restrict__ bead_cell, int nb, const int *__restrict__ cell_ptr,
+                                 int *__restrict__ cell_fill, int *__restrict__ cell_beads, int check_flag,
+                                 const DevStatus *st)
+{
+    if (check_flag && !st->rebuild) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    int c = bead_cell[i];
+    int pos = cell_ptr[c] + atomicAdd(&cell_fill[c], 1);
+    cell_beads[pos] = i;
+}
+
+// One thread per bead; candidates from the 27 surrounding cells.
+__global__ __launch_bounds__(256) void nbr_cell_kernel(const double *__restrict__ crd, int nb, BoxArg bx, double rc2,
+                                                       int ncx, int ncy, int ncz, const int *__restrict__ bead_cell,
+                                                       const int *__restrict__ cell_ptr, const int *__restrict__ cell_beads,
+                                                       const int *__restrict__ ex_ptr, const int *__restrict__ ex_col,
+                                                       int *__restrict__ nbr_cnt, int *__restrict__ nbr_idx, int cap,
+                                                       double *__restrict__ save, DevStatus *st)
+{
+    if (!st->rebuild) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    int c = bead_cell[i];
+    int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+    int cnt = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                int jx = (cx + dx + ncx) % ncx, jy = (cy + dy + ncy) % ncy, jz = (cz + dz + ncz) % ncz;
+                int jc = (jz * ncy + jy) * ncx + jx;
+                for (int p = cell_ptr[jc]; p < cell_ptr[jc + 1]; p++) {
+                    int j = cell_beads[p];
+                    if (j == i) continue;
+                    double x, y, z;
+                    pair_vec(crd, i, j, bx, x, y, z);
+                    if (rsq3(x, y, z) < rc2 && !excluded(ex_ptr, ex_col, i, j)) {
+                        if (cnt < cap) nbr_idx[(long)cnt * nb + i] = j;
+                        cnt++;
+                    }
+                }
+            }
+    nbr_cnt[i] = cnt;
+    if (cnt > cap) atomicExch(&st->overflow, 1);
+    save[3 * i] = crd[3 * i];
+    save[3 * i + 1] = crd[3 * i + 1];
+    save[3 * i + 2] = crd[3 * i + 2];
+}
+
+__global__ void zero_int_kernel(int *p, int n, int check_flag, const DevStatus *st)
+{
+    if (check_flag && !st->rebuild) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void count_rebuild_kernel(DevStatus *st, int *counter)
+{
+    if (st->rebuild) *counter += 1;
+}
+
+// ------------------------------------------------------------ host
+static std::vector<int> host_exclusions(int nb, const std::vector<int> &bi, const std::vector<int> &bj, int excl,
+                                        std::vector<int> &ptr)
+{
+    // 1-ring adjacency (connectivity_m.f90:202-246), unique and sorted
+    std::vector<std::vector<int>> adj(nb);
+    for (size_t b = 0; b < bi.size(); b++) {
+        if (bi[b] != bj[b]) {
+            adj[bi[b]].push_back(bj[b]);
+            adj[bj[b]].push_back(bi[b]);
+        }
+    }
+    for (auto &v : adj) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    std::vector<int> col;
+    ptr.assign(nb + 1, 0);
+    std::vector<int> buf;
+    for (int i = 0; i < nb; i++) {
+        buf.clear();
+        if (excl > 0) {
+            buf.push_back(i);
+            for (int j : adj[i]) {
+                buf.push_back(j);
+                if (excl > 1)
+                    for (int j2 : adj[j]) {
+                        buf.push_back(j2);
+                        if (excl > 2)
+                            for (int j3 : adj[j2]) buf.push_back(j3);
+                    }
+            }
+        }
+        std::sort(buf.begin(), buf.end());
+        buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+        col.insert(col.end(), buf.begin(), buf.end());
+        ptr[i + 1] = (int)col.size();
+    }
+    return col;
+}
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &v)
+{
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    BD_HIP(hipMalloc(dst, bytes));
+    if (!v.empty()) BD_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return BD_OK;
+}
+
+int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const int *bond_styles,
+                 const double *bond_params, int natom_types, const int *atom_types, int nvdw_types,
+                 const int *vdw_styles, const double *vdw_params)
+{
+    const int nb = c.nb;
+    std::vector<int> bij(2 * std::max(nbonds, 0)), btype(std::max(nbonds, 0));
+    std::vector<int> bi(nbonds), bj(nbonds);
+    for (int b = 0; b < nbonds; b++) {
+        int t = bonds[3 * b] - 1, i = bonds[3 * b + 1] - 1, j = bonds[3 * b + 2] - 1;
+        if (t < 0 || t >= nbond_types || i < 0 || i >= nb || j < 0 || j >= nb) {
+            set_error("bond " + std::to_string(b + 1) + " has an out-of-range type or bead index");
+            return BD_ERR_ARG;
+        }
+        bij[2 * b] = i; bij[2 * b + 1] = j; btype[b] = t;
+        bi[b] = i; bj[b] = j;
+    }
+    std::vector<int> bstyle(std::max(nbond_types, 1), 0);
+    std::vector<double> bpar((size_t)std::max(nbond_types, 1) * MXPARAM, 0.0);
+    for (int t = 0; t < nbond_types; t++) {
+        bstyle[t] = bond_styles[t];
+        if (bstyle[t] < 0 || bstyle[t] > 4) { set_error("bond style must be 0..4"); return BD_ERR_ARG; }
+        for (int k = 0; k < MXPARAM; k++) bpar[t * MXPARAM + k] = bond_params[t * MXPARAM + k];
+        double *p = &bpar[t * MXPARAM];
+        if (bstyle[t] == 2) p[3] = p[1] * p[1];                         // bond_fene_set
+        if (bstyle[t] == 3) { p[4] = p[1] * p[1]; p[5] = 1.122462048309373 * p[3]; }  // bond_kg_set
+    }
+    // bead -> bonds CSR in bond order (atbo_build, connectivity_m.f90:43-90)
+    std::vector<int> bptr(nb + 1, 0), bidx;
+    {
+        std::vector<std::vector<int>> per(nb);
+        for (int b = 0; b < nbonds; b++) {
+            per[bi[b]].push_back(b);
+            if (bj[b] != bi[b]) per[bj[b]].push_back(b);
+        }
+        for (int i = 0; i < nb; i++) {
+            bidx.insert(bidx.end(), per[i].begin(), per[i].end());
+            bptr[i + 1] = (int)bidx.size();
+        }
+    }
+    std::vector<int> atype(nb, 0);
+    for (int i = 0; i < nb; i++) {
+        atype[i] = atom_types ? atom_types[i] - 1 : 0;
+        if (atype[i] < 0 || atype[i] >= std::max(natom_types, 1)) {
+            set_error("atom type out of range");
+            return BD_ERR_ARG;
+        }
+    }
+    const int nat = std::max(natom_types, 1);
+    std::vector<int> ptype(nat * nat, 0);
+    for (int a = 1; a <= nat; a++)
+        for (int b = 1; b <= nat; b++) {
+            // interaction_m.f90:182-186
+            int typ = (a < b) ? b + (2 * nat - a) * (a - 1) / 2 : a + (2 * nat - b) * (b - 1) / 2;
+            ptype[(a - 1) * nat + (b - 1)] = typ - 1;
+        }
+    std::vector<int> vstyle(std::max(nvdw_types, 1), 0);
+    std::vector<double> vpar((size_t)std::max(nvdw_types, 1) * MXPARAM, 0.0);
+    if (c.lvdw && nvdw_types > 0 && nvdw_types < nat * (nat + 1) / 2) {
+        set_error("VDW section must list all nat*(nat+1)/2 type pairs");
+        return BD_ERR_ARG;
+    }
+    for (int t = 0; t < nvdw_types; t++) {
+        vstyle[t] = vdw_styles[t];
+        if (vstyle[t] != 0 && vstyle[t] != 1) {
+            set_error("vdw style " + std::to_string(vstyle[t]) + " is not supported (only 1: LJ)");
+            return BD_ERR_ARG;
+        }
+        for (int k = 0; k < MXPARAM; k++) vpar[t * MXPARAM + k] = vdw_params[t * MXPARAM + k];
+        if (vstyle[t] == 1) {  // vdw_lj_set, ia_vdw_m.f90:107-151
+            double *p = &vpar[t * MXPARAM];
+            double sir = p[1] / p[2], sir2 = sir * sir, sir6 = sir2 * sir2 * sir2, sir12 = sir6 * sir6;
+            p[3] = 4 * p[0] * (sir12 - sir6);
+            p[4] = -24 * p[0] * (2 * sir12 - sir6) / p[2];
+        }
+    }
+    std::vector<int> eptr;
+    std::vector<int> ecol = host_exclusions(nb, bi, bj, c.excluded_atoms, eptr);
+
+    c.nbonds = nbonds;
+    c.nbond_types = nbond_types;
+    c.natom_types = nat;
+    c.nvdw_types = nvdw_types;
+    BD_TRY(upload(&c.bond_ij, bij));
+    BD_TRY(upload(&c.bond_type, btype));
+    BD_TRY(upload(&c.bead_bond_ptr, bptr));
+    BD_TRY(upload(&c.bead_bond_idx, bidx));
+    BD_TRY(upload(&c.bond_styles, bstyle));
+    BD_TRY(upload(&c.bond_params, bpar));
+    BD_TRY(upload(&c.atom_types, atype));
+    BD_TRY(upload(&c.vdw_styles, vstyle));
+    BD_TRY(upload(&c.vdw_params, vpar));
+    BD_TRY(upload(&c.ex_ptr, eptr));
+    BD_TRY(upload(&c.ex_col, ecol));
+    BD_TRY(upload(&c.scan_tmp, ptype));  // pair-type table (nat x nat)
+    c.ex_total = (int)ecol.size();
+
+    if (!c.nbr_cnt) {
+        BD_HIP(hipMalloc(&c.nbr_cnt, (size_t)std::max(nb, 1) * sizeof(int)));
+        BD_HIP(hipMemset(c.nbr_cnt, 0, (size_t)std::max(nb, 1) * sizeof(int)));
+    }
+    if (!c.nbr_idx) {
+        c.pair_cap = NBR_CAP0;
+        BD_HIP(hipMalloc(&c.nbr_idx, (size_t)c.pair_cap * std::max(nb, 1) * sizeof(int)));
+    }
+    if (!c.crd_save) BD_HIP(hipMalloc(&c.crd_save, (size_t)3 * std::max(nb, 1) * sizeof(double)));
+    int nblk = (nb + FB - 1) / FB;
+    if (c.red_cap < nblk * NACC) {
+        if (c.red) (void)hipFree(c.red);
+        c.red_cap = std::max(nblk, 1) * NACC;
+        BD_HIP(hipMalloc(&c.red, (size_t)c.red_cap * sizeof(double)));
+    }
+    c.verlet_first = true;
+    c.pairs_valid = false;
+    // periodic cell grid for the Verlet build (cells of at least rc + skin)
+    c.ncell_tot = 0;
+    if (c.imcon && c.mth_ptgen == BD_PT_VER) {
+        double rl = c.rcutoff + c.tskin;
+        int nc[3];
+        for (int d = 0; d < 3; d++) nc[d] = (int)std::floor(c.box[d] / rl);
+        if (nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3) {
+            for (int d = 0; d < 3; d++) c.ncell[d] = nc[d];
+            c.ncell_tot = nc[0] * nc[1] * nc[2];
+            if (c.cell_cnt) (void)hipFree(c.cell_cnt);
+            if (c.cell_ptr) (void)hipFree(c.cell_ptr);
+            if (!c.cell_beads) BD_HIP(hipMalloc(&c.cell_beads, (size_t)nb * sizeof(int)));
+            if (!c.bead_cell) BD_HIP(hipMalloc(&c.bead_cell, (size_t)nb * sizeof(int)));
+            BD_HIP(hipMalloc(&c.cell_cnt, (size_t)c.ncell_tot * sizeof(int)));
+            BD_HIP(hipMalloc(&c.cell_ptr, (size_t)(c.ncell_tot + 1) * sizeof(int)));
+        }
+    }
+    c.have_topology = true;
+    return BD_OK;
+}
+
+void forces_free(Ctx &c)
+{
+    int **ip[] = {&c.bond_ij, &c.bond_type, &c.bead_bond_ptr, &c.bead_bond_idx, &c.bond_styles, &c.atom_types,
+                  &c.vdw_styles, &c.ex_ptr, &c.ex_col, &c.nbr_cnt, &c.nbr_ptr, &c.nbr_idx, &c.cell_cnt,
+                  &c.cell_ptr, &c.cell_beads, &c.bead_cell, &c.scan_tmp};
+    for (auto p : ip) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    double **dp[] = {&c.bond_params, &c.vdw_params, &c.crd_save, &c.red};
+    for (auto p : dp) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+}
+
+static BoxArg box_arg(const Ctx &c)
+{
+    BoxArg b;
+    b.imcon = c.imcon;
+    for (int d = 0; d < 3; d++) b.L[d] = c.box[d];
+    return b;
+}
+
+// Builds (or keeps) the neighbour list for the current coordinates.
+static int build_pairs(Ctx &c)
+{
+    const int nb = c.nb;
+    hipStream_t sm = c.stream;
+    BoxArg bx = box_arg(c);
+    if (c.mth_ptgen == BD_PT_CL) {
+        set_error("mth_ptgen CL is not implemented on the device; use DIR or VER");
+        return BD_ERR_ARG;
+    }
+    const bool ver = (c.mth_ptgen == BD_PT_VER);
+    const double rl = ver ? c.rcutoff + c.tskin : c.rcutoff;
+    const double rc2 = rl * rl;
+    for (int attempt = 0; attempt < 8; attempt++) {
+        if (ver) {
+            hipLaunchKernelGGL(verlet_check_kernel, dim3(1), dim3(1024), 0, sm, c.crd, c.crd_save, nb, bx,
+                               c.tskin * c.tskin, c.verlet_first ? 1 : 0, c.st);
+        } else {
+            // DIR rebuilds on every call
+            hipLaunchKernelGGL(verlet_check_kernel, dim3(1), dim3(1024), 0, sm, c.crd, c.crd_save, nb, bx, 0.0, 1,
+                               c.st);
+        }
+        if (ver && c.ncell_tot > 0) {
+            int nbk = (nb + 255) / 256;
+            hipLaunchKernelGGL(zero_int_kernel, dim3((c.ncell_tot + 255) / 256), dim3(256), 0, sm, c.cell_cnt,
+                               c.ncell_tot, 1, c.st);
+            hipLaunchKernelGGL(cell_assign_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, nb, bx, c.ncell[0], c.ncell[1],
+                               c.ncell[2], c.bead_cell, c.cell_cnt, 1, c.st);
+            hipLaunchKernelGGL(cell_scan_kernel, dim3(1), dim3(1024), 0, sm, c.cell_cnt, c.cell_ptr, c.ncell_tot, 1,
+                               c.st);
+            hipLaunchKernelGGL(cell_fill_kernel, dim3(nbk), dim3(256), 0, sm, c.bead_cell, nb, c.cell_ptr, c.cell_cnt,
+                               c.cell_beads, 1, c.st);
+            hipLaunchKernelGGL(nbr_cell_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, nb, bx, rc2, c.ncell[0], c.ncell[1],
+                               c.ncell[2], c.bead_cell, c.cell_ptr, c.cell_beads, c.ex_ptr, c.ex_col, c.nbr_cnt,
+                               c.nbr_idx, c.pair_cap, c.crd_save, c.st);
+        } else {
+            hipLaunchKernelGGL(nbr_n2_kernel, dim3((nb + 3) / 4), dim3(256), 0, sm, c.crd, nb, bx, rc2, c.ex_ptr,
+                               c.ex_col, c.nbr_cnt, c.nbr_idx, c.pair_cap, 1, ver ? c.crd_save : nullptr, c.st);
+        }
+        BD_HIP(hipGetLastError());
+        // overflow check needs the host; only on rebuilds that can overflow
+        BD_HIP(hipMemcpyAsync(c.st_host, c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, sm));
+        BD_HIP(hipStreamSynchronize(sm));
+        c.last_rebuilt = c.st_host->rebuild;
+        if (c.st_host->rebuild && ver) c.info.n_pair_rebuilds++;
+        if (!c.st_host->overflow) {
+            c.verlet_first = false;
+            c.pairs_valid = true;
+            return BD_OK;
+        }
+        // grow and rebuild from scratch
+        int zero = 0;
+        BD_HIP(hipMemcpy(&c.st->overflow, &zero, sizeof(int), hipMemcpyHostToDevice));
+        (void)hipFree(c.nbr_idx);
+        c.pair_cap *= 2;
+        BD_HIP(hipMalloc(&c.nbr_idx, (size_t)c.pair_cap * nb * sizeof(int)));
+        c.verlet_first = true;
+        if (ver && c.st_host->rebuild) c.info.n_pair_rebuilds--;
+    }
+    set_error("neighbour list capacity could not be satisfied");
+    return BD_ERR_CAPACITY;
+}
+
+int compute_forces(Ctx &c, bool want_energy)
+{
+    if (!c.have_topology) { set_error("topology not set"); return BD_ERR_STATE; }
+    const int nb = c.nb;
+    const bool pairs = c.lvdw && c.nvdw_types > 0;
+    if (pairs) BD_TRY(build_pairs(c));
+    BoxArg bx = box_arg(c);
+    int nblk = (nb + FB - 1) / FB;
+    hipLaunchKernelGGL(force_gather_kernel, dim3(std::max(nblk, 1)), dim3(FB), 0, c.stream, c.crd, nb, bx,
+                       c.bead_bond_ptr, c.bead_bond_idx, c.bond_ij, c.bond_type, c.bond_styles, c.bond_params,
+                       pairs ? 1 : 0, c.nbr_cnt, c.nbr_idx, c.pair_cap, c.atom_types, c.natom_types, c.scan_tmp,
+                       c.vdw_styles, c.vdw_params, c.frc, want_energy ? 1 : 0, c.red, c.st);
+    BD_HIP(hipGetLastError());
+    if (want_energy) {
+        double vol = c.box[0] * c.box[1] * c.box[2];
+        hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(64), 0, c.stream, c.red, std::max(nblk, 1), c.nbonds,
+                           c.imcon, vol, c.scal);
+        BD_HIP(hipGetLastError());
+    }
+    return BD_OK;
+}
+
+int pair_table_host(Ctx &c, std::vector<int> &pairs)
+{
+    pairs.clear();
+    if (!c.pairs_valid) { set_error("no pair table has been built"); return BD_ERR_STATE; }
+    const int nb = c.nb;
+    std::vector<int> cnt(nb), idx((size_t)c.pair_cap * nb);
+    BD_HIP(hipStreamSynchronize(c.stream));
+    BD_HIP(hipMemcpy(cnt.data(), c.nbr_cnt, nb * sizeof(int), hipMemcpyDeviceToHost));
+    BD_HIP(hipMemcpy(idx.data(), c.nbr_idx, idx.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < nb; i++) {
+        std::vector<int> row;
+        for (int k = 0; k < std::min(cnt[i], c.pair_cap); k++) {
+            int j = idx[(size_t)k * nb + i];
+            if (j > i) row.push_back(j);
+        }
+        std::sort(row.begin(), row.end());
+        for (int j : row) {
+            pairs.push_back(i + 1);
+            pairs.push_back(j + 1);
+        }
+    }
+    return BD_OK;
+}
+
+}  // namespace bd

@@ -1,0 +1,193 @@
+// bd_internal.h -- shared state and helpers of libbdstep (gfx950 / CDNA4).
+//
+// Data layout in HBM
+//   crd, frc, drift      n = 3N FP64, xyz interleaved (the reference's (3,N)).
+//   mobility D           lower-triangle tiles of TILE x TILE (TILE = 64):
+//                        tile (I, J), I >= J, at index I(I+1)/2 + J, each tile
+//                        column-major; diagonal tiles hold both triangles;
+//                        rows/columns beyond n are zero.  FP64 or FP32.
+//   Krylov basis V       npad x (s * dimk_max) FP64 column-major, npad =
+//                        ntile * TILE, padding rows zero.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/bdstep.h"
+
+namespace bd {
+
+constexpr int TILE = 64;
+constexpr int MXPARAM = 12;
+constexpr int MAX_S = 8;          // largest Lanczos block size supported
+constexpr int NQUAD_MAX = 160;    // quadrature nodes for H^{-1/2}
+constexpr int NBOUND = 48;        // Cholesky probes for the lower spectral bound
+
+void set_error(const std::string &msg);
+
+#define BD_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ::bd::set_error(std::string("HIP error ") + hipGetErrorString(e_) +    \
+                            " at " + __FILE__ + ":" + std::to_string(__LINE__));  \
+            return BD_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define BD_TRY(expr)                   \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != BD_OK) return rc_;  \
+    } while (0)
+
+// Device-side status words (one struct in device memory per context).
+struct DevStatus {
+    int ierr;          // 1: bond overstretched; 2: factorisation failed
+    int converged_j;   // Lanczos: first j with err <= tol (0 = not yet)
+    int last_j;        // last Lanczos iteration executed
+    int rebuild;       // Verlet list rebuild flag for the current step
+    int npairs;        // half-list pair count of the current table
+    int overflow;      // pair list capacity exceeded
+    int ticket;        // work counter for the mobility product
+    int nq;            // quadrature nodes used by the last small problem
+    int steps_done;    // EM steps completed since the last bd_run started
+    int pad;
+    double err;        // last Lanczos relative change
+    double lam_lo, lam_hi;  // spectral bounds of the last small problem
+};
+
+struct Ctx {
+    int nb = 0, n = 0, npad = 0, ntile = 0, ntiles_lower = 0;
+    bd_opts opt{};
+    int s = 1;               // Brownian block width used with HI
+    int dimk_max = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    int imcon = 0;
+    double box[3] = {0, 0, 0};
+
+    // coordinates and per-step vectors
+    double *crd = nullptr, *frc = nullptr, *drift = nullptr, *diff = nullptr;
+    double *crd_mob = nullptr;   // coordinates D was assembled at (unused by EM)
+
+    // mobility
+    double *mob64 = nullptr;
+    float *mob32 = nullptr;
+    bool mob_valid = false;
+    double *chol = nullptr;      // dense n x n work for the Cholesky factor
+    int chol_ld = 0;
+
+    // symmetric product scratch
+    int nunits = 0;
+    int *units = nullptr;        // (J, Ibeg, Iend) per work unit
+    int *unit_of_tile = nullptr;
+    double *rowpart = nullptr;   // per tile: TILE x MAX_S
+    double *colpart = nullptr;   // per unit: TILE x MAX_S
+    int *colunit_ptr = nullptr;  // units of tile column J: [ptr[J], ptr[J+1])
+
+    // Lanczos workspace
+    double *V = nullptr;         // npad x (s*dimk_max)
+    double *W = nullptr;         // npad x s
+    double *Y = nullptr;         // 2 x npad x s (ping-pong B.xi iterates)
+    double *Ablk = nullptr;      // s x s x dimk_max  diagonal blocks
+    double *Bblk = nullptr;      // s x s x dimk_max  sub-diagonal blocks
+    double *R0 = nullptr;        // s x s
+    double *gpart = nullptr;     // Gram partials (nblk x s x s)
+    int gblocks = 0;
+    double *Cq = nullptr;        // quadrature partials NQUAD_MAX x (js x s)
+    double *Ccoef = nullptr;     // js x s coefficients
+    int *bound_ok = nullptr;     // NBOUND flags
+    double *npart = nullptr;     // norm partials
+    DevStatus *st = nullptr;
+    DevStatus *st_host = nullptr;  // pinned mirror
+
+    // topology
+    bool have_topology = false;
+    int nbonds = 0, natom_types = 0, nvdw_types = 0;
+    int lvdw = 0, excluded_atoms = 0, mth_ptgen = 0;
+    double rcutoff = 0, tskin = 0;
+    int *bond_ij = nullptr;        // nbonds x 2 (0-based)
+    int *bond_type = nullptr;      // nbonds (0-based)
+    int *bead_bond_ptr = nullptr;  // CSR bead -> bonds (bond index ascending)
+    int *bead_bond_idx = nullptr;
+    int nbond_types = 0;
+    int *bond_styles = nullptr;
+    double *bond_params = nullptr; // 12 x ntypes (derived slots set)
+    int *atom_types = nullptr;     // 0-based
+    int *vdw_styles = nullptr;
+    double *vdw_params = nullptr;
+    int *ex_ptr = nullptr, *ex_col = nullptr;  // exclusions, 0-based
+    int ex_total = 0;
+
+    // pair lists
+    int pair_cap = 0;            // capacity of the full list
+    int *nbr_cnt = nullptr;      // N
+    int *nbr_ptr = nullptr;      // N+1
+    int *nbr_idx = nullptr;      // pair_cap (full list, both directions)
+    double *crd_save = nullptr;  // Verlet reference positions
+    bool verlet_first = true;
+    int *cell_cnt = nullptr, *cell_ptr = nullptr, *cell_beads = nullptr, *bead_cell = nullptr;
+    int ncell[3] = {0, 0, 0};
+    int ncell_tot = 0;
+    int *scan_tmp = nullptr;
+    int last_rebuilt = 0;
+    bool pairs_valid = false;
+
+    // reductions
+    double *red = nullptr;       // generic partial buffer
+    int red_cap = 0;
+    double *scal = nullptr;      // small device scalars
+    double *host_scal = nullptr; // pinned
+
+    // run state
+    int cntr_mobsam = 0;
+    bd_runinfo info{};
+    uint64_t rng_seed = 0x5eed, rng_offset = 0;
+    double *xi_dev = nullptr;    // staging for injected xi (n x s)
+
+    // kernel timing of the tile product (bd_prof_*)
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_ev;   // pairs (start, stop)
+    size_t prof_used = 0;
+    double prof_ms = 0.0;
+    long long prof_count = 0;
+};
+void prof_flush(Ctx &c);
+
+// --- mobility (bd_mobility.hip)
+int mob_alloc(Ctx &c);
+void mob_free(Ctx &c);
+int rpy_assemble(Ctx &c);
+// y(npad, s) = D x(npad, s) on the context stream; x, y device, ld = npad.
+int mob_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy);
+// y = alpha * (D x + add) with optional add vector (s = 1); used for the drift.
+int mob_unpack(Ctx &c, double *dense, int ld);
+
+// --- Brownian (bd_lanczos.hip)
+int lanczos_alloc(Ctx &c);
+void lanczos_free(Ctx &c);
+// xi (npad x s, device) -> out (npad x s, device); returns ierr-style code.
+int brownian_lanczos(Ctx &c, const double *xi, double *out, int s, int *lconv, int *iters);
+int brownian_cholesky(Ctx &c, const double *xi, double *out, int s, int method);
+
+// --- forces (bd_forces.hip)
+int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types,
+                 const int *bond_styles, const double *bond_params, int natom_types,
+                 const int *atom_types, int nvdw_types, const int *vdw_styles,
+                 const double *vdw_params);
+void forces_free(Ctx &c);
+// Computes forces into c.frc at c.crd.  Energies/stress reductions are
+// computed when want_energy is set (written to c.scal[0..14]).
+int compute_forces(Ctx &c, bool want_energy);
+int pair_table_host(Ctx &c, std::vector<int> &pairs);
+
+// --- step pieces (bd_api.hip)
+int launch_gauss(Ctx &c, double *out, long long count, uint64_t seed, uint64_t offset, int ld, int rows, int cols);
+
+}  // namespace bd

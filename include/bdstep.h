@@ -1,0 +1,203 @@
+/*
+ * bdstep.h -- C ABI of the MI355X Brownian-dynamics step (libbdstep.so).
+ *
+ * Replaces, one for one, the call sites of BROWNPAK's EM step:
+ *
+ *   calc_rpy_tensor(coordinates, mob)        src/bd_solver_m.f90:377, 458-522
+ *   brn_init / brn_finish                    src/brown_m.f90:73-118, 122-143
+ *   brn_calc_bdw(mob, bdw, ierr, lconv, f)   src/brown_m.f90:147-181
+ *     calc_bdw_cholesky                      src/brown_m.f90:516-554
+ *     calc_bdw_lanc                          src/brown_m.f90:185-288
+ *     calc_bdw_blanc                         src/brown_m.f90:292-512
+ *   ia_setup / ia_calc_forces                src/interaction_m.f90:30-71, 84-147
+ *   pt_init / pt_build                       src/pairtab_m.fpp:36-98, 129-148
+ *   dsymv('L', n, 1, mob, n, F, 1, 1, drift) src/bd_solver_m.f90:446
+ *   integrate_em / bds_run loop body         src/bd_solver_m.f90:167-238
+ *
+ * Conventions (those of the reference):
+ *   - coordinates and forces: (3, N) FP64, Fortran column-major, i.e. xyz of
+ *     bead 1, then bead 2, ...;  n = 3N is the order of the mobility matrix;
+ *   - the Brownian block bdw / xi: (n, s) FP64 column-major;
+ *   - bonds: (3, nb) int32 [type, i, j] with 1-based bead indices;
+ *   - bond/vdw parameters: (12, ntypes) FP64 column-major, user slots as in
+ *     the .cfg file (the derived slots are filled here, as ia_*_setup does);
+ *   - atom types, pair types and styles are 1-based, as in the .cfg file.
+ *
+ * Every call is synchronous with respect to the host buffers it takes.  A
+ * context owns all device memory and one HIP stream; it is not thread-safe
+ * (use one context per host thread).  No exception crosses the ABI.
+ *
+ * Return codes mirror the reference's ierr:
+ *   BD_OK 0, BD_ERR_BOND 1 (bond overstretched, ia_bond_m.f90:213-217),
+ *   BD_ERR_FACTOR 2 (Cholesky/eigen failure, brown_m.f90:540-544),
+ *   BD_WARN_NOTCONV 3 (Lanczos not converged; results valid, bd_solver_m.f90:
+ *   393-397), BD_ERR_ARG -1, BD_ERR_HIP -2, BD_ERR_STATE -3, BD_ERR_CELL -4
+ *   (negative coordinate in cell-list mode, cell_list_m.f90:193-196),
+ *   BD_ERR_CAPACITY -5.  bd_last_error() returns a message for the calling
+ *   thread's last failure.
+ */
+#ifndef BDSTEP_H
+#define BDSTEP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bd_ctx bd_ctx;
+
+enum {
+    BD_OK = 0,
+    BD_ERR_BOND = 1,
+    BD_ERR_FACTOR = 2,
+    BD_WARN_NOTCONV = 3,
+    BD_ERR_ARG = -1,
+    BD_ERR_HIP = -2,
+    BD_ERR_STATE = -3,
+    BD_ERR_CELL = -4,
+    BD_ERR_CAPACITY = -5
+};
+
+/* mob_fctr (control key mob_fctr, bd_solver_m.f90:64) */
+enum {
+    BD_CHOL = 0,    /* 'CHOL': B.xi = U.xi with D = U^T U (reference behaviour) */
+    BD_LANC = 1,    /* 'LANC': Lanczos (s = 1) / block Lanczos (s > 1)          */
+    BD_CHOL_LT = 2  /* Cholesky with U^T.xi, whose covariance is D               */
+};
+
+/* precision of the mobility storage and of the D.z / D.V products */
+enum {
+    BD_FP64 = 0,    /* FP64 storage, FP64 arithmetic                            */
+    BD_FP32 = 1     /* FP32 storage, FP64 accumulation; Lanczos small problem,
+                       QR and recurrences stay FP64                             */
+};
+
+/* mth_ptgen (pairtab_m.fpp:23-24) */
+enum { BD_PT_DIR = 0, BD_PT_VER = 1, BD_PT_CL = 2 };
+
+typedef struct {
+    int lhdia;            /* hydrodynamic interaction (RPY) on/off            */
+    int mob_fctr;         /* BD_CHOL / BD_LANC / BD_CHOL_LT                    */
+    int nts_mobsam;       /* s: mobility reuse interval == Lanczos block size */
+    int lanc_mxitr;       /* Krylov iteration cap (dimk_max = max(2,.)+1)     */
+    double lanc_tol;      /* relative-change tolerance                        */
+    double tim_stp;       /* dt                                               */
+    int precision;        /* BD_FP64 / BD_FP32                                 */
+    int keep_reuse_noise; /* 0: reference behaviour, the diffusion block is
+                             zeroed on every step (bd_solver_m.f90:364), so
+                             steps 2..s of a block get no noise; 1: keep it    */
+    int flow_style;       /* 0 none, 1 shear, 2 planar, 3 uniaxial (420-441)   */
+    double flow_rate;     /* flow_params(1)                                    */
+    int lanc_reorth;      /* 1: full FP64 reorthogonalisation of every new
+                             Krylov block (not in the reference, whose Lanczos
+                             loses orthogonality; see brown_m.f90:402-435)     */
+} bd_opts;
+
+/* Context.  Replaces bds_init/brn_init (bd_solver_m.f90:50-104,
+ * brown_m.f90:73-118): allocates D, the diffusion block and the Krylov
+ * workspaces on the current HIP device.  Returns NULL on failure. */
+bd_ctx *bd_ctx_create(int nbeads, const bd_opts *opts);
+/* bds_finish/brn_finish/ia_finish (bd_solver_m.f90:108-124). */
+void bd_ctx_destroy(bd_ctx *ctx);
+const char *bd_last_error(void);
+
+/* Simulation box: imcon 0 unbounded, 1 periodic orthorhombic (simbox_m). */
+int bd_set_box(bd_ctx *ctx, int imcon, const double box[3]);
+
+/* ia_setup + pt_init (interaction_m.f90:30-71, pairtab_m.fpp:36-98):
+ * topology, exclusions (connectivity_m.f90:43-317) and potential
+ * parameters.  Bond styles 1-4 and vdw style 1 are supported; other vdw
+ * styles are rejected with BD_ERR_ARG. */
+int bd_set_topology(bd_ctx *ctx, int nbonds, const int *bonds, int nbond_types,
+                    const int *bond_styles, const double *bond_params,
+                    int natom_types, const int *atom_types, int nvdw_types,
+                    const int *vdw_styles, const double *vdw_params, int lvdw,
+                    int excluded_atoms, int mth_ptgen, double rcutoff,
+                    double tskin);
+
+/* Host <-> device coordinates, (3, N) FP64. */
+int bd_set_coordinates(bd_ctx *ctx, const double *crd);
+int bd_get_coordinates(bd_ctx *ctx, double *crd);
+
+/* calc_rpy_tensor (bd_solver_m.f90:458-522) on the device coordinates. */
+int bd_rpy_assemble(bd_ctx *ctx);
+/* Copies the assembled mobility to the host as the full n x n column-major
+ * matrix (unit diagonal), widening FP32 storage to FP64. */
+int bd_get_mobility(bd_ctx *ctx, double *mob);
+
+/* dsymv/dsymm('L') products (bd_solver_m.f90:446, brown_m.f90:386):
+ * y(n,s) := D x(n,s) with the assembled D. */
+int bd_mob_symv(bd_ctx *ctx, const double *x, int s, double *y);
+
+/* brn_calc_dw + brn_calc_bdw (brown_m.f90:147-181, 558-573) on the
+ * assembled D, with the injected Gaussian block xi (n, s) in place of MKL
+ * VSL.  method: BD_CHOL, BD_LANC, BD_CHOL_LT.  out receives B.xi (n, s).
+ * lconv/iters as the reference's lconv/f.  With BD_CHOL the assembled D is
+ * kept intact (the factor lives in its own buffer). */
+int bd_brownian(bd_ctx *ctx, const double *xi, int s, int method, double *out,
+                int *lconv, int *iters);
+
+/* ia_calc_forces (interaction_m.f90:84-147) at the device coordinates,
+ * building the pair table with the method chosen in bd_set_topology.
+ * Any output pointer may be NULL.  energies = [bond, vdw, total];
+ * stress9 (3x3 column-major, divided by the volume if periodic);
+ * bndlen3 = [mean, min, max] bond length. */
+int bd_forces(bd_ctx *ctx, double *frc, double *energies, double *stress9,
+              double *bndlen3);
+
+/* The pair table used by the last bd_forces call, as a half list of
+ * unordered pairs (i < j, 1-based), sorted by (i, j).  *npairs receives
+ * the count; returns BD_ERR_CAPACITY if cap is too small (count still
+ * reported).  rebuilt (may be NULL) reports whether the last call rebuilt
+ * the Verlet list (always 1 for DIR/CL). */
+int bd_pair_table(bd_ctx *ctx, int *pairs2, int cap, int *npairs, int *rebuilt);
+
+/* nsteps iterations of the bds_run loop body (bd_solver_m.f90:167-189):
+ * integrate_em, nts+1, centre-of-mass removal for an isolated molecule or
+ * PBC wrap.  xi is the injected Gaussian stream consumed exactly as
+ * brn_calc_dw would (n*s values per mobility update with HI, n per step
+ * without); *used receives the number of values consumed.  With xi ==
+ * NULL the device Philox generator (seeded by bd_set_rng) supplies it. */
+int bd_run(bd_ctx *ctx, int nsteps, const double *xi, long long nxi,
+           long long *used);
+/* Seeds the device Gaussian generator (Philox-4x32-10 + Box-Muller). */
+int bd_set_rng(bd_ctx *ctx, uint64_t seed, uint64_t offset);
+
+/* Run statistics: nts, molecule_com[3] (accumulated COM shift),
+ * Lanczos iterations of the last update, total iterations, mobility
+ * updates, non-converged updates. */
+typedef struct {
+    long long nts;
+    double molecule_com[3];
+    int last_lanczos_iters;
+    int total_lanczos_iters;
+    int n_mob_updates;
+    int n_not_converged;
+    int n_pair_rebuilds;
+} bd_runinfo;
+int bd_get_runinfo(bd_ctx *ctx, bd_runinfo *info);
+/* Resets the step counters and the reuse counter cntr_mobsam (a fresh
+ * bds_run), keeping coordinates and topology. */
+int bd_reset_run(bd_ctx *ctx);
+
+/* Rg^2 of the device coordinates (compute_ic0, stats_m.f90:441-446). */
+int bd_rgsq(bd_ctx *ctx, double *rgsq);
+
+/* Synchronises the context's stream (timing helper). */
+int bd_sync(bd_ctx *ctx);
+
+/* Measurement helpers (no reference counterpart; the reference's only timer
+ * is the wall clock around bds_run, setup_m.f90:116-124).  With profiling
+ * on, every launch of the mobility tile-product kernel is bracketed by HIP
+ * events on the context's stream; bd_prof_get returns the summed kernel
+ * time (ms), the launch count and the bytes of stored mobility tiles one
+ * launch streams.  bd_prof_enable(ctx, 1) also clears the totals. */
+int bd_prof_enable(bd_ctx *ctx, int on);
+int bd_prof_get(bd_ctx *ctx, double *total_ms, long long *launches,
+                double *tile_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BDSTEP_H */

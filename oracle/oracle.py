@@ -1,0 +1,344 @@
+"""ctypes wrapper around the CPU oracle (oracle/bdoracle.c).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may import
+this module; the product (``brownpak_amd``) never does.
+
+Arrays follow the reference's layout: coordinates are (N, 3) C-order here,
+which is byte-identical to Fortran's column-major (3, N); the mobility matrix
+is returned as an (n, n) array whose memory is column-major (n = 3N).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libbdoracle.so")
+_lib = None
+
+MXPARAM = 12
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+
+
+class OrcSystem(C.Structure):
+    _fields_ = [
+        ("nbeads", C.c_int), ("imcon", C.c_int), ("box", C.c_double * 3),
+        ("nbonds", C.c_int), ("bonds", IP),
+        ("nbond_types", C.c_int), ("bond_styles", IP), ("bond_params", DP),
+        ("natom_types", C.c_int), ("atom_types", IP),
+        ("nvdw_types", C.c_int), ("vdw_styles", IP), ("vdw_params", DP),
+        ("lvdw", C.c_int), ("excluded_atoms", C.c_int), ("mth_ptgen", C.c_int),
+        ("rcutoff", C.c_double), ("tskin", C.c_double),
+    ]
+
+
+class OrcEnergies(C.Structure):
+    _fields_ = [
+        ("energy_bond", C.c_double), ("energy_vdw", C.c_double), ("energy_tot", C.c_double),
+        ("stress", C.c_double * 9),
+        ("bndlen", C.c_double), ("bndlen_min", C.c_double), ("bndlen_max", C.c_double),
+    ]
+
+
+class OrcBdopts(C.Structure):
+    _fields_ = [
+        ("lhdia", C.c_int), ("mob_fctr", C.c_int), ("nts_mobsam", C.c_int),
+        ("lanc_mxitr", C.c_int), ("lanc_tol", C.c_double), ("tim_stp", C.c_double),
+        ("keep_reuse_noise", C.c_int), ("flow_style", C.c_int), ("flow_rate", C.c_double),
+        ("lanc_reorth", C.c_int),
+    ]
+
+
+class OrcRuninfo(C.Structure):
+    _fields_ = [
+        ("nts", C.c_longlong), ("molecule_com", C.c_double * 3),
+        ("last_lanczos_iters", C.c_int), ("n_not_converged", C.c_int), ("ierr", C.c_int),
+        ("total_lanczos_iters", C.c_int), ("n_mob_updates", C.c_int),
+    ]
+
+
+class OrcVerlet(C.Structure):
+    _fields_ = [("first_call", C.c_int), ("crd_save", DP),
+                ("rskin_sq", C.c_double), ("tskin_sq", C.c_double)]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE],
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = C.CDLL(_LIB_PATH)
+        L = _lib
+        L.orc_rpy.argtypes = [C.c_int, DP, DP]
+        L.orc_symv_lower.argtypes = [C.c_int, C.c_double, DP, DP, C.c_double, DP]
+        L.orc_chol_bdw.argtypes = [C.c_int, C.c_int, DP, DP]
+        L.orc_chol_bdw.restype = C.c_int
+        L.orc_lanczos.argtypes = [C.c_int, DP, DP, C.c_int, C.c_double, IP, IP]
+        L.orc_block_lanczos.argtypes = [C.c_int, C.c_int, DP, DP, C.c_int, C.c_double, IP, IP]
+        L.orc_brn_calc_bdw.argtypes = [C.c_int, C.c_int, C.c_int, DP, DP, C.c_int,
+                                       C.c_double, IP, IP]
+        L.orc_brn_calc_bdw_ex.argtypes = [C.c_int, C.c_int, C.c_int, DP, DP, C.c_int,
+                                          C.c_double, C.c_int, IP, IP]
+        L.orc_geqp3.argtypes = [C.c_int, C.c_int, DP, IP, DP]
+        L.orc_orgqr.argtypes = [C.c_int, C.c_int, DP, DP]
+        L.orc_syev.argtypes = [C.c_int, DP, DP]
+        L.orc_setup_params.argtypes = [C.POINTER(OrcSystem)]
+        L.orc_exclusions.argtypes = [C.c_int, C.c_int, IP, C.c_int, IP, IP, C.c_int]
+        L.orc_pairs_n2.argtypes = [C.POINTER(OrcSystem), IP, IP, DP, C.c_double, IP, IP, C.c_int]
+        L.orc_pairs_verlet.argtypes = [C.POINTER(OrcSystem), IP, IP, DP, C.POINTER(OrcVerlet),
+                                       IP, IP, C.c_int, IP]
+        L.orc_pairs_cell_list.argtypes = [C.POINTER(OrcSystem), IP, IP, DP, IP, IP, C.c_int]
+        L.orc_forces.argtypes = [C.POINTER(OrcSystem), DP, IP, IP, DP, C.POINTER(OrcEnergies)]
+        L.orc_get_image.argtypes = [DP, DP]
+        L.orc_wrap_all.argtypes = [DP, C.c_int, DP]
+        L.orc_to_origin.argtypes = [C.c_int, DP, DP]
+        L.orc_rgsq.argtypes = [C.c_int, DP]
+        L.orc_rgsq.restype = C.c_double
+        L.orc_bd_run.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcBdopts), DP, C.c_int, DP,
+                                 C.c_longlong, C.POINTER(OrcRuninfo)]
+        L.orc_bd_run.restype = C.c_longlong
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(IP)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- mobility
+def rpy(crd):
+    """calc_rpy_tensor: returns the full n x n mobility (symmetric)."""
+    crd = _f64(crd)
+    nb = crd.shape[0]
+    mob = np.empty((3 * nb, 3 * nb), dtype=np.float64)
+    lib().orc_rpy(nb, _dp(crd), _dp(mob))
+    return mob.T.copy()  # column-major memory -> logical (row, col)
+
+
+def symv_lower(mob_colmajor_logical, x, alpha=1.0, beta=0.0, y=None):
+    a = np.asfortranarray(mob_colmajor_logical, dtype=np.float64)
+    n = a.shape[0]
+    x = _f64(x)
+    y = np.zeros(n) if y is None else _f64(y).copy()
+    lib().orc_symv_lower(n, alpha, a.ctypes.data_as(DP), _dp(x), beta, _dp(y))
+    return y
+
+
+# ------------------------------------------------------------ Brownian B.xi
+CHOL, LANC, CHOL_LT = 0, 1, 2
+
+
+def brn_calc_bdw(method, mob, xi, lanc_mxitr=50, lanc_tol=1e-6, reorth=False):
+    """brn_calc_bdw.  mob (n,n) logical, xi (n,s).  Returns (bdw, lconv, iters, ierr)."""
+    a = np.asfortranarray(mob, dtype=np.float64).copy(order="F")
+    xi = np.asfortranarray(xi, dtype=np.float64)
+    if xi.ndim == 1:
+        xi = xi[:, None]
+    n, s = xi.shape
+    b = xi.copy(order="F")
+    lconv = C.c_int(0)
+    iters = C.c_int(0)
+    ierr = lib().orc_brn_calc_bdw_ex(method, n, s, a.ctypes.data_as(DP), b.ctypes.data_as(DP),
+                                     lanc_mxitr, lanc_tol, int(bool(reorth)), C.byref(lconv),
+                                     C.byref(iters))
+    return b, bool(lconv.value), iters.value, ierr
+
+
+def geqp3(a):
+    a = np.asfortranarray(a, dtype=np.float64).copy(order="F")
+    m, n = a.shape
+    jpvt = np.zeros(n, dtype=np.int32)
+    tau = np.zeros(min(m, n))
+    lib().orc_geqp3(m, n, a.ctypes.data_as(DP), _ip(jpvt), _dp(tau))
+    return a, jpvt, tau
+
+
+def orgqr(a, tau):
+    a = np.asfortranarray(a, dtype=np.float64).copy(order="F")
+    m, n = a.shape
+    lib().orc_orgqr(m, n, a.ctypes.data_as(DP), _dp(_f64(tau)))
+    return a
+
+
+def syev(a):
+    a = np.asfortranarray(a, dtype=np.float64).copy(order="F")
+    n = a.shape[0]
+    w = np.zeros(n)
+    info = lib().orc_syev(n, a.ctypes.data_as(DP), _dp(w))
+    return w, a, info
+
+
+# ----------------------------------------------------------------- system
+class System:
+    """Holds the topology/parameter arrays alive for an OrcSystem struct.
+
+    bonds: (nb, 3) int [type, i, j], 1-based (memory == Fortran (3, nb)).
+    bond_params / vdw_params: (ntypes, 12) (memory == Fortran (12, ntypes)).
+    """
+
+    def __init__(self, nbeads, bonds, bond_styles, bond_params, atom_types=None,
+                 vdw_styles=(), vdw_params=None, natom_types=1, lvdw=True,
+                 excluded_atoms=0, mth_ptgen=0, rcutoff=0.0, tskin=0.0,
+                 imcon=0, box=(0.0, 0.0, 0.0)):
+        self.bonds = _i32(np.asarray(bonds).reshape(-1, 3)) if len(bonds) else np.zeros((0, 3), np.int32)
+        self.bond_styles = _i32(bond_styles) if len(bond_styles) else np.zeros(1, np.int32)
+        bp = np.zeros((max(len(bond_styles), 1), MXPARAM))
+        if bond_params is not None and len(bond_styles):
+            bpi = np.atleast_2d(np.asarray(bond_params, dtype=np.float64))
+            bp[:, :bpi.shape[1]] = bpi
+        self.bond_params = bp
+        self.atom_types = _i32(atom_types if atom_types is not None else np.ones(nbeads, np.int32))
+        self.vdw_styles = _i32(vdw_styles) if len(vdw_styles) else np.zeros(1, np.int32)
+        vp = np.zeros((max(len(vdw_styles), 1), MXPARAM))
+        if vdw_params is not None and len(vdw_styles):
+            vpi = np.atleast_2d(np.asarray(vdw_params, dtype=np.float64))
+            vp[:, :vpi.shape[1]] = vpi
+        self.vdw_params = vp
+        s = OrcSystem()
+        s.nbeads = nbeads
+        s.imcon = imcon
+        s.box = (C.c_double * 3)(*box)
+        s.nbonds = self.bonds.shape[0]
+        s.bonds = _ip(self.bonds)
+        s.nbond_types = len(bond_styles)
+        s.bond_styles = _ip(self.bond_styles)
+        s.bond_params = _dp(self.bond_params)
+        s.natom_types = natom_types
+        s.atom_types = _ip(self.atom_types)
+        s.nvdw_types = len(vdw_styles)
+        s.vdw_styles = _ip(self.vdw_styles)
+        s.vdw_params = _dp(self.vdw_params)
+        s.lvdw = int(bool(lvdw))
+        s.excluded_atoms = excluded_atoms
+        s.mth_ptgen = mth_ptgen
+        s.rcutoff = rcutoff
+        s.tskin = tskin
+        self.s = s
+        lib().orc_setup_params(C.byref(s))
+
+    @property
+    def nbeads(self):
+        return self.s.nbeads
+
+
+def exclusions(sysobj):
+    nb = sysobj.nbeads
+    cap = nb * 64 + 16
+    ptr = np.zeros(nb + 1, np.int32)
+    col = np.zeros(cap, np.int32)
+    m = lib().orc_exclusions(nb, sysobj.bonds.shape[0], _ip(sysobj.bonds),
+                             sysobj.s.excluded_atoms, _ip(ptr), _ip(col), cap)
+    assert m >= 0
+    return ptr, col[:m].copy()
+
+
+def pairs(sysobj, crd, method="DIR", cap=None, verlet_state=None):
+    """Pair table as (row_ptr, col) with 1-based columns (pairtab_m.fpp)."""
+    crd = _f64(crd)
+    nb = sysobj.nbeads
+    cap = cap or max(nb * nb // 2 + 16, 1024)
+    ex_ptr, ex_col = exclusions(sysobj)
+    ex_col = ex_col if ex_col.size else np.zeros(1, np.int32)
+    ptr = np.zeros(nb + 1, np.int32)
+    col = np.zeros(cap, np.int32)
+    L = lib()
+    if method == "DIR":
+        np_ = L.orc_pairs_n2(C.byref(sysobj.s), _ip(ex_ptr), _ip(ex_col), _dp(crd),
+                             sysobj.s.rcutoff ** 2, _ip(ptr), _ip(col), cap)
+    elif method == "VER":
+        st = verlet_state
+        rebuilt = C.c_int(0)
+        np_ = L.orc_pairs_verlet(C.byref(sysobj.s), _ip(ex_ptr), _ip(ex_col), _dp(crd),
+                                 C.byref(st.v), _ip(ptr), _ip(col), cap, C.byref(rebuilt))
+        if np_ == -2:
+            return None
+    elif method == "CL":
+        np_ = L.orc_pairs_cell_list(C.byref(sysobj.s), _ip(ex_ptr), _ip(ex_col), _dp(crd),
+                                    _ip(ptr), _ip(col), cap)
+    else:
+        raise ValueError(method)
+    if np_ < 0:
+        raise RuntimeError(f"pair build failed ({np_})")
+    return ptr, col[:np_].copy()
+
+
+class VerletState:
+    def __init__(self, sysobj):
+        nb = sysobj.nbeads
+        self.save = np.zeros(3 * nb)
+        self.v = OrcVerlet()
+        self.v.first_call = 1
+        self.v.crd_save = _dp(self.save)
+        self.v.rskin_sq = (sysobj.s.rcutoff + sysobj.s.tskin) ** 2
+        self.v.tskin_sq = sysobj.s.tskin ** 2
+
+
+def pair_set(ptr, col):
+    """Set of unordered 1-based pairs (min, max) from a CSR pair table."""
+    out = set()
+    for i in range(len(ptr) - 1):
+        for k in range(ptr[i], ptr[i + 1]):
+            j = int(col[k])
+            a, b = (i + 1, j) if i + 1 < j else (j, i + 1)
+            out.add((a, b))
+    return out
+
+
+def forces(sysobj, crd, table=None):
+    """ia_calc_forces over the given pair table.  Returns (frc (N,3), energies)."""
+    crd = _f64(crd)
+    nb = sysobj.nbeads
+    frc = np.zeros((nb, 3))
+    en = OrcEnergies()
+    if table is None:
+        ptr_p = None
+        col_p = None
+        ierr = lib().orc_forces(C.byref(sysobj.s), _dp(crd), None, None, _dp(frc), C.byref(en))
+    else:
+        ptr, col = table
+        col = col if col.size else np.zeros(1, np.int32)
+        ptr_p, col_p = _ip(ptr), _ip(col)
+        ierr = lib().orc_forces(C.byref(sysobj.s), _dp(crd), ptr_p, col_p, _dp(frc), C.byref(en))
+    return frc, en, ierr
+
+
+def rgsq(crd):
+    crd = _f64(crd)
+    return lib().orc_rgsq(crd.shape[0], _dp(crd))
+
+
+def bd_run(sysobj, crd, nsteps, xi, lhdia=True, mob_fctr=CHOL, nts_mobsam=1,
+           lanc_mxitr=50, lanc_tol=1e-6, tim_stp=1e-3, keep_reuse_noise=False,
+           flow_style=0, flow_rate=0.0, lanc_reorth=False):
+    """bds_run (EM) for nsteps with the injected Gaussian stream xi."""
+    crd = _f64(crd).copy()
+    xi = _f64(np.ravel(xi))
+    opt = OrcBdopts(int(lhdia), mob_fctr, nts_mobsam, lanc_mxitr, lanc_tol, tim_stp,
+                    int(keep_reuse_noise), flow_style, flow_rate, int(bool(lanc_reorth)))
+    info = OrcRuninfo()
+    used = lib().orc_bd_run(C.byref(sysobj.s), C.byref(opt), _dp(crd), nsteps, _dp(xi),
+                            xi.size, C.byref(info))
+    return crd, used, info

@@ -1,0 +1,115 @@
+"""Shared builders for oracle/GPU parity tests (test infrastructure)."""
+import numpy as np
+
+from brownpak_amd.configs import chain_bonds, fene_wca_params, random_walk_chain
+
+SIGMA = 2.0
+
+
+def fene_wca_chain(nbeads, seed):
+    """BASELINE single-chain model in reference units (a = 1, sigma = 2):
+    FENE k = 7.5, Q0 = 3.0, r0 = 0 (bond style 2); WCA eps = 1, sigma = 2,
+    rc = 2^(1/6) sigma (vdw style 1); no exclusions; random walk, bond 1.94."""
+    crd = random_walk_chain(nbeads, seed)
+    bond, vdw = fene_wca_params()
+    return dict(crd=crd, bonds=chain_bonds(nbeads), bond_styles=[2], bond_params=[bond],
+                vdw_styles=[1], vdw_params=[vdw], natom_types=1, excluded_atoms=0,
+                rcutoff=vdw[2], tskin=0.0, mth_ptgen="DIR", imcon=0, box=(0.0, 0.0, 0.0))
+
+
+def oracle_system(O, m, mth_ptgen=None):
+    mth = {"DIR": 0, "VER": 1, "CL": 2}[mth_ptgen or m["mth_ptgen"]]
+    nb = len(m["crd"])
+    return O.System(nb, m["bonds"], m["bond_styles"], m["bond_params"],
+                    atom_types=m.get("atom_types"), vdw_styles=m["vdw_styles"],
+                    vdw_params=m["vdw_params"], natom_types=m["natom_types"], lvdw=True,
+                    excluded_atoms=m["excluded_atoms"], mth_ptgen=mth, rcutoff=m["rcutoff"],
+                    tskin=m["tskin"], imcon=m["imcon"], box=m["box"])
+
+
+def gpu_system(m, **opts):
+    from brownpak_amd import BdStep
+    nb = len(m["crd"])
+    g = BdStep(nb, **opts)
+    g.set_box(m["imcon"], m["box"])
+    g.ia_setup(m["bonds"], m["bond_styles"], m["bond_params"], atom_types=m.get("atom_types"),
+               natom_types=m["natom_types"], vdw_styles=m["vdw_styles"], vdw_params=m["vdw_params"],
+               excluded_atoms=m["excluded_atoms"], mth_ptgen=m["mth_ptgen"], rcutoff=m["rcutoff"],
+               tskin=m["tskin"])
+    g.set_coordinates(m["crd"])
+    return g
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def read_cfg(path):
+    """Minimal reader of the reference .cfg format (config_io_m.f90:247-544)
+    for SIMBOX, ATOM_TYPES, ATOMS, BOND_TYPES, BONDS and VDW sections."""
+    lines = []
+    with open(path) as fh:
+        for ln in fh:
+            ln = ln.split("#")[0].strip()
+            if ln:
+                lines.append(ln)
+    out = dict(imcon=0, box=(0.0, 0.0, 0.0))
+    i = 0
+    while i < len(lines):
+        w = lines[i].split()
+        key = w[0]
+        if key == "SIMBOX":
+            if len(w) > 1:
+                imcon = int(w[1])
+                mat = [list(map(float, lines[i + k].split())) for k in (1, 2, 3)]
+                i += 4
+            else:
+                mat = [list(map(float, lines[i + k].split())) for k in (1, 2, 3)]
+                imcon = int(lines[i + 4].split()[-1]) if lines[i + 4].split()[0] == "IMCON" else int(lines[i + 4].split()[0])
+                i += 5
+            out["imcon"] = imcon
+            out["box"] = (mat[0][0], mat[1][1], mat[2][2])
+            continue
+        if key == "ATOM_TYPES":
+            nt = int(w[1])
+            out["natom_types"] = nt
+            i += 1 + nt
+            continue
+        if key == "ATOMS":
+            n = int(w[1])
+            rows = [lines[i + 1 + k].split() for k in range(n)]
+            out["atom_types"] = np.array([int(r[0]) for r in rows], np.int32)
+            out["crd"] = np.array([[float(x) for x in r[2:5]] for r in rows])
+            i += 1 + n
+            continue
+        if key == "BOND_TYPES":
+            nt = int(w[1])
+            sty, par = [], []
+            for k in range(nt):
+                r = lines[i + 1 + k].split()
+                sty.append(int(r[0]))
+                npar = int(r[1])
+                par.append([float(x) for x in r[2:2 + npar]])
+            out["bond_styles"], out["bond_params"] = sty, par
+            i += 1 + nt
+            continue
+        if key == "BONDS":
+            n = int(w[1])
+            out["bonds"] = np.array([[int(x) for x in lines[i + 1 + k].split()[:3]] for k in range(n)], np.int32)
+            i += 1 + n
+            continue
+        if key == "VDW":
+            nt = int(w[1])
+            sty, par = [], []
+            for k in range(nt):
+                r = lines[i + 1 + k].split()
+                sty.append(int(r[2]))
+                npar = int(r[3])
+                par.append([float(x) for x in r[4:4 + npar]])
+            out["vdw_styles"], out["vdw_params"] = sty, par
+            i += 1 + nt
+            continue
+        i += 1
+    return out

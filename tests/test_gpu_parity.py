@@ -1,0 +1,305 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the
+same positions and the same injected Gaussian vectors.
+
+Bars:
+  * pair sets: bit-exact;
+  * mobility, D.V, Cholesky B.xi, forces, energies, stress, steps: 1e-10
+    relative in FP64 (measured norm-wise);
+  * Lanczos: 1e-10 at a fixed iteration count and with FP64
+    reorthogonalisation; converged runs of the reference algorithm (no
+    reorthogonalisation) within 10 * lanc_tol and +-1 iteration, because that
+    algorithm's own result moves by ~tol under 1e-16 perturbations
+    (tests/test_oracle.py::test_reference_lanczos_sensitivity);
+  * FP32-storage mixed precision: <= 1e-6 relative against FP64.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import fene_wca_chain, gpu_system, oracle_system, read_cfg, rel
+from brownpak_amd.configs import chain_bonds, periodic_melt, random_walk_chain
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-10
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def chain100():
+    return fene_wca_chain(100, 101)
+
+
+@pytest.fixture(scope="module")
+def chain2000():
+    return fene_wca_chain(2000, 2000)
+
+
+# ------------------------------------------------------------- mobility
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_rpy_assembly_n100(oracle, chain100, prec):
+    g = gpu_system(chain100, mob_fctr="CHOL", precision=prec)
+    g.calc_rpy_tensor()
+    D = g.get_mobility()
+    Dref = oracle.rpy(chain100["crd"])
+    bound = 1e-15 if prec == "fp64" else 6e-8
+    assert np.abs(D - Dref).max() <= bound * np.abs(Dref).max()
+    assert np.array_equal(D, D.T)
+
+
+def test_rpy_assembly_and_product_n2000(oracle, chain2000):
+    _say("case: rpy n2000")
+    g = gpu_system(chain2000, mob_fctr="LANC", nts_mobsam=4)
+    g.calc_rpy_tensor()
+    D = g.get_mobility()
+    Dref = oracle.rpy(chain2000["crd"])
+    assert np.abs(D - Dref).max() <= 1e-15
+    x = np.random.default_rng(7).standard_normal((6000, 4))
+    y = g.mob_symv(x)
+    assert rel(y, Dref @ x) < 1e-13
+    y1 = g.mob_symv(x[:, 0])
+    assert rel(y1, Dref @ x[:, 0]) < 1e-13
+
+
+@pytest.mark.parametrize("s", [1, 2, 3, 4, 5, 8])
+def test_mob_symv_block_sizes(oracle, chain100, s):
+    g = gpu_system(chain100, mob_fctr="CHOL", nts_mobsam=s)
+    g.calc_rpy_tensor()
+    Dref = oracle.rpy(chain100["crd"])
+    x = np.random.default_rng(s).standard_normal((300, s))
+    assert rel(g.mob_symv(x), Dref @ x) < 1e-13
+
+
+# ------------------------------------------------------------- Cholesky
+@pytest.mark.parametrize("method", ["CHOL", "CHOL_LT"])
+def test_cholesky_bdw(oracle, chain100, method):
+    g = gpu_system(chain100, mob_fctr=method, nts_mobsam=2)
+    g.calc_rpy_tensor()
+    xi = np.random.default_rng(11).standard_normal((300, 2))
+    out, lc, it = g.brn_calc_bdw(xi, method=method)
+    ref, _, _, ierr = oracle.brn_calc_bdw(oracle.CHOL if method == "CHOL" else oracle.CHOL_LT,
+                                          oracle.rpy(chain100["crd"]), xi)
+    assert ierr == 0 and rel(out, ref) < TOL
+
+
+# ------------------------------------------------------------- Lanczos
+@pytest.mark.parametrize("nb,s,mx", [(100, 1, 12), (100, 4, 7), (100, 3, 7), (2000, 4, 7)])
+def test_lanczos_fixed_iterations(oracle, chain100, chain2000, nb, s, mx):
+    _say(f"case: lanczos fixed nb={nb} s={s}")
+    m = chain100 if nb == 100 else chain2000
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=s, lanc_mxitr=mx, lanc_tol=0.0)
+    g.calc_rpy_tensor()
+    xi = np.random.default_rng(nb + s).standard_normal((3 * nb, s))
+    out, lc, it = g.brn_calc_bdw(xi)
+    ref, lc2, it2, ierr = oracle.brn_calc_bdw(oracle.LANC, oracle.rpy(m["crd"]), xi, lanc_mxitr=mx,
+                                              lanc_tol=0.0)
+    assert not lc and not lc2 and it == it2 == mx + 1
+    assert rel(out, ref) < TOL
+
+
+@pytest.mark.parametrize("nb,s", [(100, 1), (100, 4), (2000, 4)])
+def test_lanczos_reorth_converged(oracle, chain100, chain2000, nb, s):
+    _say(f"case: lanczos reorth nb={nb} s={s}")
+    m = chain100 if nb == 100 else chain2000
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, lanc_reorth=True)
+    g.calc_rpy_tensor()
+    xi = np.random.default_rng(nb * s).standard_normal((3 * nb, s))
+    out, lc, it = g.brn_calc_bdw(xi)
+    ref, lc2, it2, _ = oracle.brn_calc_bdw(oracle.LANC, oracle.rpy(m["crd"]), xi, lanc_mxitr=50,
+                                           lanc_tol=1e-6, reorth=True)
+    _say(f"  iters gpu {it} oracle {it2} rel {rel(out, ref):.2e}")
+    assert lc and lc2 and it == it2
+    assert rel(out, ref) < TOL
+
+
+@pytest.mark.parametrize("nb,s", [(100, 1), (100, 4), (2000, 4)])
+def test_lanczos_reference_converged(oracle, chain100, chain2000, nb, s):
+    _say(f"case: lanczos reference nb={nb} s={s}")
+    m = chain100 if nb == 100 else chain2000
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6)
+    g.calc_rpy_tensor()
+    xi = np.random.default_rng(nb + 7 * s).standard_normal((3 * nb, s))
+    out, lc, it = g.brn_calc_bdw(xi)
+    D = oracle.rpy(m["crd"])
+    ref, lc2, it2, _ = oracle.brn_calc_bdw(oracle.LANC, D, xi, lanc_mxitr=50, lanc_tol=1e-6)
+    _say(f"  iters gpu {it} oracle {it2} rel {rel(out, ref):.2e}")
+    assert lc and lc2 and abs(it - it2) <= 1
+    assert rel(out, ref) < 1e-5
+
+
+def test_lanczos_not_converged_flag(chain100):
+    g = gpu_system(chain100, mob_fctr="LANC", nts_mobsam=4, lanc_mxitr=1, lanc_tol=1e-12)
+    g.calc_rpy_tensor()
+    out, lc, it = g.brn_calc_bdw(np.random.default_rng(3).standard_normal((300, 4)))
+    assert not lc and it == 3 and np.all(np.isfinite(out))
+
+
+def test_mixed_precision_error_n2000(chain2000):
+    _say("case: mixed precision n2000")
+    xi = np.random.default_rng(5).standard_normal((6000, 4))
+    outs = {}
+    for prec in ("fp64", "fp32"):
+        g = gpu_system(chain2000, mob_fctr="LANC", nts_mobsam=4, lanc_mxitr=50, lanc_tol=1e-6,
+                       precision=prec, lanc_reorth=True)
+        g.calc_rpy_tensor()
+        outs[prec] = g.brn_calc_bdw(xi)
+        f = np.random.default_rng(6).standard_normal(6000)
+        outs[prec + "_df"] = g.mob_symv(f)
+    e_b = rel(outs["fp32"][0], outs["fp64"][0])
+    e_df = rel(outs["fp32_df"], outs["fp64_df"])
+    _say(f"  mixed rel err: B.xi {e_b:.2e}  D.F {e_df:.2e}")
+    assert e_b < 1e-6 and e_df < 1e-6
+
+
+# ------------------------------------------------------------- forces
+def test_forces_chain(oracle, chain100):
+    g = gpu_system(chain100, mob_fctr="CHOL")
+    f, e, st, bl = g.ia_calc_forces()
+    so = oracle_system(oracle, chain100)
+    tab = oracle.pairs(so, chain100["crd"], "DIR")
+    fr, en, ierr = oracle.forces(so, chain100["crd"], tab)
+    assert ierr == 0
+    assert rel(f, fr) < TOL
+    assert abs(e[0] - en.energy_bond) <= TOL * abs(en.energy_bond)
+    assert abs(e[1] - en.energy_vdw) <= TOL * max(abs(en.energy_vdw), 1e-300)
+    assert rel(st.ravel(order="F"), np.array(en.stress)) < TOL
+    assert abs(bl[0] - en.bndlen) < 1e-12 and bl[1] == en.bndlen_min and bl[2] == en.bndlen_max
+    pairs, _ = g.pair_table()
+    assert set(map(tuple, pairs.tolist())) == oracle.pair_set(*tab)
+
+
+def test_forces_and_pairs_chns10(oracle):
+    m = read_cfg(os.path.join(GOLDEN, "chns-10.cfg"))
+    m.update(natom_types=1, excluded_atoms=1, rcutoff=m["vdw_params"][0][2], tskin=0.0, mth_ptgen="DIR")
+    g = gpu_system(m, lhdia=False)
+    f, e, st, bl = g.ia_calc_forces()
+    so = oracle_system(oracle, m)
+    tab = oracle.pairs(so, m["crd"], "DIR")
+    fr, en, _ = oracle.forces(so, m["crd"], tab)
+    assert rel(f, fr) < TOL
+    assert abs(e[2] - en.energy_tot) <= TOL * abs(en.energy_tot)
+    assert rel(st.ravel(order="F"), np.array(en.stress)) < TOL
+    pairs, _ = g.pair_table()
+    assert set(map(tuple, pairs.tolist())) == oracle.pair_set(*tab)
+
+
+def _melt(nchains, nper, seed):
+    crd, L = periodic_melt(nchains, nper, 0.3, seed)
+    bonds = np.concatenate([chain_bonds(nper, offset=c * nper) for c in range(nchains)])
+    return dict(crd=crd, bonds=bonds, bond_styles=[1], bond_params=[[25.0, 2.0]], vdw_styles=[1],
+                vdw_params=[[1.0, 2.0, 5.0]], natom_types=1, excluded_atoms=1, rcutoff=5.0, tskin=0.6,
+                mth_ptgen="VER", imcon=1, box=(L, L, L))
+
+
+@pytest.mark.parametrize("nchains,nper", [(20, 100), (200, 100)])
+def test_verlet_pairs_bit_exact_over_steps(oracle, nchains, nper):
+    _say(f"case: verlet melt {nchains}x{nper}")
+    m = _melt(nchains, nper, 31)
+    nb = nchains * nper
+    g = gpu_system(m, lhdia=False, tim_stp=1e-3)
+    so = oracle_system(oracle, m, "VER")
+    vs = oracle.VerletState(so)
+    rng = np.random.default_rng(31)
+    oracle_tab = None
+    rebuilds = 0
+    for step in range(6):
+        crd = g.get_coordinates()
+        f, _, _, _ = g.ia_calc_forces()
+        t = oracle.pairs(so, crd, "VER", verlet_state=vs)
+        if t is not None:
+            oracle_tab = t
+            rebuilds += 1
+        pairs, rebuilt = g.pair_table()
+        assert rebuilt == (t is not None)
+        assert set(map(tuple, pairs.tolist())) == oracle.pair_set(*oracle_tab)
+        fr, _, _ = oracle.forces(so, crd, oracle_tab)
+        assert rel(f, fr) < TOL
+        # a big random kick forces rebuilds on some steps
+        xi = rng.standard_normal(3 * nb) * (6.0 if step % 2 else 0.5)
+        g.bds_run(1, xi)
+    assert rebuilds >= 2
+
+
+# ------------------------------------------------------------- full steps
+def test_bd_run_cholesky_c1(oracle, chain100):
+    _say("case: bd_run chol c1")
+    nsteps = 20
+    xi = np.random.default_rng(41).standard_normal(300 * nsteps)
+    g = gpu_system(chain100, mob_fctr="CHOL", nts_mobsam=1)
+    used = g.bds_run(nsteps, xi)
+    out = g.get_coordinates()
+    so = oracle_system(oracle, chain100)
+    ref, used2, info = oracle.bd_run(so, chain100["crd"], nsteps, xi, mob_fctr=oracle.CHOL, nts_mobsam=1)
+    assert used == used2 == 300 * nsteps
+    assert rel(out, ref) < TOL
+    ri = g.runinfo()
+    assert ri["nts"] == nsteps
+    assert rel(np.array(ri["molecule_com"]), np.array(info.molecule_com)) < 1e-8
+
+
+@pytest.mark.parametrize("keep", [False, True])
+def test_bd_run_block_lanczos_reorth(oracle, chain100, keep):
+    nsteps, s = 8, 4
+    xi = np.random.default_rng(42).standard_normal(300 * s * 2)
+    g = gpu_system(chain100, mob_fctr="LANC", nts_mobsam=s, lanc_reorth=True, keep_reuse_noise=keep)
+    g.bds_run(nsteps, xi)
+    out = g.get_coordinates()
+    so = oracle_system(oracle, chain100)
+    ref, used, info = oracle.bd_run(so, chain100["crd"], nsteps, xi, mob_fctr=oracle.LANC, nts_mobsam=s,
+                                    lanc_reorth=True, keep_reuse_noise=keep)
+    assert rel(out, ref) < 1e-9
+    assert g.runinfo()["n_mob_updates"] == info.n_mob_updates == 2
+
+
+def test_bd_run_free_draining_periodic(oracle):
+    m = _melt(10, 50, 43)
+    nsteps = 10
+    xi = np.random.default_rng(43).standard_normal(1500 * nsteps)
+    g = gpu_system(m, lhdia=False)
+    g.bds_run(nsteps, xi)
+    out = g.get_coordinates()
+    so = oracle_system(oracle, m, "VER")
+    ref, _, _ = oracle.bd_run(so, m["crd"], nsteps, xi, lhdia=False)
+    assert rel(out, ref) < TOL
+
+
+def test_bond_overstretch_reports_ierr(chain100):
+    from brownpak_amd import BdError
+    m = dict(chain100)
+    crd = chain100["crd"].copy()
+    crd[50:] += np.array([5.0, 0.0, 0.0])   # bond 50-51 beyond rmax = 3
+    m["crd"] = crd
+    g = gpu_system(m, mob_fctr="CHOL")
+    with pytest.raises(BdError) as ei:
+        g.ia_calc_forces()
+    assert ei.value.code == 1
+    with pytest.raises(BdError) as ei:
+        g.bds_run(3, np.zeros(900))
+    assert ei.value.code == 1
+    assert g.runinfo()["nts"] == 0
+
+
+def test_rgsq(oracle, chain100):
+    g = gpu_system(chain100, mob_fctr="CHOL")
+    assert abs(g.rgsq() - oracle.rgsq(chain100["crd"])) < 1e-12 * oracle.rgsq(chain100["crd"])
+
+
+def test_device_rng_statistics(chain100):
+    g = gpu_system(chain100, lhdia=False, tim_stp=1e-3)
+    g.set_rng(1234)
+    # pure noise: zero forces by using no bonds and no vdw
+    from brownpak_amd import BdStep
+    h = BdStep(20000, lhdia=False, tim_stp=0.5)
+    h.set_box(0, (0, 0, 0))
+    h.ia_setup([], [], None, lvdw=False)
+    h.set_coordinates(np.zeros((20000, 3)))
+    h.set_rng(99)
+    h.bds_run(1)
+    d = h.get_coordinates().ravel()   # sqrt(2 dt) xi = xi for dt = 0.5, then COM removed
+    assert abs(d.mean()) < 1e-12
+    assert abs(d.std() - 1.0) < 0.01
+    assert abs(np.mean(d ** 4) / np.mean(d ** 2) ** 2 - 3.0) < 0.1
