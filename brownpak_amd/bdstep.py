@@ -49,6 +49,7 @@ EXPORTS = (
     "bd_set_coordinates", "bd_get_coordinates", "bd_rpy_assemble", "bd_get_mobility",
     "bd_mob_symv", "bd_brownian", "bd_forces", "bd_pair_table", "bd_run", "bd_set_rng",
     "bd_get_runinfo", "bd_reset_run", "bd_rgsq", "bd_sync", "bd_prof_enable", "bd_prof_get",
+    "bd_set_theta", "bd_run_record", "bd_run_replay", "bd_adjoint_rgsq",
 )
 
 
@@ -111,6 +112,10 @@ def load_library(path: str = LIB_PATH):
     L.bd_sync.argtypes = [vp]
     L.bd_prof_enable.argtypes = [vp, C.c_int]
     L.bd_prof_get.argtypes = [vp, DP, C.POINTER(C.c_longlong), DP]
+    L.bd_set_theta.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int]
+    L.bd_run_record.argtypes = [vp, C.c_int, DP, C.c_longlong, C.POINTER(C.c_longlong), DP]
+    L.bd_run_replay.argtypes = [vp, C.c_int, DP]
+    L.bd_adjoint_rgsq.argtypes = [vp, C.c_int, DP, C.c_int, C.c_int, DP, DP]
     for name in EXPORTS:
         if name not in ("bd_ctx_create", "bd_ctx_destroy", "bd_last_error"):
             getattr(L, name).restype = C.c_int
@@ -296,6 +301,34 @@ class BdStep:
 
     def sync(self):
         self._chk(self.L.bd_sync(self.h))
+
+    # -- differentiable trajectory (fixed noise)
+    def set_theta(self, eps, sigma, kbond, tie_rc=True):
+        self._chk(self.L.bd_set_theta(self.h, float(eps), float(sigma), float(kbond), int(bool(tie_rc))))
+
+    def run_record(self, nsteps, xi=None):
+        """bds_run that also returns the per-step displacements (nsteps, N, 3)."""
+        used = C.c_longlong(0)
+        out = np.zeros((int(nsteps), self.nbeads, 3))
+        if xi is None:
+            self._chk(self.L.bd_run_record(self.h, int(nsteps), None, 0, C.byref(used), _dp(out)))
+        else:
+            xa = _f64(np.ravel(xi))
+            self._chk(self.L.bd_run_record(self.h, int(nsteps), _dp(xa), xa.size, C.byref(used), _dp(out)))
+        return out
+
+    def run_replay(self, disp):
+        d = _f64(disp).reshape(-1, self.nbeads, 3)
+        self._chk(self.L.bd_run_replay(self.h, d.shape[0], _dp(d)))
+
+    def adjoint_rgsq(self, disp, ckpt=20, tie_rc=True):
+        """(final Rg^2, d/d(eps, sigma, k)) of a replayed trajectory."""
+        d = _f64(disp).reshape(-1, self.nbeads, 3)
+        loss = C.c_double(0.0)
+        g = np.zeros(3)
+        self._chk(self.L.bd_adjoint_rgsq(self.h, d.shape[0], _dp(d), int(ckpt), int(bool(tie_rc)),
+                                         C.byref(loss), _dp(g)))
+        return loss.value, g
 
     def prof_enable(self, on=True):
         self._chk(self.L.bd_prof_enable(self.h, int(bool(on))))

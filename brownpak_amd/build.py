@@ -49,7 +49,34 @@ def build_library(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran_driver(verbose)
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(OUT, "bdrun")
+
+
+def build_fortran_driver(verbose: bool = False) -> str:
+    """Fortran driver (iso_c_binding) linked against libbdstep.so."""
+    mod = os.path.join(OUT, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    for src in ("bdstep_c.f90", "bdrun.f90"):
+        obj = os.path.join(mod, src + ".o")
+        cmd = [FLANG, "-O2", "-c", os.path.join(FDIR, src), "-o", obj, "-module-dir", mod, "-I", mod]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {src}:\n{r.stderr[-4000:]}")
+        objs.append(obj)
+    cmd = [FLANG, *objs, "-o", DRIVER, "-L", OUT, "-lbdstep", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver link failed:\n{r.stderr[-4000:]}")
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

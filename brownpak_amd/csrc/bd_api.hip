@@ -186,6 +186,13 @@ __global__ __launch_bounds__(1024) void rgsq_kernel(const double *crd, int nb, d
     }
 }
 
+int launch_to_origin(Ctx &c, double *com_acc)
+{
+    hipLaunchKernelGGL(to_origin_kernel, dim3(1), dim3(1024), 0, c.stream, c.crd, c.nb, com_acc, c.st);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
 // ------------------------------------------------------------ helpers
 static int upload_xi(Ctx &c, const double *xi_host, int s)
 {
@@ -203,7 +210,7 @@ static int download_block(Ctx &c, const double *dev, double *host, int s)
     return BD_OK;
 }
 
-static int check_status(Ctx &c, bool clear)
+int check_status(Ctx &c, bool clear)
 {
     BD_HIP(hipMemcpyAsync(c.st_host, c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, c.stream));
     BD_HIP(hipStreamSynchronize(c.stream));
@@ -229,7 +236,11 @@ static int brownian_dev(Ctx &c, int method, int s, int *lconv, int *iters)
 // One EM step (integrate_em + the bds_run loop tail).  xi_host: next block of
 // the injected stream (or null for the device generator).  Returns the number
 // of xi values consumed via *consumed.
-static int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed)
+// disp_dev (npad, may be null): replay a recorded Brownian displacement
+// instead of computing B.xi; rec_dev (npad, may be null): record the
+// displacement this step applied (zero on steps without noise).
+int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed, const double *disp_dev,
+            double *rec_dev)
 {
     const int n = c.n, nb = c.nb;
     const bool hi = c.opt.lhdia != 0;
@@ -241,7 +252,8 @@ static int em_step(Ctx &c, const double *xi_host, long long avail, long long *co
     bool noise = true;
     // ---- calc_diffusion (bd_solver_m.f90:355-403)
     if (!hi) {
-        if (xi_host) {
+        if (disp_dev) {
+        } else if (xi_host) {
             if (avail < n) { set_error("injected xi stream exhausted"); return BD_ERR_ARG; }
             BD_TRY(upload_xi(c, xi_host, 1));
             *consumed = n;
@@ -249,39 +261,44 @@ static int em_step(Ctx &c, const double *xi_host, long long avail, long long *co
             BD_TRY(launch_gauss(c, c.xi_dev, n, c.rng_seed, c.rng_offset, c.npad, n, 1));
             c.rng_offset += (uint64_t)n + (n & 1);
         }
-        BD_HIP(hipMemcpyAsync(c.diff, c.xi_dev, (size_t)c.npad * sizeof(double), hipMemcpyDeviceToDevice, sm));
-        hipLaunchKernelGGL(scale_kernel, dim3((n + 255) / 256), dim3(256), 0, sm, c.diff, (long)n, sqrt_two_dt, c.st);
+        if (!disp_dev) {
+            BD_HIP(hipMemcpyAsync(c.diff, c.xi_dev, (size_t)c.npad * sizeof(double), hipMemcpyDeviceToDevice, sm));
+            hipLaunchKernelGGL(scale_kernel, dim3((n + 255) / 256), dim3(256), 0, sm, c.diff, (long)n, sqrt_two_dt,
+                               c.st);
+        }
     } else {
         if (c.cntr_mobsam == 0) {
             BD_TRY(rpy_assemble(c));
-            long long need = (long long)n * s;
-            if (xi_host) {
-                if (avail < need) { set_error("injected xi stream exhausted"); return BD_ERR_ARG; }
-                BD_TRY(upload_xi(c, xi_host, s));
-                *consumed = need;
-            } else {
-                BD_TRY(launch_gauss(c, c.xi_dev, need, c.rng_seed, c.rng_offset, c.npad, n, s));
-                c.rng_offset += (uint64_t)need + (need & 1);
-            }
-            int lconv = 1, iters = 0;
-            int method = c.opt.mob_fctr;
-            int rc = brownian_dev(c, method, s, &lconv, &iters);
-            if (rc != BD_OK) return rc;
-            if (method == BD_LANC) {
-                c.info.last_lanczos_iters = iters;
-                c.info.total_lanczos_iters += iters;
-                if (!lconv) c.info.n_not_converged++;
+            if (!disp_dev) {
+                long long need = (long long)n * s;
+                if (xi_host) {
+                    if (avail < need) { set_error("injected xi stream exhausted"); return BD_ERR_ARG; }
+                    BD_TRY(upload_xi(c, xi_host, s));
+                    *consumed = need;
+                } else {
+                    BD_TRY(launch_gauss(c, c.xi_dev, need, c.rng_seed, c.rng_offset, c.npad, n, s));
+                    c.rng_offset += (uint64_t)need + (need & 1);
+                }
+                int lconv = 1, iters = 0;
+                int method = c.opt.mob_fctr;
+                int rc = brownian_dev(c, method, s, &lconv, &iters);
+                if (rc != BD_OK) return rc;
+                if (method == BD_LANC) {
+                    c.info.last_lanczos_iters = iters;
+                    c.info.total_lanczos_iters += iters;
+                    if (!lconv) c.info.n_not_converged++;
+                }
+                hipLaunchKernelGGL(scale_kernel, dim3((unsigned)(((long)c.npad * s + 255) / 256)), dim3(256), 0, sm,
+                                   c.diff, (long)c.npad * s, sqrt_two_dt, c.st);
             }
             c.info.n_mob_updates++;
-            hipLaunchKernelGGL(scale_kernel, dim3((unsigned)(((long)c.npad * s + 255) / 256)), dim3(256), 0, sm,
-                               c.diff, (long)c.npad * s, sqrt_two_dt, c.st);
         } else if (!c.opt.keep_reuse_noise) {
             noise = false;  // the reference zeroed the block (bd_solver_m.f90:364)
         }
         c.cntr_mobsam++;
     }
     // ---- calc_drift (bd_solver_m.f90:407-454)
-    BD_TRY(compute_forces(c, false));
+    BD_TRY(compute_forces(c, false, false));
     const double *v = c.frc;
     if (hi) {
         BD_TRY(mob_product(c, c.frc, c.W, 1, c.npad, c.npad));
@@ -291,7 +308,12 @@ static int em_step(Ctx &c, const double *xi_host, long long avail, long long *co
                        c.opt.tim_stp, c.drift, c.st);
     // ---- update (integrate_em, bd_solver_m.f90:231-236)
     const double *dcol = nullptr;
-    if (noise) dcol = c.diff + (size_t)(hi ? c.cntr_mobsam - 1 : 0) * c.npad;
+    if (disp_dev) dcol = disp_dev;
+    else if (noise) dcol = c.diff + (size_t)(hi ? c.cntr_mobsam - 1 : 0) * c.npad;
+    if (rec_dev) {
+        if (dcol) BD_HIP(hipMemcpyAsync(rec_dev, dcol, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, sm));
+        else BD_HIP(hipMemsetAsync(rec_dev, 0, (size_t)n * sizeof(double), sm));
+    }
     hipLaunchKernelGGL(update_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, c.drift, dcol, nb, c.imcon, c.box[0],
                        c.box[1], c.box[2], c.st);
     if (hi && c.cntr_mobsam == s) c.cntr_mobsam = 0;
@@ -304,10 +326,6 @@ static int em_step(Ctx &c, const double *xi_host, long long avail, long long *co
 }  // namespace bd
 
 using namespace bd;
-
-struct bd_ctx {
-    Ctx c;
-};
 
 extern "C" {
 
@@ -352,7 +370,7 @@ bd_ctx *bd_ctx_create(int nbeads, const bd_opts *opts)
         return fail(0);
     }
     if (hipMalloc(&c.st, sizeof(DevStatus)) != hipSuccess || hipMemset(c.st, 0, sizeof(DevStatus)) != hipSuccess ||
-        hipHostMalloc(&c.st_host, sizeof(DevStatus)) != hipSuccess) {
+        hipHostMalloc(&c.st_host, 2 * sizeof(DevStatus)) != hipSuccess) {
         set_error("status allocation failed");
         return fail(0);
     }
@@ -518,7 +536,7 @@ int bd_forces(bd_ctx *h, double *frc, double *energies, double *stress9, double 
 {
     if (!h) { set_error("null context"); return BD_ERR_ARG; }
     Ctx &c = h->c;
-    BD_TRY(compute_forces(c, true));
+    BD_TRY(compute_forces(c, true, true));
     int rc = check_status(c, true);
     if (rc != BD_OK) return rc;
     double e[16];
@@ -567,13 +585,15 @@ int bd_run(bd_ctx *h, int nsteps, const double *xi, long long nxi, long long *us
     int rc = BD_OK;
     for (int k = 0; k < nsteps && rc == BD_OK; k++) {
         long long cons = 0;
-        rc = em_step(c, xi ? xi + pos : nullptr, xi ? nxi - pos : 0, &cons);
+        rc = em_step(c, xi ? xi + pos : nullptr, xi ? nxi - pos : 0, &cons, nullptr, nullptr);
         pos += cons;
         if (rc == BD_OK && c.opt.lhdia == 0 && (k % 64) == 63) rc = check_status(c, false);
     }
     if (used) *used = pos;
     int rc2 = check_status(c, false);
     if (rc == BD_OK) rc = rc2;
+    if (c.st_host->overflow) BD_TRY(grow_pairs(c));  // next build gets more room
+    c.info.n_pair_rebuilds = c.st_host->nrebuild;
     // nts counts completed steps only (bds_run returns before nts+1 on error)
     c.info.nts = nts0 + c.st_host->steps_done;
     return rc;

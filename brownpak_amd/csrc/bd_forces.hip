@@ -107,6 +107,8 @@ __device__ inline void lj_force(double r, const double *p, double &enrg, double 
     }
 }
 
+__device__ inline bool excluded(const int *ex_ptr, const int *ex_col, int i, int j);
+
 // ------------------------------------------------------------ energy sums
 // acc layout: 0 e_bond, 1 e_vdw, 2..10 stress(3x3 col-major), 11 bndlen sum,
 // 12 bndlen min, 13 bndlen max
@@ -142,7 +144,8 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
     const int *__restrict__ bond_idx, const int *__restrict__ bond_ij, const int *__restrict__ bond_type,
     const int *__restrict__ bstyles, const double *__restrict__ bparams, int use_pairs,
     const int *__restrict__ nbr_cnt, const int *__restrict__ nbr_idx, int nbr_cap,
-    const int *__restrict__ atype, int natype, const int *__restrict__ ptype, const int *__restrict__ vstyles,
+    const double *__restrict__ crd_ref, double ref_rc2, const int *__restrict__ ex_ptr,
+    const int *__restrict__ ex_col, const int *__restrict__ atype, int natype, const int *__restrict__ ptype, const int *__restrict__ vstyles,
     const double *__restrict__ vparams, double *__restrict__ frc, int want_energy, double *__restrict__ epart,
     DevStatus *st)
 {
@@ -178,9 +181,20 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
         }
         if (use_pairs) {
             const int ati = atype[i];
-            const int cnt = min(nbr_cnt[i], nbr_cap);
-            for (int k = 0; k < cnt; k++) {
-                int j = nbr_idx[(long)k * nb + i];
+            const int cnt = nbr_cnt[i];
+            const bool full = cnt > nbr_cap;  // list overflowed: scan all beads
+            const int kend = full ? nb : cnt;
+            for (int k = 0; k < kend; k++) {
+                int j;
+                if (full) {
+                    j = k;
+                    if (j == i) continue;
+                    double x, y, z;
+                    pair_vec(crd_ref, i, j, bx, x, y, z);
+                    if (!(rsq3(x, y, z) < ref_rc2) || excluded(ex_ptr, ex_col, i, j)) continue;
+                } else {
+                    j = nbr_idx[(long)k * nb + i];
+                }
                 int t = ptype[ati * natype + atype[j]];
                 if (vstyles[t] != 1) continue;
                 double x, y, z;
@@ -228,6 +242,7 @@ __global__ void energy_final_kernel(const double *__restrict__ epart, int nblk, 
 // ------------------------------------------------------------ pair lists
 __device__ inline bool excluded(const int *ex_ptr, const int *ex_col, int i, int j)
 {
+    // rows are sorted ascending
     for (int k = ex_ptr[i]; k < ex_ptr[i + 1]; k++)
         if (ex_col[k] == j) return true;
     return false;
@@ -405,6 +420,150 @@ __global__ void count_rebuild_kernel(DevStatus *st, int *counter)
     if (st->rebuild) *counter += 1;
 }
 
+// ------------------------------------------------------------ adjoint
+// dfrc/dr of the bond styles (the derivative of the force magnitude that
+// ia_get_bond_force returns) and of the shifted LJ force.
+__device__ inline double bond_dforce(double r, int sty, const double *p)
+{
+    switch (sty) {
+    case 1: return p[0];
+    case 2: {
+        double e = r - p[2], q = e * e / p[3];
+        return p[0] * (1.0 + q) / ((1.0 - q) * (1.0 - q));
+    }
+    case 3: {
+        double q = r * r / p[4];
+        double d = p[0] * (1.0 + q) / ((1.0 - q) * (1.0 - q));
+        if (r < p[5]) {
+            double sir = p[3] / r, s6 = sir * sir * sir * sir * sir * sir, s12 = s6 * s6;
+            d += 24 * p[2] * (26 * s12 - 7 * s6) / (r * r);
+        }
+        return d;
+    }
+    case 4: {
+        double x = r / p[1], rr = 1.0 / (1.0 - x);
+        return (-1.0 - 0.5 * rr * rr * rr) / (p[1] * p[0]);
+    }
+    default: return 0.0;
+    }
+}
+
+__device__ inline double lj_dforce(double r, const double *p)
+{
+    if (!(r < p[2])) return 0.0;
+    double sir = p[1] / r, s6 = sir * sir * sir * sir * sir * sir, s12 = s6 * s6;
+    return 24 * p[0] * (26 * s12 - 7 * s6) / (r * r);
+}
+
+// J_g(x) u for the pair force g(x) = frc(|x|) x/|x|.
+__device__ inline void jg_apply(double x, double y, double z, double r, double f, double df, const double *u,
+                                double &ox, double &oy, double &oz)
+{
+    double a = f / r, b = (df - a) / (r * r);
+    double xu = x * u[0] + y * u[1] + z * u[2];
+    ox += a * u[0] + b * x * xu;
+    oy += a * u[1] + b * y * xu;
+    oz += a * u[2] + b * z * xu;
+}
+
+// out_i = sum_j J_g(r_j - r_i)(u_j - u_i)  (the force Jacobian, symmetric),
+// tpart[blk*3 + m] = sum_i u_i . dF_i/dtheta_m for theta = (eps, sigma) of
+// vdw type 1 and k of bond type 1 (FENE/harmonic).  tie_rc: rc = 2^(1/6) sigma.
+__global__ __launch_bounds__(FB) void adj_force_kernel(
+    const double *__restrict__ crd, int nb, BoxArg bx, const int *__restrict__ bond_ptr,
+    const int *__restrict__ bond_idx, const int *__restrict__ bond_ij, const int *__restrict__ bond_type,
+    const int *__restrict__ bstyles, const double *__restrict__ bparams, int use_pairs,
+    const int *__restrict__ nbr_cnt, const int *__restrict__ nbr_idx, int nbr_cap,
+    const double *__restrict__ crd_ref, double ref_rc2, const int *__restrict__ ex_ptr,
+    const int *__restrict__ ex_col, const int *__restrict__ atype, int natype, const int *__restrict__ ptype,
+    const int *__restrict__ vstyles, const double *__restrict__ vparams, int tie_rc, const double *__restrict__ u,
+    double *__restrict__ out, double *__restrict__ tpart)
+{
+    const int i = blockIdx.x * FB + threadIdx.x;
+    double t[3] = {0.0, 0.0, 0.0};
+    if (i < nb) {
+        double ox = 0.0, oy = 0.0, oz = 0.0;
+        const double ui[3] = {u[3 * i], u[3 * i + 1], u[3 * i + 2]};
+        for (int k = bond_ptr[i]; k < bond_ptr[i + 1]; k++) {
+            int b = bond_idx[k];
+            int bi = bond_ij[2 * b], bj = bond_ij[2 * b + 1];
+            int j = (bi == i) ? bj : bi;
+            if (j == i) continue;
+            double x, y, z;
+            pair_vec(crd, i, j, bx, x, y, z);
+            double r = sqrt(x * x + y * y + z * z);
+            int ty = bond_type[b];
+            const double *p = bparams + ty * MXPARAM;
+            double en, f;
+            bond_force(r, bstyles[ty], p, en, f);
+            double df = bond_dforce(r, bstyles[ty], p);
+            double du[3] = {u[3 * j] - ui[0], u[3 * j + 1] - ui[1], u[3 * j + 2] - ui[2]};
+            jg_apply(x, y, z, r, f, df, du, ox, oy, oz);
+            if (ty == 0 && (bstyles[ty] == 1 || bstyles[ty] == 2) && p[0] != 0.0) {
+                double dk = f / p[0];  // both styles are linear in k
+                t[2] += dk * (x * ui[0] + y * ui[1] + z * ui[2]) / r;
+            }
+        }
+        if (use_pairs) {
+            const int ati = atype[i];
+            const int cnt = nbr_cnt[i];
+            const bool full = cnt > nbr_cap;
+            const int kend = full ? nb : cnt;
+            for (int k = 0; k < kend; k++) {
+                int j;
+                if (full) {
+                    j = k;
+                    if (j == i) continue;
+                    double x, y, z;
+                    pair_vec(crd_ref, i, j, bx, x, y, z);
+                    if (!(rsq3(x, y, z) < ref_rc2) || excluded(ex_ptr, ex_col, i, j)) continue;
+                } else {
+                    j = nbr_idx[(long)k * nb + i];
+                }
+                int ty = ptype[ati * natype + atype[j]];
+                if (vstyles[ty] != 1) continue;
+                const double *p = vparams + ty * MXPARAM;
+                double x, y, z;
+                pair_vec(crd, i, j, bx, x, y, z);
+                double r = sqrt(x * x + y * y + z * z);
+                double en, f;
+                lj_force(r, p, en, f);
+                double df = lj_dforce(r, p);
+                double du[3] = {u[3 * j] - ui[0], u[3 * j + 1] - ui[1], u[3 * j + 2] - ui[2]};
+                jg_apply(x, y, z, r, f, df, du, ox, oy, oz);
+                if (ty == 0 && r < p[2]) {
+                    double eps = p[0], sig = p[1];
+                    double proj = (x * ui[0] + y * ui[1] + z * ui[2]) / r;
+                    t[0] += (f / eps) * proj;
+                    double sr = sig / r, s6 = sr * sr * sr * sr * sr * sr, s12 = s6 * s6;
+                    double dsig = -144.0 * eps * (4 * s12 - s6) / (sig * r);
+                    if (!tie_rc) {
+                        double sc = sig / p[2], c6 = sc * sc * sc * sc * sc * sc, c12 = c6 * c6;
+                        dsig -= -144.0 * eps * (4 * c12 - c6) / (sig * p[2]);
+                    }
+                    t[1] += dsig * proj;
+                }
+            }
+        }
+        out[3 * i] = ox;
+        out[3 * i + 1] = oy;
+        out[3 * i + 2] = oz;
+    }
+    __shared__ double sh[FB / 64][3];
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    for (int m = 0; m < 3; m++) {
+        double v = t[m];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) sh[wv][m] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double v = 0.0;
+        for (int w = 0; w < FB / 64; w++) v += sh[w][threadIdx.x];
+        tpart[blockIdx.x * 3 + threadIdx.x] = v;
+    }
+}
+
 // ------------------------------------------------------------ host
 static std::vector<int> host_exclusions(int nb, const std::vector<int> &bi, const std::vector<int> &bj, int excl,
                                         std::vector<int> &ptr)
@@ -535,6 +694,10 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const in
     std::vector<int> eptr;
     std::vector<int> ecol = host_exclusions(nb, bi, bj, c.excluded_atoms, eptr);
 
+    c.h_bpar = bpar;
+    c.h_vpar = vpar;
+    c.h_bsty = bstyle;
+    c.h_vsty = vstyle;
     c.nbonds = nbonds;
     c.nbond_types = nbond_types;
     c.natom_types = nat;
@@ -615,8 +778,13 @@ static BoxArg box_arg(const Ctx &c)
     return b;
 }
 
-// Builds (or keeps) the neighbour list for the current coordinates.
-static int build_pairs(Ctx &c)
+__global__ void set_rebuild_kernel(DevStatus *st) { st->rebuild = 1; }
+
+// Builds (or keeps) the neighbour list for the current coordinates.  With
+// sync the overflow flag is checked on the host and the capacity grown;
+// without it (the step loop) nothing waits: beads whose list overflowed are
+// handled by the all-pairs fallback of force_gather_kernel.
+static int build_pairs(Ctx &c, bool sync)
 {
     const int nb = c.nb;
     hipStream_t sm = c.stream;
@@ -629,14 +797,11 @@ static int build_pairs(Ctx &c)
     const double rl = ver ? c.rcutoff + c.tskin : c.rcutoff;
     const double rc2 = rl * rl;
     for (int attempt = 0; attempt < 8; attempt++) {
-        if (ver) {
+        if (ver)
             hipLaunchKernelGGL(verlet_check_kernel, dim3(1), dim3(1024), 0, sm, c.crd, c.crd_save, nb, bx,
                                c.tskin * c.tskin, c.verlet_first ? 1 : 0, c.st);
-        } else {
-            // DIR rebuilds on every call
-            hipLaunchKernelGGL(verlet_check_kernel, dim3(1), dim3(1024), 0, sm, c.crd, c.crd_save, nb, bx, 0.0, 1,
-                               c.st);
-        }
+        else
+            hipLaunchKernelGGL(set_rebuild_kernel, dim3(1), dim3(1), 0, sm, c.st);  // DIR: every call
         if (ver && c.ncell_tot > 0) {
             int nbk = (nb + 255) / 256;
             hipLaunchKernelGGL(zero_int_kernel, dim3((c.ncell_tot + 255) / 256), dim3(256), 0, sm, c.cell_cnt,
@@ -654,42 +819,49 @@ static int build_pairs(Ctx &c)
             hipLaunchKernelGGL(nbr_n2_kernel, dim3((nb + 3) / 4), dim3(256), 0, sm, c.crd, nb, bx, rc2, c.ex_ptr,
                                c.ex_col, c.nbr_cnt, c.nbr_idx, c.pair_cap, 1, ver ? c.crd_save : nullptr, c.st);
         }
+        if (ver) hipLaunchKernelGGL(count_rebuild_kernel, dim3(1), dim3(1), 0, sm, c.st, &c.st->nrebuild);
         BD_HIP(hipGetLastError());
-        // overflow check needs the host; only on rebuilds that can overflow
+        c.verlet_first = false;
+        c.pairs_valid = true;
+        if (!sync) return BD_OK;
         BD_HIP(hipMemcpyAsync(c.st_host, c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, sm));
         BD_HIP(hipStreamSynchronize(sm));
         c.last_rebuilt = c.st_host->rebuild;
-        if (c.st_host->rebuild && ver) c.info.n_pair_rebuilds++;
-        if (!c.st_host->overflow) {
-            c.verlet_first = false;
-            c.pairs_valid = true;
-            return BD_OK;
-        }
+        if (!c.st_host->overflow) return BD_OK;
         // grow and rebuild from scratch
-        int zero = 0;
-        BD_HIP(hipMemcpy(&c.st->overflow, &zero, sizeof(int), hipMemcpyHostToDevice));
-        (void)hipFree(c.nbr_idx);
-        c.pair_cap *= 2;
-        BD_HIP(hipMalloc(&c.nbr_idx, (size_t)c.pair_cap * nb * sizeof(int)));
+        BD_TRY(grow_pairs(c));
         c.verlet_first = true;
-        if (ver && c.st_host->rebuild) c.info.n_pair_rebuilds--;
     }
     set_error("neighbour list capacity could not be satisfied");
     return BD_ERR_CAPACITY;
 }
 
-int compute_forces(Ctx &c, bool want_energy)
+int grow_pairs(Ctx &c)
+{
+    int zero = 0;
+    BD_HIP(hipStreamSynchronize(c.stream));
+    BD_HIP(hipMemcpy(&c.st->overflow, &zero, sizeof(int), hipMemcpyHostToDevice));
+    (void)hipFree(c.nbr_idx);
+    c.pair_cap *= 2;
+    BD_HIP(hipMalloc(&c.nbr_idx, (size_t)c.pair_cap * c.nb * sizeof(int)));
+    return BD_OK;
+}
+
+int compute_forces(Ctx &c, bool want_energy, bool sync)
 {
     if (!c.have_topology) { set_error("topology not set"); return BD_ERR_STATE; }
     const int nb = c.nb;
     const bool pairs = c.lvdw && c.nvdw_types > 0;
-    if (pairs) BD_TRY(build_pairs(c));
+    if (pairs) BD_TRY(build_pairs(c, sync));
+    const bool ver = (c.mth_ptgen == BD_PT_VER);
+    const double rl = ver ? c.rcutoff + c.tskin : c.rcutoff;
     BoxArg bx = box_arg(c);
     int nblk = (nb + FB - 1) / FB;
     hipLaunchKernelGGL(force_gather_kernel, dim3(std::max(nblk, 1)), dim3(FB), 0, c.stream, c.crd, nb, bx,
                        c.bead_bond_ptr, c.bead_bond_idx, c.bond_ij, c.bond_type, c.bond_styles, c.bond_params,
-                       pairs ? 1 : 0, c.nbr_cnt, c.nbr_idx, c.pair_cap, c.atom_types, c.natom_types, c.scan_tmp,
-                       c.vdw_styles, c.vdw_params, c.frc, want_energy ? 1 : 0, c.red, c.st);
+                       pairs ? 1 : 0, c.nbr_cnt, c.nbr_idx, c.pair_cap, ver ? c.crd_save : c.crd, rl * rl, c.ex_ptr,
+                       c.ex_col, c.atom_types, c.natom_types, c.scan_tmp, c.vdw_styles, c.vdw_params, c.frc,
+                       want_energy ? 1 : 0, c.red, c.st);
     BD_HIP(hipGetLastError());
     if (want_energy) {
         double vol = c.box[0] * c.box[1] * c.box[2];
@@ -697,6 +869,47 @@ int compute_forces(Ctx &c, bool want_energy)
                            c.imcon, vol, c.scal);
         BD_HIP(hipGetLastError());
     }
+    return BD_OK;
+}
+
+// Force Jacobian-vector product and theta partials at the device
+// coordinates (the neighbour list must be the one of the last compute_forces).
+int adj_force(Ctx &c, const double *u, double *out, double *tpart, int tie_rc)
+{
+    const int nb = c.nb;
+    const bool pairs = c.lvdw && c.nvdw_types > 0;
+    const bool ver = (c.mth_ptgen == BD_PT_VER);
+    const double rl = ver ? c.rcutoff + c.tskin : c.rcutoff;
+    BoxArg bx = box_arg(c);
+    int nblk = std::max((nb + FB - 1) / FB, 1);
+    hipLaunchKernelGGL(adj_force_kernel, dim3(nblk), dim3(FB), 0, c.stream, c.crd, nb, bx, c.bead_bond_ptr,
+                       c.bead_bond_idx, c.bond_ij, c.bond_type, c.bond_styles, c.bond_params, pairs ? 1 : 0,
+                       c.nbr_cnt, c.nbr_idx, c.pair_cap, ver ? c.crd_save : c.crd, rl * rl, c.ex_ptr, c.ex_col,
+                       c.atom_types, c.natom_types, c.scan_tmp, c.vdw_styles, c.vdw_params, tie_rc, u, out, tpart);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+// Updates eps, sigma (vdw type 1; rc = 2^(1/6) sigma when tie_rc) and k
+// (bond type 1) on the device, re-deriving the shifted-force constants.
+int set_theta(Ctx &c, double eps, double sigma, double kb, int tie_rc)
+{
+    if (c.h_vpar.size() >= (size_t)MXPARAM && c.nvdw_types > 0 && c.h_vsty[0] == 1) {
+        double *p = &c.h_vpar[0];
+        p[0] = eps;
+        p[1] = sigma;
+        if (tie_rc) p[2] = 1.122462048309373 * sigma;
+        double sir = p[1] / p[2], sir2 = sir * sir, sir6 = sir2 * sir2 * sir2, sir12 = sir6 * sir6;
+        p[3] = 4 * p[0] * (sir12 - sir6);
+        p[4] = -24 * p[0] * (2 * sir12 - sir6) / p[2];
+        BD_HIP(hipMemcpy(c.vdw_params, c.h_vpar.data(), c.h_vpar.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (tie_rc) c.rcutoff = p[2];
+    }
+    if (c.h_bpar.size() >= (size_t)MXPARAM && c.nbond_types > 0) {
+        c.h_bpar[0] = kb;
+        BD_HIP(hipMemcpy(c.bond_params, c.h_bpar.data(), c.h_bpar.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c.verlet_first = true;
     return BD_OK;
 }
 
@@ -710,8 +923,9 @@ int pair_table_host(Ctx &c, std::vector<int> &pairs)
     BD_HIP(hipMemcpy(cnt.data(), c.nbr_cnt, nb * sizeof(int), hipMemcpyDeviceToHost));
     BD_HIP(hipMemcpy(idx.data(), c.nbr_idx, idx.size() * sizeof(int), hipMemcpyDeviceToHost));
     for (int i = 0; i < nb; i++) {
+        if (cnt[i] > c.pair_cap) { set_error("pair list overflowed; call bd_forces first"); return BD_ERR_CAPACITY; }
         std::vector<int> row;
-        for (int k = 0; k < std::min(cnt[i], c.pair_cap); k++) {
+        for (int k = 0; k < cnt[i]; k++) {
             int j = idx[(size_t)k * nb + i];
             if (j > i) row.push_back(j);
         }

@@ -26,6 +26,7 @@ constexpr int MXPARAM = 12;
 constexpr int MAX_S = 8;          // largest Lanczos block size supported
 constexpr int NQUAD_MAX = 160;    // quadrature nodes for H^{-1/2}
 constexpr int NBOUND = 48;        // Cholesky probes for the lower spectral bound
+constexpr int LZ_NEV = 8;         // events handing Lanczos iterations to the side stream
 
 void set_error(const std::string &msg);
 
@@ -56,7 +57,7 @@ struct DevStatus {
     int ticket;        // work counter for the mobility product
     int nq;            // quadrature nodes used by the last small problem
     int steps_done;    // EM steps completed since the last bd_run started
-    int pad;
+    int nrebuild;      // Verlet list rebuilds
     double err;        // last Lanczos relative change
     double lam_lo, lam_hi;  // spectral bounds of the last small problem
 };
@@ -105,7 +106,11 @@ struct Ctx {
     int *bound_ok = nullptr;     // NBOUND flags
     double *npart = nullptr;     // norm partials
     DevStatus *st = nullptr;
-    DevStatus *st_host = nullptr;  // pinned mirror
+    DevStatus *st_host = nullptr;  // pinned mirror, 2 slots
+    hipStream_t side = nullptr;    // small-problem stream of the Lanczos solve
+    hipEvent_t lz_ev[LZ_NEV] = {};
+    hipEvent_t lz_chunk_ev[2] = {};
+    double *Rq = nullptr, *Rcoef = nullptr;  // reorthogonalisation scratch
 
     // topology
     bool have_topology = false;
@@ -124,6 +129,8 @@ struct Ctx {
     double *vdw_params = nullptr;
     int *ex_ptr = nullptr, *ex_col = nullptr;  // exclusions, 0-based
     int ex_total = 0;
+    std::vector<double> h_bpar, h_vpar;       // host copies (12 x ntypes)
+    std::vector<int> h_bsty, h_vsty;
 
     // pair lists
     int pair_cap = 0;            // capacity of the full list
@@ -166,6 +173,8 @@ void mob_free(Ctx &c);
 int rpy_assemble(Ctx &c);
 // y(npad, s) = D x(npad, s) on the context stream; x, y device, ld = npad.
 int mob_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy);
+// Same product inside a Lanczos solve: skipped once the solve has converged.
+int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy);
 // y = alpha * (D x + add) with optional add vector (s = 1); used for the drift.
 int mob_unpack(Ctx &c, double *dense, int ld);
 
@@ -184,10 +193,22 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types,
 void forces_free(Ctx &c);
 // Computes forces into c.frc at c.crd.  Energies/stress reductions are
 // computed when want_energy is set (written to c.scal[0..14]).
-int compute_forces(Ctx &c, bool want_energy);
+int compute_forces(Ctx &c, bool want_energy, bool sync);
+int grow_pairs(Ctx &c);
+int adj_force(Ctx &c, const double *u, double *out, double *tpart, int tie_rc);
+int set_theta(Ctx &c, double eps, double sigma, double kb, int tie_rc);
 int pair_table_host(Ctx &c, std::vector<int> &pairs);
 
 // --- step pieces (bd_api.hip)
+int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed, const double *disp_dev,
+            double *rec_dev);
+int check_status(Ctx &c, bool clear);
+int rpy_assemble_at(Ctx &c, const double *crd);
+int launch_to_origin(Ctx &c, double *com_acc);
 int launch_gauss(Ctx &c, double *out, long long count, uint64_t seed, uint64_t offset, int ld, int rows, int cols);
 
 }  // namespace bd
+
+struct bd_ctx {
+    bd::Ctx c;
+};

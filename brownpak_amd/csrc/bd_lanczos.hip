@@ -660,12 +660,27 @@ int lanczos_alloc(Ctx &c)
     BD_HIP(hipMalloc(&c.Ccoef, (size_t)ncol * s * sizeof(double)));
     BD_HIP(hipMalloc(&c.bound_ok, NBOUND * sizeof(int)));
     BD_HIP(hipMalloc(&c.npart, (size_t)GBLK * 2 * MAX_S * sizeof(double)));
+    if (c.opt.lanc_reorth) {
+        BD_HIP(hipMalloc(&c.Rq, (size_t)GBLK * ncol * s * sizeof(double)));
+        BD_HIP(hipMalloc(&c.Rcoef, (size_t)ncol * s * sizeof(double)));
+    }
+    BD_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    for (int k = 0; k < LZ_NEV; k++) BD_HIP(hipEventCreateWithFlags(&c.lz_ev[k], hipEventDisableTiming));
+    for (int k = 0; k < 2; k++) BD_HIP(hipEventCreateWithFlags(&c.lz_chunk_ev[k], hipEventDisableTiming));
     return BD_OK;
 }
 
 void lanczos_free(Ctx &c)
 {
-    double **ps[] = {&c.V, &c.W, &c.Y, &c.Ablk, &c.Bblk, &c.R0, &c.gpart, &c.Cq, &c.Ccoef, &c.npart};
+    if (c.side) (void)hipStreamSynchronize(c.side);
+    for (int k = 0; k < LZ_NEV; k++)
+        if (c.lz_ev[k]) (void)hipEventDestroy(c.lz_ev[k]);
+    for (int k = 0; k < 2; k++)
+        if (c.lz_chunk_ev[k]) (void)hipEventDestroy(c.lz_chunk_ev[k]);
+    if (c.side) (void)hipStreamDestroy(c.side);
+    c.side = nullptr;
+    double **ps[] = {&c.V, &c.W, &c.Y, &c.Ablk, &c.Bblk, &c.R0, &c.gpart, &c.Cq, &c.Ccoef, &c.npart,
+                     &c.Rq, &c.Rcoef};
     for (auto p : ps) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
@@ -676,6 +691,13 @@ void lanczos_free(Ctx &c)
     c.chol = nullptr;
 }
 
+// Host driver of one block-Lanczos solve.  The Krylov recurrence (product,
+// three-term update, CholeskyQR2) runs on the context stream; the small
+// problem and the convergence test of iteration j run on a second stream as
+// soon as iteration j's blocks exist, overlapping the next products.  The
+// host looks at the convergence flag once per chunk of iterations, one chunk
+// behind, so it never stalls the device; iterations enqueued past the first
+// converged j find the flag set and return at once.
 template <int S>
 static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *iters)
 {
@@ -683,7 +705,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     const int dmax = c.dimk_max;
     const bool two_norm = (S == 1);
     double *g0 = c.gpart, *g1 = c.gpart + GBLK * MAX_S * MAX_S;
-    hipStream_t sm = c.stream;
+    hipStream_t sm = c.stream, ss = c.side;
 
     hipLaunchKernelGGL(status_reset_kernel, dim3(1), dim3(1), 0, sm, c.st);
     // xi = V_1 R0 (CholeskyQR2)
@@ -695,26 +717,31 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     BD_HIP(hipGetLastError());
 
     const int chunk = 4;
-    int jdone = 0;
-    for (int j = 1; j <= dmax; j++) {
+    int pending = -1;   // slot of the status copy of the previous chunk
+    int slot = 0;
+    bool stop = false;
+    for (int j = 1; j <= dmax && !stop; j++) {
         double *Vj = c.V + (size_t)(j - 1) * S * ld;
         const double *Vp = (j > 1) ? c.V + (size_t)(j - 2) * S * ld : nullptr;
         const double *Bp = (j > 1) ? c.Bblk + (size_t)(j - 2) * S * S : nullptr;
-        BD_TRY(mob_product(c, Vj, c.W, S, ld, ld));
+        BD_TRY(mob_product_lz(c, Vj, c.W, S, ld, ld));
         hipLaunchKernelGGL((lanczos_a_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vj, Vp, Bp, ld, rows, g0, c.st);
         const int upd = (j < dmax) ? 1 : 0;
         hipLaunchKernelGGL((lanczos_b_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vj, ld, rows, g0,
                            c.Ablk + (size_t)(j - 1) * S * S, upd, g1, c.st);
+        // A_j and V_1..V_j are final here: hand iteration j to the side stream
+        hipEvent_t ev = c.lz_ev[j % LZ_NEV];
+        BD_HIP(hipEventRecord(ev, sm));
         if (upd && c.opt.lanc_reorth) {
             // two classical Gram-Schmidt passes against V_1..V_j (FP64)
             const int ncol = j * S;
             for (int pass = 0; pass < 2; pass++) {
                 hipLaunchKernelGGL((reorth_coef_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.V, c.W, ld, rows, ncol,
-                                   c.Cq, c.st);
-                hipLaunchKernelGGL(reorth_sum_kernel, dim3((ncol * S + 255) / 256), dim3(256), 0, sm, c.Cq,
-                                   ncol * S, c.Ccoef, c.st);
+                                   c.Rq, c.st);
+                hipLaunchKernelGGL(reorth_sum_kernel, dim3((ncol * S + 255) / 256), dim3(256), 0, sm, c.Rq,
+                                   ncol * S, c.Rcoef, c.st);
                 hipLaunchKernelGGL((reorth_apply_kernel<S>), dim3(GBLK), dim3(RB), (size_t)ncol * S * sizeof(double),
-                                   sm, c.V, c.W, ld, rows, ncol, c.Ccoef, pass == 1 ? g1 : (double *)nullptr, c.st);
+                                   sm, c.V, c.W, ld, rows, ncol, c.Rcoef, pass == 1 ? g1 : (double *)nullptr, c.st);
             }
         }
         if (upd) {
@@ -725,35 +752,42 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             hipLaunchKernelGGL((cholqr_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, Vn, Vn, ld, rows, g0,
                                (double *)nullptr, Bj, (const double *)(c.R0 + S * S), c.st);
         }
+        BD_HIP(hipGetLastError());
         if (j >= 2) {
+            BD_HIP(hipStreamWaitEvent(ss, ev, 0));
             const int js = j * S;
             size_t band_bytes = (size_t)(S + 1) * js * sizeof(double);
-            hipLaunchKernelGGL((bounds_kernel<S>), dim3(NBOUND), dim3(64), band_bytes, sm, c.Ablk, c.Bblk, js,
+            hipLaunchKernelGGL((bounds_kernel<S>), dim3(NBOUND), dim3(64), band_bytes, ss, c.Ablk, c.Bblk, js,
                                c.bound_ok, c.st);
             const int nq = 2 * 125 + 1;
-            hipLaunchKernelGGL((quad_kernel<S>), dim3(nq), dim3(64), band_bytes + (size_t)js * S * sizeof(double), sm,
+            hipLaunchKernelGGL((quad_kernel<S>), dim3(nq), dim3(64), band_bytes + (size_t)js * S * sizeof(double), ss,
                                c.Ablk, c.Bblk, c.R0, js, c.bound_ok, c.Cq, c.st);
-            hipLaunchKernelGGL((nodesum_kernel<S>), dim3((js * S + 63) / 64), dim3(256), 0, sm, c.Cq, js, c.Ccoef,
+            hipLaunchKernelGGL((nodesum_kernel<S>), dim3((js * S + 63) / 64), dim3(256), 0, ss, c.Cq, js, c.Ccoef,
                                c.st);
             double *ycur = c.Y + (size_t)(j % 2) * S * ld;
             const double *yold = c.Y + (size_t)((j + 1) % 2) * S * ld;
             const int want = (j > 2) ? 1 : 0;
-            hipLaunchKernelGGL((ycomb_kernel<S>), dim3(GBLK), dim3(RB), (size_t)js * S * sizeof(double), sm, c.V, ld,
+            hipLaunchKernelGGL((ycomb_kernel<S>), dim3(GBLK), dim3(RB), (size_t)js * S * sizeof(double), ss, c.V, ld,
                                rows, js, c.Ablk, c.Bblk, c.Ccoef, ycur, yold, want, two_norm ? 1 : 0, c.npart, c.st);
-            hipLaunchKernelGGL((finalize_kernel<S>), dim3(1), dim3(64), 0, sm, c.npart, j, want, two_norm ? 1 : 0,
+            hipLaunchKernelGGL((finalize_kernel<S>), dim3(1), dim3(64), 0, ss, c.npart, j, want, two_norm ? 1 : 0,
                                c.opt.lanc_tol, c.st);
+            BD_HIP(hipGetLastError());
         }
-        BD_HIP(hipGetLastError());
-        jdone = j;
         if (j % chunk == 0 || j == dmax) {
-            BD_HIP(hipMemcpyAsync(c.st_host, c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, sm));
-            BD_HIP(hipStreamSynchronize(sm));
-            if (c.st_host->ierr || c.st_host->converged_j) break;
+            // snapshot the status after this chunk's tests; read the previous one
+            BD_HIP(hipMemcpyAsync(&c.st_host[slot], c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, ss));
+            BD_HIP(hipEventRecord(c.lz_chunk_ev[slot], ss));
+            if (pending >= 0) {
+                BD_HIP(hipEventSynchronize(c.lz_chunk_ev[pending]));
+                if (c.st_host[pending].ierr || c.st_host[pending].converged_j) stop = true;
+            }
+            pending = slot;
+            slot ^= 1;
         }
     }
-    (void)jdone;
-    BD_HIP(hipMemcpyAsync(c.st_host, c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, sm));
+    BD_HIP(hipStreamSynchronize(ss));
     BD_HIP(hipStreamSynchronize(sm));
+    BD_HIP(hipMemcpy(c.st_host, c.st, sizeof(DevStatus), hipMemcpyDeviceToHost));
     if (c.st_host->ierr == 1) {
         set_error("bond length too large (overstretched bond)");
         return BD_ERR_BOND;

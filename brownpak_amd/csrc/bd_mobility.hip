@@ -108,8 +108,10 @@ template <typename E, int S>
 __global__ __launch_bounds__(NT) void symm_tiles_kernel(
     const E *__restrict__ mob, const double *__restrict__ x, int ldx, const int *__restrict__ units,
     int nunits, int ntile, int nsc, double *__restrict__ rowpart, double *__restrict__ colpart,
-    int *__restrict__ ticket)
+    int *__restrict__ ticket, const DevStatus *stop)
 {
+    // inside a Lanczos solve: nothing to do once it has converged
+    if (stop && (stop->converged_j != 0 || stop->ierr != 0)) return;
     // the reduction buffer aliases the staged tile (never live together)
     __shared__ union {
         TileLds<E> t;
@@ -295,23 +297,25 @@ void mob_free(Ctx &c)
     c.rowpart = nullptr; c.colpart = nullptr;
 }
 
-int rpy_assemble(Ctx &c)
+int rpy_assemble_at(Ctx &c, const double *crd)
 {
     if (c.nb > 0) {
         if (c.mob32)
             hipLaunchKernelGGL(rpy_tile_kernel<float>, dim3(c.ntiles_lower), dim3(256), 0, c.stream,
-                               c.crd, c.nb, c.n, c.ntile, c.mob32);
+                               crd, c.nb, c.n, c.ntile, c.mob32);
         else
             hipLaunchKernelGGL(rpy_tile_kernel<double>, dim3(c.ntiles_lower), dim3(256), 0, c.stream,
-                               c.crd, c.nb, c.n, c.ntile, c.mob64);
+                               crd, c.nb, c.n, c.ntile, c.mob64);
         BD_HIP(hipGetLastError());
     }
     c.mob_valid = true;
     return BD_OK;
 }
 
+int rpy_assemble(Ctx &c) { return rpy_assemble_at(c, c.crd); }
+
 template <typename E, int S>
-static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, int ldy)
+static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, int ldy, const DevStatus *stop)
 {
     const int nsc = (c.ntile + WSC - 1) / WSC;
     int cus = 256;
@@ -334,7 +338,7 @@ static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, 
         BD_HIP(hipEventRecord(e0, c.stream));
     }
     hipLaunchKernelGGL((symm_tiles_kernel<E, S>), dim3(grid), dim3(NT), 0, c.stream, mob, x, ldx,
-                       c.units, c.nunits, c.ntile, nsc, c.rowpart, c.colpart, ticket);
+                       c.units, c.nunits, c.ntile, nsc, c.rowpart, c.colpart, ticket, stop);
     BD_HIP(hipGetLastError());
     if (c.prof_on) BD_HIP(hipEventRecord(e1, c.stream));
     long tot = (long)c.ntile * TILE * S;
@@ -345,17 +349,18 @@ static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, 
 }
 
 template <typename E>
-static int product_dispatch(Ctx &c, const E *mob, const double *x, double *y, int s, int ldx, int ldy)
+static int product_dispatch(Ctx &c, const E *mob, const double *x, double *y, int s, int ldx, int ldy,
+                            const DevStatus *stop)
 {
     switch (s) {
-    case 1: return product_t<E, 1>(c, mob, x, y, ldx, ldy);
-    case 2: return product_t<E, 2>(c, mob, x, y, ldx, ldy);
-    case 3: return product_t<E, 3>(c, mob, x, y, ldx, ldy);
-    case 4: return product_t<E, 4>(c, mob, x, y, ldx, ldy);
-    case 5: return product_t<E, 5>(c, mob, x, y, ldx, ldy);
-    case 6: return product_t<E, 6>(c, mob, x, y, ldx, ldy);
-    case 7: return product_t<E, 7>(c, mob, x, y, ldx, ldy);
-    case 8: return product_t<E, 8>(c, mob, x, y, ldx, ldy);
+    case 1: return product_t<E, 1>(c, mob, x, y, ldx, ldy, stop);
+    case 2: return product_t<E, 2>(c, mob, x, y, ldx, ldy, stop);
+    case 3: return product_t<E, 3>(c, mob, x, y, ldx, ldy, stop);
+    case 4: return product_t<E, 4>(c, mob, x, y, ldx, ldy, stop);
+    case 5: return product_t<E, 5>(c, mob, x, y, ldx, ldy, stop);
+    case 6: return product_t<E, 6>(c, mob, x, y, ldx, ldy, stop);
+    case 7: return product_t<E, 7>(c, mob, x, y, ldx, ldy, stop);
+    case 8: return product_t<E, 8>(c, mob, x, y, ldx, ldy, stop);
     default: set_error("block size s must be in 1..8"); return BD_ERR_ARG;
     }
 }
@@ -363,8 +368,15 @@ static int product_dispatch(Ctx &c, const E *mob, const double *x, double *y, in
 int mob_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy)
 {
     if (!c.mob_valid) { set_error("mobility not assembled"); return BD_ERR_STATE; }
-    if (c.mob32) return product_dispatch<float>(c, c.mob32, x, y, s, ldx, ldy);
-    return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy);
+    if (c.mob32) return product_dispatch<float>(c, c.mob32, x, y, s, ldx, ldy, nullptr);
+    return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy, nullptr);
+}
+
+int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy)
+{
+    if (!c.mob_valid) { set_error("mobility not assembled"); return BD_ERR_STATE; }
+    if (c.mob32) return product_dispatch<float>(c, c.mob32, x, y, s, ldx, ldy, c.st);
+    return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy, c.st);
 }
 
 // Sums the recorded (start, stop) pairs into prof_ms and recycles them.

@@ -193,6 +193,30 @@ int bd_sync(bd_ctx *ctx);
  * events on the context's stream; bd_prof_get returns the summed kernel
  * time (ms), the launch count and the bytes of stored mobility tiles one
  * launch streams.  bd_prof_enable(ctx, 1) also clears the totals. */
+/* Differentiable trajectory (no reference counterpart; BROWNPAK has no
+ * gradients).  The Brownian displacement of every step is recorded once and
+ * then replayed ("noise held fixed"), so a trajectory is a deterministic
+ * function of the force-field parameters theta = (eps, sigma) of vdw type 1
+ * and k of bond type 1.
+ *   bd_run_record: like bd_run, and writes the displacement each step
+ *     applied, disp_out (n, nsteps) column-major (zero columns on the steps
+ *     the reference leaves without noise).
+ *   bd_run_replay: nsteps EM steps with disp (n, nsteps) as the Brownian
+ *     term (mobility, forces and drift as in bd_run).
+ *   bd_set_theta: sets eps, sigma (and rc = 2^(1/6) sigma if tie_rc) of
+ *     vdw type 1 and k of bond type 1, re-deriving the shifted-force terms.
+ *   bd_adjoint_rgsq: replays nsteps from the device coordinates, returns the
+ *     final Rg^2 (stats_m.f90:441-446) in *loss and d loss / d(eps, sigma, k)
+ *     in grad3 from a hand-written adjoint sweep with checkpoints every
+ *     ckpt steps (a multiple of nts_mobsam); the coordinates end at the final
+ *     state.  Isolated chains (imcon 0) only. */
+int bd_set_theta(bd_ctx *ctx, double eps, double sigma, double kbond, int tie_rc);
+int bd_run_record(bd_ctx *ctx, int nsteps, const double *xi, long long nxi,
+                  long long *used, double *disp_out);
+int bd_run_replay(bd_ctx *ctx, int nsteps, const double *disp);
+int bd_adjoint_rgsq(bd_ctx *ctx, int nsteps, const double *disp, int ckpt,
+                    int tie_rc, double *loss, double *grad3);
+
 int bd_prof_enable(bd_ctx *ctx, int on);
 int bd_prof_get(bd_ctx *ctx, double *total_ms, long long *launches,
                 double *tile_bytes);

@@ -1111,6 +1111,15 @@ int orc_pairs_cell_list(const orc_system *sys, const int *ex_ptr,
 long long orc_bd_run(const orc_system *sys, const orc_bdopts *opt, double *crd,
                      int nsteps, const double *xi, long long nxi, orc_runinfo *info)
 {
+    return orc_bd_run_ex(sys, opt, crd, nsteps, xi, nxi, NULL, info);
+}
+
+/* disp (n, nsteps), if given, replaces the Brownian term of every step
+ * (a recorded displacement replayed with the noise held fixed). */
+long long orc_bd_run_ex(const orc_system *sys, const orc_bdopts *opt, double *crd,
+                        int nsteps, const double *xi, long long nxi,
+                        const double *disp, orc_runinfo *info)
+{
     const int nb = sys->nbeads, n = 3 * nb;
     const int s = opt->lhdia ? opt->nts_mobsam : 1;
     const double sqrt_two_dt = sqrt(2.0 * opt->tim_stp);
@@ -1154,7 +1163,15 @@ long long orc_bd_run(const orc_system *sys, const orc_bdopts *opt, double *crd,
          * mobility get no Brownian term.  keep_reuse_noise keeps the block. */
         if (!opt->lhdia || !opt->keep_reuse_noise || cntr == 0)
             memset(diff, 0, sizeof(double) * (size_t)n * s);
-        if (!opt->lhdia) {
+        if (disp) {
+            if (opt->lhdia) {
+                if (cntr == 0) {
+                    orc_rpy(nb, crd, mob);
+                    info->n_mob_updates++;
+                }
+                cntr++;
+            }
+        } else if (!opt->lhdia) {
             if (used + n > nxi) { ret = -11; goto done; }
             for (int i = 0; i < n; i++) diff[i] = sqrt_two_dt * xi[used + i];
             used += n;
@@ -1228,7 +1245,7 @@ long long orc_bd_run(const orc_system *sys, const orc_bdopts *opt, double *crd,
         for (int i = 0; i < n; i++) drift[i] = opt->tim_stp * drift[i];
         /* ---- update ---- */
         {
-            const double *dcol = diff + (size_t)(opt->lhdia ? cntr - 1 : 0) * n;
+            const double *dcol = disp ? disp + (size_t)step * n : diff + (size_t)(opt->lhdia ? cntr - 1 : 0) * n;
             for (int i = 0; i < n; i++) crd[i] = crd[i] + drift[i] + dcol[i];
         }
         if (opt->lhdia && cntr == s) cntr = 0;

@@ -146,6 +146,9 @@ typedef struct {
 long long orc_bd_run(const orc_system *sys, const orc_bdopts *opt, double *crd,
                      int nsteps, const double *xi, long long nxi,
                      orc_runinfo *info);
+long long orc_bd_run_ex(const orc_system *sys, const orc_bdopts *opt, double *crd,
+                        int nsteps, const double *xi, long long nxi,
+                        const double *disp, orc_runinfo *info);
 
 #ifdef __cplusplus
 }

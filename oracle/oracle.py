@@ -109,6 +109,9 @@ def lib():
         L.orc_bd_run.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcBdopts), DP, C.c_int, DP,
                                  C.c_longlong, C.POINTER(OrcRuninfo)]
         L.orc_bd_run.restype = C.c_longlong
+        L.orc_bd_run_ex.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcBdopts), DP, C.c_int, DP,
+                                    C.c_longlong, DP, C.POINTER(OrcRuninfo)]
+        L.orc_bd_run_ex.restype = C.c_longlong
     return _lib
 
 
@@ -342,3 +345,17 @@ def bd_run(sysobj, crd, nsteps, xi, lhdia=True, mob_fctr=CHOL, nts_mobsam=1,
     used = lib().orc_bd_run(C.byref(sysobj.s), C.byref(opt), _dp(crd), nsteps, _dp(xi),
                             xi.size, C.byref(info))
     return crd, used, info
+
+
+def bd_run_replay(sysobj, crd, disp, lhdia=True, nts_mobsam=1, tim_stp=1e-3):
+    """bds_run with the Brownian term of every step replaced by disp (K, N, 3)."""
+    crd = _f64(crd).copy()
+    d = _f64(disp)
+    nsteps = d.shape[0]
+    opt = OrcBdopts(int(lhdia), CHOL, nts_mobsam, 0, 0.0, tim_stp, 0, 0, 0.0, 0)
+    info = OrcRuninfo()
+    r = lib().orc_bd_run_ex(C.byref(sysobj.s), C.byref(opt), _dp(crd), nsteps, None, 0, _dp(d),
+                            C.byref(info))
+    if r < 0:
+        raise RuntimeError(f"orc_bd_run_ex failed ({r})")
+    return crd

@@ -113,3 +113,43 @@ def read_cfg(path):
             continue
         i += 1
     return out
+
+
+def write_cfg(path, m):
+    """Writes a system in the reference .cfg format (config_io_m.f90:247-544)."""
+    crd = np.asarray(m["crd"])
+    nb = len(crd)
+    at = m.get("atom_types")
+    at = np.ones(nb, int) if at is None else at
+    L = m["box"]
+    with open(path, "w") as fh:
+        fh.write("generated\nSIMBOX\n")
+        fh.write(f"{L[0]:.16g} 0.0 0.0\n0.0 {L[1]:.16g} 0.0\n0.0 0.0 {L[2]:.16g}\n\n")
+        fh.write(f"IMCON {m['imcon']}\n\nATOM_TYPES {m['natom_types']}\n")
+        for t in range(m["natom_types"]):
+            fh.write(f"M{t + 1} 1 1.0\n")
+        fh.write(f"\nATOMS {nb}\n")
+        for i in range(nb):
+            fh.write(f"{at[i]} 0.0 {crd[i, 0]:.17g} {crd[i, 1]:.17g} {crd[i, 2]:.17g}\n")
+        fh.write(f"\nBOND_TYPES {len(m['bond_styles'])}\n")
+        for sty, par in zip(m["bond_styles"], m["bond_params"]):
+            fh.write(f"{sty} {len(par)} " + " ".join(f"{p:.17g}" for p in par) + "\n")
+        fh.write(f"\nBONDS {len(m['bonds'])}\n")
+        for b in m["bonds"]:
+            fh.write(f"{b[0]} {b[1]} {b[2]}\n")
+        nv = len(m["vdw_styles"])
+        fh.write(f"\nVDW {nv}\n")
+        k = 0
+        for a in range(1, m["natom_types"] + 1):
+            for b in range(a, m["natom_types"] + 1):
+                par = m["vdw_params"][k]
+                fh.write(f"{a} {b} {m['vdw_styles'][k]} {len(par)} " + " ".join(f"{p:.17g}" for p in par) + "\n")
+                k += 1
+
+
+def fortran_seed(tag):
+    """Seed the Fortran driver derives from job_tag (bdrun.f90:seed_from)."""
+    s = 88172645463325252
+    for ch in tag:
+        s = ((s * 31) & 0xFFFFFFFFFFFFFFFF) ^ ord(ch)
+    return s
