@@ -1,0 +1,948 @@
+// bd_batch.hip -- many independent chains advanced together.
+//
+// Each chain is one BROWNPAK run (linear bead-spring chain, one bond type,
+// one LJ/WCA type, RPY, Cholesky-free block Lanczos with s = nts_mobsam,
+// unbounded domain): the same EM step as bds_run (bd_solver_m.f90:128-238),
+// chain by chain, but every kernel covers all chains of the batch in one
+// launch.  The mobility tiles of all chains share one MobPlan (work units of
+// all chains feed one persistent product kernel); each Lanczos iteration runs
+// the Krylov recurrence of every still-active chain with one block per chain,
+// and the small problem of every chain with one wavefront per (chain,
+// quadrature node).  Chains converge independently: a chain that passed the
+// reference's test (brown_m.f90:497-506) is switched off for the rest of the
+// solve, exactly as if it had left its own loop.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bd_forces_dev.h"
+#include "bd_internal.h"
+#include "bd_lanczos_dev.h"
+
+namespace bd {
+
+constexpr int BT = 256;            // threads per per-chain block
+constexpr int BNBOUND = 32;        // spectral-bound probes per chain
+constexpr int BNQ = 2 * 79 + 1;    // quadrature nodes per chain (kappa <= 1e8)
+
+struct BChainDev {
+    int nb, n, npad, pad;
+    long row;       // row offset of the chain in the padded vectors
+};
+
+struct Batch {
+    int C = 0, NB = 0;
+    long NP = 0;
+    bd_opts opt{};
+    int s = 1, dimk_max = 0;
+    hipStream_t stream = nullptr, side = nullptr;
+    int device = 0;
+    std::vector<int> nb;
+    std::vector<long> row;
+    MobPlan plan;
+    float *mob32 = nullptr;
+    double *mob64 = nullptr;
+    BChainDev *chains = nullptr;
+    int *bead_chain = nullptr;    // per bead (global): chain
+    int *bead_local = nullptr;    // per bead: index inside its chain
+    long *bead_row = nullptr;     // per bead: row of its x component
+    double *crd = nullptr, *frc = nullptr, *dfr = nullptr, *diff = nullptr, *xi = nullptr;
+    double *V = nullptr, *W = nullptr, *Y = nullptr;
+    double *A = nullptr, *B = nullptr, *R0 = nullptr, *Cq = nullptr, *Cy = nullptr;
+    // active: the chain still takes Lanczos iterations (cleared by the
+    // convergence test); run: its Krylov recurrence still advances (also
+    // cleared on a breakdown); failed: iteration whose CholeskyQR broke down
+    int *active = nullptr, *run = nullptr, *failed = nullptr, *conv = nullptr, *bound_ok = nullptr, *nq = nullptr;
+    int *counters = nullptr;      // [0] active chains, [1] error code
+    int *host_counters = nullptr; // pinned, 2 slots x 2
+    double *lam_hi = nullptr;
+    DevStatus *st = nullptr;
+    hipEvent_t ev[LZ_NEV] = {};
+    hipEvent_t chunk_ev[2] = {};
+    // model
+    int bond_style = 2;
+    double bpar[MXPARAM] = {}, vpar[MXPARAM] = {};
+    double *dev_bpar = nullptr, *dev_vpar = nullptr;
+    double rcutoff = 0.0;
+    int excl = 0, lvdw = 1;
+    bool have_model = false;
+    int cntr = 0;
+    long long nts = 0;
+    int n_updates = 0, total_iters = 0, n_not_conv = 0;
+    uint64_t seed = 0x5eed, offset = 0;
+    std::vector<double> stage;    // host staging for injected xi
+    // timing of the product
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_ev;
+    size_t prof_used = 0;
+    double prof_ms = 0.0;
+    long long prof_count = 0;
+};
+
+// ------------------------------------------------------------ kernels
+// Gaussian block for all chains: rows beyond a chain's n are zero.
+__global__ void b_gauss_kernel(double *out, long NP, int s, const int *__restrict__ rowblk,
+                               const BChainDev *__restrict__ ch, uint64_t seed, uint64_t offset)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NP * s) return;
+    long rowg = g % NP;
+    int c = rowblk[8 * (rowg / TILE)];
+    long loc = rowg - ch[c].row;
+    if (loc >= ch[c].n) {
+        out[g] = 0.0;
+        return;
+    }
+    uint64_t ctr = offset + (uint64_t)(g >> 1);
+    uint32_t q[4] = {(uint32_t)ctr, (uint32_t)(ctr >> 32), 0x13198A2Eu, 0x03707344u};
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * q[0], p1 = (uint64_t)0xCD9E8D57u * q[2];
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ q[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ q[3] ^ k1;
+        q[0] = n0; q[1] = (uint32_t)p1; q[2] = n2; q[3] = (uint32_t)p0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    uint64_t a = ((uint64_t)q[0] << 32) | q[1], b = ((uint64_t)q[2] << 32) | q[3];
+    double u1 = ((a >> 11) + 1) * 0x1.0p-53, u2 = (b >> 11) * 0x1.0p-53;
+    double rr = sqrt(-2.0 * log(u1)), sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    out[g] = (g & 1) ? rr * sn : rr * cs;
+}
+
+// Block-wide sum of an S x S matrix accumulated per thread (BT threads).
+template <int S>
+__device__ void bsum(double (&acc)[S * S], double *out)
+{
+    __shared__ double sh[BT / 64][S * S];
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+#pragma unroll
+    for (int e = 0; e < S * S; e++) {
+        double v = acc[e];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) sh[wv][e] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < S * S) {
+        double v = 0.0;
+        for (int w = 0; w < BT / 64; w++) v += sh[w][threadIdx.x];
+        out[threadIdx.x] = v;
+    }
+    __syncthreads();
+}
+
+// CholeskyQR2 of X (rows of one chain) into Q; R = R2 R1 -> Rout.
+template <int S>
+__device__ bool b_cholqr2(const double *X, double *Q, long ld, int rows, double *Rout)
+{
+    __shared__ double g[S * S];
+    double R1[S][S], R2[S][S];
+    for (int pass = 0; pass < 2; pass++) {
+        const double *src = pass == 0 ? X : Q;
+        double acc[S * S];
+#pragma unroll
+        for (int e = 0; e < S * S; e++) acc[e] = 0.0;
+        for (int r = threadIdx.x; r < rows; r += BT) {
+            double v[S];
+#pragma unroll
+            for (int k = 0; k < S; k++) v[k] = src[(long)k * ld + r];
+#pragma unroll
+            for (int cj = 0; cj < S; cj++)
+#pragma unroll
+                for (int ci = 0; ci < S; ci++) acc[ci + cj * S] += v[ci] * v[cj];
+        }
+        bsum<S>(acc, g);
+        double (&R)[S][S] = pass == 0 ? R1 : R2;
+        if (!small_chol<S>(g, R)) return false;
+        for (int r = threadIdx.x; r < rows; r += BT) {
+            double x[S], q[S];
+#pragma unroll
+            for (int k = 0; k < S; k++) x[k] = src[(long)k * ld + r];
+#pragma unroll
+            for (int c = 0; c < S; c++) {
+                double t = x[c];
+#pragma unroll
+                for (int k = 0; k < c; k++) t -= q[k] * R[k][c];
+                q[c] = t / R[c][c];
+            }
+#pragma unroll
+            for (int k = 0; k < S; k++) Q[(long)k * ld + r] = q[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int i = 0; i < S; i++)
+            for (int j = 0; j < S; j++) {
+                double t = 0.0;
+                for (int k = 0; k < S; k++) t += R2[i][k] * R1[k][j];
+                Rout[i + j * S] = t;
+            }
+    return true;
+}
+
+// Initial block: xi = V_1 R0 for every chain.
+template <int S>
+__global__ __launch_bounds__(BT) void b_qr0_kernel(const double *xi, double *V, long NP, const BChainDev *ch,
+                                                   double *R0, int *counters)
+{
+    const int c = blockIdx.x;
+    const long r0 = ch[c].row;
+    if (!b_cholqr2<S>(xi + r0, V + r0, NP, ch[c].npad, R0 + (long)c * S * S) && threadIdx.x == 0)
+        atomicExch(&counters[1], 2);
+}
+
+// One Krylov step for every active chain (one block per chain):
+//   W -= V_{j-1} B_{j-1}^T; A_j = V_j^T W; W -= V_j symL(A_j); W = V_{j+1} B_j.
+template <int S>
+__global__ __launch_bounds__(BT) void b_krylov_kernel(double *W, double *V, long NP, int dmax, int j,
+                                                      const BChainDev *ch, double *A, double *B, int *run,
+                                                      int *failed)
+{
+    const int c = blockIdx.x;
+    if (!run[c]) return;
+    const long r0 = ch[c].row;
+    const int rows = ch[c].npad;
+    const long ldv = NP;
+    double *w = W + r0;
+    const double *vj = V + (long)(j - 1) * S * ldv + r0;
+    const double *vp = (j > 1) ? V + (long)(j - 2) * S * ldv + r0 : nullptr;
+    const double *bp = (j > 1) ? B + ((long)c * dmax + (j - 2)) * S * S : nullptr;
+    double *aj = A + ((long)c * dmax + (j - 1)) * S * S;
+    __shared__ double sA[S * S];
+    double acc[S * S];
+#pragma unroll
+    for (int e = 0; e < S * S; e++) acc[e] = 0.0;
+    for (int r = threadIdx.x; r < rows; r += BT) {
+        double x[S], v[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) { x[k] = w[(long)k * NP + r]; v[k] = vj[(long)k * ldv + r]; }
+        if (vp) {
+#pragma unroll
+            for (int cc = 0; cc < S; cc++) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < S; k++) t += vp[(long)k * ldv + r] * bp[cc + k * S];
+                x[cc] = x[cc] - t;
+            }
+#pragma unroll
+            for (int k = 0; k < S; k++) w[(long)k * NP + r] = x[k];
+        }
+#pragma unroll
+        for (int cj = 0; cj < S; cj++)
+#pragma unroll
+            for (int ci = 0; ci < S; ci++) acc[ci + cj * S] += v[ci] * x[cj];
+    }
+    bsum<S>(acc, sA);
+    if (threadIdx.x < S * S) aj[threadIdx.x] = sA[threadIdx.x];
+    if (j >= dmax) return;
+    for (int r = threadIdx.x; r < rows; r += BT) {
+        double x[S], v[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) { x[k] = w[(long)k * NP + r]; v[k] = vj[(long)k * ldv + r]; }
+#pragma unroll
+        for (int cc = 0; cc < S; cc++) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < S; k++) t += v[k] * ((k >= cc) ? sA[k + cc * S] : sA[cc + k * S]);
+            x[cc] = x[cc] - t;
+        }
+#pragma unroll
+        for (int k = 0; k < S; k++) w[(long)k * NP + r] = x[k];
+    }
+    __syncthreads();
+    double *vn = V + (long)j * S * ldv + r0;
+    if (!b_cholqr2<S>(w, vn, NP, rows, B + ((long)c * dmax + (j - 1)) * S * S) && threadIdx.x == 0) {
+        // breakdown: V_{j+1} does not exist; harmless if iteration j converges
+        failed[c] = j;
+        run[c] = 0;
+    }
+}
+
+// H_j of chain c exists if no CholeskyQR before iteration j broke down
+__device__ inline bool b_live(const int *active, const int *failed, int c, int j)
+{
+    return active[c] && !(failed[c] && failed[c] < j);
+}
+
+template <int S>
+__global__ __launch_bounds__(64) void b_bounds_kernel(const double *A, const double *B, int dmax, int j,
+                                                      const int *active, const int *failed, int *ok, double *lam_hi)
+{
+    const int c = blockIdx.x / BNBOUND, b = blockIdx.x % BNBOUND, js = j * S;
+    if (!b_live(active, failed, c, j)) return;
+    extern __shared__ double band[];
+    const double *Ab = A + (long)c * dmax * S * S, *Bb = B + (long)c * dmax * S * S;
+    double M = gersh_hi<S>(Ab, Bb, js);
+    double mu = ldexp(M, -(b + 1));
+    load_band<S>(Ab, Bb, js, mu, band, js);
+    bool good = band_chol<S>(band, js, js);
+    if (threadIdx.x == 0) ok[c * BNBOUND + b] = good ? 1 : 0;
+    if (b == 0 && threadIdx.x == 0) lam_hi[c] = M;
+}
+
+template <int S>
+__global__ __launch_bounds__(64) void b_quad_kernel(const double *A, const double *B, const double *R0, int dmax,
+                                                    int j, const int *active, const int *failed, const int *ok,
+                                                    const double *lam_hi, double *part, int *nq, int *counters)
+{
+    const int c = blockIdx.x / BNQ, node = blockIdx.x % BNQ, js = j * S;
+    if (!b_live(active, failed, c, j)) return;
+    extern __shared__ double smem[];
+    double *band = smem, *z = smem + (S + 1) * js;
+    const double *Ab = A + (long)c * dmax * S * S, *Bb = B + (long)c * dmax * S * S;
+    const double M = lam_hi[c];
+    int kb = -1;
+    for (int b = 0; b < BNBOUND; b++)
+        if (ok[c * BNBOUND + b]) { kb = b; break; }
+    if (kb < 0) {
+        if (threadIdx.x == 0) atomicExch(&counters[1], 2);
+        return;
+    }
+    const double m = ldexp(M, -(kb + 1));
+    int dec = (int)ceil(log10(M / m));
+    if (dec > 8) {  // the node table of this kernel ends at kappa = 1e8
+        if (threadIdx.x == 0) atomicExch(&counters[1], 3);
+        return;
+    }
+    dec = dec < 1 ? 1 : dec;
+    const int N = c_qN[dec];
+    const double h = c_qh[dec];
+    if (node == 0 && threadIdx.x == 0) nq[c] = 2 * N + 1;
+    if (node > 2 * N) return;
+    const double u0 = 0.25 * (log(m) + log(M));
+    const double v = (node - N) * h;
+    const double u = u0 + sinh(v);
+    const double wgt = (2.0 / M_PI) * h * cosh(v) * exp(u);
+    load_band<S>(Ab, Bb, js, -exp(2.0 * u), band, js);
+    if (!band_chol<S>(band, js, js)) {
+        if (threadIdx.x == 0) atomicExch(&counters[1], 2);
+        return;
+    }
+    for (int e = threadIdx.x; e < js * S; e += blockDim.x) {
+        int i = e % js, cc = e / js;
+        z[e] = (i < S) ? R0[(long)c * S * S + i + cc * S] : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x < S) {
+        double *zc = z + threadIdx.x * js;
+        for (int i = 0; i < js; i++) {
+            double t = zc[i];
+            for (int d = 1; d <= S && d <= i; d++) t -= band[d * js + i - d] * zc[i - d];
+            zc[i] = t / band[i];
+        }
+        for (int i = js - 1; i >= 0; i--) {
+            double t = zc[i];
+            for (int d = 1; d <= S && i + d < js; d++) t -= band[d * js + i] * zc[i + d];
+            zc[i] = t / band[i];
+        }
+    }
+    __syncthreads();
+    double *out = part + ((long)c * BNQ + node) * js * S;
+    for (int e = threadIdx.x; e < js * S; e += blockDim.x) out[e] = wgt * z[e];
+}
+
+// Per chain (one block): ysum = sum of node parts; C = H ysum; y_j = V C;
+// relative change against y_{j-1}; convergence test.
+template <int S>
+__global__ __launch_bounds__(BT) void b_combine_kernel(const double *A, const double *B, int dmax, int j,
+                                                       const double *part, const int *nq, const double *V, long NP,
+                                                       const BChainDev *ch, double *Y, int *active, int *run,
+                                                       const int *failed, int *conv, double tol, int *counters)
+{
+    const int c = blockIdx.x, js = j * S;
+    if (!active[c]) return;
+    if (!b_live(active, failed, c, j)) {
+        // the Krylov basis ended before this chain converged: stop it here
+        // and report it as failed at the end of the solve
+        if (threadIdx.x == 0) {
+            active[c] = 0;
+            atomicSub(&counters[0], 1);
+        }
+        return;
+    }
+    extern __shared__ double sm[];
+    double *ys = sm, *Cc = sm + js * S;
+    const double *Ab = A + (long)c * dmax * S * S, *Bb = B + (long)c * dmax * S * S;
+    const int nn = nq[c];
+    for (int e = threadIdx.x; e < js * S; e += BT) {
+        double v = 0.0;
+        for (int k = 0; k < nn; k++) v += part[((long)c * BNQ + k) * js * S + e];
+        ys[e] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < js * S; e += BT) {
+        int i = e % js, cc = e / js;
+        double v = 0.0;
+        for (int l = max(0, i - S); l <= min(js - 1, i + S); l++) {
+            double hv = (i >= l) ? h_entry<S>(Ab, Bb, js, i, l) : h_entry<S>(Ab, Bb, js, l, i);
+            v += hv * ys[l + cc * js];
+        }
+        Cc[e] = v;
+    }
+    __syncthreads();
+    const long r0 = ch[c].row;
+    const int rows = ch[c].npad;
+    double *y = Y + (long)(j % 2) * S * NP + r0;
+    const double *yo = Y + (long)((j + 1) % 2) * S * NP + r0;
+    const bool want = j > 2;
+    double dn[S], on[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) { dn[k] = 0.0; on[k] = 0.0; }
+    for (int r = threadIdx.x; r < rows; r += BT) {
+        double acc[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) acc[k] = 0.0;
+        for (int m = 0; m < js; m++) {
+            double vm = V[(long)m * NP + r0 + r];
+#pragma unroll
+            for (int k = 0; k < S; k++) acc[k] += vm * Cc[m + k * js];
+        }
+#pragma unroll
+        for (int k = 0; k < S; k++) {
+            y[(long)k * NP + r] = acc[k];
+            if (want) {
+                double o = yo[(long)k * NP + r], d = acc[k] - o;
+                if (S == 1) { dn[k] += d * d; on[k] += o * o; }
+                else { dn[k] += fabs(d); on[k] += fabs(o); }
+            }
+        }
+    }
+    if (!want) return;
+    double tot[2 * S];
+    __shared__ double sh[BT / 64][2 * S];
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+#pragma unroll
+    for (int k = 0; k < 2 * S; k++) {
+        double v = k < S ? dn[k] : on[k - S];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) sh[wv][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 2 * S; k++) {
+        double v = 0.0;
+        for (int w = 0; w < BT / 64; w++) v += sh[w][k];
+        tot[k] = v;
+    }
+    double num = 0.0, den = 0.0;
+    if (S == 1) {
+        num = sqrt(tot[0]);
+        den = sqrt(tot[1]);
+    } else {
+        for (int k = 0; k < S; k++) { num = fmax(num, tot[k]); den = fmax(den, tot[S + k]); }
+    }
+    if (num / den <= tol) {
+        conv[c] = j;
+        active[c] = 0;
+        run[c] = 0;
+        atomicSub(&counters[0], 1);
+    }
+}
+
+// Forces of every bead: chain bonds (i-1, i+1) and the chain's LJ pairs
+// r^2 < rc^2 with |i - j| > excl (the 1..3-ring exclusions of a linear chain).
+__global__ __launch_bounds__(256) void b_force_kernel(const double *__restrict__ crd, int NB,
+                                                      const int *__restrict__ bead_chain,
+                                                      const int *__restrict__ bead_local, const long *__restrict__ bead_row,
+                                                      const BChainDev *__restrict__ ch, int bsty, const double *bp,
+                                                      int lvdw, const double *vp, double rc2, int excl,
+                                                      double *__restrict__ frc, int *counters)
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NB) return;
+    const int c = bead_chain[g], i = bead_local[g], nbc = ch[c].nb;
+    const double *cr = crd + ch[c].row;
+    double f[3] = {0, 0, 0};
+    // bonds (i-1, i) then (i, i+1), the bond order of a linear chain
+    for (int side = -1; side <= 1; side += 2) {
+        int j = i + side;
+        if (j < 0 || j >= nbc) continue;
+        double x = __dsub_rn(cr[3 * j], cr[3 * i]), y = __dsub_rn(cr[3 * j + 1], cr[3 * i + 1]),
+               z = __dsub_rn(cr[3 * j + 2], cr[3 * i + 2]);
+        double r = sqrt(rsq3(x, y, z)), en, fr;
+        if (bond_force(r, bsty, bp, en, fr)) atomicExch(&counters[1], 1);
+        f[0] += fr * x / r;
+        f[1] += fr * y / r;
+        f[2] += fr * z / r;
+    }
+    if (lvdw) {
+        for (int j = 0; j < nbc; j++) {
+            if (j == i || abs(j - i) <= excl) continue;
+            double x = __dsub_rn(cr[3 * j], cr[3 * i]), y = __dsub_rn(cr[3 * j + 1], cr[3 * i + 1]),
+                   z = __dsub_rn(cr[3 * j + 2], cr[3 * i + 2]);
+            double r2 = rsq3(x, y, z);
+            if (!(r2 < rc2)) continue;
+            double r = sqrt(r2), en, fr;
+            lj_force(r, vp, en, fr);
+            f[0] += fr * x / r;
+            f[1] += fr * y / r;
+            f[2] += fr * z / r;
+        }
+    }
+    long o = bead_row[g];
+    frc[o] = f[0]; frc[o + 1] = f[1]; frc[o + 2] = f[2];
+}
+
+// r += dt * (D F) + diff (diff may be null); one thread per bead.
+__global__ void b_update_kernel(double *crd, const double *dfr, const double *diff, int NB, const long *bead_row,
+                                double dt)
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NB) return;
+    long o = bead_row[g];
+    for (int d = 0; d < 3; d++) {
+        double r = crd[o + d] + dt * dfr[o + d];
+        if (diff) r = r + diff[o + d];
+        crd[o + d] = r;
+    }
+}
+
+// centre of mass to the origin, one block per chain (simbox_m.f90:182-202)
+__global__ __launch_bounds__(256) void b_com_kernel(double *crd, const BChainDev *ch)
+{
+    const int c = blockIdx.x;
+    double *cr = crd + ch[c].row;
+    const int nb = ch[c].nb;
+    __shared__ double sh[3][4];
+    __shared__ double m[3];
+    double s[3] = {0, 0, 0};
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        for (int d = 0; d < 3; d++) s[d] += cr[3 * i + d];
+    for (int d = 0; d < 3; d++) {
+        double v = s[d];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (threadIdx.x % 64 == 0) sh[d][threadIdx.x / 64] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) m[threadIdx.x] = (sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3]) / nb;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        for (int d = 0; d < 3; d++) cr[3 * i + d] = cr[3 * i + d] - m[d];
+}
+
+__global__ void b_scale_copy_kernel(double *dst, const double *src, long n, double a)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = a * src[i];
+}
+
+__global__ void b_lz_init_kernel(int *active, int *run, int *failed, int *conv, int C, int *counters)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < C) {
+        active[c] = 1;
+        run[c] = 1;
+        failed[c] = 0;
+        conv[c] = 0;
+    }
+    if (c == 0) counters[0] = C;
+}
+
+// results of non-converged chains: y of the last iteration
+__global__ void b_finish_kernel(const double *Y, double *out, long NP, int s, int jlast, const int *conv,
+                                const int *rowblk)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NP * s) return;
+    long row = g % NP, k = g / NP;
+    int c = rowblk[8 * (row / TILE)];
+    int j = conv[c] ? conv[c] : jlast;
+    out[g] = Y[(long)(j % 2) * s * NP + k * NP + row];
+}
+
+// ------------------------------------------------------------ host
+template <int S>
+static int b_lanczos(Batch &b, int *iters_out)
+{
+    hipStream_t sm = b.stream, ss = b.side;
+    const int C = b.C, dmax = b.dimk_max;
+    const long NP = b.NP;
+    hipLaunchKernelGGL(b_lz_init_kernel, dim3((C + 255) / 256), dim3(256), 0, sm, b.active, b.run, b.failed, b.conv, C,
+                       b.counters);
+    hipLaunchKernelGGL((b_qr0_kernel<S>), dim3(C), dim3(BT), 0, sm, b.xi, b.V, NP, b.chains, b.R0, b.counters);
+    BD_HIP(hipGetLastError());
+    int pending = -1, slot = 0, jlast = 0;
+    bool stop = false;
+    for (int j = 1; j <= dmax && !stop; j++) {
+        double *Vj = b.V + (size_t)(j - 1) * S * NP;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (b.prof_on) {
+            if (b.prof_used + 2 > b.prof_ev.size()) {
+                hipEvent_t x, y;
+                BD_HIP(hipEventCreate(&x));
+                BD_HIP(hipEventCreate(&y));
+                b.prof_ev.push_back(x);
+                b.prof_ev.push_back(y);
+            }
+            e0 = b.prof_ev[b.prof_used];
+            e1 = b.prof_ev[b.prof_used + 1];
+            b.prof_used += 2;
+        }
+        if (b.mob32)
+            BD_TRY((plan_product<float, S>(b.plan, b.mob32, Vj, b.W, (int)NP, (int)NP, &b.st->ticket, nullptr,
+                                           b.run, sm, b.device, e0, e1)));
+        else
+            BD_TRY((plan_product<double, S>(b.plan, b.mob64, Vj, b.W, (int)NP, (int)NP, &b.st->ticket, nullptr,
+                                            b.run, sm, b.device, e0, e1)));
+        hipLaunchKernelGGL((b_krylov_kernel<S>), dim3(C), dim3(BT), 0, sm, b.W, b.V, NP, dmax, j, b.chains, b.A,
+                           b.B, b.run, b.failed);
+        BD_HIP(hipGetLastError());
+        jlast = j;
+        if (j >= 2) {
+            hipEvent_t ev = b.ev[j % LZ_NEV];
+            BD_HIP(hipEventRecord(ev, sm));
+            BD_HIP(hipStreamWaitEvent(ss, ev, 0));
+            const int js = j * S;
+            size_t bb = (size_t)(S + 1) * js * sizeof(double);
+            hipLaunchKernelGGL((b_bounds_kernel<S>), dim3(C * BNBOUND), dim3(64), bb, ss, b.A, b.B, dmax, j, b.active,
+                               b.failed, b.bound_ok, b.lam_hi);
+            hipLaunchKernelGGL((b_quad_kernel<S>), dim3(C * BNQ), dim3(64), bb + (size_t)js * S * sizeof(double), ss,
+                               b.A, b.B, b.R0, dmax, j, b.active, b.failed, b.bound_ok, b.lam_hi, b.Cq, b.nq,
+                               b.counters);
+            hipLaunchKernelGGL((b_combine_kernel<S>), dim3(C), dim3(BT), (size_t)2 * js * S * sizeof(double), ss, b.A,
+                               b.B, dmax, j, b.Cq, b.nq, b.V, NP, b.chains, b.Y, b.active, b.run, b.failed, b.conv,
+                               b.opt.lanc_tol, b.counters);
+            BD_HIP(hipGetLastError());
+        }
+        if (j % 4 == 0 || j == dmax) {
+            BD_HIP(hipMemcpyAsync(b.host_counters + 2 * slot, b.counters, 2 * sizeof(int), hipMemcpyDeviceToHost, ss));
+            BD_HIP(hipEventRecord(b.chunk_ev[slot], ss));
+            if (pending >= 0) {
+                BD_HIP(hipEventSynchronize(b.chunk_ev[pending]));
+                if (b.host_counters[2 * pending] == 0 || b.host_counters[2 * pending + 1]) stop = true;
+            }
+            pending = slot;
+            slot ^= 1;
+        }
+    }
+    BD_HIP(hipStreamSynchronize(ss));
+    BD_HIP(hipStreamSynchronize(sm));
+    int cnt[2];
+    BD_HIP(hipMemcpy(cnt, b.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (cnt[1] == 1) { set_error("bond length too large (overstretched bond)"); return BD_ERR_BOND; }
+    if (cnt[1] == 3) { set_error("spectral range of a Lanczos matrix beyond 1e8"); return BD_ERR_FACTOR; }
+    if (cnt[1]) { set_error("batched Lanczos failed (matrix not SPD?)"); return BD_ERR_FACTOR; }
+    hipLaunchKernelGGL(b_finish_kernel, dim3((unsigned)((NP * S + 255) / 256)), dim3(256), 0, sm, b.Y, b.diff, NP, S,
+                       jlast, b.conv, b.plan.rowblk);
+    BD_HIP(hipGetLastError());
+    std::vector<int> conv(C), failed(C);
+    BD_HIP(hipMemcpy(conv.data(), b.conv, C * sizeof(int), hipMemcpyDeviceToHost));
+    BD_HIP(hipMemcpy(failed.data(), b.failed, C * sizeof(int), hipMemcpyDeviceToHost));
+    for (int c = 0; c < C; c++)
+        if (failed[c] && !conv[c]) {
+            set_error("Krylov breakdown before convergence in chain " + std::to_string(c));
+            return BD_ERR_FACTOR;
+        }
+    for (int c = 0; c < C; c++) {
+        int it = conv[c] ? conv[c] : dmax;
+        if (iters_out) iters_out[c] = it;
+        b.total_iters += it;
+        if (!conv[c]) b.n_not_conv++;
+    }
+    return BD_OK;
+}
+
+static int b_step(Batch &b, const double *xi_host, long long avail, long long *consumed)
+{
+    const int s = b.s;
+    const long NP = b.NP;
+    hipStream_t sm = b.stream;
+    *consumed = 0;
+    bool noise = true;
+    if (b.cntr == 0) {
+        if (b.mob32) BD_TRY(rpy_launch<float>(b.plan, b.crd, b.mob32, sm));
+        else BD_TRY(rpy_launch<double>(b.plan, b.crd, b.mob64, sm));
+        long long need = 0;
+        for (int c = 0; c < b.C; c++) need += 3LL * b.nb[c] * s;
+        if (xi_host) {
+            if (avail < need) { set_error("injected xi stream exhausted"); return BD_ERR_ARG; }
+            // chain by chain (n_c x s column-major) into the padded layout
+            b.stage.assign((size_t)NP * s, 0.0);
+            long long p = 0;
+            for (int c = 0; c < b.C; c++) {
+                const int n = 3 * b.nb[c];
+                for (int k = 0; k < s; k++)
+                    for (int r = 0; r < n; r++) b.stage[(size_t)k * NP + b.row[c] + r] = xi_host[p++];
+            }
+            BD_HIP(hipMemcpyAsync(b.xi, b.stage.data(), (size_t)NP * s * sizeof(double), hipMemcpyHostToDevice, sm));
+            *consumed = need;
+        } else {
+            hipLaunchKernelGGL(b_gauss_kernel, dim3((unsigned)((NP * s + 255) / 256)), dim3(256), 0, sm, b.xi, NP, s,
+                               b.plan.rowblk, b.chains, b.seed, b.offset);
+            BD_HIP(hipGetLastError());
+            b.offset += (uint64_t)NP * s;
+        }
+        int rc;
+        switch (s) {
+        case 1: rc = b_lanczos<1>(b, nullptr); break;
+        case 2: rc = b_lanczos<2>(b, nullptr); break;
+        case 3: rc = b_lanczos<3>(b, nullptr); break;
+        case 4: rc = b_lanczos<4>(b, nullptr); break;
+        default: set_error("batched path supports nts_mobsam 1..4"); return BD_ERR_ARG;
+        }
+        if (rc != BD_OK) return rc;
+        b.n_updates++;
+        hipLaunchKernelGGL(b_scale_copy_kernel, dim3((unsigned)((NP * s + 255) / 256)), dim3(256), 0, sm, b.diff,
+                           b.diff, NP * s, std::sqrt(2.0 * b.opt.tim_stp));
+    } else if (!b.opt.keep_reuse_noise) {
+        noise = false;  // bd_solver_m.f90:364
+    }
+    b.cntr++;
+    const int nbk = (b.NB + 255) / 256;
+    hipLaunchKernelGGL(b_force_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.NB, b.bead_chain, b.bead_local,
+                       b.bead_row, b.chains, b.bond_style, b.dev_bpar, b.lvdw, b.dev_vpar, b.rcutoff * b.rcutoff,
+                       b.excl, b.frc, b.counters);
+    if (b.mob32)
+        BD_TRY((plan_product<float, 1>(b.plan, b.mob32, b.frc, b.dfr, (int)NP, (int)NP, &b.st->ticket, nullptr,
+                                       nullptr, sm, b.device, nullptr, nullptr)));
+    else
+        BD_TRY((plan_product<double, 1>(b.plan, b.mob64, b.frc, b.dfr, (int)NP, (int)NP, &b.st->ticket, nullptr,
+                                        nullptr, sm, b.device, nullptr, nullptr)));
+    const double *dcol = noise ? b.diff + (size_t)(b.cntr - 1) * NP : nullptr;
+    hipLaunchKernelGGL(b_update_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.dfr, dcol, b.NB, b.bead_row,
+                       b.opt.tim_stp);
+    if (b.cntr == s) b.cntr = 0;
+    hipLaunchKernelGGL(b_com_kernel, dim3(b.C), dim3(256), 0, sm, b.crd, b.chains);
+    BD_HIP(hipGetLastError());
+    b.nts++;
+    return BD_OK;
+}
+
+}  // namespace bd
+
+using namespace bd;
+
+struct bd_batch {
+    Batch b;
+};
+
+extern "C" {
+
+bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
+{
+    if (nchains <= 0 || !nbeads || !opts) { set_error("bad arguments"); return nullptr; }
+    if (!opts->lhdia || opts->mob_fctr != BD_LANC || opts->nts_mobsam < 1 || opts->nts_mobsam > 4) {
+        set_error("the batched path runs RPY with (block) Lanczos, nts_mobsam 1..4");
+        return nullptr;
+    }
+    bd_batch *h = new bd_batch();
+    Batch &b = h->b;
+    auto fail = [&](const char *m) -> bd_batch * {
+        set_error(m);
+        bd_batch_destroy(h);
+        return nullptr;
+    };
+    b.C = nchains;
+    b.opt = *opts;
+    b.s = opts->nts_mobsam;
+    b.dimk_max = (opts->lanc_mxitr > 2 ? opts->lanc_mxitr : 2) + 1;
+    if (b.dimk_max * b.s > 4 * 79) return fail("lanc_mxitr too large for the batched path");
+    b.nb.assign(nbeads, nbeads + nchains);
+    long rowp = 0;
+    std::vector<BChainDev> ch(nchains);
+    std::vector<int> bc, bl;
+    std::vector<long> br;
+    for (int c = 0; c < nchains; c++) {
+        if (nbeads[c] < 2) return fail("chains need at least 2 beads");
+        const int n = 3 * nbeads[c], np = ((n + TILE - 1) / TILE) * TILE;
+        b.row.push_back(rowp);
+        ch[c] = BChainDev{nbeads[c], n, np, 0, rowp};
+        for (int i = 0; i < nbeads[c]; i++) {
+            bc.push_back(c);
+            bl.push_back(i);
+            br.push_back(rowp + 3L * i);
+        }
+        rowp += np;
+    }
+    b.NP = rowp;
+    b.NB = (int)bc.size();
+    if (hipGetDevice(&b.device) != hipSuccess) return fail("no HIP device");
+    if (hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking) != hipSuccess)
+        return fail("stream creation failed");
+    if (plan_build(b.plan, b.nb, b.row) != BD_OK) return fail("plan construction failed");
+    const size_t elems = (size_t)b.plan.ntiles * TILE * TILE;
+    bool ok = true;
+    auto dal = [&](double **p, size_t count) {
+        if (ok) ok = hipMalloc(p, std::max<size_t>(count, 1) * sizeof(double)) == hipSuccess &&
+                     hipMemset(*p, 0, std::max<size_t>(count, 1) * sizeof(double)) == hipSuccess;
+    };
+    auto ial = [&](int **p, size_t count) {
+        if (ok) ok = hipMalloc(p, std::max<size_t>(count, 1) * sizeof(int)) == hipSuccess &&
+                     hipMemset(*p, 0, std::max<size_t>(count, 1) * sizeof(int)) == hipSuccess;
+    };
+    if (opts->precision == BD_FP32) ok = hipMalloc(&b.mob32, elems * sizeof(float)) == hipSuccess;
+    else ok = hipMalloc(&b.mob64, elems * sizeof(double)) == hipSuccess;
+    const int s = b.s, dm = b.dimk_max, jsmax = s * dm;
+    dal(&b.crd, b.NP); dal(&b.frc, b.NP); dal(&b.dfr, b.NP); dal(&b.diff, (size_t)b.NP * s);
+    dal(&b.xi, (size_t)b.NP * s); dal(&b.V, (size_t)b.NP * s * dm); dal(&b.W, (size_t)b.NP * s);
+    dal(&b.Y, (size_t)2 * b.NP * s);
+    dal(&b.A, (size_t)nchains * dm * s * s); dal(&b.B, (size_t)nchains * dm * s * s);
+    dal(&b.R0, (size_t)nchains * s * s); dal(&b.Cq, (size_t)nchains * BNQ * jsmax * s);
+    dal(&b.lam_hi, nchains); dal(&b.dev_bpar, MXPARAM); dal(&b.dev_vpar, MXPARAM);
+    ial(&b.active, nchains); ial(&b.run, nchains); ial(&b.failed, nchains); ial(&b.conv, nchains); ial(&b.bound_ok, (size_t)nchains * BNBOUND);
+    ial(&b.nq, nchains); ial(&b.counters, 2); ial(&b.bead_chain, b.NB); ial(&b.bead_local, b.NB);
+    if (!ok) return fail("device allocation failed");
+    if (hipMalloc(&b.bead_row, b.NB * sizeof(long)) != hipSuccess ||
+        hipMalloc(&b.chains, nchains * sizeof(BChainDev)) != hipSuccess ||
+        hipMalloc(&b.st, sizeof(DevStatus)) != hipSuccess || hipMemset(b.st, 0, sizeof(DevStatus)) != hipSuccess ||
+        hipHostMalloc(&b.host_counters, 4 * sizeof(int)) != hipSuccess)
+        return fail("device allocation failed");
+    if (hipMemcpy(b.chains, ch.data(), nchains * sizeof(BChainDev), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b.bead_chain, bc.data(), b.NB * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b.bead_local, bl.data(), b.NB * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b.bead_row, br.data(), b.NB * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
+        return fail("upload failed");
+    for (int k = 0; k < LZ_NEV; k++)
+        if (hipEventCreateWithFlags(&b.ev[k], hipEventDisableTiming) != hipSuccess) return fail("event failed");
+    for (int k = 0; k < 2; k++)
+        if (hipEventCreateWithFlags(&b.chunk_ev[k], hipEventDisableTiming) != hipSuccess) return fail("event failed");
+    return h;
+}
+
+void bd_batch_destroy(bd_batch *h)
+{
+    if (!h) return;
+    Batch &b = h->b;
+    if (b.stream) (void)hipStreamSynchronize(b.stream);
+    if (b.side) (void)hipStreamSynchronize(b.side);
+    plan_free(b.plan);
+    double **dp[] = {&b.crd, &b.frc, &b.dfr, &b.diff, &b.xi, &b.V, &b.W, &b.Y, &b.A, &b.B, &b.R0, &b.Cq,
+                     &b.lam_hi, &b.dev_bpar, &b.dev_vpar, &b.mob64};
+    for (auto p : dp)
+        if (*p) (void)hipFree(*p);
+    int **ip[] = {&b.active, &b.run, &b.failed, &b.conv, &b.bound_ok, &b.nq, &b.counters, &b.bead_chain, &b.bead_local};
+    for (auto p : ip)
+        if (*p) (void)hipFree(*p);
+    if (b.mob32) (void)hipFree(b.mob32);
+    if (b.bead_row) (void)hipFree(b.bead_row);
+    if (b.chains) (void)hipFree(b.chains);
+    if (b.st) (void)hipFree(b.st);
+    if (b.host_counters) (void)hipHostFree(b.host_counters);
+    for (auto e : b.ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : b.chunk_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : b.prof_ev) (void)hipEventDestroy(e);
+    if (b.side) (void)hipStreamDestroy(b.side);
+    if (b.stream) (void)hipStreamDestroy(b.stream);
+    delete h;
+}
+
+int bd_batch_set_model(bd_batch *h, int bond_style, const double *bond_params, int lvdw, const double *vdw_params,
+                       double rcutoff, int excluded_atoms)
+{
+    if (!h || !bond_params || (lvdw && !vdw_params) || (bond_style != 1 && bond_style != 2) || excluded_atoms < 0 ||
+        excluded_atoms > 3) {
+        set_error("bad model arguments (bond style 1 or 2, vdw style 1)");
+        return BD_ERR_ARG;
+    }
+    Batch &b = h->b;
+    b.bond_style = bond_style;
+    for (int k = 0; k < MXPARAM; k++) b.bpar[k] = bond_params[k];
+    if (bond_style == 2) b.bpar[3] = b.bpar[1] * b.bpar[1];  // bond_fene_set
+    b.lvdw = lvdw ? 1 : 0;
+    if (lvdw) {
+        for (int k = 0; k < MXPARAM; k++) b.vpar[k] = vdw_params[k];
+        double sir = b.vpar[1] / b.vpar[2], s2 = sir * sir, s6 = s2 * s2 * s2, s12 = s6 * s6;  // vdw_lj_set
+        b.vpar[3] = 4 * b.vpar[0] * (s12 - s6);
+        b.vpar[4] = -24 * b.vpar[0] * (2 * s12 - s6) / b.vpar[2];
+    }
+    b.rcutoff = rcutoff;
+    b.excl = excluded_atoms;
+    BD_HIP(hipMemcpy(b.dev_bpar, b.bpar, sizeof(b.bpar), hipMemcpyHostToDevice));
+    BD_HIP(hipMemcpy(b.dev_vpar, b.vpar, sizeof(b.vpar), hipMemcpyHostToDevice));
+    b.have_model = true;
+    return BD_OK;
+}
+
+int bd_batch_set_coordinates(bd_batch *h, const double *crd)
+{
+    if (!h || !crd) { set_error("null argument"); return BD_ERR_ARG; }
+    Batch &b = h->b;
+    std::vector<double> st((size_t)b.NP, 0.0);
+    long p = 0;
+    for (int c = 0; c < b.C; c++)
+        for (int i = 0; i < 3 * b.nb[c]; i++) st[b.row[c] + i] = crd[p++];
+    BD_HIP(hipMemcpy(b.crd, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice));
+    return BD_OK;
+}
+
+int bd_batch_get_coordinates(bd_batch *h, double *crd)
+{
+    if (!h || !crd) { set_error("null argument"); return BD_ERR_ARG; }
+    Batch &b = h->b;
+    std::vector<double> st((size_t)b.NP);
+    BD_HIP(hipStreamSynchronize(b.stream));
+    BD_HIP(hipMemcpy(st.data(), b.crd, st.size() * sizeof(double), hipMemcpyDeviceToHost));
+    long p = 0;
+    for (int c = 0; c < b.C; c++)
+        for (int i = 0; i < 3 * b.nb[c]; i++) crd[p++] = st[b.row[c] + i];
+    return BD_OK;
+}
+
+int bd_batch_set_rng(bd_batch *h, uint64_t seed, uint64_t offset)
+{
+    if (!h) { set_error("null batch"); return BD_ERR_ARG; }
+    h->b.seed = seed;
+    h->b.offset = offset;
+    return BD_OK;
+}
+
+int bd_batch_run(bd_batch *h, int nsteps, const double *xi, long long nxi, long long *used)
+{
+    if (!h || nsteps < 0) { set_error("bad arguments"); return BD_ERR_ARG; }
+    Batch &b = h->b;
+    if (!b.have_model) { set_error("model not set"); return BD_ERR_STATE; }
+    long long pos = 0;
+    hipLaunchKernelGGL(b_com_kernel, dim3(b.C), dim3(256), 0, b.stream, b.crd, b.chains);
+    int rc = BD_OK;
+    for (int k = 0; k < nsteps && rc == BD_OK; k++) {
+        long long cons = 0;
+        rc = b_step(b, xi ? xi + pos : nullptr, xi ? nxi - pos : 0, &cons);
+        pos += cons;
+    }
+    if (used) *used = pos;
+    BD_HIP(hipStreamSynchronize(b.stream));
+    int cnt[2];
+    BD_HIP(hipMemcpy(cnt, b.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (rc == BD_OK && cnt[1] == 1) { set_error("bond length too large (overstretched bond)"); rc = BD_ERR_BOND; }
+    return rc;
+}
+
+int bd_batch_info(bd_batch *h, long long *nts, int *mob_updates, int *total_iters, int *n_not_converged)
+{
+    if (!h) { set_error("null batch"); return BD_ERR_ARG; }
+    Batch &b = h->b;
+    if (nts) *nts = b.nts;
+    if (mob_updates) *mob_updates = b.n_updates;
+    if (total_iters) *total_iters = b.total_iters;
+    if (n_not_converged) *n_not_converged = b.n_not_conv;
+    return BD_OK;
+}
+
+int bd_batch_prof(bd_batch *h, int on, double *total_ms, long long *launches, double *tile_bytes)
+{
+    if (!h) { set_error("null batch"); return BD_ERR_ARG; }
+    Batch &b = h->b;
+    BD_HIP(hipStreamSynchronize(b.stream));
+    for (size_t k = 0; k + 1 < b.prof_used; k += 2) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, b.prof_ev[k], b.prof_ev[k + 1]) == hipSuccess) {
+            b.prof_ms += ms;
+            b.prof_count++;
+        }
+    }
+    b.prof_used = 0;
+    if (total_ms) *total_ms = b.prof_ms;
+    if (launches) *launches = b.prof_count;
+    if (tile_bytes) *tile_bytes = (double)b.plan.ntiles * TILE * TILE * (b.mob32 ? 4.0 : 8.0);
+    if (on >= 0) {
+        b.prof_on = on != 0;
+        b.prof_ms = 0.0;
+        b.prof_count = 0;
+    }
+    return BD_OK;
+}
+
+}  // extern "C"

@@ -62,6 +62,28 @@ struct DevStatus {
     double lam_lo, lam_hi;  // spectral bounds of the last small problem
 };
 
+// Tile layout and work decomposition of the mobility of one or more chains
+// (bd_mobility.hip).  chain_i: nb, n, ntile, nsc per chain; chain_l: row
+// offset of the chain in the vectors, first tile, first row-partial block.
+struct MobPlan {
+    int nchains = 0;
+    long ntiles = 0;
+    int nunits = 0, nrowblk = 0;
+    int *tiles = nullptr;     // 4 ints per stored tile: chain, I, J, 0
+    int *units = nullptr;     // 4 ints per unit: chain, super-column, Ib, Ie
+    int *rowblk = nullptr;    // 8 ints per output block
+    int *chain_i = nullptr;
+    long *chain_l = nullptr;
+    double *rowpart = nullptr, *colpart = nullptr;
+};
+int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off);
+void plan_free(MobPlan &p);
+template <typename E>
+int rpy_launch(const MobPlan &p, const double *crd, E *mob, hipStream_t sm);
+template <typename E, int S>
+int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int ldx, int ldy, int *ticket,
+                 const DevStatus *stop, const int *active, hipStream_t sm, int device, hipEvent_t e0, hipEvent_t e1);
+
 struct Ctx {
     int nb = 0, n = 0, npad = 0, ntile = 0, ntiles_lower = 0;
     bd_opts opt{};
@@ -84,13 +106,9 @@ struct Ctx {
     double *chol = nullptr;      // dense n x n work for the Cholesky factor
     int chol_ld = 0;
 
-    // symmetric product scratch
+    // symmetric product plan
+    MobPlan plan;
     int nunits = 0;
-    int *units = nullptr;        // (J, Ibeg, Iend) per work unit
-    int *unit_of_tile = nullptr;
-    double *rowpart = nullptr;   // per tile: TILE x MAX_S
-    double *colpart = nullptr;   // per unit: TILE x MAX_S
-    int *colunit_ptr = nullptr;  // units of tile column J: [ptr[J], ptr[J+1])
 
     // Lanczos workspace
     double *V = nullptr;         // npad x (s*dimk_max)

@@ -12,6 +12,8 @@
 // per (tile row, super-column), and column products are summed over the unit's
 // tile rows.  A second kernel adds the partials of every output block in a
 // fixed order, so results are deterministic run to run.
+#include <type_traits>
+
 #include "bd_internal.h"
 
 namespace bd {
@@ -27,17 +29,15 @@ __host__ __device__ inline long tile_index(int I, int J) { return (long)I * (I +
 // pairs the tile touches are computed once into LDS (6 unique entries each),
 // then the 64 x 64 entries are written column-major (coalesced along rows).
 template <typename E>
-__global__ __launch_bounds__(256) void rpy_tile_kernel(const double *__restrict__ crd, int nb,
-                                                       int n, int ntile, E *__restrict__ mob)
+__global__ __launch_bounds__(256) void rpy_tile_kernel(const double *__restrict__ crd_all, const int *__restrict__ tiles,
+                                                       const int *__restrict__ chain_i, const long *__restrict__ chain_l,
+                                                       E *__restrict__ mob_all)
 {
     __shared__ double sblk[22 * 22 * 6];
-    // map linear block id to (I, J) with I >= J
-    long t = blockIdx.x;
-    int I = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-    while ((long)I * (I + 1) / 2 > t) I--;
-    while ((long)(I + 1) * (I + 2) / 2 <= t) I++;
-    int J = (int)(t - (long)I * (I + 1) / 2);
-
+    const int chain = tiles[4 * blockIdx.x], I = tiles[4 * blockIdx.x + 1], J = tiles[4 * blockIdx.x + 2];
+    const int nb = chain_i[4 * chain], n = chain_i[4 * chain + 1];
+    const double *crd = crd_all + chain_l[3 * chain];
+    E *mob = mob_all + chain_l[3 * chain + 1] * (TILE * TILE);
     const int gi0 = I * TILE, gj0 = J * TILE;
     const int bi0 = gi0 / 3, bj0 = gj0 / 3;
     const int nbi = min((gi0 + TILE - 1) / 3, nb - 1) - bi0 + 1;
@@ -106,9 +106,9 @@ struct TileLds {
 // rowpart: (ntile * nsc) x TILE x S; colpart: nunits x WSC x TILE x S.
 template <typename E, int S>
 __global__ __launch_bounds__(NT) void symm_tiles_kernel(
-    const E *__restrict__ mob, const double *__restrict__ x, int ldx, const int *__restrict__ units,
-    int nunits, int ntile, int nsc, double *__restrict__ rowpart, double *__restrict__ colpart,
-    int *__restrict__ ticket, const DevStatus *stop)
+    const E *__restrict__ mob_all, const double *__restrict__ x_all, int ldx, const int *__restrict__ units,
+    int nunits, const int *__restrict__ chain_i, const long *__restrict__ chain_l, double *__restrict__ rowpart,
+    double *__restrict__ colpart, int *__restrict__ ticket, const DevStatus *stop, const int *__restrict__ active)
 {
     // inside a Lanczos solve: nothing to do once it has converged
     if (stop && (stop->converged_j != 0 || stop->ierr != 0)) return;
@@ -132,7 +132,15 @@ __global__ __launch_bounds__(NT) void symm_tiles_kernel(
         __syncthreads();
         const int u = s_unit;
         if (u >= nunits) break;
-        const int sc = units[3 * u], Ib = units[3 * u + 1], Ie = units[3 * u + 2];
+        const int chain = units[4 * u], sc = units[4 * u + 1], Ib = units[4 * u + 2], Ie = units[4 * u + 3];
+        if (active && !active[chain]) {
+            __syncthreads();
+            continue;
+        }
+        const int ntile = chain_i[4 * chain + 2], nsc = chain_i[4 * chain + 3];
+        const double *x = x_all + chain_l[3 * chain];
+        const E *mob = mob_all + chain_l[3 * chain + 1] * (TILE * TILE);
+        const long rpo = chain_l[3 * chain + 2];
         const int J0 = sc * WSC;
         // V_J blocks for the unit's tile columns
         for (int e = tid; e < WSC * TILE * S; e += NT) {
@@ -195,7 +203,7 @@ __global__ __launch_bounds__(NT) void symm_tiles_kernel(
                 double v = 0.0;
 #pragma unroll
                 for (int qq = 0; qq < NT / TILE; qq++) v += sred[qq][r][k];
-                rowpart[((long)I * nsc + sc) * (TILE * S) + e] = v;
+                rowpart[(rpo + (long)I * nsc + sc) * (TILE * S) + e] = v;
             }
         }
         // column partials of this unit
@@ -216,24 +224,28 @@ __global__ __launch_bounds__(NT) void symm_tiles_kernel(
     }
 }
 
-// y(K block) = sum_{sc <= K/WSC} rowpart[K, sc] + sum_{units of sc(K)} colpart[u, K % WSC]
+// Output block G (a tile row of some chain):
+//   y = sum_{sc <= K/WSC} rowpart[K, sc] + sum_{units u of super-column K/WSC} colpart[u, K % WSC]
+// rowblk (8 ints per block): chain, K, nsc, u0, u1, rp_off, y row offset.
 template <int S>
 __global__ void symm_reduce_kernel(const double *__restrict__ rowpart, const double *__restrict__ colpart,
-                                   const int *__restrict__ sc_unit_ptr, int ntile, int nsc,
-                                   double *__restrict__ y, int ldy, int *__restrict__ ticket)
+                                   const int *__restrict__ rowblk, int nrowblk, double *__restrict__ y, int ldy,
+                                   int *__restrict__ ticket, const int *__restrict__ active)
 {
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g == 0) *ticket = 0;  // re-arm the work counter for the next product
-    if (g >= (long)ntile * TILE * S) return;
-    int K = (int)(g / (TILE * S));
-    int e = (int)(g % (TILE * S));
-    int r = e / S, k = e % S;
+    if (g >= (long)nrowblk * TILE * S) return;
+    const int G = (int)(g / (TILE * S));
+    const int e = (int)(g % (TILE * S));
+    const int *rb = rowblk + 8 * G;
+    const int chain = rb[0], K = rb[1], nsc = rb[2], u0 = rb[3], u1 = rb[4], rpo = rb[5], yrow = rb[6];
+    if (active && !active[chain]) return;
+    const int r = e / S, k = e % S;
     double v = 0.0;
-    for (int sc = 0; sc <= K / WSC; sc++) v += rowpart[((long)K * nsc + sc) * (TILE * S) + e];
-    int scK = K / WSC, jj = K % WSC;
-    for (int u = sc_unit_ptr[scK]; u < sc_unit_ptr[scK + 1]; u++)
-        v += colpart[((long)u * WSC + jj) * (TILE * S) + e];
-    y[(long)k * ldy + (long)K * TILE + r] = v;
+    for (int sc = 0; sc <= K / WSC; sc++) v += rowpart[((long)rpo + (long)K * nsc + sc) * (TILE * S) + e];
+    const int jj = K % WSC;
+    for (int u = u0; u < u1; u++) v += colpart[((long)u * WSC + jj) * (TILE * S) + e];
+    y[(long)k * ldy + yrow + r] = v;
 }
 
 // Expands the tiled lower storage to a dense n x n column-major matrix.
@@ -254,6 +266,83 @@ __global__ void unpack_kernel(const E *__restrict__ mob, int n, double *__restri
 }
 
 // ---------------------------------------------------------------- host side
+// Builds the tile list, work units and output-block table of a set of
+// chains with n_c rows each, stored back to back (row offsets row_off_c,
+// tile offsets in the order given).
+int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off)
+{
+    const int C = (int)nb.size();
+    std::vector<int> tiles, units, rowblk, ci(4 * C);
+    std::vector<long> cl(3 * C);
+    long toff = 0, rpo = 0;
+    for (int ch = 0; ch < C; ch++) {
+        const int n = 3 * nb[ch], nt = (n + TILE - 1) / TILE, nsc = (nt + WSC - 1) / WSC;
+        ci[4 * ch] = nb[ch];
+        ci[4 * ch + 1] = n;
+        ci[4 * ch + 2] = nt;
+        ci[4 * ch + 3] = nsc;
+        cl[3 * ch] = row_off[ch];
+        cl[3 * ch + 1] = toff;
+        cl[3 * ch + 2] = rpo;
+        for (int I = 0; I < nt; I++)
+            for (int J = 0; J <= I; J++) {
+                tiles.push_back(ch); tiles.push_back(I); tiles.push_back(J); tiles.push_back(0);
+            }
+        std::vector<int> scu(nsc + 1, 0);
+        for (int sc = 0; sc < nsc; sc++) {
+            scu[sc] = (int)(units.size() / 4);
+            for (int Ib = sc * WSC; Ib < nt; Ib += CHR) {
+                units.push_back(ch); units.push_back(sc); units.push_back(Ib);
+                units.push_back(std::min(Ib + CHR, nt));
+            }
+        }
+        scu[nsc] = (int)(units.size() / 4);
+        for (int K = 0; K < nt; K++) {
+            int sc = K / WSC;
+            rowblk.push_back(ch); rowblk.push_back(K); rowblk.push_back(nsc); rowblk.push_back(scu[sc]);
+            rowblk.push_back(scu[sc + 1]); rowblk.push_back((int)rpo); rowblk.push_back((int)(row_off[ch] + (long)K * TILE));
+            rowblk.push_back(0);
+        }
+        toff += (long)nt * (nt + 1) / 2;
+        rpo += (long)nt * nsc;
+    }
+    if (rpo > 0x7fffffffL || row_off.back() + 3L * nb.back() > 0x7fffffffL) {
+        set_error("batch too large for 32-bit block offsets");
+        return BD_ERR_ARG;
+    }
+    p.nchains = C;
+    p.ntiles = toff;
+    p.nunits = (int)(units.size() / 4);
+    p.nrowblk = (int)(rowblk.size() / 8);
+    auto up = [](auto **dst, const auto &v) -> int {
+        using T = typename std::remove_reference<decltype(v)>::type::value_type;
+        BD_HIP(hipMalloc(dst, std::max<size_t>(v.size(), 1) * sizeof(T)));
+        if (!v.empty()) BD_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return BD_OK;
+    };
+    BD_TRY(up(&p.tiles, tiles));
+    BD_TRY(up(&p.units, units));
+    BD_TRY(up(&p.rowblk, rowblk));
+    BD_TRY(up(&p.chain_i, ci));
+    BD_TRY(up(&p.chain_l, cl));
+    BD_HIP(hipMalloc(&p.rowpart, (size_t)std::max(rpo, 1L) * TILE * MAX_S * sizeof(double)));
+    BD_HIP(hipMalloc(&p.colpart, (size_t)std::max(p.nunits, 1) * WSC * TILE * MAX_S * sizeof(double)));
+    return BD_OK;
+}
+
+void plan_free(MobPlan &p)
+{
+    int **ip[] = {&p.tiles, &p.units, &p.rowblk, &p.chain_i};
+    for (auto q : ip) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+    if (p.chain_l) (void)hipFree(p.chain_l);
+    if (p.rowpart) (void)hipFree(p.rowpart);
+    if (p.colpart) (void)hipFree(p.colpart);
+    p.chain_l = nullptr; p.rowpart = nullptr; p.colpart = nullptr;
+}
+
 int mob_alloc(Ctx &c)
 {
     c.ntile = (c.n + TILE - 1) / TILE;
@@ -262,26 +351,8 @@ int mob_alloc(Ctx &c)
     size_t elems = (size_t)c.ntiles_lower * TILE * TILE;
     if (c.opt.precision == BD_FP32) BD_HIP(hipMalloc(&c.mob32, elems * sizeof(float)));
     else BD_HIP(hipMalloc(&c.mob64, elems * sizeof(double)));
-
-    const int nsc = (c.ntile + WSC - 1) / WSC;
-    std::vector<int> units, scptr(nsc + 1, 0);
-    for (int sc = 0; sc < nsc; sc++) {
-        scptr[sc] = (int)(units.size() / 3);
-        int J0 = sc * WSC;
-        for (int Ib = J0; Ib < c.ntile; Ib += CHR) {
-            units.push_back(sc);
-            units.push_back(Ib);
-            units.push_back(std::min(Ib + CHR, c.ntile));
-        }
-    }
-    scptr[nsc] = (int)(units.size() / 3);
-    c.nunits = (int)(units.size() / 3);
-    BD_HIP(hipMalloc(&c.units, units.size() * sizeof(int)));
-    BD_HIP(hipMemcpy(c.units, units.data(), units.size() * sizeof(int), hipMemcpyHostToDevice));
-    BD_HIP(hipMalloc(&c.colunit_ptr, scptr.size() * sizeof(int)));
-    BD_HIP(hipMemcpy(c.colunit_ptr, scptr.data(), scptr.size() * sizeof(int), hipMemcpyHostToDevice));
-    BD_HIP(hipMalloc(&c.rowpart, (size_t)c.ntile * nsc * TILE * MAX_S * sizeof(double)));
-    BD_HIP(hipMalloc(&c.colpart, (size_t)c.nunits * WSC * TILE * MAX_S * sizeof(double)));
+    BD_TRY(plan_build(c.plan, std::vector<int>{c.nb}, std::vector<long>{0}));
+    c.nunits = c.plan.nunits;
     return BD_OK;
 }
 
@@ -289,39 +360,64 @@ void mob_free(Ctx &c)
 {
     if (c.mob64) (void)hipFree(c.mob64);
     if (c.mob32) (void)hipFree(c.mob32);
-    if (c.units) (void)hipFree(c.units);
-    if (c.colunit_ptr) (void)hipFree(c.colunit_ptr);
-    if (c.rowpart) (void)hipFree(c.rowpart);
-    if (c.colpart) (void)hipFree(c.colpart);
-    c.mob64 = nullptr; c.mob32 = nullptr; c.units = nullptr; c.colunit_ptr = nullptr;
-    c.rowpart = nullptr; c.colpart = nullptr;
+    c.mob64 = nullptr; c.mob32 = nullptr;
+    plan_free(c.plan);
 }
+
+template <typename E>
+int rpy_launch(const MobPlan &p, const double *crd, E *mob, hipStream_t sm)
+{
+    if (p.ntiles > 0)
+        hipLaunchKernelGGL(rpy_tile_kernel<E>, dim3((unsigned)p.ntiles), dim3(256), 0, sm, crd, p.tiles, p.chain_i,
+                           p.chain_l, mob);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+template int rpy_launch<float>(const MobPlan &, const double *, float *, hipStream_t);
+template int rpy_launch<double>(const MobPlan &, const double *, double *, hipStream_t);
 
 int rpy_assemble_at(Ctx &c, const double *crd)
 {
-    if (c.nb > 0) {
-        if (c.mob32)
-            hipLaunchKernelGGL(rpy_tile_kernel<float>, dim3(c.ntiles_lower), dim3(256), 0, c.stream,
-                               crd, c.nb, c.n, c.ntile, c.mob32);
-        else
-            hipLaunchKernelGGL(rpy_tile_kernel<double>, dim3(c.ntiles_lower), dim3(256), 0, c.stream,
-                               crd, c.nb, c.n, c.ntile, c.mob64);
-        BD_HIP(hipGetLastError());
-    }
+    if (c.mob32) BD_TRY(rpy_launch<float>(c.plan, crd, c.mob32, c.stream));
+    else BD_TRY(rpy_launch<double>(c.plan, crd, c.mob64, c.stream));
     c.mob_valid = true;
     return BD_OK;
 }
 
 int rpy_assemble(Ctx &c) { return rpy_assemble_at(c, c.crd); }
 
+// Launches the tile product of a plan: y = D x for every (active) chain.
+template <typename E, int S>
+int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int ldx, int ldy, int *ticket,
+                 const DevStatus *stop, const int *active, hipStream_t sm, int device, hipEvent_t e0, hipEvent_t e1)
+{
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    int grid = std::max(1, std::min(p.nunits, cus * 3));
+    if (e0) BD_HIP(hipEventRecord(e0, sm));
+    hipLaunchKernelGGL((symm_tiles_kernel<E, S>), dim3(grid), dim3(NT), 0, sm, mob, x, ldx, p.units, p.nunits,
+                       p.chain_i, p.chain_l, p.rowpart, p.colpart, ticket, stop, active);
+    BD_HIP(hipGetLastError());
+    if (e1) BD_HIP(hipEventRecord(e1, sm));
+    long tot = (long)p.nrowblk * TILE * S;
+    hipLaunchKernelGGL((symm_reduce_kernel<S>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, sm, p.rowpart,
+                       p.colpart, p.rowblk, p.nrowblk, y, ldy, ticket, active);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+#define BD_INST(E, S)                                                                                          \
+    template int plan_product<E, S>(const MobPlan &, const E *, const double *, double *, int, int, int *,     \
+                                    const DevStatus *, const int *, hipStream_t, int, hipEvent_t, hipEvent_t);
+BD_INST(float, 1) BD_INST(float, 2) BD_INST(float, 3) BD_INST(float, 4)
+BD_INST(float, 5) BD_INST(float, 6) BD_INST(float, 7) BD_INST(float, 8)
+BD_INST(double, 1) BD_INST(double, 2) BD_INST(double, 3) BD_INST(double, 4)
+BD_INST(double, 5) BD_INST(double, 6) BD_INST(double, 7) BD_INST(double, 8)
+#undef BD_INST
+
 template <typename E, int S>
 static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, int ldy, const DevStatus *stop)
 {
-    const int nsc = (c.ntile + WSC - 1) / WSC;
-    int cus = 256;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device);
-    int grid = std::min(c.nunits, cus * 3);
-    int *ticket = &c.st->ticket;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c.prof_on) {
         if (c.prof_used + 2 > c.prof_ev.size()) prof_flush(c);
@@ -335,17 +431,8 @@ static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, 
         e0 = c.prof_ev[c.prof_used];
         e1 = c.prof_ev[c.prof_used + 1];
         c.prof_used += 2;
-        BD_HIP(hipEventRecord(e0, c.stream));
     }
-    hipLaunchKernelGGL((symm_tiles_kernel<E, S>), dim3(grid), dim3(NT), 0, c.stream, mob, x, ldx,
-                       c.units, c.nunits, c.ntile, nsc, c.rowpart, c.colpart, ticket, stop);
-    BD_HIP(hipGetLastError());
-    if (c.prof_on) BD_HIP(hipEventRecord(e1, c.stream));
-    long tot = (long)c.ntile * TILE * S;
-    hipLaunchKernelGGL((symm_reduce_kernel<S>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                       c.stream, c.rowpart, c.colpart, c.colunit_ptr, c.ntile, nsc, y, ldy, ticket);
-    BD_HIP(hipGetLastError());
-    return BD_OK;
+    return plan_product<E, S>(c.plan, mob, x, y, ldx, ldy, &c.st->ticket, stop, nullptr, c.stream, c.device, e0, e1);
 }
 
 template <typename E>

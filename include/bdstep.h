@@ -187,12 +187,6 @@ int bd_rgsq(bd_ctx *ctx, double *rgsq);
 /* Synchronises the context's stream (timing helper). */
 int bd_sync(bd_ctx *ctx);
 
-/* Measurement helpers (no reference counterpart; the reference's only timer
- * is the wall clock around bds_run, setup_m.f90:116-124).  With profiling
- * on, every launch of the mobility tile-product kernel is bracketed by HIP
- * events on the context's stream; bd_prof_get returns the summed kernel
- * time (ms), the launch count and the bytes of stored mobility tiles one
- * launch streams.  bd_prof_enable(ctx, 1) also clears the totals. */
 /* Differentiable trajectory (no reference counterpart; BROWNPAK has no
  * gradients).  The Brownian displacement of every step is recorded once and
  * then replayed ("noise held fixed"), so a trajectory is a deterministic
@@ -217,9 +211,56 @@ int bd_run_replay(bd_ctx *ctx, int nsteps, const double *disp);
 int bd_adjoint_rgsq(bd_ctx *ctx, int nsteps, const double *disp, int ckpt,
                     int tie_rc, double *loss, double *grad3);
 
+/* Measurement helpers (no reference counterpart; the reference's only timer
+ * is the wall clock around bds_run, setup_m.f90:116-124).  With profiling
+ * on, every launch of the mobility tile-product kernel is bracketed by HIP
+ * events on the context's stream; bd_prof_get returns the summed kernel
+ * time (ms), the launch count and the bytes of stored mobility tiles one
+ * launch streams.  bd_prof_enable(ctx, 1) also clears the totals. */
 int bd_prof_enable(bd_ctx *ctx, int on);
 int bd_prof_get(bd_ctx *ctx, double *total_ms, long long *launches,
                 double *tile_bytes);
+
+/* Batched ensemble: many independent chains of uneven length advanced
+ * together on one device, each exactly one bds_run (bd_solver_m.f90:128-238)
+ * of a linear chain with HI (RPY, calc_rpy_tensor bd_solver_m.f90:458-522)
+ * and (block) Lanczos noise (calc_bdw_lanc / calc_bdw_blanc, brown_m.f90:
+ * 185-512), unbounded domain (imcon 0, centre of mass to the origin after
+ * every step as smbx_to_origin, simbox_m.f90:182-202).  The reference runs
+ * one chain per process; this is the same step for every chain, sharing the
+ * launches.
+ *   bd_batch_create: nbeads[c] beads of chain c (>= 2), bonds i-(i+1);
+ *     opts: lhdia 1, mob_fctr BD_LANC, nts_mobsam 1..4, lanc_mxitr,
+ *     lanc_tol, tim_stp, precision (tile storage), keep_reuse_noise.
+ *   bd_batch_set_model: one bond type (style 1 harmonic or 2 FENE, the .cfg
+ *     slots of ia_bond_m.f90) for every bond, LJ style 1 (ia_vdw_m.f90:
+ *     107-180) when lvdw, pair cutoff rcutoff (r^2 < rc^2 as pt_build), and
+ *     the 1..excluded_atoms ring exclusions of a linear chain (|i-j| <= k).
+ *   bd_batch_set_coordinates / get: chains back to back, each (3, nb[c]).
+ *   bd_batch_run: nsteps EM steps of every chain.  xi (optional) is the
+ *     injected Gaussian stream: at every mobility update it supplies, chain
+ *     by chain, an (n_c, s) column-major block (n_c = 3 nb[c]); *used returns
+ *     the values consumed.  Without xi a device Philox generator is used.
+ *     Errors: BD_ERR_BOND if any bond overstretched during the call.
+ *   bd_batch_info: steps taken, mobility updates, summed Lanczos iterations
+ *     over chains and updates, and chain-updates that hit lanc_mxitr.
+ *   bd_batch_prof: as bd_prof_enable + bd_prof_get for the batch (on < 0
+ *     reads without changing the state). */
+typedef struct bd_batch bd_batch;
+bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts);
+void bd_batch_destroy(bd_batch *batch);
+int bd_batch_set_model(bd_batch *batch, int bond_style, const double *bond_params,
+                       int lvdw, const double *vdw_params, double rcutoff,
+                       int excluded_atoms);
+int bd_batch_set_coordinates(bd_batch *batch, const double *crd);
+int bd_batch_get_coordinates(bd_batch *batch, double *crd);
+int bd_batch_set_rng(bd_batch *batch, uint64_t seed, uint64_t offset);
+int bd_batch_run(bd_batch *batch, int nsteps, const double *xi, long long nxi,
+                 long long *used);
+int bd_batch_info(bd_batch *batch, long long *nts, int *mob_updates,
+                  int *total_iters, int *n_not_converged);
+int bd_batch_prof(bd_batch *batch, int on, double *total_ms,
+                  long long *launches, double *tile_bytes);
 
 #ifdef __cplusplus
 }
