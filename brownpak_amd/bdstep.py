@@ -38,6 +38,7 @@ CHOL, LANC, CHOL_LT = 0, 1, 2
 FP64, FP32 = 0, 1
 PT = {"DIR": 0, "VER": 1, "CL": 2}
 MOB_FCTR = {"CHOL": CHOL, "LANC": LANC, "CHOL_LT": CHOL_LT}
+INTEGRATOR = {"EM": 0, "PC": 1}
 MXPARAM = 12
 
 DP = C.POINTER(C.c_double)
@@ -50,6 +51,8 @@ EXPORTS = (
     "bd_mob_symv", "bd_brownian", "bd_forces", "bd_pair_table", "bd_run", "bd_set_rng",
     "bd_get_runinfo", "bd_reset_run", "bd_rgsq", "bd_sync", "bd_prof_enable", "bd_prof_get",
     "bd_set_theta", "bd_run_record", "bd_run_replay", "bd_adjoint_rgsq",
+    "bd_batch_create", "bd_batch_destroy", "bd_batch_set_model", "bd_batch_set_coordinates",
+    "bd_batch_get_coordinates", "bd_batch_set_rng", "bd_batch_run", "bd_batch_info", "bd_batch_prof",
 )
 
 
@@ -58,7 +61,7 @@ class BdOpts(C.Structure):
         ("lhdia", C.c_int), ("mob_fctr", C.c_int), ("nts_mobsam", C.c_int),
         ("lanc_mxitr", C.c_int), ("lanc_tol", C.c_double), ("tim_stp", C.c_double),
         ("precision", C.c_int), ("keep_reuse_noise", C.c_int), ("flow_style", C.c_int),
-        ("flow_rate", C.c_double), ("lanc_reorth", C.c_int),
+        ("flow_rate", C.c_double), ("lanc_reorth", C.c_int), ("integrator", C.c_int),
     ]
 
 
@@ -116,8 +119,20 @@ def load_library(path: str = LIB_PATH):
     L.bd_run_record.argtypes = [vp, C.c_int, DP, C.c_longlong, C.POINTER(C.c_longlong), DP]
     L.bd_run_replay.argtypes = [vp, C.c_int, DP]
     L.bd_adjoint_rgsq.argtypes = [vp, C.c_int, DP, C.c_int, C.c_int, DP, DP]
+    L.bd_batch_create.argtypes = [C.c_int, IP, C.POINTER(BdOpts)]
+    L.bd_batch_create.restype = vp
+    L.bd_batch_destroy.argtypes = [vp]
+    L.bd_batch_destroy.restype = None
+    L.bd_batch_set_model.argtypes = [vp, C.c_int, DP, C.c_int, DP, C.c_double, C.c_int]
+    L.bd_batch_set_coordinates.argtypes = [vp, DP]
+    L.bd_batch_get_coordinates.argtypes = [vp, DP]
+    L.bd_batch_set_rng.argtypes = [vp, C.c_uint64, C.c_uint64]
+    L.bd_batch_run.argtypes = [vp, C.c_int, DP, C.c_longlong, C.POINTER(C.c_longlong)]
+    L.bd_batch_info.argtypes = [vp, C.POINTER(C.c_longlong), IP, IP, IP]
+    L.bd_batch_prof.argtypes = [vp, C.c_int, DP, C.POINTER(C.c_longlong), DP]
     for name in EXPORTS:
-        if name not in ("bd_ctx_create", "bd_ctx_destroy", "bd_last_error"):
+        if name not in ("bd_ctx_create", "bd_ctx_destroy", "bd_last_error", "bd_batch_create",
+                        "bd_batch_destroy"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -152,7 +167,7 @@ class BdStep:
 
     def __init__(self, nbeads, *, lhdia=True, mob_fctr="LANC", nts_mobsam=1, lanc_mxitr=50,
                  lanc_tol=1e-6, tim_stp=1e-3, precision="fp64", keep_reuse_noise=False,
-                 flow_style=0, flow_rate=0.0, lanc_reorth=False):
+                 flow_style=0, flow_rate=0.0, lanc_reorth=False, integrator="EM"):
         self.L = load_library()
         self.nbeads = int(nbeads)
         self.n = 3 * self.nbeads
@@ -161,7 +176,8 @@ class BdStep:
         prec = {"fp64": FP64, "fp32": FP32}[precision] if isinstance(precision, str) else int(precision)
         self.opts = BdOpts(int(bool(lhdia)), mf, int(nts_mobsam), int(lanc_mxitr), float(lanc_tol),
                            float(tim_stp), prec, int(bool(keep_reuse_noise)), int(flow_style),
-                           float(flow_rate), int(bool(lanc_reorth)))
+                           float(flow_rate), int(bool(lanc_reorth)),
+                           INTEGRATOR[integrator] if isinstance(integrator, str) else int(integrator))
         self.h = self.L.bd_ctx_create(self.nbeads, C.byref(self.opts))
         if not self.h:
             raise BdError(BD_ERR_ARG, self.L.bd_last_error().decode())
@@ -339,4 +355,98 @@ class BdStep:
         cnt = C.c_longlong(0)
         tb = C.c_double(0.0)
         self._chk(self.L.bd_prof_get(self.h, C.byref(ms), C.byref(cnt), C.byref(tb)))
+        return ms.value, cnt.value, tb.value
+
+
+class BdBatch:
+    """Many independent linear chains (RPY + Lanczos noise) advanced together.
+
+    Each chain follows bds_run (bd_solver_m.f90:128-238) for an isolated chain
+    exactly as one BdStep would; the batch shares every launch.
+    """
+
+    def __init__(self, nbeads, *, nts_mobsam=1, lanc_mxitr=50, lanc_tol=1e-6, tim_stp=1e-3,
+                 precision="fp64", keep_reuse_noise=False):
+        self.L = load_library()
+        self.nbeads = np.asarray(nbeads, dtype=np.int32).ravel().copy()
+        self.nchains = int(self.nbeads.size)
+        self.s = int(nts_mobsam)
+        prec = {"fp64": FP64, "fp32": FP32}[precision] if isinstance(precision, str) else int(precision)
+        self.opts = BdOpts(1, LANC, int(nts_mobsam), int(lanc_mxitr), float(lanc_tol), float(tim_stp), prec,
+                           int(bool(keep_reuse_noise)), 0, 0.0, 0)
+        self.h = self.L.bd_batch_create(self.nchains, _ip(self.nbeads), C.byref(self.opts))
+        if not self.h:
+            raise BdError(BD_ERR_ARG, self.L.bd_last_error().decode())
+
+    def _chk(self, rc, ok=(BD_OK,)):
+        if rc not in ok:
+            raise BdError(rc, self.L.bd_last_error().decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_model(self, bond_style, bond_params, *, lvdw=True, vdw_params=None, rcutoff=0.0,
+                  excluded_atoms=0):
+        """One bond type for every bond i-(i+1), LJ style 1 for every pair (.cfg slots)."""
+        bp = _params(bond_params, 1)
+        vp = _params(vdw_params if vdw_params is not None else [], 1)
+        self._chk(self.L.bd_batch_set_model(self.h, int(bond_style), _dp(bp), int(bool(lvdw)), _dp(vp),
+                                            float(rcutoff), int(excluded_atoms)))
+
+    def set_coordinates(self, chains):
+        """chains: list of (nb_c, 3) arrays (or one flat array, chains back to back)."""
+        flat = _f64(np.concatenate([np.ravel(c) for c in chains]) if isinstance(chains, (list, tuple))
+                    else np.ravel(chains))
+        if flat.size != 3 * int(self.nbeads.sum()):
+            raise ValueError("coordinate count does not match the chain lengths")
+        self._chk(self.L.bd_batch_set_coordinates(self.h, _dp(flat)))
+
+    def get_coordinates(self):
+        flat = np.zeros(3 * int(self.nbeads.sum()))
+        self._chk(self.L.bd_batch_get_coordinates(self.h, _dp(flat)))
+        out, p = [], 0
+        for nb in self.nbeads:
+            out.append(flat[p:p + 3 * nb].reshape(nb, 3).copy())
+            p += 3 * nb
+        return out
+
+    def set_rng(self, seed, offset=0):
+        self._chk(self.L.bd_batch_set_rng(self.h, int(seed), int(offset)))
+
+    def run(self, nsteps, xi=None):
+        used = C.c_longlong(0)
+        if xi is None:
+            self._chk(self.L.bd_batch_run(self.h, int(nsteps), None, 0, C.byref(used)))
+        else:
+            xa = _f64(np.ravel(xi))
+            self._chk(self.L.bd_batch_run(self.h, int(nsteps), _dp(xa), xa.size, C.byref(used)))
+        return used.value
+
+    def info(self):
+        nts = C.c_longlong(0)
+        upd, it, nc = C.c_int(0), C.c_int(0), C.c_int(0)
+        self._chk(self.L.bd_batch_info(self.h, C.byref(nts), C.byref(upd), C.byref(it), C.byref(nc)))
+        return {"nts": nts.value, "mob_updates": upd.value, "total_lanczos_iters": it.value,
+                "n_not_converged": nc.value}
+
+    def prof(self, on=-1):
+        ms = C.c_double(0.0)
+        cnt = C.c_longlong(0)
+        tb = C.c_double(0.0)
+        self._chk(self.L.bd_batch_prof(self.h, int(on), C.byref(ms), C.byref(cnt), C.byref(tb)))
         return ms.value, cnt.value, tb.value

@@ -37,6 +37,24 @@ def random_walk_chain(nbeads: int, seed: int, bond: float = 0.97 * SIGMA,
     return crd
 
 
+def overlap_free_chain(nbeads: int, seed: int, bond: float = 0.97 * SIGMA,
+                       sep: float = 1.9, maxitr: int = 5000) -> np.ndarray:
+    """random_walk_chain, regrown from derived seeds (seed, attempt) until no
+    two non-bonded beads are closer than sep.  The plain walk accepts its last
+    trial position when a bead is trapped, which occasionally leaves an
+    overlap whose WCA force overstretches a bond on the first step; long
+    ensembles need every chain clean."""
+    for attempt in range(1000):
+        s = seed if attempt == 0 else int(np.random.SeedSequence([seed, attempt]).generate_state(1)[0])
+        crd = random_walk_chain(nbeads, s, bond, sep, maxitr)
+        d = crd[:, None, :] - crd[None, :, :]
+        r2 = np.einsum("ijk,ijk->ij", d, d)
+        iu = np.triu_indices(nbeads, 2)
+        if nbeads < 3 or r2[iu].min() >= sep * sep * (1.0 - 1e-12):
+            return crd
+    raise RuntimeError(f"no overlap-free walk of {nbeads} beads from seed {seed}")
+
+
 def chain_bonds(nbeads: int, offset: int = 0, btype: int = 1) -> np.ndarray:
     """(nbeads-1, 3) int32 rows [type, i, j], 1-based, for a linear chain."""
     i = np.arange(1, nbeads, dtype=np.int32) + offset

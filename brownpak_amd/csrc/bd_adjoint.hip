@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "bd_internal.h"
+#include "bd_lanczos_dev.h"
 
 namespace bd {
 
@@ -148,9 +149,7 @@ __global__ void adj_combine_kernel(const double *__restrict__ mu, const double *
     if (i < n) lam[i] = mu[i] + dt * h[i];
     if (i == 0) {
         for (int m = 0; m < 3; m++) {
-            double v = 0.0;
-            for (int b = 0; b < ntp; b++) v += tpart[b * 3 + m];
-            theta[m] += dt * v;
+            theta[m] += dt * sum_strided(tpart + m, 3, ntp);
         }
     }
 }
@@ -258,6 +257,7 @@ int bd_adjoint_rgsq(bd_ctx *h, int nsteps, const double *disp, int ckpt, int tie
     Ctx &c = h->c;
     if (!c.have_topology) { set_error("topology not set"); return BD_ERR_STATE; }
     if (c.imcon != 0) { set_error("the Rg^2 adjoint is for an isolated chain (imcon = 0)"); return BD_ERR_ARG; }
+    if (c.opt.integrator != BD_EM) { set_error("the Rg^2 adjoint differentiates the EM step"); return BD_ERR_ARG; }
     const int s = c.opt.lhdia ? c.s : 1;
     if (ckpt <= 0 || ckpt % s != 0) { set_error("ckpt must be a positive multiple of nts_mobsam"); return BD_ERR_ARG; }
     if (c.cntr_mobsam != 0) { set_error("the adjoint needs a run that starts a mobility block"); return BD_ERR_STATE; }

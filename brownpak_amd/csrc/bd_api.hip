@@ -121,6 +121,36 @@ __global__ void update_kernel(double *__restrict__ crd, const double *__restrict
     }
 }
 
+// Predictor of the predictor-corrector step: r0 <- r, d0 <- d and
+// r <- (r + d) + w.  No periodic wrap here: the drift at the predictor sees
+// the same absolute positions (flow field) as the reference's r would.
+__global__ void pc_predict_kernel(double *__restrict__ crd, const double *__restrict__ drift,
+                                  const double *__restrict__ w, double *__restrict__ r0, double *__restrict__ d0, int n,
+                                  const DevStatus *st)
+{
+    if (st->ierr) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double r = crd[i], d = drift[i];
+    r0[i] = r;
+    d0[i] = d;
+    double p = r + d;
+    if (w) p = p + w[i];
+    crd[i] = p;
+}
+
+// Corrector: r <- r0 and drift <- (d0 + d(r*)) / 2; the EM update then adds
+// the same w.
+__global__ void pc_correct_kernel(double *__restrict__ crd, double *__restrict__ drift, const double *__restrict__ r0,
+                                  const double *__restrict__ d0, int n, const DevStatus *st)
+{
+    if (st->ierr) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    crd[i] = r0[i];
+    drift[i] = 0.5 * (d0[i] + drift[i]);
+}
+
 // smbx_to_origin (simbox_m.f90:182-202) in one block; com_acc += com.
 __global__ __launch_bounds__(1024) void to_origin_kernel(double *crd, int nb, double *com_acc, const DevStatus *st)
 {
@@ -297,19 +327,33 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
         }
         c.cntr_mobsam++;
     }
-    // ---- calc_drift (bd_solver_m.f90:407-454)
-    BD_TRY(compute_forces(c, false, false));
-    const double *v = c.frc;
-    if (hi) {
-        BD_TRY(mob_product(c, c.frc, c.W, 1, c.npad, c.npad));
-        v = c.W;
-    }
-    hipLaunchKernelGGL(drift_kernel, dim3(nbk), dim3(256), 0, sm, v, c.crd, nb, c.opt.flow_style, c.opt.flow_rate,
-                       c.opt.tim_stp, c.drift, c.st);
-    // ---- update (integrate_em, bd_solver_m.f90:231-236)
+    // ---- calc_drift (bd_solver_m.f90:407-454): drift = dt (U(r) + D F(r))
+    auto drift_at_crd = [&]() -> int {
+        BD_TRY(compute_forces(c, false, false));
+        const double *v = c.frc;
+        if (hi) {
+            BD_TRY(mob_product(c, c.frc, c.W, 1, c.npad, c.npad));
+            v = c.W;
+        }
+        hipLaunchKernelGGL(drift_kernel, dim3(nbk), dim3(256), 0, sm, v, c.crd, nb, c.opt.flow_style,
+                           c.opt.flow_rate, c.opt.tim_stp, c.drift, c.st);
+        return BD_OK;
+    };
+    BD_TRY(drift_at_crd());
+    // Brownian displacement w of this step
     const double *dcol = nullptr;
     if (disp_dev) dcol = disp_dev;
     else if (noise) dcol = c.diff + (size_t)(hi ? c.cntr_mobsam - 1 : 0) * c.npad;
+    // ---- predictor-corrector: drift <- (d(r) + d(r + d(r) + w)) / 2
+    if (c.opt.integrator == BD_PC) {
+        const int nbn = (n + 255) / 256;
+        hipLaunchKernelGGL(pc_predict_kernel, dim3(nbn), dim3(256), 0, sm, c.crd, c.drift, dcol, c.pc_r0, c.pc_d0,
+                           n, c.st);
+        BD_TRY(drift_at_crd());
+        hipLaunchKernelGGL(pc_correct_kernel, dim3(nbn), dim3(256), 0, sm, c.crd, c.drift, c.pc_r0, c.pc_d0, n,
+                           c.st);
+    }
+    // ---- update (integrate_em, bd_solver_m.f90:231-236)
     if (rec_dev) {
         if (dcol) BD_HIP(hipMemcpyAsync(rec_dev, dcol, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, sm));
         else BD_HIP(hipMemsetAsync(rec_dev, 0, (size_t)n * sizeof(double), sm));
@@ -340,6 +384,7 @@ bd_ctx *bd_ctx_create(int nbeads, const bd_opts *opts)
     }
     if (opts->mob_fctr < BD_CHOL || opts->mob_fctr > BD_CHOL_LT) { set_error("unknown mob_fctr"); return nullptr; }
     if (opts->precision != BD_FP64 && opts->precision != BD_FP32) { set_error("unknown precision"); return nullptr; }
+    if (opts->integrator != BD_EM && opts->integrator != BD_PC) { set_error("unknown integrator"); return nullptr; }
     bd_ctx *h = new bd_ctx();
     Ctx &c = h->c;
     c.nb = nbeads;
@@ -369,6 +414,10 @@ bd_ctx *bd_ctx_create(int nbeads, const bd_opts *opts)
         set_error("device allocation failed");
         return fail(0);
     }
+    if (opts->integrator == BD_PC && (!dalloc(&c.pc_r0, c.npad) || !dalloc(&c.pc_d0, c.npad))) {
+        set_error("device allocation failed");
+        return fail(0);
+    }
     if (hipMalloc(&c.st, sizeof(DevStatus)) != hipSuccess || hipMemset(c.st, 0, sizeof(DevStatus)) != hipSuccess ||
         hipHostMalloc(&c.st_host, 2 * sizeof(DevStatus)) != hipSuccess) {
         set_error("status allocation failed");
@@ -395,7 +444,7 @@ void bd_ctx_destroy(bd_ctx *h)
     mob_free(c);
     lanczos_free(c);
     forces_free(c);
-    double **dp[] = {&c.crd, &c.frc, &c.drift, &c.diff, &c.xi_dev, &c.scal};
+    double **dp[] = {&c.crd, &c.frc, &c.drift, &c.diff, &c.xi_dev, &c.scal, &c.pc_r0, &c.pc_d0};
     for (auto p : dp)
         if (*p) (void)hipFree(*p);
     if (c.W) (void)hipFree(c.W);

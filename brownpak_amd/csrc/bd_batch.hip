@@ -7,8 +7,8 @@
 // launch.  The mobility tiles of all chains share one MobPlan (work units of
 // all chains feed one persistent product kernel); each Lanczos iteration runs
 // the Krylov recurrence of every still-active chain with one block per chain,
-// and the small problem of every chain with one wavefront per (chain,
-// quadrature node).  Chains converge independently: a chain that passed the
+// and the small problem of every chain with one block per chain (one thread
+// per quadrature node, sp::small_problem).  Chains converge independently: a chain that passed the
 // reference's test (brown_m.f90:497-506) is switched off for the rest of the
 // solve, exactly as if it had left its own loop.
 #include <algorithm>
@@ -24,7 +24,8 @@ namespace bd {
 
 constexpr int BT = 256;            // threads per per-chain block
 constexpr int BNBOUND = 32;        // spectral-bound probes per chain
-constexpr int BNQ = 2 * 79 + 1;    // quadrature nodes per chain (kappa <= 1e8)
+constexpr int BDEC = 8;            // node table rows used (kappa <= 1e8)
+constexpr int BNQP = 160;          // node slots per chain (2 * 79 + 1 <= 160)
 
 struct BChainDev {
     int nb, n, npad, pad;
@@ -49,14 +50,15 @@ struct Batch {
     long *bead_row = nullptr;     // per bead: row of its x component
     double *crd = nullptr, *frc = nullptr, *dfr = nullptr, *diff = nullptr, *xi = nullptr;
     double *V = nullptr, *W = nullptr, *Y = nullptr;
-    double *A = nullptr, *B = nullptr, *R0 = nullptr, *Cq = nullptr, *Cy = nullptr;
+    double *A = nullptr, *B = nullptr, *R0 = nullptr;
+    double *scr = nullptr, *Cc = nullptr;   // small-problem scratch and coefficients, per chain
+    long scr_stride = 0;
     // active: the chain still takes Lanczos iterations (cleared by the
     // convergence test); run: its Krylov recurrence still advances (also
     // cleared on a breakdown); failed: iteration whose CholeskyQR broke down
-    int *active = nullptr, *run = nullptr, *failed = nullptr, *conv = nullptr, *bound_ok = nullptr, *nq = nullptr;
+    int *active = nullptr, *run = nullptr, *failed = nullptr, *conv = nullptr;
     int *counters = nullptr;      // [0] active chains, [1] error code
     int *host_counters = nullptr; // pinned, 2 slots x 2
-    double *lam_hi = nullptr;
     DevStatus *st = nullptr;
     hipEvent_t ev[LZ_NEV] = {};
     hipEvent_t chunk_ev[2] = {};
@@ -264,94 +266,22 @@ __device__ inline bool b_live(const int *active, const int *failed, int c, int j
     return active[c] && !(failed[c] && failed[c] < j);
 }
 
+// Per chain (one block): the small problem (sp::small_problem: bounds,
+// quadrature, C = sqrt(H_j) E1 R0), y_j = V C, the relative change against
+// y_{j-1} and the convergence test.
 template <int S>
-__global__ __launch_bounds__(64) void b_bounds_kernel(const double *A, const double *B, int dmax, int j,
-                                                      const int *active, const int *failed, int *ok, double *lam_hi)
-{
-    const int c = blockIdx.x / BNBOUND, b = blockIdx.x % BNBOUND, js = j * S;
-    if (!b_live(active, failed, c, j)) return;
-    extern __shared__ double band[];
-    const double *Ab = A + (long)c * dmax * S * S, *Bb = B + (long)c * dmax * S * S;
-    double M = gersh_hi<S>(Ab, Bb, js);
-    double mu = ldexp(M, -(b + 1));
-    load_band<S>(Ab, Bb, js, mu, band, js);
-    bool good = band_chol<S>(band, js, js);
-    if (threadIdx.x == 0) ok[c * BNBOUND + b] = good ? 1 : 0;
-    if (b == 0 && threadIdx.x == 0) lam_hi[c] = M;
-}
-
-template <int S>
-__global__ __launch_bounds__(64) void b_quad_kernel(const double *A, const double *B, const double *R0, int dmax,
-                                                    int j, const int *active, const int *failed, const int *ok,
-                                                    const double *lam_hi, double *part, int *nq, int *counters)
-{
-    const int c = blockIdx.x / BNQ, node = blockIdx.x % BNQ, js = j * S;
-    if (!b_live(active, failed, c, j)) return;
-    extern __shared__ double smem[];
-    double *band = smem, *z = smem + (S + 1) * js;
-    const double *Ab = A + (long)c * dmax * S * S, *Bb = B + (long)c * dmax * S * S;
-    const double M = lam_hi[c];
-    int kb = -1;
-    for (int b = 0; b < BNBOUND; b++)
-        if (ok[c * BNBOUND + b]) { kb = b; break; }
-    if (kb < 0) {
-        if (threadIdx.x == 0) atomicExch(&counters[1], 2);
-        return;
-    }
-    const double m = ldexp(M, -(kb + 1));
-    int dec = (int)ceil(log10(M / m));
-    if (dec > 8) {  // the node table of this kernel ends at kappa = 1e8
-        if (threadIdx.x == 0) atomicExch(&counters[1], 3);
-        return;
-    }
-    dec = dec < 1 ? 1 : dec;
-    const int N = c_qN[dec];
-    const double h = c_qh[dec];
-    if (node == 0 && threadIdx.x == 0) nq[c] = 2 * N + 1;
-    if (node > 2 * N) return;
-    const double u0 = 0.25 * (log(m) + log(M));
-    const double v = (node - N) * h;
-    const double u = u0 + sinh(v);
-    const double wgt = (2.0 / M_PI) * h * cosh(v) * exp(u);
-    load_band<S>(Ab, Bb, js, -exp(2.0 * u), band, js);
-    if (!band_chol<S>(band, js, js)) {
-        if (threadIdx.x == 0) atomicExch(&counters[1], 2);
-        return;
-    }
-    for (int e = threadIdx.x; e < js * S; e += blockDim.x) {
-        int i = e % js, cc = e / js;
-        z[e] = (i < S) ? R0[(long)c * S * S + i + cc * S] : 0.0;
-    }
-    __syncthreads();
-    if (threadIdx.x < S) {
-        double *zc = z + threadIdx.x * js;
-        for (int i = 0; i < js; i++) {
-            double t = zc[i];
-            for (int d = 1; d <= S && d <= i; d++) t -= band[d * js + i - d] * zc[i - d];
-            zc[i] = t / band[i];
-        }
-        for (int i = js - 1; i >= 0; i--) {
-            double t = zc[i];
-            for (int d = 1; d <= S && i + d < js; d++) t -= band[d * js + i] * zc[i + d];
-            zc[i] = t / band[i];
-        }
-    }
-    __syncthreads();
-    double *out = part + ((long)c * BNQ + node) * js * S;
-    for (int e = threadIdx.x; e < js * S; e += blockDim.x) out[e] = wgt * z[e];
-}
-
-// Per chain (one block): ysum = sum of node parts; C = H ysum; y_j = V C;
-// relative change against y_{j-1}; convergence test.
-template <int S>
-__global__ __launch_bounds__(BT) void b_combine_kernel(const double *A, const double *B, int dmax, int j,
-                                                       const double *part, const int *nq, const double *V, long NP,
-                                                       const BChainDev *ch, double *Y, int *active, int *run,
-                                                       const int *failed, int *conv, double tol, int *counters)
+__global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const double *B, const double *R0, int dmax,
+                                                     int j, double *scr, long scr_stride, double *Cbuf,
+                                                     const double *V, long NP, const BChainDev *ch, double *Y,
+                                                     int *active, int *run, const int *failed, int *conv, double tol,
+                                                     int *counters)
 {
     const int c = blockIdx.x, js = j * S;
-    if (!active[c]) return;
-    if (!b_live(active, failed, c, j)) {
+    __shared__ int s_state;
+    if (threadIdx.x == 0) s_state = !active[c] ? 0 : (b_live(active, failed, c, j) ? 1 : 2);
+    __syncthreads();
+    if (s_state == 0) return;
+    if (s_state == 2) {
         // the Krylov basis ended before this chain converged: stop it here
         // and report it as failed at the end of the solve
         if (threadIdx.x == 0) {
@@ -360,24 +290,13 @@ __global__ __launch_bounds__(BT) void b_combine_kernel(const double *A, const do
         }
         return;
     }
-    extern __shared__ double sm[];
-    double *ys = sm, *Cc = sm + js * S;
     const double *Ab = A + (long)c * dmax * S * S, *Bb = B + (long)c * dmax * S * S;
-    const int nn = nq[c];
-    for (int e = threadIdx.x; e < js * S; e += BT) {
-        double v = 0.0;
-        for (int k = 0; k < nn; k++) v += part[((long)c * BNQ + k) * js * S + e];
-        ys[e] = v;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < js * S; e += BT) {
-        int i = e % js, cc = e / js;
-        double v = 0.0;
-        for (int l = max(0, i - S); l <= min(js - 1, i + S); l++) {
-            double hv = (i >= l) ? h_entry<S>(Ab, Bb, js, i, l) : h_entry<S>(Ab, Bb, js, l, i);
-            v += hv * ys[l + cc * js];
-        }
-        Cc[e] = v;
+    double *Cc = Cbuf + (long)c * dmax * S * S;
+    const sp::SmallOut o =
+        sp::small_problem<S>(Ab, Bb, R0 + (long)c * S * S, j, BNBOUND, BDEC, scr + c * scr_stride, BNQP, Cc);
+    if (o.err) {
+        if (threadIdx.x == 0) atomicExch(&counters[1], o.err);
+        return;
     }
     __syncthreads();
     const long r0 = ch[c].row;
@@ -392,6 +311,7 @@ __global__ __launch_bounds__(BT) void b_combine_kernel(const double *A, const do
         double acc[S];
 #pragma unroll
         for (int k = 0; k < S; k++) acc[k] = 0.0;
+#pragma unroll 4
         for (int m = 0; m < js; m++) {
             double vm = V[(long)m * NP + r0 + r];
 #pragma unroll
@@ -592,15 +512,8 @@ static int b_lanczos(Batch &b, int *iters_out)
             hipEvent_t ev = b.ev[j % LZ_NEV];
             BD_HIP(hipEventRecord(ev, sm));
             BD_HIP(hipStreamWaitEvent(ss, ev, 0));
-            const int js = j * S;
-            size_t bb = (size_t)(S + 1) * js * sizeof(double);
-            hipLaunchKernelGGL((b_bounds_kernel<S>), dim3(C * BNBOUND), dim3(64), bb, ss, b.A, b.B, dmax, j, b.active,
-                               b.failed, b.bound_ok, b.lam_hi);
-            hipLaunchKernelGGL((b_quad_kernel<S>), dim3(C * BNQ), dim3(64), bb + (size_t)js * S * sizeof(double), ss,
-                               b.A, b.B, b.R0, dmax, j, b.active, b.failed, b.bound_ok, b.lam_hi, b.Cq, b.nq,
-                               b.counters);
-            hipLaunchKernelGGL((b_combine_kernel<S>), dim3(C), dim3(BT), (size_t)2 * js * S * sizeof(double), ss, b.A,
-                               b.B, dmax, j, b.Cq, b.nq, b.V, NP, b.chains, b.Y, b.active, b.run, b.failed, b.conv,
+            hipLaunchKernelGGL((b_small_kernel<S>), dim3(C), dim3(BT), 0, ss, b.A, b.B, b.R0, dmax, j, b.scr,
+                               b.scr_stride, b.Cc, b.V, NP, b.chains, b.Y, b.active, b.run, b.failed, b.conv,
                                b.opt.lanc_tol, b.counters);
             BD_HIP(hipGetLastError());
         }
@@ -725,6 +638,7 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
         set_error("the batched path runs RPY with (block) Lanczos, nts_mobsam 1..4");
         return nullptr;
     }
+    if (opts->integrator != BD_EM) { set_error("the batched path integrates with EM"); return nullptr; }
     bd_batch *h = new bd_batch();
     Batch &b = h->b;
     auto fail = [&](const char *m) -> bd_batch * {
@@ -736,7 +650,7 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
     b.opt = *opts;
     b.s = opts->nts_mobsam;
     b.dimk_max = (opts->lanc_mxitr > 2 ? opts->lanc_mxitr : 2) + 1;
-    if (b.dimk_max * b.s > 4 * 79) return fail("lanc_mxitr too large for the batched path");
+    if (b.dimk_max * b.s > 512) return fail("lanc_mxitr * nts_mobsam too large for the batched path (order <= 512)");
     b.nb.assign(nbeads, nbeads + nchains);
     long rowp = 0;
     std::vector<BChainDev> ch(nchains);
@@ -773,15 +687,16 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
     };
     if (opts->precision == BD_FP32) ok = hipMalloc(&b.mob32, elems * sizeof(float)) == hipSuccess;
     else ok = hipMalloc(&b.mob64, elems * sizeof(double)) == hipSuccess;
-    const int s = b.s, dm = b.dimk_max, jsmax = s * dm;
+    const int s = b.s, dm = b.dimk_max;
     dal(&b.crd, b.NP); dal(&b.frc, b.NP); dal(&b.dfr, b.NP); dal(&b.diff, (size_t)b.NP * s);
     dal(&b.xi, (size_t)b.NP * s); dal(&b.V, (size_t)b.NP * s * dm); dal(&b.W, (size_t)b.NP * s);
     dal(&b.Y, (size_t)2 * b.NP * s);
     dal(&b.A, (size_t)nchains * dm * s * s); dal(&b.B, (size_t)nchains * dm * s * s);
-    dal(&b.R0, (size_t)nchains * s * s); dal(&b.Cq, (size_t)nchains * BNQ * jsmax * s);
-    dal(&b.lam_hi, nchains); dal(&b.dev_bpar, MXPARAM); dal(&b.dev_vpar, MXPARAM);
-    ial(&b.active, nchains); ial(&b.run, nchains); ial(&b.failed, nchains); ial(&b.conv, nchains); ial(&b.bound_ok, (size_t)nchains * BNBOUND);
-    ial(&b.nq, nchains); ial(&b.counters, 2); ial(&b.bead_chain, b.NB); ial(&b.bead_local, b.NB);
+    b.scr_stride = (long)dm * (3 * s * s + s) * BNQP;
+    dal(&b.R0, (size_t)nchains * s * s); dal(&b.scr, (size_t)nchains * b.scr_stride);
+    dal(&b.Cc, (size_t)nchains * dm * s * s); dal(&b.dev_bpar, MXPARAM); dal(&b.dev_vpar, MXPARAM);
+    ial(&b.active, nchains); ial(&b.run, nchains); ial(&b.failed, nchains); ial(&b.conv, nchains);
+    ial(&b.counters, 2); ial(&b.bead_chain, b.NB); ial(&b.bead_local, b.NB);
     if (!ok) return fail("device allocation failed");
     if (hipMalloc(&b.bead_row, b.NB * sizeof(long)) != hipSuccess ||
         hipMalloc(&b.chains, nchains * sizeof(BChainDev)) != hipSuccess ||
@@ -807,11 +722,11 @@ void bd_batch_destroy(bd_batch *h)
     if (b.stream) (void)hipStreamSynchronize(b.stream);
     if (b.side) (void)hipStreamSynchronize(b.side);
     plan_free(b.plan);
-    double **dp[] = {&b.crd, &b.frc, &b.dfr, &b.diff, &b.xi, &b.V, &b.W, &b.Y, &b.A, &b.B, &b.R0, &b.Cq,
-                     &b.lam_hi, &b.dev_bpar, &b.dev_vpar, &b.mob64};
+    double **dp[] = {&b.crd, &b.frc, &b.dfr, &b.diff, &b.xi, &b.V, &b.W, &b.Y, &b.A, &b.B, &b.R0, &b.scr,
+                     &b.Cc, &b.dev_bpar, &b.dev_vpar, &b.mob64};
     for (auto p : dp)
         if (*p) (void)hipFree(*p);
-    int **ip[] = {&b.active, &b.run, &b.failed, &b.conv, &b.bound_ok, &b.nq, &b.counters, &b.bead_chain, &b.bead_local};
+    int **ip[] = {&b.active, &b.run, &b.failed, &b.conv, &b.counters, &b.bead_chain, &b.bead_local};
     for (auto p : ip)
         if (*p) (void)hipFree(*p);
     if (b.mob32) (void)hipFree(b.mob32);

@@ -58,6 +58,7 @@ struct DevStatus {
     int nq;            // quadrature nodes used by the last small problem
     int steps_done;    // EM steps completed since the last bd_run started
     int nrebuild;      // Verlet list rebuilds
+    int ycnt;          // blocks of the Lanczos combine kernel done (last one finalises)
     double err;        // last Lanczos relative change
     double lam_lo, lam_hi;  // spectral bounds of the last small problem
 };
@@ -65,6 +66,10 @@ struct DevStatus {
 // Tile layout and work decomposition of the mobility of one or more chains
 // (bd_mobility.hip).  chain_i: nb, n, ntile, nsc per chain; chain_l: row
 // offset of the chain in the vectors, first tile, first row-partial block.
+namespace sp {
+struct SplitState;
+}
+
 struct MobPlan {
     int nchains = 0;
     long ntiles = 0;
@@ -76,13 +81,19 @@ struct MobPlan {
     long *chain_l = nullptr;
     double *rowpart = nullptr, *colpart = nullptr;
 };
+// Lanczos epilogue of a product (see symm_reduce_lz_kernel).
+struct LzEpi {
+    const double *Vj, *Vp, *Bp;
+    double *part;
+};
 int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off);
 void plan_free(MobPlan &p);
 template <typename E>
 int rpy_launch(const MobPlan &p, const double *crd, E *mob, hipStream_t sm);
 template <typename E, int S>
 int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int ldx, int ldy, int *ticket,
-                 const DevStatus *stop, const int *active, hipStream_t sm, int device, hipEvent_t e0, hipEvent_t e1);
+                 const DevStatus *stop, const int *active, hipStream_t sm, int device, hipEvent_t e0, hipEvent_t e1,
+                 const LzEpi *epi = nullptr);
 
 struct Ctx {
     int nb = 0, n = 0, npad = 0, ntile = 0, ntiles_lower = 0;
@@ -98,6 +109,7 @@ struct Ctx {
     // coordinates and per-step vectors
     double *crd = nullptr, *frc = nullptr, *drift = nullptr, *diff = nullptr;
     double *crd_mob = nullptr;   // coordinates D was assembled at (unused by EM)
+    double *pc_r0 = nullptr, *pc_d0 = nullptr;   // predictor-corrector: r and d(r) of the step
 
     // mobility
     double *mob64 = nullptr;
@@ -122,6 +134,8 @@ struct Ctx {
     double *Cq = nullptr;        // quadrature partials NQUAD_MAX x (js x s)
     double *Ccoef = nullptr;     // js x s coefficients
     int *bound_ok = nullptr;     // NBOUND flags
+    sp::SplitState *split = nullptr;   // state of the split small problem
+    double *lzpart = nullptr;          // per-block partials of the product epilogue
     double *npart = nullptr;     // norm partials
     DevStatus *st = nullptr;
     DevStatus *st_host = nullptr;  // pinned mirror, 2 slots
@@ -192,7 +206,7 @@ int rpy_assemble(Ctx &c);
 // y(npad, s) = D x(npad, s) on the context stream; x, y device, ld = npad.
 int mob_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy);
 // Same product inside a Lanczos solve: skipped once the solve has converged.
-int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy);
+int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy, const LzEpi *epi = nullptr);
 // y = alpha * (D x + add) with optional add vector (s = 1); used for the drift.
 int mob_unpack(Ctx &c, double *dense, int ld);
 

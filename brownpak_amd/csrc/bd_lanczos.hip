@@ -19,11 +19,11 @@
 // discretised by a sinh-substituted trapezoid rule (t = exp(u), u = u0 +
 // sinh(v)) whose node count is chosen from the spectral bounds [m, M] of H_j
 // for 3e-15 relative accuracy.  M is the Gershgorin bound; m is the largest
-// M 2^-k for which the banded Cholesky of H_j - m I succeeds.  Each node is
-// one banded SPD solve of bandwidth s, run by its own wavefront, so a whole
-// evaluation costs a few tens of microseconds instead of a serial O(js^3)
-// eigensolve.  In exact arithmetic the result equals the reference's
-// P sqrt(Lambda) P^T E_1 R.
+// M 2^-k for which the Cholesky of H_j - m I succeeds.  Each node (and each
+// probe of m) is one block-tridiagonal SPD solve with s x s blocks held in
+// registers, run by its own thread, so a whole evaluation is one block of
+// one kernel instead of a serial O(js^3) eigensolve.  In exact arithmetic
+// the result equals the reference's P sqrt(Lambda) P^T E_1 R.
 #include "bd_internal.h"
 #include "bd_lanczos_dev.h"
 
@@ -194,6 +194,275 @@ __global__ __launch_bounds__(RB) void cholqr_kernel(const double *__restrict__ X
     if (gpart_out) block_reduce_store<S>(acc, gpart_out + blockIdx.x * S * S);
 }
 
+// ------------------------------------------------------------ fused step
+// The Krylov step of one iteration in one workgroup, for n <= KT * KR rows:
+//   W -= V_{j-1} B_{j-1}^T ; A_j = V_j^T W ; W -= V_j symL(A_j) ;
+//   CholeskyQR2: W = Q R1, Q = V_{j+1} R2, B_j = R2 R1.
+// The rows stay in registers across the three block reductions, so the
+// four multi-block kernels (and their launch gaps) become one launch.  A
+// thread owns KR consecutive rows, so one base address per column serves
+// all of them (immediate offsets) and the registers go to the rows.
+constexpr int KT = 1024;   // threads of the fused step
+constexpr int KR = 6;      // rows per thread (n <= 6144)
+
+template <int S>
+__device__ inline void kblock_sum(double (&acc)[S * S], double (*sh)[S * S], double *out)
+{
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+#pragma unroll
+    for (int e = 0; e < S * S; e++) {
+        double v = acc[e];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) sh[wv][e] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < S * S) {
+        double v = 0.0;
+        for (int w = 0; w < KT / 64; w++) v += sh[w][threadIdx.x];
+        out[threadIdx.x] = v;
+    }
+    __syncthreads();
+}
+
+template <int S>
+__global__ __launch_bounds__(KT) void krylov_fused_kernel(const double *__restrict__ W, const double *__restrict__ Vj,
+                                                          const double *__restrict__ Vp, const double *__restrict__ Bp,
+                                                          double *__restrict__ Vn, int ld, int rows,
+                                                          double *__restrict__ Aout, double *__restrict__ Bout,
+                                                          int upd, DevStatus *st)
+{
+    // the S x S matrices live in LDS and are read as broadcasts, which keeps
+    // the registers for the rows
+    __shared__ int s_done, s_fail;
+    __shared__ double sh[KT / 64][S * S];
+    __shared__ double sM[S * S], sB[S * S], sR1[S * S], sR2[S * S], sI1[S], sI2[S];
+    if (threadIdx.x == 0) {
+        s_done = lz_done(st) ? 1 : 0;
+        s_fail = 0;
+    }
+    if (threadIdx.x < S * S) sB[threadIdx.x] = Vp ? Bp[threadIdx.x] : 0.0;
+    __syncthreads();
+    if (s_done) return;
+    double w[KR][S];
+    double acc[S * S];
+#pragma unroll
+    for (int e = 0; e < S * S; e++) acc[e] = 0.0;
+    // W -= V_{j-1} B^T ; A = V_j^T W.  Loads come from clamped in-range rows,
+    // so they carry no branch; rows past the end are zeroed afterwards.
+#pragma unroll
+    for (int m = 0; m < KR; m++) {
+        const int r = min((int)threadIdx.x * KR + m, rows - 1);
+        const bool in = (int)threadIdx.x * KR + m < rows;
+        double v[S], p[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) {
+            w[m][k] = W[(long)k * ld + r];
+            v[k] = Vj[(long)k * ld + r];
+            p[k] = Vp ? Vp[(long)k * ld + r] : 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < S; c++) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < S; k++) t += p[k] * sB[c + k * S];
+            w[m][c] = in ? w[m][c] - t : 0.0;
+        }
+#pragma unroll
+        for (int cj = 0; cj < S; cj++)
+#pragma unroll
+            for (int ci = 0; ci < S; ci++) acc[ci + cj * S] += (in ? v[ci] : 0.0) * w[m][cj];
+        // two rows of loads in flight per thread; hoisting all would spill
+        if (m % 2 == 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    kblock_sum<S>(acc, sh, sM);
+    if (threadIdx.x < S * S) Aout[threadIdx.x] = sM[threadIdx.x];
+    if (!upd) return;
+    // W -= V_j symL(A) ; G1 = W^T W
+#pragma unroll
+    for (int e = 0; e < S * S; e++) acc[e] = 0.0;
+#pragma unroll
+    for (int m = 0; m < KR; m++) {
+        const int r = min((int)threadIdx.x * KR + m, rows - 1);
+        const bool in = (int)threadIdx.x * KR + m < rows;
+        double v[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) v[k] = Vj[(long)k * ld + r];
+#pragma unroll
+        for (int c = 0; c < S; c++) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < S; k++) t += v[k] * ((k >= c) ? sM[k + c * S] : sM[c + k * S]);
+            w[m][c] = in ? w[m][c] - t : 0.0;
+        }
+#pragma unroll
+        for (int cj = 0; cj < S; cj++)
+#pragma unroll
+            for (int ci = 0; ci < S; ci++) acc[ci + cj * S] += w[m][ci] * w[m][cj];
+    }
+    // CholeskyQR2 with the factors in LDS
+    kblock_sum<S>(acc, sh, sM);
+    if (threadIdx.x == 0) {
+        double R[S][S];
+        if (!small_chol<S>(sM, R)) s_fail = 1;
+        for (int i = 0; i < S; i++) {
+            sI1[i] = 1.0 / R[i][i];
+            for (int j = 0; j < S; j++) sR1[i + j * S] = R[i][j];
+        }
+    }
+    __syncthreads();
+    if (s_fail) {
+        if (threadIdx.x == 0) atomicExch(&st->ierr, 2);
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < S * S; e++) acc[e] = 0.0;
+#pragma unroll
+    for (int m = 0; m < KR; m++) {
+        double q[S];
+#pragma unroll
+        for (int c = 0; c < S; c++) {
+            double t = w[m][c];
+#pragma unroll
+            for (int k = 0; k < c; k++) t -= q[k] * sR1[k + c * S];
+            q[c] = t * sI1[c];
+        }
+#pragma unroll
+        for (int k = 0; k < S; k++) w[m][k] = q[k];
+#pragma unroll
+        for (int cj = 0; cj < S; cj++)
+#pragma unroll
+            for (int ci = 0; ci < S; ci++) acc[ci + cj * S] += q[ci] * q[cj];
+    }
+    kblock_sum<S>(acc, sh, sM);
+    if (threadIdx.x == 0) {
+        double R[S][S];
+        if (!small_chol<S>(sM, R)) s_fail = 1;
+        for (int i = 0; i < S; i++) {
+            sI2[i] = 1.0 / R[i][i];
+            for (int j = 0; j < S; j++) sR2[i + j * S] = R[i][j];
+        }
+        // B_j = R2 R1
+        for (int i = 0; i < S; i++)
+            for (int j = 0; j < S; j++) {
+                double t = 0.0;
+                for (int k = 0; k < S; k++) t += R[i][k] * sR1[k + j * S];
+                Bout[i + j * S] = t;
+            }
+    }
+    __syncthreads();
+    if (s_fail) {
+        if (threadIdx.x == 0) atomicExch(&st->ierr, 2);
+        return;
+    }
+#pragma unroll
+    for (int m = 0; m < KR; m++) {
+        const int r = (int)threadIdx.x * KR + m;
+        double q[S];
+#pragma unroll
+        for (int c = 0; c < S; c++) {
+            double t = w[m][c];
+#pragma unroll
+            for (int k = 0; k < c; k++) t -= q[k] * sR2[k + c * S];
+            q[c] = t * sI2[c];
+        }
+        if (r < rows) {
+#pragma unroll
+            for (int k = 0; k < S; k++) Vn[(long)k * ld + r] = q[k];
+        }
+    }
+}
+
+// ------------------------------------------------------------ two-pass step
+// After the product's epilogue (W1 = W - V_{j-1} B^T and the partials of
+// A = V_j^T W1, P = W1^T W1), one kernel finishes A, forms the Gram matrix of
+// W2 = W1 - V_j symL(A) without another pass over the rows,
+//   W2^T W2 = P - A^T Ah - Ah^T A + Ah^T Ah     (V_j^T V_j = I, Ah = symL(A)),
+// factors it (R1) and writes Q = W2 R1^{-1} with the partial Gram matrix of
+// Q; the second CholeskyQR pass (cholqr_kernel) completes V_{j+1} and B_j.
+template <int S>
+__global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict__ W, const double *__restrict__ Vj,
+                                                       double *__restrict__ Q, int ld, int rows,
+                                                       const double *__restrict__ lzpart, int nblk,
+                                                       double *__restrict__ Aout, double *__restrict__ R1save,
+                                                       int upd, double *__restrict__ gpart, DevStatus *st)
+{
+    __shared__ int s_done, s_fail;
+    __shared__ double sAP[2 * S * S], sAh[S * S], sR[S * S], sI[S], s_scr[RB];
+    if (threadIdx.x == 0) {
+        s_done = lz_done(st) ? 1 : 0;
+        s_fail = 0;
+    }
+    // fixed-order sums of the per-block partials (ends with a barrier)
+    block_sum_parts(lzpart, 2 * S * S, nblk, s_scr, sAP);
+    if (s_done) return;
+    if (blockIdx.x == 0 && threadIdx.x < S * S) Aout[threadIdx.x] = sAP[threadIdx.x];
+    if (!upd) return;
+    if (threadIdx.x == 0) {
+        const double *A = sAP, *P = sAP + S * S;
+        for (int a = 0; a < S; a++)
+            for (int b = 0; b < S; b++) sAh[a + b * S] = (a >= b) ? A[a + b * S] : A[b + a * S];
+        double G[S * S];
+        for (int a = 0; a < S; a++)
+            for (int b = 0; b < S; b++) {
+                double t = P[a + b * S];
+                for (int k = 0; k < S; k++) {
+                    t -= A[k + a * S] * sAh[k + b * S];     // (A^T Ah)(a,b)
+                    t -= sAh[k + a * S] * A[k + b * S];     // (Ah^T A)(a,b)
+                    t += sAh[k + a * S] * sAh[k + b * S];   // (Ah^T Ah)(a,b)
+                }
+                G[a + b * S] = t;
+            }
+        // symmetrise (the three products are symmetric only up to rounding)
+        for (int a = 0; a < S; a++)
+            for (int b = 0; b < a; b++) {
+                const double t = 0.5 * (G[a + b * S] + G[b + a * S]);
+                G[a + b * S] = t;
+                G[b + a * S] = t;
+            }
+        double R[S][S];
+        if (!small_chol<S>(G, R)) s_fail = 1;
+        for (int i = 0; i < S; i++) {
+            sI[i] = 1.0 / R[i][i];
+            for (int j = 0; j < S; j++) sR[i + j * S] = R[i][j];
+        }
+    }
+    __syncthreads();
+    if (s_fail) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(&st->ierr, 2);
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < S * S) R1save[threadIdx.x] = sR[threadIdx.x];
+    double acc[S * S];
+#pragma unroll
+    for (int e = 0; e < S * S; e++) acc[e] = 0.0;
+    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += GBLK * RB) {
+        double w[S], v[S], q[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) { w[k] = W[(long)k * ld + r]; v[k] = Vj[(long)k * ld + r]; }
+#pragma unroll
+        for (int c = 0; c < S; c++) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < S; k++) t += v[k] * sAh[k + c * S];
+            w[c] = w[c] - t;
+        }
+#pragma unroll
+        for (int c = 0; c < S; c++) {
+            double t = w[c];
+#pragma unroll
+            for (int k = 0; k < c; k++) t -= q[k] * sR[k + c * S];
+            q[c] = t * sI[c];
+        }
+#pragma unroll
+        for (int k = 0; k < S; k++) Q[(long)k * ld + r] = q[k];
+#pragma unroll
+        for (int cj = 0; cj < S; cj++)
+#pragma unroll
+            for (int ci = 0; ci < S; ci++) acc[ci + cj * S] += q[ci] * q[cj];
+    }
+    block_reduce_store<S>(acc, gpart + blockIdx.x * S * S);
+}
+
 // ------------------------------------------------------------ reorthogonalisation
 // part[b][m*S + k] = sum over the block's rows of V[r, m] W[r, k], m < ncol.
 template <int S>
@@ -282,210 +551,314 @@ __global__ __launch_bounds__(RB) void reorth_apply_kernel(const double *__restri
 
 // ------------------------------------------------------------ small problem
 // One block (64 threads) per probe: Cholesky of H - mu_b I, mu_b = M 2^-(b+1).
+// The small problem of iteration j in one block (sp::small_problem):
+// spectral bounds, quadrature, and the coefficients C = sqrt(H_j) E_1 R_0
+// of y_j = V_{1:j} C.
 template <int S>
-__global__ __launch_bounds__(64) void bounds_kernel(const double *__restrict__ Ablk, const double *__restrict__ Bblk,
-                                                    int js, int *__restrict__ ok, DevStatus *st)
+__global__ __launch_bounds__(256) void small_kernel(const double *__restrict__ Ablk, const double *__restrict__ Bblk,
+                                                    const double *__restrict__ R0, int nb, double *__restrict__ scr,
+                                                    double *__restrict__ Cout, DevStatus *st)
 {
-    if (lz_done(st)) return;
-    extern __shared__ double band[];
-    const int ldb = js;
-    double M = gersh_hi<S>(Ablk, Bblk, js);
-    double mu = ldexp(M, -(int)(blockIdx.x + 1));
-    load_band<S>(Ablk, Bblk, js, mu, band, ldb);
-    bool good = band_chol<S>(band, ldb, js);
-    if (threadIdx.x == 0) ok[blockIdx.x] = good ? 1 : 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->lam_hi = M;
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    __syncthreads();
+    if (s_done) return;
+    const sp::SmallOut o = sp::small_problem<S>(Ablk, Bblk, R0, nb, NBOUND, 13, scr, 256, Cout);
+    if (threadIdx.x == 0) {
+        st->lam_hi = o.lam_hi;
+        st->lam_lo = o.lam_lo;
+        st->nq = o.nq;
+        if (o.err) atomicExch(&st->ierr, 2);
+    }
 }
 
-
-// Node k of the quadrature: solves (H + t_k^2 I) Z = X, X = E1 R0, and
-// stores w_k Z into part[k] (js x S col-major, ld js).
+// The small problem split over workgroups (sp::small_problem_split).
 template <int S>
-__global__ __launch_bounds__(64) void quad_kernel(const double *__restrict__ Ablk, const double *__restrict__ Bblk,
-                                                  const double *__restrict__ R0, int js, const int *__restrict__ ok,
-                                                  double *__restrict__ part, DevStatus *st)
+__global__ __launch_bounds__(sp::WG_T) void small_split_kernel(const double *__restrict__ Ablk,
+                                                               const double *__restrict__ Bblk,
+                                                               const double *__restrict__ R0, int nb,
+                                                               double *__restrict__ part, double *__restrict__ Cout,
+                                                               sp::SplitState *ss, DevStatus *st)
 {
-    if (lz_done(st)) return;
-    extern __shared__ double smem[];
-    double *band = smem;               // (S+1) x js
-    double *z = smem + (S + 1) * js;   // js x S
-    const double M = st->lam_hi;
-    int kb = -1;
-    for (int b = 0; b < NBOUND; b++)
-        if (ok[b]) { kb = b; break; }
-    if (kb < 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(&st->ierr, 2);
-        return;
-    }
-    const double m = ldexp(M, -(kb + 1));
-    double kappa = M / m;
-    int dec = (int)ceil(log10(kappa));
-    if (dec < 1) dec = 1;
-    if (dec > 13) dec = 13;
-    const int N = c_qN[dec];
-    const double h = c_qh[dec];
-    const int k = (int)blockIdx.x - N;
-    double *out = part + (long)blockIdx.x * js * S;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->lam_lo = m;
-        st->nq = 2 * N + 1;
-    }
-    if ((int)blockIdx.x > 2 * N) return;
-    const double u0 = 0.25 * (log(m) + log(M));
-    const double v = k * h;
-    const double u = u0 + sinh(v);
-    const double w = (2.0 / M_PI) * h * cosh(v) * exp(u);
-    const double t2 = exp(2.0 * u);
-    load_band<S>(Ablk, Bblk, js, -t2, band, js);
-    if (!band_chol<S>(band, js, js)) {
-        if (threadIdx.x == 0) atomicExch(&st->ierr, 2);
-        return;
-    }
-    // right-hand sides: rows 0..S-1 = R0, others 0
-    for (int e = threadIdx.x; e < js * S; e += blockDim.x) {
-        int i = e % js, c = e / js;
-        z[e] = (i < S) ? R0[i + c * S] : 0.0;
-    }
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
     __syncthreads();
-    if (threadIdx.x < S) {
-        double *zc = z + threadIdx.x * js;
-        for (int i = 0; i < js; i++) {        // L y = x
-            double t = zc[i];
-            for (int d = 1; d <= S && d <= i; d++) t -= band[d * js + i - d] * zc[i - d];
-            zc[i] = t / band[i];
-        }
-        for (int i = js - 1; i >= 0; i--) {   // L^T z = y
-            double t = zc[i];
-            for (int d = 1; d <= S && i + d < js; d++) t -= band[d * js + i] * zc[i + d];
-            zc[i] = t / band[i];
-        }
+    if (s_done) return;
+    const bool fin = sp::small_problem_split<S>(Ablk, Bblk, R0, nb, NBOUND, 13, part, Cout, ss);
+    if (fin && threadIdx.x == 0) {
+        st->lam_hi = ss->lam_hi;
+        st->lam_lo = ss->lam_lo;
+        st->nq = ss->nq;
+        if (ss->err) atomicExch(&st->ierr, 2);
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < js * S; e += blockDim.x) out[e] = w * z[e];
 }
 
-// ysum = sum_k part[k] over the st->nq nodes, fixed order.  Block of 256
-// threads handles 64 entries x 4 node stripes.
+constexpr int YR = 32, YG = 8, YC = 32;   // combine: rows per block, column groups, columns per group
+
+// ------------------------------------------------------------ grouped tests
+// The small problems of LG consecutive iterations j0 .. j0+LG-1 run as one
+// launch (blockIdx.y = member, each split over workgroups), and one combine
+// launch forms their y_j and applies the reference's test to each in order.
+// Everything stays on the solve's stream: no cross-stream events, and the
+// Krylov recurrence never waits for a test it does not need.
+constexpr int LG = 4;
+
 template <int S>
-__global__ __launch_bounds__(256) void nodesum_kernel(const double *__restrict__ part, int js,
-                                                      double *__restrict__ ysum, const DevStatus *st)
+__global__ __launch_bounds__(sp::WG_T) void small_group_kernel(const double *__restrict__ Ablk,
+                                                               const double *__restrict__ Bblk,
+                                                               const double *__restrict__ R0, int j0, int jend,
+                                                               double *__restrict__ part, long part_stride,
+                                                               double *__restrict__ Cg, long c_stride,
+                                                               sp::SplitState *ssv, DevStatus *st)
 {
-    if (lz_done(st)) return;
-    __shared__ double sh[4][64];
-    const int e = blockIdx.x * 64 + threadIdx.x % 64, q = threadIdx.x / 64;
-    const int nq = st->nq;
-    double v = 0.0;
-    if (e < js * S)
-        for (int k = q; k < nq; k += 4) v += part[(long)k * js * S + e];
-    sh[q][threadIdx.x % 64] = v;
+    const int j = j0 + (int)blockIdx.y;
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
     __syncthreads();
-    if (q == 0 && e < js * S) ysum[e] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+    if (s_done || j > jend || j < 2) return;
+    sp::SplitState *ss = ssv + blockIdx.y;
+    const bool fin = sp::small_problem_split<S>(Ablk, Bblk, R0, j, NBOUND, 13, part + blockIdx.y * part_stride,
+                                                Cg + blockIdx.y * c_stride, ss);
+    if (fin && threadIdx.x == 0) {
+        st->lam_hi = ss->lam_hi;
+        st->lam_lo = ss->lam_lo;
+        st->nq = ss->nq;
+        if (ss->err) atomicExch(&st->ierr, 2);
+    }
 }
 
-// y = V(:, 1:js) C ; partial norms of (y - yold) and yold per column.
+// y_j = V(:, 1:jS) C_j for the group's members (Y is a ring of LG + 1
+// vectors indexed by j mod (LG + 1)); partial norms of y_j - y_{j-1} and
+// y_{j-1}; the last block tests the members in order and records the first
+// j that passes.
 template <int S>
-__global__ __launch_bounds__(RB) void ycomb_kernel(const double *__restrict__ V, int ld, int rows, int js,
-                                                   const double *__restrict__ Ablk, const double *__restrict__ Bblk,
-                                                   const double *__restrict__ ysum, double *__restrict__ y,
-                                                   const double *__restrict__ yold, int want_err, int two_norm,
-                                                   double *__restrict__ part, const DevStatus *st)
+__global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restrict__ V, int ld, int rows, int j0,
+                                                          int jend, const double *__restrict__ Cg, long c_stride,
+                                                          double *__restrict__ Y, int two_norm,
+                                                          double *__restrict__ part, double tol, DevStatus *st)
 {
-    if (lz_done(st)) return;
-    extern __shared__ double sC[];
-    // C = H ysum (banded, symmetric)
-    for (int e = threadIdx.x; e < js * S; e += blockDim.x) {
-        int i = e % js, c = e / js;
-        double v = 0.0;
-        for (int l = max(0, i - S); l <= min(js - 1, i + S); l++) {
-            double hv = (i >= l) ? h_entry<S>(Ablk, Bblk, js, i, l) : h_entry<S>(Ablk, Bblk, js, l, i);
-            v += hv * ysum[l + c * js];
-        }
-        sC[e] = v;
+    constexpr int NR = LG + 1;
+    __shared__ int s_done, s_last;
+    __shared__ double s_acc[YG][LG][S][YR];   // row index fastest: conflict-free
+    __shared__ double s_nrm[LG][2 * S];
+    __shared__ double s_C[LG][208 * S];
+    __shared__ double s_scr[256];
+    const int ng = jend - j0 + 1;               // members (<= LG)
+    const int jsmax = jend * S;
+    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    for (int g = 0; g < ng; g++) {
+        const int js = (j0 + g) * S;
+        if (j0 + g < 2) continue;
+        for (int e = threadIdx.x; e < js * S; e += blockDim.x)
+            s_C[g][(e % js) * S + e / js] = Cg[g * c_stride + e];
     }
     __syncthreads();
-    double dn[S], on[S];
+    if (s_done) return;
+    const int lr = threadIdx.x % YR, gq = threadIdx.x / YR;
+    const int r = blockIdx.x * YR + lr;
+    const int rr = min(r, rows - 1);
+    const int per = (jsmax + YG - 1) / YG, m0 = gq * per, m1 = min(jsmax, m0 + per);
+    double acc[LG][S];
 #pragma unroll
-    for (int k = 0; k < S; k++) { dn[k] = 0.0; on[k] = 0.0; }
-    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += GBLK * RB) {
-        double acc[S];
+    for (int g = 0; g < LG; g++)
 #pragma unroll
-        for (int k = 0; k < S; k++) acc[k] = 0.0;
-        for (int m = 0; m < js; m++) {
-            double vm = V[(long)m * ld + r];
+        for (int k = 0; k < S; k++) acc[g][k] = 0.0;
+    for (int c0 = m0; c0 < m1; c0 += YC) {
+        double vm[YC];
 #pragma unroll
-            for (int k = 0; k < S; k++) acc[k] += vm * sC[m + k * js];
-        }
+        for (int q = 0; q < YC; q++) vm[q] = V[(long)min(c0 + q, jsmax - 1) * ld + rr];
 #pragma unroll
-        for (int k = 0; k < S; k++) {
-            y[(long)k * ld + r] = acc[k];
-            if (want_err) {
-                double o = yold[(long)k * ld + r];
-                double d = acc[k] - o;
-                if (two_norm) { dn[k] += d * d; on[k] += o * o; }
-                else { dn[k] += fabs(d); on[k] += fabs(o); }
+        for (int q = 0; q < YC; q++) {
+            const int m = c0 + q;
+            if (m < m1) {
+#pragma unroll
+                for (int g = 0; g < LG; g++)
+                    if (g < ng && j0 + g >= 2 && m < (j0 + g) * S) {
+#pragma unroll
+                        for (int k = 0; k < S; k++) acc[g][k] += vm[q] * s_C[g][m * S + k];
+                    }
             }
         }
     }
-    if (!want_err) return;
-    __shared__ double sh[RB / 64][2 * S];
-    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
 #pragma unroll
-    for (int k = 0; k < 2 * S; k++) {
-        double v = k < S ? dn[k] : on[k - S];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if (lane == 0) sh[wv][k] = v;
-    }
+    for (int g = 0; g < LG; g++)
+#pragma unroll
+        for (int k = 0; k < S; k++) s_acc[gq][g][k][lr] = acc[g][k];
     __syncthreads();
-    if (threadIdx.x < 2 * S) {
-        double v = 0.0;
-        for (int w = 0; w < RB / 64; w++) v += sh[w][threadIdx.x];
-        part[blockIdx.x * 2 * S + threadIdx.x] = v;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        double dn[LG][S], on[LG][S];
+        double prev[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) prev[k] = 0.0;
+        if (lane < YR && r < rows && j0 >= 3) {
+#pragma unroll
+            for (int k = 0; k < S; k++) prev[k] = Y[((long)((j0 - 1) % NR) * S + k) * ld + r];
+        }
+#pragma unroll
+        for (int g = 0; g < LG; g++) {
+            const int j = j0 + g;
+#pragma unroll
+            for (int k = 0; k < S; k++) {
+                dn[g][k] = 0.0;
+                on[g][k] = 0.0;
+            }
+            if (g < ng && j >= 2 && lane < YR && r < rows) {
+#pragma unroll
+                for (int k = 0; k < S; k++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int q = 0; q < YG; q++) v += s_acc[q][g][k][lane];
+                    Y[((long)(j % NR) * S + k) * ld + r] = v;
+                    if (j > 2) {
+                        const double o = prev[k], d = v - o;
+                        if (two_norm) { dn[g][k] = d * d; on[g][k] = o * o; }
+                        else { dn[g][k] = fabs(d); on[g][k] = fabs(o); }
+                    }
+                    prev[k] = v;
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < LG; g++)
+#pragma unroll
+            for (int e = 0; e < 2 * S; e++) {
+                double v = e < S ? dn[g][e] : on[g][e - S];
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                if (lane == 0) part[((long)blockIdx.x * LG + g) * 2 * S + e] = v;
+            }
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = (atomicAdd(&st->ycnt, 1) == (int)gridDim.x - 1) ? 1 : 0;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // part[b][g][e] summed over the blocks b (ends with a barrier)
+    block_sum_parts(part, LG * 2 * S, gridDim.x, s_scr, &s_nrm[0][0]);
+    if (threadIdx.x != 0) return;
+    st->ycnt = 0;
+    st->last_j = jend;
+    for (int g = 0; g < ng; g++) {
+        const int j = j0 + g;
+        if (j <= 2) continue;
+        double num = 0.0, den = 0.0;
+        if (two_norm) {
+            for (int k = 0; k < S; k++) { num += s_nrm[g][k]; den += s_nrm[g][S + k]; }
+            num = sqrt(num);
+            den = sqrt(den);
+        } else {
+            for (int k = 0; k < S; k++) { num = fmax(num, s_nrm[g][k]); den = fmax(den, s_nrm[g][S + k]); }
+        }
+        const double err = num / den;
+        st->err = err;
+        if (err <= tol) {
+            st->converged_j = j;
+            break;
+        }
     }
 }
 
+// y = V(:, 1:js) C, 32 rows per block and the js columns split over eight
+// column groups (up to 32 columns each, all loads issued together); then
+// the partial norms of (y - yold) and yold; the last block to finish applies
+// the reference's test (brown_m.f90:497-506) and records j.
 template <int S>
-__global__ __launch_bounds__(64) void finalize_kernel(const double *__restrict__ part, int j, int want_err,
-                                                      int two_norm, double tol, DevStatus *st)
+__global__ __launch_bounds__(256) void ycomb_kernel(const double *__restrict__ V, int ld, int rows, int js,
+                                                    const double *__restrict__ C, double *__restrict__ y,
+                                                    const double *__restrict__ yold, int want_err, int two_norm,
+                                                    double *__restrict__ part, int j, double tol, DevStatus *st)
 {
-    if (lz_done(st)) return;
-    const int t = threadIdx.x;
-    __shared__ double tot[2 * S];
-    if (want_err) {
-        // lane t: entry t % (2S), block stripe t / (2S)
-        constexpr int E = 2 * S, STR = 64 / E;
-        double v = 0.0;
-        if (t < E * STR)
-            for (int b = t / E; b < GBLK; b += STR) v += part[b * E + t % E];
-        __shared__ double sh[64];
-        sh[t] = v;
-        __syncthreads();
-        if (t < E) {
-            double a = 0.0;
-            for (int q = 0; q < STR; q++) a += sh[q * E + t];
-            tot[t] = a;
+    __shared__ int s_done, s_last;
+    __shared__ double s_acc[YG][S][YR];   // row index fastest: conflict-free
+    __shared__ double s_nrm[2 * S];
+    __shared__ double s_C[512 * S];
+    __shared__ double s_scr[256];
+    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    for (int e = threadIdx.x; e < js * S; e += blockDim.x) s_C[(e % js) * S + e / js] = C[e];
+    __syncthreads();
+    if (s_done) return;
+    const int lr = threadIdx.x % YR, g = threadIdx.x / YR;
+    const int r = blockIdx.x * YR + lr;
+    const int rr = min(r, rows - 1);   // clamped: loads stay unconditional
+    const int per = (js + YG - 1) / YG, m0 = g * per, m1 = min(js, m0 + per);
+    double acc[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) acc[k] = 0.0;
+    for (int c0 = m0; c0 < m1; c0 += YC) {   // one pass while js <= YG * YC
+        double vm[YC];
+#pragma unroll
+        for (int q = 0; q < YC; q++) vm[q] = V[(long)min(c0 + q, js - 1) * ld + rr];
+#pragma unroll
+        for (int q = 0; q < YC; q++) {
+            if (c0 + q < m1) {
+#pragma unroll
+                for (int k = 0; k < S; k++) acc[k] += vm[q] * s_C[(c0 + q) * S + k];
+            }
         }
-        __syncthreads();
     }
-    if (t != 0) return;
+#pragma unroll
+    for (int k = 0; k < S; k++) s_acc[g][k][lr] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        double dn[S], on[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) { dn[k] = 0.0; on[k] = 0.0; }
+        if (lane < YR && r < rows) {
+#pragma unroll
+            for (int k = 0; k < S; k++) {
+                double v = 0.0;
+#pragma unroll
+                for (int gg = 0; gg < YG; gg++) v += s_acc[gg][k][lane];
+                y[(long)k * ld + r] = v;
+                if (want_err) {
+                    const double o = yold[(long)k * ld + r], d = v - o;
+                    if (two_norm) { dn[k] = d * d; on[k] = o * o; }
+                    else { dn[k] = fabs(d); on[k] = fabs(o); }
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 2 * S; e++) {
+            double v = e < S ? dn[e] : on[e - S];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) part[blockIdx.x * 2 * S + e] = v;
+        }
+    }
+    // last block: sum the partials in block order and test
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = (atomicAdd(&st->ycnt, 1) == (int)gridDim.x - 1) ? 1 : 0;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    block_sum_parts(part, 2 * S, gridDim.x, s_scr, s_nrm);   // ends with a barrier
+    if (threadIdx.x != 0) return;
+    st->ycnt = 0;
     st->last_j = j;
     if (!want_err) return;
     double num = 0.0, den = 0.0;
     if (two_norm) {
-        for (int k = 0; k < S; k++) { num += tot[k]; den += tot[S + k]; }
-        num = sqrt(num); den = sqrt(den);
+        for (int k = 0; k < S; k++) { num += s_nrm[k]; den += s_nrm[S + k]; }
+        num = sqrt(num);
+        den = sqrt(den);
     } else {
-        for (int k = 0; k < S; k++) { num = fmax(num, tot[k]); den = fmax(den, tot[S + k]); }
+        for (int k = 0; k < S; k++) { num = fmax(num, s_nrm[k]); den = fmax(den, s_nrm[S + k]); }
     }
-    double err = num / den;
+    const double err = num / den;
     st->err = err;
     if (err <= tol) st->converged_j = j;
 }
 
-__global__ void status_reset_kernel(DevStatus *st)
+__global__ void status_reset_kernel(DevStatus *st, sp::SplitState *ss)
 {
     st->converged_j = 0;
     st->last_j = 0;
     st->err = 0.0;
+    st->ycnt = 0;
+    if (ss)
+        for (int g = 0; g < LG; g++) ss[g].cnt = 0;
 }
 
 // ---------------------------------------------------------------- host
@@ -497,16 +870,29 @@ int lanczos_alloc(Ctx &c)
     BD_HIP(hipMemsetAsync(c.V, 0, (size_t)c.npad * ncol * sizeof(double), c.stream));
     BD_HIP(hipMalloc(&c.W, (size_t)c.npad * s * sizeof(double)));
     BD_HIP(hipMemsetAsync(c.W, 0, (size_t)c.npad * s * sizeof(double), c.stream));
-    BD_HIP(hipMalloc(&c.Y, (size_t)2 * c.npad * s * sizeof(double)));
-    BD_HIP(hipMemsetAsync(c.Y, 0, (size_t)2 * c.npad * s * sizeof(double), c.stream));
+    BD_HIP(hipMalloc(&c.Y, (size_t)(LG + 1) * c.npad * s * sizeof(double)));
+    BD_HIP(hipMemsetAsync(c.Y, 0, (size_t)(LG + 1) * c.npad * s * sizeof(double), c.stream));
     BD_HIP(hipMalloc(&c.Ablk, (size_t)s * s * c.dimk_max * sizeof(double)));
     BD_HIP(hipMalloc(&c.Bblk, (size_t)s * s * c.dimk_max * sizeof(double)));
     BD_HIP(hipMalloc(&c.R0, (size_t)s * s * 2 * sizeof(double)));
     BD_HIP(hipMalloc(&c.gpart, (size_t)2 * GBLK * MAX_S * MAX_S * sizeof(double)));
-    BD_HIP(hipMalloc(&c.Cq, (size_t)(2 * 125 + 1) * ncol * s * sizeof(double)));
-    BD_HIP(hipMalloc(&c.Ccoef, (size_t)ncol * s * sizeof(double)));
+    if (ncol > 512) {
+        set_error("lanc_mxitr * nts_mobsam too large (Krylov order <= 512)");
+        return BD_ERR_ARG;
+    }
+    // small-problem scratch: per block of H, factors of 256 node shifts
+    BD_HIP(hipMalloc(&c.Cq, (size_t)c.dimk_max * (3 * s * s + s) * 256 * sizeof(double)));
+    BD_HIP(hipMalloc(&c.Ccoef, (size_t)LG * ncol * s * sizeof(double)));
     BD_HIP(hipMalloc(&c.bound_ok, NBOUND * sizeof(int)));
-    BD_HIP(hipMalloc(&c.npart, (size_t)GBLK * 2 * MAX_S * sizeof(double)));
+    BD_HIP(hipMalloc(&c.lzpart, (size_t)(c.npad / TILE + 1) * 2 * MAX_S * MAX_S * sizeof(double)));
+    BD_HIP(hipMalloc(&c.split, LG * sizeof(sp::SplitState)));
+    {
+        sp::SplitState z[LG];
+        for (int g = 0; g < LG; g++) z[g] = sp::SplitState{0, -1, 0, 0, 0.0, 0.0};
+        BD_HIP(hipMemcpy(c.split, z, sizeof(z), hipMemcpyHostToDevice));
+    }
+    // combine partials: one set of 2 s norms per block, per group member
+    BD_HIP(hipMalloc(&c.npart, (size_t)(c.npad / YR + 1) * LG * 2 * MAX_S * sizeof(double)));
     if (c.opt.lanc_reorth) {
         BD_HIP(hipMalloc(&c.Rq, (size_t)GBLK * ncol * s * sizeof(double)));
         BD_HIP(hipMalloc(&c.Rcoef, (size_t)ncol * s * sizeof(double)));
@@ -534,6 +920,10 @@ void lanczos_free(Ctx &c)
     }
     if (c.bound_ok) (void)hipFree(c.bound_ok);
     c.bound_ok = nullptr;
+    if (c.split) (void)hipFree(c.split);
+    c.split = nullptr;
+    if (c.lzpart) (void)hipFree(c.lzpart);
+    c.lzpart = nullptr;
     if (c.chol) (void)hipFree(c.chol);
     c.chol = nullptr;
 }
@@ -551,10 +941,35 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     const int ld = c.npad, rows = c.npad;
     const int dmax = c.dimk_max;
     const bool two_norm = (S == 1);
+    // one-workgroup Krylov step when the rows fit its registers
+    const bool fused = S <= 2 && rows <= KT * KR && !c.opt.lanc_reorth;
+    // otherwise the Krylov step rides on the product's reduction (no FP64
+    // reorthogonalisation requested)
+    const bool twopass = !fused && !c.opt.lanc_reorth;
     double *g0 = c.gpart, *g1 = c.gpart + GBLK * MAX_S * MAX_S;
     hipStream_t sm = c.stream, ss = c.side;
 
-    hipLaunchKernelGGL(status_reset_kernel, dim3(1), dim3(1), 0, sm, c.st);
+    hipLaunchKernelGGL(status_reset_kernel, dim3(1), dim3(1), 0, sm, c.st, c.split);
+    // small problem split over workgroups when its blocks fit their LDS
+    const bool split = S <= 4 && dmax <= sp::WG_NBM<S>;
+    const int nwg = (2 * 125 + 1 + sp::WG_NPB<S> - 1) / sp::WG_NPB<S>;
+    // grouped tests: LG iterations' small problems per launch, on the solve's
+    // stream (needs the split small problem)
+    const bool grouped = split;
+    (void)nwg;
+    {
+        // host-side capacity checks of the scratch every launch below indexes
+        const size_t ncol = (size_t)S * dmax;
+        const size_t cq = (size_t)dmax * (3 * S * S + S) * 256;
+        const size_t need_part = grouped ? (size_t)LG * nwg * dmax * S * S
+                                         : (size_t)dmax * (3 * S * S + S) * 256;
+        const size_t nblk_y = (size_t)(rows + YR - 1) / YR;
+        if (need_part > cq || nblk_y * LG * 2 * S > (size_t)(c.npad / YR + 1) * LG * 2 * MAX_S ||
+            (size_t)LG * dmax * S * S > (size_t)LG * ncol * S || (size_t)c.plan.nrowblk > (size_t)(c.npad / TILE + 1)) {
+            set_error("internal: Lanczos scratch smaller than its launches need");
+            return BD_ERR_STATE;
+        }
+    }
     // xi = V_1 R0 (CholeskyQR2)
     hipLaunchKernelGGL((gram_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, xi, xi, ld, rows, g0, c.st);
     hipLaunchKernelGGL((cholqr_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, xi, c.V, ld, rows, g0, g1,
@@ -563,7 +978,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                        (double *)nullptr, c.R0, (const double *)(c.R0 + S * S), c.st);
     BD_HIP(hipGetLastError());
 
-    const int chunk = 4;
+    const int chunk = LG;
     int pending = -1;   // slot of the status copy of the previous chunk
     int slot = 0;
     bool stop = false;
@@ -571,15 +986,38 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
         double *Vj = c.V + (size_t)(j - 1) * S * ld;
         const double *Vp = (j > 1) ? c.V + (size_t)(j - 2) * S * ld : nullptr;
         const double *Bp = (j > 1) ? c.Bblk + (size_t)(j - 2) * S * S : nullptr;
-        BD_TRY(mob_product_lz(c, Vj, c.W, S, ld, ld));
-        hipLaunchKernelGGL((lanczos_a_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vj, Vp, Bp, ld, rows, g0, c.st);
         const int upd = (j < dmax) ? 1 : 0;
-        hipLaunchKernelGGL((lanczos_b_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vj, ld, rows, g0,
-                           c.Ablk + (size_t)(j - 1) * S * S, upd, g1, c.st);
+        if (twopass) {
+            // product + epilogue, then one kernel to Q and the last CholeskyQR pass
+            LzEpi epi{Vj, Vp, Bp, c.lzpart};
+            BD_TRY(mob_product_lz(c, Vj, c.W, S, ld, ld, &epi));
+            double *Vn = c.V + (size_t)j * S * ld;
+            hipLaunchKernelGGL((krylov_k2_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vj, Vn, ld, rows, c.lzpart,
+                               c.plan.nrowblk, c.Ablk + (size_t)(j - 1) * S * S, c.R0 + S * S, upd, g0, c.st);
+            if (upd)
+                hipLaunchKernelGGL((cholqr_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, Vn, Vn, ld, rows, g0,
+                                   (double *)nullptr, c.Bblk + (size_t)(j - 1) * S * S,
+                                   (const double *)(c.R0 + S * S), c.st);
+            BD_HIP(hipGetLastError());
+        } else {
+            BD_TRY(mob_product_lz(c, Vj, c.W, S, ld, ld));
+        }
+        if (twopass) {
+        } else if (fused) {
+            hipLaunchKernelGGL((krylov_fused_kernel<S>), dim3(1), dim3(KT), 0, sm, c.W, Vj, Vp, Bp,
+                               c.V + (size_t)j * S * ld, ld, rows, c.Ablk + (size_t)(j - 1) * S * S,
+                               c.Bblk + (size_t)(j - 1) * S * S, upd, c.st);
+            BD_HIP(hipGetLastError());
+        } else {
+            hipLaunchKernelGGL((lanczos_a_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vj, Vp, Bp, ld, rows, g0,
+                               c.st);
+            hipLaunchKernelGGL((lanczos_b_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vj, ld, rows, g0,
+                               c.Ablk + (size_t)(j - 1) * S * S, upd, g1, c.st);
+        }
         // A_j and V_1..V_j are final here: hand iteration j to the side stream
         hipEvent_t ev = c.lz_ev[j % LZ_NEV];
-        BD_HIP(hipEventRecord(ev, sm));
-        if (upd && c.opt.lanc_reorth) {
+        if (!grouped) BD_HIP(hipEventRecord(ev, sm));
+        if (!twopass && !fused && upd && c.opt.lanc_reorth) {
             // two classical Gram-Schmidt passes against V_1..V_j (FP64)
             const int ncol = j * S;
             for (int pass = 0; pass < 2; pass++) {
@@ -591,7 +1029,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                                    sm, c.V, c.W, ld, rows, ncol, c.Rcoef, pass == 1 ? g1 : (double *)nullptr, c.st);
             }
         }
-        if (upd) {
+        if (!twopass && !fused && upd) {
             double *Vn = c.V + (size_t)j * S * ld;
             double *Bj = c.Bblk + (size_t)(j - 1) * S * S;
             hipLaunchKernelGGL((cholqr_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vn, ld, rows, g1, g0,
@@ -600,30 +1038,46 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                                (double *)nullptr, Bj, (const double *)(c.R0 + S * S), c.st);
         }
         BD_HIP(hipGetLastError());
-        if (j >= 2) {
+        if (grouped && (j % LG == 0 || j == dmax)) {
+            // the tests of iterations jg0..j, on this stream
+            const int jg0 = ((j - 1) / LG) * LG + 1;
+            if constexpr (S <= 4) {
+                const long part_stride = (long)nwg * dmax * S * S;
+                const long c_stride = (long)c.dimk_max * S * S;
+                hipLaunchKernelGGL((small_group_kernel<S>), dim3(nwg, j - jg0 + 1), dim3(sp::WG_T), 0, sm, c.Ablk,
+                                   c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split, c.st);
+                hipLaunchKernelGGL((ycomb_group_kernel<S>), dim3((rows + YR - 1) / YR), dim3(256), 0, sm, c.V, ld,
+                                   rows, jg0, j, c.Ccoef, c_stride, c.Y, two_norm ? 1 : 0, c.npart, c.opt.lanc_tol,
+                                   c.st);
+            }
+            BD_HIP(hipGetLastError());
+        }
+        if (!grouped && j >= 2) {
             BD_HIP(hipStreamWaitEvent(ss, ev, 0));
             const int js = j * S;
-            size_t band_bytes = (size_t)(S + 1) * js * sizeof(double);
-            hipLaunchKernelGGL((bounds_kernel<S>), dim3(NBOUND), dim3(64), band_bytes, ss, c.Ablk, c.Bblk, js,
-                               c.bound_ok, c.st);
-            const int nq = 2 * 125 + 1;
-            hipLaunchKernelGGL((quad_kernel<S>), dim3(nq), dim3(64), band_bytes + (size_t)js * S * sizeof(double), ss,
-                               c.Ablk, c.Bblk, c.R0, js, c.bound_ok, c.Cq, c.st);
-            hipLaunchKernelGGL((nodesum_kernel<S>), dim3((js * S + 63) / 64), dim3(256), 0, ss, c.Cq, js, c.Ccoef,
-                               c.st);
+            bool launched = false;
+            if constexpr (S <= 4) {
+                if (split) {
+                    hipLaunchKernelGGL((small_split_kernel<S>), dim3(nwg), dim3(sp::WG_T), 0, ss, c.Ablk, c.Bblk, c.R0,
+                                       j, c.Cq, c.Ccoef, c.split, c.st);
+                    launched = true;
+                }
+            }
+            if (!launched)
+                hipLaunchKernelGGL((small_kernel<S>), dim3(1), dim3(256), 0, ss, c.Ablk, c.Bblk, c.R0, j, c.Cq,
+                                   c.Ccoef, c.st);
             double *ycur = c.Y + (size_t)(j % 2) * S * ld;
             const double *yold = c.Y + (size_t)((j + 1) % 2) * S * ld;
             const int want = (j > 2) ? 1 : 0;
-            hipLaunchKernelGGL((ycomb_kernel<S>), dim3(GBLK), dim3(RB), (size_t)js * S * sizeof(double), ss, c.V, ld,
-                               rows, js, c.Ablk, c.Bblk, c.Ccoef, ycur, yold, want, two_norm ? 1 : 0, c.npart, c.st);
-            hipLaunchKernelGGL((finalize_kernel<S>), dim3(1), dim3(64), 0, ss, c.npart, j, want, two_norm ? 1 : 0,
-                               c.opt.lanc_tol, c.st);
+            hipLaunchKernelGGL((ycomb_kernel<S>), dim3((rows + YR - 1) / YR), dim3(256), 0, ss, c.V, ld, rows, js, c.Ccoef,
+                               ycur, yold, want, two_norm ? 1 : 0, c.npart, j, c.opt.lanc_tol, c.st);
             BD_HIP(hipGetLastError());
         }
         if (j % chunk == 0 || j == dmax) {
             // snapshot the status after this chunk's tests; read the previous one
-            BD_HIP(hipMemcpyAsync(&c.st_host[slot], c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, ss));
-            BD_HIP(hipEventRecord(c.lz_chunk_ev[slot], ss));
+            hipStream_t ts = grouped ? sm : ss;
+            BD_HIP(hipMemcpyAsync(&c.st_host[slot], c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, ts));
+            BD_HIP(hipEventRecord(c.lz_chunk_ev[slot], ts));
             if (pending >= 0) {
                 BD_HIP(hipEventSynchronize(c.lz_chunk_ev[pending]));
                 if (c.st_host[pending].ierr || c.st_host[pending].converged_j) stop = true;
@@ -646,7 +1100,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     int jf = c.st_host->converged_j ? c.st_host->converged_j : c.st_host->last_j;
     *lconv = c.st_host->converged_j ? 1 : 0;
     *iters = c.st_host->converged_j ? c.st_host->converged_j : dmax;
-    const double *yres = c.Y + (size_t)(jf % 2) * S * ld;
+    const double *yres = c.Y + (size_t)(jf % (grouped ? LG + 1 : 2)) * S * ld;
     BD_HIP(hipMemcpyAsync(out, yres, (size_t)S * ld * sizeof(double), hipMemcpyDeviceToDevice, sm));
     return BD_OK;
 }

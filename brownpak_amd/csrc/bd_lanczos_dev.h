@@ -37,6 +37,45 @@ __device__ void block_reduce_store(double (&acc)[S * S], double *out)
     }
 }
 
+// p[0] + p[stride] + ... (cnt terms), added in index order with sixteen
+// loads in flight: the same additions as the plain loop, without a memory
+// round trip per term (partials written by other XCDs cost ~0.3 us each).
+__device__ __forceinline__ double sum_strided(const double *__restrict__ p, long stride, int cnt)
+{
+    constexpr int U = 16;
+    double t = 0.0;
+    for (int b = 0; b < cnt; b += U) {
+        double v[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) v[q] = p[(long)min(b + q, cnt - 1) * stride];
+#pragma unroll
+        for (int q = 0; q < U; q++)
+            if (b + q < cnt) t += v[q];
+    }
+    return t;
+}
+
+// out[e] = sum over b < cnt of part[b * E + e], e < E <= blockDim.x, by the
+// whole block: G = blockDim.x / E groups, group g summing b = g, g + G, ...;
+// the group sums are added in group order (a fixed order for a given block
+// size).  scr: blockDim.x doubles of shared memory; every thread of the
+// block must call; ends with a barrier.
+__device__ __forceinline__ void block_sum_parts(const double *__restrict__ part, int E, int cnt,
+                                                double *scr, double *out)
+{
+    const int t = threadIdx.x, G = blockDim.x / E, g = t / E, e = t % E;
+    double v = 0.0;
+    if (g < G && cnt > g) v = sum_strided(part + (long)g * E + e, (long)G * E, (cnt - g + G - 1) / G);
+    scr[t] = v;
+    __syncthreads();
+    if (t < E) {
+        double a = 0.0;
+        for (int q = 0; q < G; q++) a += scr[q * E + t];
+        out[t] = a;
+    }
+    __syncthreads();
+}
+
 // Sum of GBLK partials of S*S values, fixed order, by the whole block
 // (blockDim.x == RB).  Result in g (shared), valid after the call.
 template <int S>
@@ -160,5 +199,490 @@ __device__ void load_band(const double *Ablk, const double *Bblk, int js, double
     }
     __syncthreads();
 }
+
+// ------------------------------------------------------------ small problem
+// H_j (order js = nb * S) is block tridiagonal with S x S blocks: diagonal
+// A_b (symmetric, lower triangle used) and sub-diagonal B_{b-1} =
+// H(bS.., (b-1)S..), upper triangular (the R factor of CholeskyQR).  For a
+// shift sigma, H + sigma I = L L^T with L block lower bidiagonal: diagonal
+// blocks L_b (lower) and sub-diagonal blocks G_b = B_{b-1} L_{b-1}^{-T}
+// (upper).  One thread owns one shift; its blocks live in registers with
+// compile-time indices, and the blocks the backward sweep needs go to a
+// node-interleaved scratch array (coalesced over the threads of a block).
+namespace sp {
+
+template <int S>
+constexpr int NE = 3 * S * S + S;   // per block: L (S*S), 1/diag (S), G (S*S), F then w*Z (S*S)
+template <int S>
+constexpr int OFF_L = 0;
+template <int S>
+constexpr int OFF_I = S * S;
+template <int S>
+constexpr int OFF_G = S * S + S;
+template <int S>
+constexpr int OFF_F = 2 * S * S + S;
+
+template <int S>
+__device__ inline void load_blk(const double *__restrict__ p, double (&a)[S][S])
+{
+#pragma unroll
+    for (int c = 0; c < S; c++)
+#pragma unroll
+        for (int r = 0; r < S; r++) a[r][c] = p[r + c * S];
+}
+
+// In-place Cholesky of the lower triangle of a; inv = 1 / diag(L).
+template <int S>
+__device__ inline bool chol(double (&a)[S][S], double (&inv)[S])
+{
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < S; c++) {
+        double d = a[c][c];
+#pragma unroll
+        for (int k = 0; k < c; k++) d -= a[c][k] * a[c][k];
+        ok = ok && (d > 0.0);
+        const double l = sqrt(fmax(d, 1e-300)), r = 1.0 / l;
+        a[c][c] = l;
+        inv[c] = r;
+#pragma unroll
+        for (int i = c + 1; i < S; i++) {
+            double t = a[i][c];
+#pragma unroll
+            for (int k = 0; k < c; k++) t -= a[i][k] * a[c][k];
+            a[i][c] = t * r;
+        }
+    }
+    return ok;
+}
+
+// G = B Lp^{-T} (B upper triangular, so is G)
+template <int S>
+__device__ inline void gsolve(const double (&B)[S][S], const double (&Lp)[S][S], const double (&ip)[S],
+                              double (&G)[S][S])
+{
+#pragma unroll
+    for (int r = 0; r < S; r++)
+#pragma unroll
+        for (int c = 0; c < S; c++) {
+            double t = B[r][c];
+#pragma unroll
+            for (int k = 0; k < c; k++) t -= G[r][k] * Lp[c][k];
+            G[r][c] = t * ip[c];
+        }
+}
+
+// A_b + sigma I - G G^T into a (lower), then its Cholesky.
+template <int S>
+__device__ inline bool next_diag(const double *__restrict__ Ab, double sigma, const double (&G)[S][S],
+                                 double (&a)[S][S], double (&inv)[S])
+{
+    load_blk<S>(Ab, a);
+#pragma unroll
+    for (int r = 0; r < S; r++)
+#pragma unroll
+        for (int c = 0; c <= r; c++) {
+            double t = a[r][c] + (r == c ? sigma : 0.0);
+#pragma unroll
+            for (int k = 0; k < S; k++) t -= G[r][k] * G[c][k];
+            a[r][c] = t;
+        }
+    return chol<S>(a, inv);
+}
+
+// Is H + sigma I positive definite?  (no storage)
+template <int S>
+__device__ bool factor_ok(const double *__restrict__ A, const double *__restrict__ B, int nb, double sigma)
+{
+    double L[S][S], inv[S];
+    load_blk<S>(A, L);
+#pragma unroll
+    for (int r = 0; r < S; r++) L[r][r] += sigma;
+    if (!chol<S>(L, inv)) return false;
+    for (int b = 1; b < nb; b++) {
+        double Bm[S][S], G[S][S], Ln[S][S], in[S];
+        load_blk<S>(B + (b - 1) * S * S, Bm);
+        gsolve<S>(Bm, L, inv, G);
+        if (!next_diag<S>(A + b * S * S, sigma, G, Ln, in)) return false;
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            inv[r] = in[r];
+#pragma unroll
+            for (int c = 0; c < S; c++) L[r][c] = Ln[r][c];
+        }
+    }
+    return true;
+}
+
+// Stores / loads of one block's S x S slot in the node-interleaved scratch.
+template <int S>
+__device__ inline void put(double *__restrict__ scr, int nqp, int node, int slot, const double (&a)[S][S])
+{
+#pragma unroll
+    for (int c = 0; c < S; c++)
+#pragma unroll
+        for (int r = 0; r < S; r++) scr[(long)(slot + r + c * S) * nqp + node] = a[r][c];
+}
+template <int S>
+__device__ inline void get(const double *__restrict__ scr, int nqp, int node, int slot, double (&a)[S][S])
+{
+#pragma unroll
+    for (int c = 0; c < S; c++)
+#pragma unroll
+        for (int r = 0; r < S; r++) a[r][c] = scr[(long)(slot + r + c * S) * nqp + node];
+}
+
+// One quadrature node: solves (H + sigma I) Z = E1 R0 and leaves w Z in the
+// F slots of its scratch column.  Returns false if the factorisation fails.
+// PF: load the next block of the backward sweep one step ahead (worth it
+// when the scratch is global memory).
+template <int S, bool PF = true>
+__device__ bool node_solve(const double *__restrict__ A, const double *__restrict__ B, const double *__restrict__ R0,
+                           int nb, double sigma, double w, double *__restrict__ scr, int nqp, int node)
+{
+    constexpr int E = NE<S>;
+    double L[S][S], inv[S], F[S][S];
+    load_blk<S>(A, L);
+#pragma unroll
+    for (int r = 0; r < S; r++) L[r][r] += sigma;
+    bool ok = chol<S>(L, inv);
+    {
+        double R[S][S];
+        load_blk<S>(R0, R);
+#pragma unroll
+        for (int c = 0; c < S; c++)
+#pragma unroll
+            for (int i = 0; i < S; i++) {
+                double t = R[i][c];
+#pragma unroll
+                for (int k = 0; k < i; k++) t -= L[i][k] * F[k][c];
+                F[i][c] = t * inv[i];
+            }
+    }
+    // forward sweep: factor and F_b = -L_b^{-1} G_b F_{b-1}
+    for (int b = 0;; b++) {
+        put<S>(scr, nqp, node, b * E + OFF_L<S>, L);
+#pragma unroll
+        for (int r = 0; r < S; r++) scr[(long)(b * E + OFF_I<S> + r) * nqp + node] = inv[r];
+        put<S>(scr, nqp, node, b * E + OFF_F<S>, F);
+        if (b + 1 >= nb) break;
+        double Bm[S][S], G[S][S], Ln[S][S], in[S];
+        load_blk<S>(B + b * S * S, Bm);
+        gsolve<S>(Bm, L, inv, G);
+        ok = next_diag<S>(A + (b + 1) * S * S, sigma, G, Ln, in) && ok;
+        put<S>(scr, nqp, node, (b + 1) * E + OFF_G<S>, G);
+        double Fn[S][S];
+#pragma unroll
+        for (int c = 0; c < S; c++)
+#pragma unroll
+            for (int i = 0; i < S; i++) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < S; k++) t -= G[i][k] * F[k][c];
+#pragma unroll
+                for (int k = 0; k < i; k++) t -= Ln[i][k] * Fn[k][c];
+                Fn[i][c] = t * in[i];
+            }
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            inv[r] = in[r];
+#pragma unroll
+            for (int c = 0; c < S; c++) { L[r][c] = Ln[r][c]; F[r][c] = Fn[r][c]; }
+        }
+    }
+    // backward sweep: Z_b = L_b^{-T} (F_b - G_{b+1}^T Z_{b+1}); the last
+    // block's L, 1/diag and F are still in registers
+    double Z[S][S], Gn[S][S];
+#pragma unroll
+    for (int r = 0; r < S; r++)
+#pragma unroll
+        for (int c = 0; c < S; c++) { Gn[r][c] = 0.0; Z[r][c] = 0.0; }
+    for (int b = nb - 1; b >= 0; b--) {
+        double Gp[S][S], Lp[S][S], ip[S], Fp[S][S];
+        if (!PF && b < nb - 1) {   // block b now (G_{b+1}, L_b, 1/diag, F_b)
+            get<S>(scr, nqp, node, (b + 1) * E + OFF_G<S>, Gn);
+            get<S>(scr, nqp, node, b * E + OFF_L<S>, L);
+            get<S>(scr, nqp, node, b * E + OFF_F<S>, F);
+#pragma unroll
+            for (int r = 0; r < S; r++) inv[r] = scr[(long)(b * E + OFF_I<S> + r) * nqp + node];
+        }
+        if (PF && b > 0) {  // prefetch block b-1 while block b is solved
+            get<S>(scr, nqp, node, b * E + OFF_G<S>, Gp);
+            get<S>(scr, nqp, node, (b - 1) * E + OFF_L<S>, Lp);
+            get<S>(scr, nqp, node, (b - 1) * E + OFF_F<S>, Fp);
+#pragma unroll
+            for (int r = 0; r < S; r++) ip[r] = scr[(long)((b - 1) * E + OFF_I<S> + r) * nqp + node];
+        }
+        double Zn[S][S];
+#pragma unroll
+        for (int c = 0; c < S; c++)
+#pragma unroll
+            for (int ii = S - 1; ii >= 0; ii--) {
+                double t = F[ii][c];
+#pragma unroll
+                for (int k = 0; k < S; k++) t -= Gn[k][ii] * Z[k][c];
+#pragma unroll
+                for (int k = ii + 1; k < S; k++) t -= L[k][ii] * Zn[k][c];
+                Zn[ii][c] = t * inv[ii];
+            }
+        double wz[S][S];
+#pragma unroll
+        for (int r = 0; r < S; r++)
+#pragma unroll
+            for (int c = 0; c < S; c++) { Z[r][c] = Zn[r][c]; wz[r][c] = w * Zn[r][c]; }
+        put<S>(scr, nqp, node, b * E + OFF_F<S>, wz);
+        if (b == 0) break;
+        if (PF) {
+#pragma unroll
+            for (int r = 0; r < S; r++) {
+                inv[r] = ip[r];
+#pragma unroll
+                for (int c = 0; c < S; c++) { L[r][c] = Lp[r][c]; F[r][c] = Fp[r][c]; Gn[r][c] = Gp[r][c]; }
+            }
+        }
+    }
+    return ok;
+}
+
+// Gershgorin bound of H_j by the whole block (any blockDim multiple of 64).
+template <int S>
+__device__ double gersh_block(const double *Ablk, const double *Bblk, int js, double *red)
+{
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < js; i += blockDim.x) {
+        double rs = 0.0;
+        for (int l = max(0, i - S); l <= min(js - 1, i + S); l++) {
+            double h = (i >= l) ? h_entry<S>(Ablk, Bblk, js, i, l) : h_entry<S>(Ablk, Bblk, js, l, i);
+            rs += fabs(h);
+        }
+        mx = fmax(mx, rs);
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = mx;
+    __syncthreads();
+    double M = red[0];
+    for (int w = 1; w < (int)(blockDim.x / 64); w++) M = fmax(M, red[w]);
+    __syncthreads();
+    return M;
+}
+
+struct SmallOut {
+    int nq;
+    int err;        // 0 ok, 2 factorisation failed, 3 spectral range beyond the node table
+    double lam_lo, lam_hi;
+};
+
+// The whole small problem of one Lanczos iteration, by one block:
+// M = Gershgorin bound; m = the largest M 2^-(b+1), b < nbound, with H - m I
+// positive definite (one thread per probe); DE quadrature
+//   H^{-1/2} X = (2/pi) int_0^inf (H + t^2 I)^{-1} X dt,  t = e^u,
+//   u = u0 + sinh(v), u0 = (ln m + ln M) / 4, nodes v = k h, |k| <= N,
+// one thread per node; Z = sum of w Z_k; C = H Z (so y = V C = V sqrt(H) E1 R0).
+// C is written to Cout (js x S, ld js).  scr holds nb * NE<S> * nqp doubles.
+template <int S>
+__device__ SmallOut small_problem(const double *__restrict__ A, const double *__restrict__ B,
+                                  const double *__restrict__ R0, int nb, int nbound, int dec_max,
+                                  double *__restrict__ scr, int nqp, double *__restrict__ Cout)
+{
+    constexpr int E = NE<S>;
+    const int js = nb * S;
+    __shared__ double s_red[16];
+    __shared__ int s_ok[64];
+    __shared__ int s_fail;
+    __shared__ double s_Z[512 * S];      // js <= 512
+    SmallOut out{0, 0, 0.0, 0.0};
+    const double M = gersh_block<S>(A, B, js, s_red);
+    out.lam_hi = M;
+    if (threadIdx.x == 0) s_fail = 0;
+    if ((int)threadIdx.x < nbound) s_ok[threadIdx.x] = factor_ok<S>(A, B, nb, -ldexp(M, -(int)(threadIdx.x + 1))) ? 1 : 0;
+    __syncthreads();
+    int kb = -1;
+    for (int b = 0; b < nbound; b++)
+        if (s_ok[b]) { kb = b; break; }
+    if (kb < 0) { out.err = 2; return out; }
+    const double m = ldexp(M, -(kb + 1));
+    out.lam_lo = m;
+    int dec = (int)ceil(log10(M / m));
+    if (dec > dec_max) { out.err = 3; return out; }
+    if (dec < 1) dec = 1;
+    const int N = c_qN[dec];
+    const double h = c_qh[dec];
+    const int nq = 2 * N + 1;
+    out.nq = nq;
+    if (nq > nqp || nq > (int)blockDim.x) { out.err = 3; return out; }
+    const int node = threadIdx.x;
+    if (node < nq) {
+        const double u0 = 0.25 * (log(m) + log(M));
+        const double v = (node - N) * h;
+        const double u = u0 + sinh(v);
+        const double w = (2.0 / M_PI) * h * cosh(v) * exp(u);
+        if (!node_solve<S>(A, B, R0, nb, exp(2.0 * u), w, scr, nqp, node)) s_fail = 1;
+    }
+    __syncthreads();
+    if (s_fail) { out.err = 2; return out; }
+    // Z = sum over nodes (fixed order), into LDS
+    for (int e = threadIdx.x; e < js * S; e += blockDim.x) {
+        const int b = e / (S * S), rc = e % (S * S);     // rc = r + c S inside block b
+        const double *p = scr + (long)(b * E + OFF_F<S> + rc) * nqp;
+        const double t = sum_strided(p, 1, nq);
+        const int r = rc % S, c = rc / S;
+        s_Z[(b * S + r) + c * js] = t;
+    }
+    __syncthreads();
+    // C = H Z (banded)
+    for (int e = threadIdx.x; e < js * S; e += blockDim.x) {
+        const int i = e % js, c = e / js;
+        double v = 0.0;
+        for (int l = max(0, i - S); l <= min(js - 1, i + S); l++) {
+            double hv = (i >= l) ? h_entry<S>(A, B, js, i, l) : h_entry<S>(A, B, js, l, i);
+            v += hv * s_Z[l + c * js];
+        }
+        Cout[e] = v;
+    }
+    return out;
+}
+
+// ------------------------------------------------------------ split small problem
+// The same small problem spread over several workgroups of two waves:
+// wave 0 tests the spectral probes while wave 1 solves this workgroup's
+// share of the quadrature nodes, with the band of H_j and every node's
+// factor blocks in LDS (so the sequential block sweeps see LDS, not global,
+// latency).  The nodes are placed with the lower bound of the previous
+// iteration (a hint); when the probes find another bound the nodes are
+// solved again.  Each workgroup leaves the weighted sum of its nodes'
+// solutions in part[wg]; the last one to finish sums the parts in
+// workgroup order and forms C = H Z.
+template <int S>
+constexpr int WG_NBM = 208 / S;      // largest H_j block count served (order <= 208)
+template <int S>
+constexpr int WG_NPB = S == 1 ? 16 : (S == 2 ? 10 : (S == 3 ? 8 : 6));   // nodes per workgroup
+constexpr int WG_T = 128;            // threads: probes in wave 0, nodes in wave 1
+
+struct SplitState {
+    int cnt;        // workgroups finished (the last resets it)
+    int kb_hint;    // probe index of the previous lower bound (-1: none)
+    int nq, err;
+    double lam_lo, lam_hi;
+};
+
+// Returns true (uniformly) in the workgroup that finished last and wrote
+// C and the SplitState fields.
+template <int S>
+__device__ bool small_problem_split(const double *__restrict__ Ag, const double *__restrict__ Bg,
+                                    const double *__restrict__ R0, int nb, int nbound, int dec_max,
+                                    double *__restrict__ part, double *__restrict__ Cout, SplitState *ss)
+{
+    constexpr int E = NE<S>, NBM = WG_NBM<S>, NPB = WG_NPB<S>;
+    __shared__ double sA[NBM * S * S], sB[NBM * S * S], sR[S * S];
+    __shared__ double sF[NBM * E * NPB];
+    __shared__ double s_red[WG_T / 64];
+    __shared__ int s_ok[64];
+    __shared__ int s_fail, s_last;
+    const int js = nb * S, t = threadIdx.x, wv = t / 64, lane = t % 64;
+    for (int e = t; e < nb * S * S; e += WG_T) {
+        sA[e] = Ag[e];
+        sB[e] = Bg[e];
+    }
+    if (t < S * S) sR[t] = R0[t];
+    if (t == 0) s_fail = 0;
+    __syncthreads();
+    const double M = gersh_block<S>(sA, sB, js, s_red);
+    // node placement for a probe index kb: (m, N, h, nq)
+    auto place = [&](int kb, double &m, int &N, double &h) {
+        m = ldexp(M, -(kb + 1));
+        int dec = (int)ceil(log10(M / m));
+        if (dec < 1) dec = 1;
+        if (dec > dec_max) dec = dec_max + 1;
+        N = c_qN[min(dec, 13)];
+        h = c_qh[min(dec, 13)];
+        return dec;
+    };
+    const int hint = ss->kb_hint;
+    const int kb_guess = (hint >= 0 && hint < nbound) ? hint : 0;
+    auto solve_nodes = [&](int kb) {
+        double m, h;
+        int N;
+        const int dec = place(kb, m, N, h);
+        const int nq = 2 * N + 1;
+        if (dec > dec_max) return;
+        const int node = blockIdx.x * NPB + lane;
+        if (lane < NPB && node < nq) {
+            const double u0 = 0.25 * (log(m) + log(M));
+            const double v = (node - N) * h;
+            const double u = u0 + sinh(v);
+            const double w = (2.0 / M_PI) * h * cosh(v) * exp(u);
+            if (!node_solve<S, false>(sA, sB, sR, nb, exp(2.0 * u), w, sF, NPB, lane)) s_fail = 1;
+        }
+    };
+    if (wv == 0) {
+        if (lane < nbound) s_ok[lane] = factor_ok<S>(sA, sB, nb, -ldexp(M, -(lane + 1))) ? 1 : 0;
+    } else {
+        solve_nodes(kb_guess);
+    }
+    __syncthreads();
+    int kb = -1;
+    for (int b = 0; b < nbound; b++)
+        if (s_ok[b]) { kb = b; break; }
+    int err = 0;
+    if (kb < 0) err = 2;
+    double m = 0.0, h = 0.0;
+    int N = 0;
+    const int dec = kb < 0 ? 0 : place(kb, m, N, h);
+    if (!err && dec > dec_max) err = 3;
+    const int nq = 2 * N + 1;
+    const int nact = err ? 1 : (nq + NPB - 1) / NPB;
+    if ((int)blockIdx.x >= nact) return false;   // no nodes here: not counted
+    if (!err && kb != kb_guess) {
+        if (t == 0) s_fail = 0;
+        __syncthreads();
+        if (wv == 1) solve_nodes(kb);
+        __syncthreads();
+    }
+    if (s_fail && !err) err = 2;
+    // this workgroup's part: sum over its node slots
+    const int nmine = err ? 0 : max(0, min(NPB, nq - (int)blockIdx.x * NPB));
+    double *pw = part + (long)blockIdx.x * js * S;
+    for (int e = t; e < js * S; e += WG_T) {
+        const int b = e / (S * S), rc = e % (S * S);
+        double acc = 0.0;
+        for (int k = 0; k < nmine; k++) acc += sF[(b * E + OFF_F<S> + rc) * NPB + k];
+        pw[e] = acc;
+    }
+    __threadfence();
+    __syncthreads();
+    if (t == 0) s_last = (atomicAdd(&ss->cnt, 1) == nact - 1) ? 1 : 0;
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    if (t == 0) {
+        ss->cnt = 0;
+        ss->kb_hint = kb;
+        ss->nq = nq;
+        ss->err = err;
+        ss->lam_lo = m;
+        ss->lam_hi = M;
+    }
+    if (err) return true;
+    // Z (block-major, S x S blocks col-major) summed over parts, into sF[0..)
+    double *sZ = sF;
+    for (int e = t; e < js * S; e += WG_T) {
+        const double acc = sum_strided(part + e, (long)js * S, nact);
+        const int b = e / (S * S), rc = e % (S * S), r = rc % S, c = rc / S;
+        sZ[(b * S + r) + c * js] = acc;
+    }
+    __syncthreads();
+    for (int e = t; e < js * S; e += WG_T) {
+        const int i = e % js, c = e / js;
+        double v = 0.0;
+        for (int l = max(0, i - S); l <= min(js - 1, i + S); l++) {
+            double hv = (i >= l) ? h_entry<S>(sA, sB, js, i, l) : h_entry<S>(sA, sB, js, l, i);
+            v += hv * sZ[l + c * js];
+        }
+        Cout[e] = v;
+    }
+    return true;
+}
+
+}  // namespace sp
 
 }  // namespace bd

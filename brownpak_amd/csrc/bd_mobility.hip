@@ -10,8 +10,9 @@
 // work unit covers a 2-row x 4-column block of tiles; row products are summed
 // over the unit's 4 tile columns in registers before one partial is written
 // per (tile row, super-column), and column products are summed over the unit's
-// tile rows.  A second kernel adds the partials of every output block in a
-// fixed order, so results are deterministic run to run.
+// tile rows.  A second kernel adds
+// the partials of every output block in a fixed order, so results are
+// deterministic run to run.
 #include <type_traits>
 
 #include "bd_internal.h"
@@ -21,6 +22,10 @@ namespace bd {
 constexpr int WSC = 4;   // tile columns per super-column
 constexpr int CHR = 2;   // tile rows per work unit
 constexpr int NT = 256;  // threads per block of the product kernel
+// Work-unit record (16 ints): chain, super-column, Ib, Ie, ntile, nsc,
+// valid-slot mask, 0, tile base (elements, 2 ints), x row offset (2 ints),
+// row-partial offset (2 ints), 0, 0.
+constexpr int UREC = 16;
 
 __host__ __device__ inline long tile_index(int I, int J) { return (long)I * (I + 1) / 2 + J; }
 
@@ -97,130 +102,156 @@ __global__ __launch_bounds__(256) void rpy_tile_kernel(const double *__restrict_
 }
 
 // --------------------------------------------------------------- product
-template <typename E>
-struct TileLds {
-    E d[TILE][TILE + 1];            // column-major tile, padded
-};
-
-// Persistent kernel; blocks pull work units from a ticket counter.
+// Blocks of 256 threads take work units (2 tile rows x 4 tile columns of one
+// chain) in turn, each described by one precomputed record.  Two (FP64) or
+// three (FP32) tiles are in flight at any time: each thread loads 16
+// coalesced elements of a tile into registers, and a register set is refilled
+// with the tile that many slots ahead as soon as its own tile is staged in
+// LDS.  Within a tile the four
+// waves split the work by quarters:
+//   row product    y_I[r] += sum_c D(r,c) x_J[c]   (lane r, 16 columns);
+//   column product z_J[c] += sum_r D(r,c) x_I[r]   (lane c, 16 rows);
+// the padded tile makes both LDS patterns conflict-free and the x reads are
+// broadcasts.  Row sums are combined over the quarters once per unit row,
+// column sums once per unit column, through a small LDS buffer.
 // rowpart: (ntile * nsc) x TILE x S; colpart: nunits x WSC x TILE x S.
+constexpr int TPT = TILE * TILE / NT;   // tile elements per thread (16)
+// tiles in flight per block (register sets), sized so that three blocks fit
+// a CU without spilling
+template <typename E>
+constexpr int PDT = sizeof(E) == 8 ? 2 : 3;
+
+template <typename E>
+__device__ inline void load_tile16(const E *__restrict__ tp, int tid, E (&t)[TPT])
+{
+#pragma unroll
+    for (int i = 0; i < TPT; i++) t[i] = tp[tid + i * NT];
+}
+
 template <typename E, int S>
-__global__ __launch_bounds__(NT) void symm_tiles_kernel(
+__global__ __launch_bounds__(NT, 3) void symm_tiles_kernel(
     const E *__restrict__ mob_all, const double *__restrict__ x_all, int ldx, const int *__restrict__ units,
     int nunits, const int *__restrict__ chain_i, const long *__restrict__ chain_l, double *__restrict__ rowpart,
     double *__restrict__ colpart, int *__restrict__ ticket, const DevStatus *stop, const int *__restrict__ active)
 {
+    constexpr int NS = CHR * WSC, PD = PDT<E>;
+    __shared__ E T[TILE][TILE + 1];
+    __shared__ double xJs[WSC][TILE][S];
+    __shared__ double xIs[CHR][TILE][S];
+    __shared__ double sred[NT / TILE][TILE][S];
+    const int tid = threadIdx.x, lane = tid % TILE, q = tid / TILE;
+    (void)chain_i;
+    (void)chain_l;
+    (void)ticket;
     // inside a Lanczos solve: nothing to do once it has converged
     if (stop && (stop->converged_j != 0 || stop->ierr != 0)) return;
-    // the reduction buffer aliases the staged tile (never live together)
-    __shared__ union {
-        TileLds<E> t;
-        double red[NT / TILE][TILE][S];
-    } sU;
-    __shared__ double sVJ[WSC][TILE][S];
-    __shared__ double sVI[TILE][S];
-    __shared__ int s_unit;
-    auto &sT = sU.t;
-    auto &sred = sU.red;
-
-    const int tid = threadIdx.x;
-    const int lane_r = tid % TILE;      // row (row product) / column (column product)
-    const int q = tid / TILE;           // quarter: 16 columns (row prod) / 16 rows (col prod)
-
-    for (;;) {
-        if (tid == 0) s_unit = atomicAdd(ticket, 1);
-        __syncthreads();
-        const int u = s_unit;
-        if (u >= nunits) break;
-        const int chain = units[4 * u], sc = units[4 * u + 1], Ib = units[4 * u + 2], Ie = units[4 * u + 3];
-        if (active && !active[chain]) {
+    // units are dealt out statically; one record load describes a unit
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int *rec = units + (long)u * UREC;
+        const int chain = rec[0], sc = rec[1], Ib = rec[2], Ie = rec[3], ntile = rec[4], nsc = rec[5];
+        const unsigned mask = (unsigned)rec[6];
+        const long tbase = (long)(unsigned)rec[8] | ((long)rec[9] << 32);
+        const long xoff = (long)(unsigned)rec[10] | ((long)rec[11] << 32);
+        const long rpo = (long)(unsigned)rec[12] | ((long)rec[13] << 32);
+        if (active) {
+            // one thread reads the flag (another stream may clear it), so the
+            // whole block takes the same branch
+            __shared__ int s_act;
             __syncthreads();
-            continue;
+            if (tid == 0) s_act = active[chain];
+            __syncthreads();
+            if (!s_act) continue;
         }
-        const int ntile = chain_i[4 * chain + 2], nsc = chain_i[4 * chain + 3];
-        const double *x = x_all + chain_l[3 * chain];
-        const E *mob = mob_all + chain_l[3 * chain + 1] * (TILE * TILE);
-        const long rpo = chain_l[3 * chain + 2];
+        const double *x = x_all + xoff;
+        const E *mob = mob_all + tbase;
         const int J0 = sc * WSC;
-        // V_J blocks for the unit's tile columns
-        for (int e = tid; e < WSC * TILE * S; e += NT) {
-            int jj = e / (TILE * S), rem = e % (TILE * S), r = rem / S, k = rem % S;
-            int J = J0 + jj;
-            sVJ[jj][r][k] = (J < ntile) ? x[(long)k * ldx + (long)J * TILE + r] : 0.0;
+        // PD tiles in flight: slot sl lives in register set sl % PD, and the
+        // load of slot sl + PD is issued as soon as slot sl is in LDS
+        E R[PD][TPT];
+#pragma unroll
+        for (int sl = 0; sl < PD; sl++)
+            if (mask & (1u << sl)) load_tile16<E>(mob + tile_index(Ib + sl / WSC, J0 + sl % WSC) * (TILE * TILE), tid, R[sl]);
+        // x blocks (zeros past the chain's last tile), unconditional loads
+        __syncthreads();   // the previous unit no longer reads xJs / xIs / sred
+        {
+            constexpr int NX = (WSC + CHR) * TILE * S;
+#pragma unroll
+            for (int i = 0; i < (NX + NT - 1) / NT; i++) {
+                const int e = tid + i * NT;
+                if (e < NX) {
+                    const int blk = e / (TILE * S), rem = e % (TILE * S), r = rem / S, k = rem % S;
+                    const int B = blk < WSC ? min(J0 + blk, ntile - 1) : min(Ib + blk - WSC, Ie - 1);
+                    const bool in = blk < WSC ? (J0 + blk < ntile) : (Ib + blk - WSC < Ie);
+                    const double v = x[(long)k * ldx + (long)B * TILE + r];
+                    if (blk < WSC) xJs[blk][r][k] = in ? v : 0.0;
+                    else xIs[blk - WSC][r][k] = in ? v : 0.0;
+                }
+            }
         }
-        double colacc[WSC][S];
+        double racc[CHR][S], cacc[WSC][S];
 #pragma unroll
-        for (int jj = 0; jj < WSC; jj++)
+        for (int a = 0; a < CHR; a++)
 #pragma unroll
-            for (int k = 0; k < S; k++) colacc[jj][k] = 0.0;
-
-        for (int I = Ib; I < Ie; I++) {
-            double rowacc[S];
+            for (int k = 0; k < S; k++) racc[a][k] = 0.0;
 #pragma unroll
-            for (int k = 0; k < S; k++) rowacc[k] = 0.0;
-            __syncthreads();
-            for (int e = tid; e < TILE * S; e += NT) {
-                int r = e / S, k = e % S;
-                sVI[r][k] = x[(long)k * ldx + (long)I * TILE + r];
-            }
-            const int jmax = min(WSC, I - J0 + 1);
+        for (int a = 0; a < WSC; a++)
 #pragma unroll
-            for (int jj = 0; jj < WSC; jj++) {
-                if (jj >= jmax) break;
-                const int J = J0 + jj;
-                const E *tp = mob + tile_index(I, J) * (TILE * TILE);
-                __syncthreads();
-                // stage the tile: element e = c*TILE + r
-#pragma unroll 4
-                for (int e = tid; e < TILE * TILE; e += NT) sT.d[e / TILE][e % TILE] = tp[e];
-                __syncthreads();
-                // row product: rows lane_r, columns 16q..16q+15
-#pragma unroll 4
-                for (int cc = 0; cc < TILE / 4; cc++) {
-                    int c = q * (TILE / 4) + cc;
-                    double dv = (double)sT.d[c][lane_r];
+            for (int k = 0; k < S; k++) cacc[a][k] = 0.0;
 #pragma unroll
-                    for (int k = 0; k < S; k++) rowacc[k] += dv * sVJ[jj][c][k];
-                }
-                if (I != J) {
-                    // column product: column lane_r, rows 16q..16q+15
-#pragma unroll 4
-                    for (int rr = 0; rr < TILE / 4; rr++) {
-                        int r = q * (TILE / 4) + rr;
-                        double dv = (double)sT.d[lane_r][r];
+        for (int sl = 0; sl < NS; sl++) {
+            const bool here = mask & (1u << sl);
+            const int ii = sl / WSC, jj = sl % WSC;
+            const int I = Ib + ii, J = J0 + jj;
+            if (here) {
+                __syncthreads();   // the previous tile is no longer read
 #pragma unroll
-                        for (int k = 0; k < S; k++) colacc[jj][k] += dv * sVI[r][k];
-                    }
+                for (int i = 0; i < TPT; i++) {
+                    const int e = tid + i * NT;
+                    T[e / TILE][e % TILE] = R[sl % PD][i];
                 }
             }
-            // reduce row partial over quarters, write one partial per (I, sc)
+            if (sl + PD < NS && (mask & (1u << (sl + PD))))
+                load_tile16<E>(mob + tile_index(Ib + (sl + PD) / WSC, J0 + (sl + PD) % WSC) * (TILE * TILE), tid,
+                               R[sl % PD]);
+            if (!here) continue;
+            __syncthreads();
+#pragma unroll 4
+            for (int cc = 0; cc < TILE / 4; cc++) {
+                const int c = q * (TILE / 4) + cc;
+                const double d = (double)T[c][lane];
+#pragma unroll
+                for (int k = 0; k < S; k++) racc[ii][k] += d * xJs[jj][c][k];
+            }
+            if (I != J) {
+#pragma unroll 4
+                for (int rr = 0; rr < TILE / 4; rr++) {
+                    const int r = q * (TILE / 4) + rr;
+                    const double d = (double)T[lane][r];
+#pragma unroll
+                    for (int k = 0; k < S; k++) cacc[jj][k] += d * xIs[ii][r][k];
+                }
+            }
+        }
+        // combine quarters: one partial per (tile row, super-column) and per
+        // (unit, tile column)
+#pragma unroll
+        for (int a = 0; a < CHR + WSC; a++) {
+            const bool isrow = a < CHR;
+            if (isrow && Ib + a >= Ie) continue;
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < S; k++) sred[q][lane_r][k] = rowacc[k];
+            for (int k = 0; k < S; k++) sred[q][lane][k] = isrow ? racc[a][k] : cacc[a - CHR][k];
             __syncthreads();
+            double *dst = isrow ? rowpart + (rpo + (long)(Ib + a) * nsc + sc) * (TILE * S)
+                                : colpart + ((long)u * WSC + (a - CHR)) * (TILE * S);
             for (int e = tid; e < TILE * S; e += NT) {
-                int r = e / S, k = e % S;
+                const int r = e / S, k = e % S;
                 double v = 0.0;
 #pragma unroll
                 for (int qq = 0; qq < NT / TILE; qq++) v += sred[qq][r][k];
-                rowpart[(rpo + (long)I * nsc + sc) * (TILE * S) + e] = v;
+                dst[e] = v;
             }
         }
-        // column partials of this unit
-        for (int jj = 0; jj < WSC; jj++) {
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < S; k++) sred[q][lane_r][k] = colacc[jj][k];
-            __syncthreads();
-            for (int e = tid; e < TILE * S; e += NT) {
-                int r = e / S, k = e % S;
-                double v = 0.0;
-#pragma unroll
-                for (int qq = 0; qq < NT / TILE; qq++) v += sred[qq][r][k];
-                colpart[((long)u * WSC + jj) * (TILE * S) + e] = v;
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -241,11 +272,92 @@ __global__ void symm_reduce_kernel(const double *__restrict__ rowpart, const dou
     const int chain = rb[0], K = rb[1], nsc = rb[2], u0 = rb[3], u1 = rb[4], rpo = rb[5], yrow = rb[6];
     if (active && !active[chain]) return;
     const int r = e / S, k = e % S;
+    // fixed summation order; loads batched eight at a time
     double v = 0.0;
-    for (int sc = 0; sc <= K / WSC; sc++) v += rowpart[((long)rpo + (long)K * nsc + sc) * (TILE * S) + e];
+    const int nrow = K / WSC + 1;
+    const double *rp = rowpart + ((long)rpo + (long)K * nsc) * (TILE * S) + e;
+    for (int s0 = 0; s0 < nrow; s0 += 8) {
+        double t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = (s0 + q < nrow) ? rp[(long)(s0 + q) * (TILE * S)] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v += t[q];
+    }
     const int jj = K % WSC;
-    for (int u = u0; u < u1; u++) v += colpart[((long)u * WSC + jj) * (TILE * S) + e];
+    const double *cp = colpart + (long)jj * (TILE * S) + e;
+    for (int u = u0; u < u1; u += 8) {
+        double t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = (u + q < u1) ? cp[(long)(u + q) * WSC * (TILE * S)] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v += t[q];
+    }
     y[(long)k * ldy + yrow + r] = v;
+}
+
+// The same sum for one output block per workgroup (TILE * S threads), fused
+// with the first step of a Lanczos iteration:
+//   W1 = W - V_{j-1} B_{j-1}^T  (stored over W),
+//   partials of A = V_j^T W1 and P = W1^T W1 (2 S^2 per block, into lzpart).
+template <int S>
+__global__ __launch_bounds__(TILE * MAX_S) void symm_reduce_lz_kernel(
+    const double *__restrict__ rowpart, const double *__restrict__ colpart, const int *__restrict__ rowblk,
+    double *__restrict__ W, int ld, const double *__restrict__ Vj, const double *__restrict__ Vp,
+    const double *__restrict__ Bp, double *__restrict__ lzpart, const DevStatus *stop)
+{
+    __shared__ int s_done;
+    __shared__ double sw[TILE][S];
+    __shared__ double red[TILE][2 * S * S];
+    if (threadIdx.x == 0) s_done = (stop && (stop->converged_j != 0 || stop->ierr != 0)) ? 1 : 0;
+    __syncthreads();
+    if (s_done) return;
+    const int G = blockIdx.x, e = threadIdx.x;
+    const int *rb = rowblk + 8 * G;
+    const int K = rb[1], nsc = rb[2], u0 = rb[3], u1 = rb[4], rpo = rb[5], yrow = rb[6];
+    const int r = e / S, k = e % S;
+    double v = 0.0;
+    const int nrow = K / WSC + 1;
+    const double *rp = rowpart + ((long)rpo + (long)K * nsc) * (TILE * S) + e;
+    for (int s0 = 0; s0 < nrow; s0 += 8) {
+        double t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = (s0 + q < nrow) ? rp[(long)(s0 + q) * (TILE * S)] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v += t[q];
+    }
+    const int jj = K % WSC;
+    const double *cp = colpart + (long)jj * (TILE * S) + e;
+    for (int u = u0; u < u1; u += 8) {
+        double t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = (u + q < u1) ? cp[(long)(u + q) * WSC * (TILE * S)] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v += t[q];
+    }
+    const long row = yrow + r;
+    double vj[S];
+#pragma unroll
+    for (int c = 0; c < S; c++) vj[c] = Vj[(long)c * ld + row];
+    if (Vp) {
+        double t = 0.0;
+#pragma unroll
+        for (int kp = 0; kp < S; kp++) t += Vp[(long)kp * ld + row] * Bp[k + kp * S];
+        v = v - t;
+    }
+    W[(long)k * ld + row] = v;
+    sw[r][k] = v;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < S; c++) {
+        red[r][c + k * S] = vj[c] * v;             // A(c, k)
+        red[r][S * S + c + k * S] = sw[r][c] * v;  // P(c, k)
+    }
+    __syncthreads();
+    if (e < 2 * S * S) {
+        double t = 0.0;
+        for (int rr = 0; rr < TILE; rr++) t += red[rr][e];
+        lzpart[(long)G * 2 * S * S + e] = t;
+    }
 }
 
 // Expands the tiled lower storage to a dense n x n column-major matrix.
@@ -290,13 +402,22 @@ int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &
             }
         std::vector<int> scu(nsc + 1, 0);
         for (int sc = 0; sc < nsc; sc++) {
-            scu[sc] = (int)(units.size() / 4);
+            scu[sc] = (int)(units.size() / UREC);
             for (int Ib = sc * WSC; Ib < nt; Ib += CHR) {
-                units.push_back(ch); units.push_back(sc); units.push_back(Ib);
-                units.push_back(std::min(Ib + CHR, nt));
+                const int Ie = std::min(Ib + CHR, nt), J0 = sc * WSC;
+                int mask = 0;
+                for (int sl = 0; sl < CHR * WSC; sl++) {
+                    const int I = Ib + sl / WSC, J = J0 + sl % WSC;
+                    if (I < Ie && J <= I && J < nt) mask |= 1 << sl;
+                }
+                const long tb = toff * (TILE * TILE), xo = row_off[ch];
+                const int rec[UREC] = {ch, sc, Ib, Ie, nt, nsc, mask, 0,
+                                       (int)(tb & 0xffffffffL), (int)(tb >> 32), (int)(xo & 0xffffffffL),
+                                       (int)(xo >> 32), (int)(rpo & 0xffffffffL), (int)(rpo >> 32), 0, 0};
+                units.insert(units.end(), rec, rec + UREC);
             }
         }
-        scu[nsc] = (int)(units.size() / 4);
+        scu[nsc] = (int)(units.size() / UREC);
         for (int K = 0; K < nt; K++) {
             int sc = K / WSC;
             rowblk.push_back(ch); rowblk.push_back(K); rowblk.push_back(nsc); rowblk.push_back(scu[sc]);
@@ -312,7 +433,7 @@ int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &
     }
     p.nchains = C;
     p.ntiles = toff;
-    p.nunits = (int)(units.size() / 4);
+    p.nunits = (int)(units.size() / UREC);
     p.nrowblk = (int)(rowblk.size() / 8);
     auto up = [](auto **dst, const auto &v) -> int {
         using T = typename std::remove_reference<decltype(v)>::type::value_type;
@@ -389,16 +510,26 @@ int rpy_assemble(Ctx &c) { return rpy_assemble_at(c, c.crd); }
 // Launches the tile product of a plan: y = D x for every (active) chain.
 template <typename E, int S>
 int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int ldx, int ldy, int *ticket,
-                 const DevStatus *stop, const int *active, hipStream_t sm, int device, hipEvent_t e0, hipEvent_t e1)
+                 const DevStatus *stop, const int *active, hipStream_t sm, int device, hipEvent_t e0, hipEvent_t e1,
+                 const LzEpi *epi)
 {
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-    int grid = std::max(1, std::min(p.nunits, cus * 3));
+    // one block per unit up to three per CU (the occupancy the kernel is
+    // built for); beyond that blocks take units in turn
+    const int per_cu = 3;
+    int grid = std::max(1, std::min(p.nunits, cus * per_cu));
     if (e0) BD_HIP(hipEventRecord(e0, sm));
     hipLaunchKernelGGL((symm_tiles_kernel<E, S>), dim3(grid), dim3(NT), 0, sm, mob, x, ldx, p.units, p.nunits,
                        p.chain_i, p.chain_l, p.rowpart, p.colpart, ticket, stop, active);
     BD_HIP(hipGetLastError());
     if (e1) BD_HIP(hipEventRecord(e1, sm));
+    if (epi) {
+        hipLaunchKernelGGL((symm_reduce_lz_kernel<S>), dim3(p.nrowblk), dim3(TILE * S), 0, sm, p.rowpart, p.colpart,
+                           p.rowblk, y, ldy, epi->Vj, epi->Vp, epi->Bp, epi->part, stop);
+        BD_HIP(hipGetLastError());
+        return BD_OK;
+    }
     long tot = (long)p.nrowblk * TILE * S;
     hipLaunchKernelGGL((symm_reduce_kernel<S>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, sm, p.rowpart,
                        p.colpart, p.rowblk, p.nrowblk, y, ldy, ticket, active);
@@ -408,7 +539,8 @@ int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int
 
 #define BD_INST(E, S)                                                                                          \
     template int plan_product<E, S>(const MobPlan &, const E *, const double *, double *, int, int, int *,     \
-                                    const DevStatus *, const int *, hipStream_t, int, hipEvent_t, hipEvent_t);
+                                    const DevStatus *, const int *, hipStream_t, int, hipEvent_t, hipEvent_t,  \
+                                    const LzEpi *);
 BD_INST(float, 1) BD_INST(float, 2) BD_INST(float, 3) BD_INST(float, 4)
 BD_INST(float, 5) BD_INST(float, 6) BD_INST(float, 7) BD_INST(float, 8)
 BD_INST(double, 1) BD_INST(double, 2) BD_INST(double, 3) BD_INST(double, 4)
@@ -416,7 +548,8 @@ BD_INST(double, 5) BD_INST(double, 6) BD_INST(double, 7) BD_INST(double, 8)
 #undef BD_INST
 
 template <typename E, int S>
-static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, int ldy, const DevStatus *stop)
+static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, int ldy, const DevStatus *stop,
+                     const LzEpi *epi)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c.prof_on) {
@@ -432,22 +565,23 @@ static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, 
         e1 = c.prof_ev[c.prof_used + 1];
         c.prof_used += 2;
     }
-    return plan_product<E, S>(c.plan, mob, x, y, ldx, ldy, &c.st->ticket, stop, nullptr, c.stream, c.device, e0, e1);
+    return plan_product<E, S>(c.plan, mob, x, y, ldx, ldy, &c.st->ticket, stop, nullptr, c.stream, c.device, e0, e1,
+                              epi);
 }
 
 template <typename E>
 static int product_dispatch(Ctx &c, const E *mob, const double *x, double *y, int s, int ldx, int ldy,
-                            const DevStatus *stop)
+                            const DevStatus *stop, const LzEpi *epi = nullptr)
 {
     switch (s) {
-    case 1: return product_t<E, 1>(c, mob, x, y, ldx, ldy, stop);
-    case 2: return product_t<E, 2>(c, mob, x, y, ldx, ldy, stop);
-    case 3: return product_t<E, 3>(c, mob, x, y, ldx, ldy, stop);
-    case 4: return product_t<E, 4>(c, mob, x, y, ldx, ldy, stop);
-    case 5: return product_t<E, 5>(c, mob, x, y, ldx, ldy, stop);
-    case 6: return product_t<E, 6>(c, mob, x, y, ldx, ldy, stop);
-    case 7: return product_t<E, 7>(c, mob, x, y, ldx, ldy, stop);
-    case 8: return product_t<E, 8>(c, mob, x, y, ldx, ldy, stop);
+    case 1: return product_t<E, 1>(c, mob, x, y, ldx, ldy, stop, epi);
+    case 2: return product_t<E, 2>(c, mob, x, y, ldx, ldy, stop, epi);
+    case 3: return product_t<E, 3>(c, mob, x, y, ldx, ldy, stop, epi);
+    case 4: return product_t<E, 4>(c, mob, x, y, ldx, ldy, stop, epi);
+    case 5: return product_t<E, 5>(c, mob, x, y, ldx, ldy, stop, epi);
+    case 6: return product_t<E, 6>(c, mob, x, y, ldx, ldy, stop, epi);
+    case 7: return product_t<E, 7>(c, mob, x, y, ldx, ldy, stop, epi);
+    case 8: return product_t<E, 8>(c, mob, x, y, ldx, ldy, stop, epi);
     default: set_error("block size s must be in 1..8"); return BD_ERR_ARG;
     }
 }
@@ -459,11 +593,11 @@ int mob_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy)
     return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy, nullptr);
 }
 
-int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy)
+int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy, const LzEpi *epi)
 {
     if (!c.mob_valid) { set_error("mobility not assembled"); return BD_ERR_STATE; }
-    if (c.mob32) return product_dispatch<float>(c, c.mob32, x, y, s, ldx, ldy, c.st);
-    return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy, c.st);
+    if (c.mob32) return product_dispatch<float>(c, c.mob32, x, y, s, ldx, ldy, c.st, epi);
+    return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy, c.st, epi);
 }
 
 // Sums the recorded (start, stop) pairs into prof_ms and recycles them.

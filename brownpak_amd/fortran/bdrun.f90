@@ -49,10 +49,13 @@ end do
 
 call read_control(trim(fn_control))
 call read_cfg(trim(fn_cfg))
-if (trim(bdintg) /= 'EM') then
-    print '(a)', 'bdrun: only bdintg = EM runs on the device'
+select case (trim(bdintg))
+case ('EM'); opts%integrator = BD_EM
+case ('PC'); opts%integrator = BD_PC
+case default
+    print '(a)', 'bdrun: bdintg must be EM or PC on the device'
     stop 2
-end if
+end select
 
 opts%lhdia = merge(1, 0, lhdia)
 select case (trim(mob_fctr))

@@ -14,6 +14,7 @@ integer(c_int), parameter :: BD_OK = 0, BD_ERR_BOND = 1, BD_ERR_FACTOR = 2, &
 integer(c_int), parameter :: BD_CHOL = 0, BD_LANC = 1, BD_CHOL_LT = 2
 integer(c_int), parameter :: BD_FP64 = 0, BD_FP32 = 1
 integer(c_int), parameter :: BD_PT_DIR = 0, BD_PT_VER = 1, BD_PT_CL = 2
+integer(c_int), parameter :: BD_EM = 0, BD_PC = 1
 
 type, bind(C) :: bd_opts_t
     integer(c_int) :: lhdia = 1
@@ -27,6 +28,7 @@ type, bind(C) :: bd_opts_t
     integer(c_int) :: flow_style = 0
     real(c_double) :: flow_rate = 0.0_c_double
     integer(c_int) :: lanc_reorth = 0
+    integer(c_int) :: integrator = 0
 end type
 
 type, bind(C) :: bd_runinfo_t

@@ -73,6 +73,14 @@ enum {
                        QR and recurrences stay FP64                             */
 };
 
+/* integrator (control key bdintg, bd_solver_m.f90:64, 169-174).  BD_PC is the
+ * predictor-corrector (Heun) step with the mobility and the Brownian
+ * displacement w of the step held fixed:
+ *   r* = r + d(r) + w,   r' = r + (d(r) + d(r*)) / 2 + w,
+ * d(r) = dt (U(r) + D F(r)) -- one more force and D.F evaluation per step.
+ * The reference's semi-implicit 'SE' (NITSOL) is not provided. */
+enum { BD_EM = 0, BD_PC = 1 };
+
 /* mth_ptgen (pairtab_m.fpp:23-24) */
 enum { BD_PT_DIR = 0, BD_PT_VER = 1, BD_PT_CL = 2 };
 
@@ -92,6 +100,8 @@ typedef struct {
     int lanc_reorth;      /* 1: full FP64 reorthogonalisation of every new
                              Krylov block (not in the reference, whose Lanczos
                              loses orthogonality; see brown_m.f90:402-435)     */
+    int integrator;       /* BD_EM (integrate_em, bd_solver_m.f90:216-238) or
+                             BD_PC: predictor-corrector, not in the reference  */
 } bd_opts;
 
 /* Context.  Replaces bds_init/brn_init (bd_solver_m.f90:50-104,

@@ -1108,6 +1108,57 @@ int orc_pairs_cell_list(const orc_system *sys, const int *ex_ptr,
  * calc_diffusion (355-403) and calc_drift (407-454).  Statistics, dumps and
  * logging are not part of the step.  xi replaces brn_calc_dw
  * (brown_m.f90:558-573). */
+/* calc_drift (bd_solver_m.f90:407-454) at crd: pair table, forces, flow
+ * field, D.F (or F) and the factor dt, into drift.  Returns 0, -1 (bond
+ * overstretched) or -12 (pair table capacity). */
+static int drift_eval(const orc_system *sys, const orc_bdopts *opt, const double *mob, const int *ex_ptr,
+                      const int *ex_col, orc_verlet *vsp, int *pt_ptr, int *pt_col, int ptcap, const double *crd,
+                      double *frc, double *drift, orc_energies *en)
+{
+    const int nb = sys->nbeads, n = 3 * nb;
+    memset(drift, 0, sizeof(double) * n);
+    if (sys->lvdw && sys->nvdw_types > 0) {
+        int np;
+        if (sys->mth_ptgen == 1) {
+            int rebuilt;
+            np = orc_pairs_verlet(sys, ex_ptr, ex_col, crd, vsp, pt_ptr, pt_col, ptcap, &rebuilt);
+            if (np == -2) np = 0;
+        } else if (sys->mth_ptgen == 2) {
+            np = orc_pairs_cell_list(sys, ex_ptr, ex_col, crd, pt_ptr, pt_col, ptcap);
+        } else {
+            np = orc_pairs_n2(sys, ex_ptr, ex_col, crd, sys->rcutoff * sys->rcutoff, pt_ptr, pt_col, ptcap);
+        }
+        if (np < 0) return -12;
+    }
+    if (orc_forces(sys, crd, (sys->lvdw && sys->nvdw_types > 0) ? pt_ptr : NULL, pt_col, frc, en)) return -1;
+    switch (opt->flow_style) {
+    case 1:
+        for (int i = 0; i < nb; i++) drift[3 * i] = opt->flow_rate * crd[3 * i + 1];
+        break;
+    case 2:
+        for (int i = 0; i < nb; i++) {
+            drift[3 * i] = opt->flow_rate * crd[3 * i];
+            drift[3 * i + 1] = -opt->flow_rate * crd[3 * i + 1];
+        }
+        break;
+    case 3:
+        for (int i = 0; i < nb; i++) {
+            drift[3 * i] = opt->flow_rate * crd[3 * i];
+            drift[3 * i + 1] = -0.5 * opt->flow_rate * crd[3 * i + 1];
+            drift[3 * i + 2] = -0.5 * opt->flow_rate * crd[3 * i + 2];
+        }
+        break;
+    default:
+        break;
+    }
+    if (opt->lhdia)
+        orc_symv_lower(n, 1.0, mob, frc, 1.0, drift);
+    else
+        for (int i = 0; i < n; i++) drift[i] = drift[i] + frc[i];
+    for (int i = 0; i < n; i++) drift[i] = opt->tim_stp * drift[i];
+    return 0;
+}
+
 long long orc_bd_run(const orc_system *sys, const orc_bdopts *opt, double *crd,
                      int nsteps, const double *xi, long long nxi, orc_runinfo *info)
 {
@@ -1128,6 +1179,8 @@ long long orc_bd_run_ex(const orc_system *sys, const orc_bdopts *opt, double *cr
     double *diff = (double *)calloc((size_t)n * s, sizeof(double));
     double *drift = (double *)calloc((size_t)n, sizeof(double));
     double *frc = (double *)calloc((size_t)n, sizeof(double));
+    double *rpred = opt->integrator == 1 ? (double *)calloc((size_t)n, sizeof(double)) : NULL;
+    double *dpred = opt->integrator == 1 ? (double *)calloc((size_t)n, sizeof(double)) : NULL;
     int excap = nb * 64 + 16;
     int *ex_ptr = (int *)calloc((size_t)nb + 1, sizeof(int));
     int *ex_col = (int *)malloc(sizeof(int) * excap);
@@ -1199,50 +1252,22 @@ long long orc_bd_run_ex(const orc_system *sys, const orc_bdopts *opt, double *cr
             cntr++;
         }
         /* ---- calc_drift ---- */
-        memset(drift, 0, sizeof(double) * n);
-        if (sys->lvdw && sys->nvdw_types > 0) {
-            int np;
-            if (sys->mth_ptgen == 1) {
-                int rebuilt;
-                np = orc_pairs_verlet(sys, ex_ptr, ex_col, crd, &vs, pt_ptr, pt_col, ptcap, &rebuilt);
-                if (np == -2) np = 0;
-            } else if (sys->mth_ptgen == 2) {
-                np = orc_pairs_cell_list(sys, ex_ptr, ex_col, crd, pt_ptr, pt_col, ptcap);
-            } else {
-                np = orc_pairs_n2(sys, ex_ptr, ex_col, crd, sys->rcutoff * sys->rcutoff, pt_ptr, pt_col, ptcap);
-            }
-            if (np < 0) { ret = -12; goto done; }
+        {
+            int e = drift_eval(sys, opt, mob, ex_ptr, ex_col, &vs, pt_ptr, pt_col, ptcap, crd, frc, drift, &en);
+            if (e == -1) { info->ierr = 1; ret = -1; goto done; }
+            if (e) { ret = e; goto done; }
         }
-        if (orc_forces(sys, crd, (sys->lvdw && sys->nvdw_types > 0) ? pt_ptr : NULL, pt_col, frc, &en)) {
-            info->ierr = 1;
-            ret = -1;
-            goto done;
+        /* ---- predictor-corrector (not in the reference): r* = r + d(r) + w,
+         * then the drift is the mean of d(r) and d(r*); D and w are those of
+         * the step (the mobility is held over nts_mobsam steps anyway) ---- */
+        if (opt->integrator == 1) {
+            const double *dcol = disp ? disp + (size_t)step * n : diff + (size_t)(opt->lhdia ? cntr - 1 : 0) * n;
+            for (int i = 0; i < n; i++) rpred[i] = crd[i] + drift[i] + dcol[i];
+            int e = drift_eval(sys, opt, mob, ex_ptr, ex_col, &vs, pt_ptr, pt_col, ptcap, rpred, frc, dpred, &en);
+            if (e == -1) { info->ierr = 1; ret = -1; goto done; }
+            if (e) { ret = e; goto done; }
+            for (int i = 0; i < n; i++) drift[i] = 0.5 * (drift[i] + dpred[i]);
         }
-        switch (opt->flow_style) {
-        case 1:
-            for (int i = 0; i < nb; i++) drift[3 * i] = opt->flow_rate * crd[3 * i + 1];
-            break;
-        case 2:
-            for (int i = 0; i < nb; i++) {
-                drift[3 * i] = opt->flow_rate * crd[3 * i];
-                drift[3 * i + 1] = -opt->flow_rate * crd[3 * i + 1];
-            }
-            break;
-        case 3:
-            for (int i = 0; i < nb; i++) {
-                drift[3 * i] = opt->flow_rate * crd[3 * i];
-                drift[3 * i + 1] = -0.5 * opt->flow_rate * crd[3 * i + 1];
-                drift[3 * i + 2] = -0.5 * opt->flow_rate * crd[3 * i + 2];
-            }
-            break;
-        default:
-            break;
-        }
-        if (opt->lhdia)
-            orc_symv_lower(n, 1.0, mob, frc, 1.0, drift);
-        else
-            for (int i = 0; i < n; i++) drift[i] = drift[i] + frc[i];
-        for (int i = 0; i < n; i++) drift[i] = opt->tim_stp * drift[i];
         /* ---- update ---- */
         {
             const double *dcol = disp ? disp + (size_t)step * n : diff + (size_t)(opt->lhdia ? cntr - 1 : 0) * n;
@@ -1259,7 +1284,7 @@ long long orc_bd_run_ex(const orc_system *sys, const orc_bdopts *opt, double *cr
     }
     ret = used;
 done:
-    free(mob); free(diff); free(drift); free(frc); free(ex_ptr); free(ex_col);
+    free(mob); free(diff); free(drift); free(frc); free(rpred); free(dpred); free(ex_ptr); free(ex_col);
     free(pt_ptr); free(pt_col); free(vs.crd_save);
     return ret;
 }

@@ -128,6 +128,9 @@ typedef struct {
     int flow_style;       /* 0 none, 1 shear, 2 planar, 3 uniaxial      */
     double flow_rate;
     int lanc_reorth;      /* 1: FP64 full reorthogonalisation (not in the reference) */
+    int integrator;       /* 0 EM (integrate_em); 1 predictor-corrector (Heun drift,
+                             mobility and Brownian term held over the step; not in
+                             the reference) */
 } orc_bdopts;
 
 typedef struct {

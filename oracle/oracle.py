@@ -50,7 +50,7 @@ class OrcBdopts(C.Structure):
         ("lhdia", C.c_int), ("mob_fctr", C.c_int), ("nts_mobsam", C.c_int),
         ("lanc_mxitr", C.c_int), ("lanc_tol", C.c_double), ("tim_stp", C.c_double),
         ("keep_reuse_noise", C.c_int), ("flow_style", C.c_int), ("flow_rate", C.c_double),
-        ("lanc_reorth", C.c_int),
+        ("lanc_reorth", C.c_int), ("integrator", C.c_int),
     ]
 
 
@@ -152,6 +152,7 @@ def symv_lower(mob_colmajor_logical, x, alpha=1.0, beta=0.0, y=None):
 
 # ------------------------------------------------------------ Brownian B.xi
 CHOL, LANC, CHOL_LT = 0, 1, 2
+EM, PC = 0, 1          # integrator: integrate_em, predictor-corrector
 
 
 def brn_calc_bdw(method, mob, xi, lanc_mxitr=50, lanc_tol=1e-6, reorth=False):
@@ -335,12 +336,13 @@ def rgsq(crd):
 
 def bd_run(sysobj, crd, nsteps, xi, lhdia=True, mob_fctr=CHOL, nts_mobsam=1,
            lanc_mxitr=50, lanc_tol=1e-6, tim_stp=1e-3, keep_reuse_noise=False,
-           flow_style=0, flow_rate=0.0, lanc_reorth=False):
-    """bds_run (EM) for nsteps with the injected Gaussian stream xi."""
+           flow_style=0, flow_rate=0.0, lanc_reorth=False, integrator=EM):
+    """bds_run for nsteps with the injected Gaussian stream xi; integrator
+    EM (integrate_em) or PC (predictor-corrector, not in the reference)."""
     crd = _f64(crd).copy()
     xi = _f64(np.ravel(xi))
     opt = OrcBdopts(int(lhdia), mob_fctr, nts_mobsam, lanc_mxitr, lanc_tol, tim_stp,
-                    int(keep_reuse_noise), flow_style, flow_rate, int(bool(lanc_reorth)))
+                    int(keep_reuse_noise), flow_style, flow_rate, int(bool(lanc_reorth)), integrator)
     info = OrcRuninfo()
     used = lib().orc_bd_run(C.byref(sysobj.s), C.byref(opt), _dp(crd), nsteps, _dp(xi),
                             xi.size, C.byref(info))

@@ -1,0 +1,148 @@
+"""Batched ensemble (bd_batch_*): every chain of a batch follows its own
+bds_run exactly.
+
+Bars:
+  * per chain against the oracle, same injected xi: 1e-10 relative at a fixed
+    Lanczos iteration count (lanc_tol = 0), as the single-chain tests;
+  * converged runs: each chain's Brownian block within 10 * lanc_tol of the
+    exact square root (eigen-decomposition of the oracle's RPY matrix);
+  * chain independence at the batched-ensemble sizes (lengths U[50, 800]):
+    a chain's trajectory is bit-identical whether it runs alone or inside
+    the batch (the product partials and reductions are per chain and fixed
+    order, so the other chains cannot leak into its arithmetic).
+"""
+import numpy as np
+import pytest
+
+from helpers import fene_wca_chain, oracle_system, rel
+from brownpak_amd.configs import fene_wca_params, random_walk_chain
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _batch(lengths, seed, **opts):
+    from brownpak_amd import BdBatch
+    b = BdBatch(lengths, **opts)
+    bond, vdw = fene_wca_params()
+    b.set_model(2, bond, lvdw=True, vdw_params=vdw, rcutoff=vdw[2], excluded_atoms=0)
+    chains = [random_walk_chain(int(nb), seed + c) for c, nb in enumerate(lengths)]
+    b.set_coordinates(chains)
+    return b, chains
+
+
+def _split_xi(lengths, s, nupd, xi):
+    """Per-chain streams of the batch's xi layout (chain by chain per update)."""
+    per = [[] for _ in lengths]
+    p = 0
+    for _ in range(nupd):
+        for c, nb in enumerate(lengths):
+            k = 3 * nb * s
+            per[c].append(xi[p:p + k])
+            p += k
+    return [np.concatenate(x) for x in per]
+
+
+@pytest.mark.parametrize("s,prec", [(1, "fp64"), (4, "fp64"), (3, "fp64"), (4, "fp32")])
+def test_batch_matches_oracle_fixed_iterations(oracle, s, prec):
+    lengths = [40, 17, 64, 23, 50]
+    nsteps, mx = 2 * s, 5
+    nupd = (nsteps + s - 1) // s
+    _say(f"case: batch fixed-iteration s={s} {prec} lengths={lengths}")
+    tot = sum(3 * nb * s for nb in lengths) * nupd
+    xi = np.random.default_rng(7 + s).standard_normal(tot)
+    b, chains = _batch(lengths, 300, nts_mobsam=s, lanc_mxitr=mx, lanc_tol=0.0, precision=prec)
+    used = b.run(nsteps, xi)
+    assert used == tot
+    out = b.get_coordinates()
+    per = _split_xi(lengths, s, nupd, xi)
+    tol = 1e-10 if prec == "fp64" else 1e-6
+    for c, nb in enumerate(lengths):
+        m = fene_wca_chain(nb, 0)
+        m["crd"] = chains[c]
+        so = oracle_system(oracle, m)
+        ref, used_c, _ = oracle.bd_run(so, chains[c], nsteps, per[c], mob_fctr=oracle.LANC, nts_mobsam=s,
+                                       lanc_mxitr=mx, lanc_tol=0.0)
+        assert used_c == per[c].size
+        assert rel(out[c], ref) < tol, (c, nb, rel(out[c], ref))
+    info = b.info()
+    assert info["nts"] == nsteps and info["mob_updates"] == nupd
+    assert info["total_lanczos_iters"] == nupd * len(lengths) * (mx + 1)
+
+
+def test_batch_converged_noise_accuracy(oracle):
+    """One step; the displacement minus the exact drift is the Brownian term."""
+    lengths = [30, 55, 21, 80]
+    s, tol = 4, 1e-6
+    _say("case: batch converged noise")
+    xi = np.random.default_rng(11).standard_normal(sum(3 * nb * s for nb in lengths))
+    b, chains = _batch(lengths, 500, nts_mobsam=s, lanc_mxitr=60, lanc_tol=tol, tim_stp=0.5)
+    # keep_reuse_noise off: steps 2..s of a block carry no noise; one step is enough
+    before = [c - c.mean(axis=0) for c in chains]
+    b.run(1, xi)
+    after = b.get_coordinates()
+    per = _split_xi(lengths, s, 1, xi)
+    for c, nb in enumerate(lengths):
+        n = 3 * nb
+        m = fene_wca_chain(nb, 0)
+        m["crd"] = chains[c]
+        so = oracle_system(oracle, m)
+        D = oracle.rpy(chains[c])
+        w, U = np.linalg.eigh(D)
+        X = per[c].reshape(s, n).T
+        exact = (U * np.sqrt(w)) @ (U.T @ X)
+        # drift of this step from the oracle's bond + pair forces
+        f = oracle.forces(so, chains[c], oracle.pairs(so, chains[c], "DIR"))[0].ravel()
+        disp = after[c].ravel() - before[c].ravel()
+        brown = np.sqrt(2 * 0.5) * exact[:, 0]
+        expect = 0.5 * (D @ f) + brown
+        expect = expect.reshape(nb, 3)
+        expect -= expect.mean(axis=0)
+        err = np.linalg.norm(disp - expect.ravel()) / np.linalg.norm(brown)
+        assert err < 10 * tol, (c, err)
+    assert b.info()["n_not_converged"] == 0
+
+
+def test_batch_chain_independence_ensemble():
+    """Lengths U[50, 800] (seeded), FP32 tiles, s = 4, tol 1e-6, device RNG
+    replaced by an injected stream so single runs see the same numbers."""
+    rng = np.random.default_rng(2024)
+    lengths = [int(x) for x in rng.integers(50, 801, size=24)]
+    s, nsteps = 4, 8
+    _say(f"case: batch independence {len(lengths)} chains, {sum(lengths)} beads")
+    tot = sum(3 * nb * s for nb in lengths) * (nsteps // s)
+    xi = rng.standard_normal(tot)
+    b, chains = _batch(lengths, 900, nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32")
+    b.run(nsteps, xi)
+    out = b.get_coordinates()
+    assert all(np.isfinite(o).all() for o in out)
+    for o in out:
+        assert np.abs(o.mean(axis=0)).max() < 1e-9
+    per = _split_xi(lengths, s, nsteps // s, xi)
+    for c in (0, 7, len(lengths) - 1):
+        one, _ = _batch([lengths[c]], 900 + c, nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32")
+        one.run(nsteps, per[c])
+        assert np.array_equal(one.get_coordinates()[0], out[c]), c
+
+
+def test_batch_overstretch_reports_bond_error():
+    from brownpak_amd import BdError
+    lengths = [20, 30]
+    b, chains = _batch(lengths, 77, nts_mobsam=1, lanc_mxitr=20, lanc_tol=1e-6)
+    chains[1][15:] += np.array([6.0, 0.0, 0.0])
+    b.set_coordinates(chains)
+    xi = np.random.default_rng(5).standard_normal(3 * sum(lengths) * 2)
+    with pytest.raises(BdError) as ei:
+        b.run(2, xi)
+    assert ei.value.code == 1
+
+
+def test_batch_rejects_unsupported_options():
+    from brownpak_amd import BdBatch, BdError
+    with pytest.raises(BdError):
+        BdBatch([10, 20], nts_mobsam=6)
+    with pytest.raises(BdError):
+        BdBatch([1, 20])

@@ -267,6 +267,39 @@ def test_bd_run_free_draining_periodic(oracle):
     assert rel(out, ref) < TOL
 
 
+@pytest.mark.parametrize("case", ["chol_c1", "lanczos_s4", "periodic_ver"])
+def test_bd_run_predictor_corrector(oracle, chain100, case):
+    """Predictor-corrector step (not in the reference) against the oracle's
+    restatement of it: same injected xi, FP64, 1e-10 relative."""
+    _say(f"case: bd_run predictor-corrector {case}")
+    if case == "periodic_ver":
+        m = _melt(10, 50, 44)
+        nsteps, nx = 10, 1500 * 10
+        kw, okw = dict(lhdia=False), dict(lhdia=False)
+        so = oracle_system(oracle, m, "VER")
+    else:
+        m = chain100
+        so = oracle_system(oracle, m)
+        if case == "chol_c1":
+            nsteps, nx = 12, 300 * 12
+            kw = dict(mob_fctr="CHOL", nts_mobsam=1)
+            okw = dict(mob_fctr=oracle.CHOL, nts_mobsam=1)
+        else:
+            nsteps, nx = 8, 300 * 4 * 2
+            kw = dict(mob_fctr="LANC", nts_mobsam=4, lanc_reorth=True, keep_reuse_noise=True)
+            okw = dict(mob_fctr=oracle.LANC, nts_mobsam=4, lanc_reorth=True, keep_reuse_noise=True)
+    xi = np.random.default_rng(44).standard_normal(nx)
+    g = gpu_system(m, integrator="PC", **kw)
+    used = g.bds_run(nsteps, xi)
+    out = g.get_coordinates()
+    ref, used2, _ = oracle.bd_run(so, m["crd"], nsteps, xi, integrator=oracle.PC, **okw)
+    assert used == used2
+    assert rel(out, ref) < (1e-9 if case == "lanczos_s4" else TOL)
+    # and it is not the EM trajectory
+    em, _, _ = oracle.bd_run(so, m["crd"], nsteps, xi, **okw)
+    assert rel(em, ref) > 1e3 * rel(out, ref)
+
+
 def test_bond_overstretch_reports_ierr(chain100):
     from brownpak_amd import BdError
     m = dict(chain100)

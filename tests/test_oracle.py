@@ -431,6 +431,69 @@ def test_hi_step_closed_form_and_reuse_quirk(oracle):
     assert rel(out2, c2_keep) < 1e-12
 
 
+# ------------------------------------------------------------ predictor-corrector
+# Not in the reference: pinned by what Heun's method fixes for linear drifts
+# (its amplification factor 1 + h a + (h a)^2 / 2), by its order, and by the
+# Brownian term entering the predictor once and the update once.
+def test_pc_linear_spring_heun_factor(oracle):
+    """Harmonic dimer (r0 = 0), free draining, no noise: the separation obeys
+    d' = -2k d, so one step multiplies it by 1 - 2kh + 2(kh)^2 (EM: 1 - 2kh)."""
+    k, h, nst = 5.0, 0.01, 7
+    sysobj = oracle.System(2, [[1, 1, 2]], [1], [[k, 0.0]], vdw_styles=(), lvdw=False)
+    crd = np.array([[0.3, -0.2, 0.5], [1.1, 0.4, -0.7]])
+    d0 = crd[1] - crd[0]
+    xi = np.zeros(6 * nst)
+    for integ, g in ((oracle.EM, 1 - 2 * k * h), (oracle.PC, 1 - 2 * k * h + 2 * (k * h) ** 2)):
+        out, used, info = oracle.bd_run(sysobj, crd, nst, xi, lhdia=False, tim_stp=h, integrator=integ)
+        assert used == 6 * nst and info.nts == nst
+        assert np.allclose(out[1] - out[0], d0 * g ** nst, rtol=1e-13, atol=0)
+        assert np.allclose(out.mean(0), 0.0, atol=1e-15)
+
+
+def test_pc_planar_flow_with_noise_closed_form(oracle):
+    """Planar elongation (x' = e x, y' = -e y), no forces, one step with
+    noise w: x1 = x (1 + eh + (eh)^2/2) + w (1 + eh/2), the extra w eh/2
+    coming from the predictor's drift (EM: x (1 + eh) + w)."""
+    e, h, nb = 3.0, 0.02, 4
+    sysobj = oracle.System(nb, [], [], None, vdw_styles=(), lvdw=False)
+    crd = np.random.default_rng(40).standard_normal((nb, 3))
+    xi = np.random.default_rng(41).standard_normal(3 * nb)
+    w = np.sqrt(2 * h) * xi.reshape(nb, 3)
+    c0 = crd - crd.mean(0)
+    a = np.array([e, -e, 0.0]) * h
+    for integ in (oracle.EM, oracle.PC):
+        out, _, _ = oracle.bd_run(sysobj, crd, 1, xi, lhdia=False, tim_stp=h, integrator=integ,
+                                  flow_style=2, flow_rate=e)
+        if integ == oracle.EM:
+            expect = c0 * (1 + a) + w
+        else:
+            expect = c0 * (1 + a + a * a / 2) + w * (1 + a / 2)
+        expect -= expect.mean(0)
+        assert rel(out, expect) < 1e-13, integ
+
+
+def test_pc_second_order_fene_chain(oracle):
+    """Deterministic FENE chain (smooth forces), free draining: halving dt
+    divides the PC error at a fixed time by ~4 (EM: ~2).  With RPY the
+    mobility is held over the step (as over nts_mobsam steps), so the error
+    from D varying along the step stays first order, as in EM."""
+    m = fene_wca_chain(8, 42)
+    sysobj = oracle.System(8, m["bonds"], m["bond_styles"], m["bond_params"], vdw_styles=(), lvdw=False)
+    T = 0.08
+    def run(integ, h):
+        nst = int(round(T / h))
+        n = 3 * len(m["crd"])
+        out, _, _ = oracle.bd_run(sysobj, m["crd"], nst, np.zeros(n * nst), lhdia=False,
+                                  tim_stp=h, integrator=integ)
+        return out
+    ref = run(oracle.PC, T / 6400)
+    for integ, order in ((oracle.PC, 2), (oracle.EM, 1)):
+        e1 = np.linalg.norm(run(integ, T / 80) - ref)
+        e2 = np.linalg.norm(run(integ, T / 160) - ref)
+        ratio = e1 / e2
+        assert abs(np.log2(ratio) - order) < 0.25, (integ, ratio)
+
+
 def test_periodic_wrap(oracle):
     crd, L = periodic_melt(2, 10, 0.3, 22)
     nb = len(crd)
