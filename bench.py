@@ -47,8 +47,9 @@ def build_system(precision, seed=2000, reorth=False):
 
 def time_path(precision, steps, warmup):
     """Steps/s of the plain run, then the same number of steps again with
-    HIP events around every tile-product launch (the events add dependency
-    points to the stream, so the timed run goes without them)."""
+    HIP events around every Lanczos tile-product launch (block width s; the
+    events add dependency points to the stream, so the timed run goes
+    without them)."""
     import torch
     g, _ = build_system(precision)
     g.bds_run(warmup)
@@ -60,7 +61,7 @@ def time_path(precision, steps, warmup):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     info = g.runinfo()
-    g.prof_enable(True)
+    g.prof_enable(True, width=S_BLOCK)
     g.bds_run(steps)
     g.sync()
     kms, launches, tile_bytes = g.prof_get()
@@ -141,6 +142,33 @@ def cpu_baseline(budget_s=20.0):
                        f"block Lanczos s=4 tol 1e-6, FP64, oracle/bdoracle.c at 1 thread, {dt:.1f} s")
 
 
+def measured_traffic():
+    """HBM bytes per launch of symm_tiles_kernel<float, 4> from the two
+    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in profiles/,
+    already scaled by the copy-kernel calibration; None until measured."""
+    path = os.path.join(ROOT, "profiles", "symm_tiles_n2000_s4_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch_fp32:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def calibration_copy(mib=256, reps=3):
+    """A plain device copy of known size (mib MiB read + mib MiB written) for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of the profiled runs."""
+    import torch
+    a = torch.ones(mib * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"# calibration copy: {reps} x {mib} MiB", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +177,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
     ap.add_argument("--states", type=int, default=10, help="seeded states of the accuracy report")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="end with a 256 MiB device copy (counter calibration of profiled runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
@@ -168,19 +198,19 @@ def main():
     n = 3 * N_BEADS
 
     def roof(run, elem):
-        # algorithmic bytes of the tile-product launches of the timed region:
-        # each working launch streams the lower triangle of D (n(n+1)/2
-        # entries of elem bytes) and reads and writes an n x s block (s = 4
-        # for a Lanczos iteration, 1 for the drift).  Launches enqueued after
-        # a solve converged exit at once; they count in the time, not in the
-        # bytes, exactly as in a rocprof average over all launches.
-        n_lz, n_dr = run["prof_lanczos_iters"], steps
-        total = (n_lz + n_dr) * elem * n * (n + 1) / 2 + (n_lz * 2 * n * S_BLOCK + n_dr * 2 * n) * 8
+        # algorithmic bytes of the Lanczos tile-product launches (block width
+        # s = 4) of the profiled run: each working launch streams the lower
+        # triangle of D (n(n+1)/2 entries of elem bytes) and reads and writes
+        # an n x s FP64 block.  Launches enqueued after a solve converged
+        # exit at once; they count in the time, not in the bytes, exactly as
+        # in a rocprof average over all launches of the kernel.
+        n_lz = run["prof_lanczos_iters"]
+        per = elem * n * (n + 1) / 2 + 2 * n * S_BLOCK * 8
         t = run["kernel_ms"] * 1e-3
         launches = max(run["launches"], 1)
-        return dict(achieved=total / t / 1e9, avg_launch_us=t / launches * 1e6,
-                    bytes_per_launch=total / launches, launches=launches,
-                    empty_launches=launches - n_lz - n_dr)
+        return dict(achieved=n_lz * per / t / 1e9, avg_launch_us=t / launches * 1e6,
+                    bytes_per_launch=n_lz * per / launches, bytes_per_working_launch=per,
+                    launches=launches, empty_launches=launches - n_lz)
 
     rm, r64 = roof(mixed, 4), roof(fp64, 8)
     achieved = rm["achieved"]
@@ -206,10 +236,12 @@ def main():
         "accuracy": acc,
         "lanczos_iters_per_update": round(mixed["lanczos_iters"] / max(mixed["mob_updates"], 1), 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "symm_tiles_kernel<float,S>",
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(),
+                     "kernel": "symm_tiles_kernel<float, 4>",
                      "avg_launch_us": round(rm["avg_launch_us"], 2),
                      "bytes_per_launch": round(rm["bytes_per_launch"]),
+                     "bytes_per_working_launch": round(rm["bytes_per_working_launch"]),
                      "launches": rm["launches"], "empty_launches": rm["empty_launches"],
                      "fp64_achieved": round(r64["achieved"], 1),
                      "fp64_avg_launch_us": round(r64["avg_launch_us"], 2)},
@@ -218,6 +250,8 @@ def main():
     if cpu:
         out["vs_cpu_fp64"] = round(fp64["steps_per_s"] / cpu["value"], 1)
     print(json.dumps(out), flush=True)
+    if args.calibration_copy:
+        calibration_copy()
 
 
 if __name__ == "__main__":

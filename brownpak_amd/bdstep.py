@@ -346,8 +346,10 @@ class BdStep:
                                          C.byref(loss), _dp(g)))
         return loss.value, g
 
-    def prof_enable(self, on=True):
-        self._chk(self.L.bd_prof_enable(self.h, int(bool(on))))
+    def prof_enable(self, on=True, width=0):
+        """Bracket tile-product launches with HIP events (width > 0: only the
+        launches of blocks with that many columns)."""
+        self._chk(self.L.bd_prof_enable(self.h, (1 + int(width) if width else 1) if on else 0))
 
     def prof_get(self):
         """(summed tile-product kernel ms, launches, stored tile bytes per launch)."""

@@ -122,11 +122,11 @@ __global__ void update_kernel(double *__restrict__ crd, const double *__restrict
 }
 
 // Predictor of the predictor-corrector step: r0 <- r, d0 <- d and
-// r <- (r + d) + w.  No periodic wrap here: the drift at the predictor sees
-// the same absolute positions (flow field) as the reference's r would.
+// r <- (r + d) + w, wrapped into the box when periodic (as after a step, so
+// that the cell-list pair table sees coordinates in [0, L)).
 __global__ void pc_predict_kernel(double *__restrict__ crd, const double *__restrict__ drift,
                                   const double *__restrict__ w, double *__restrict__ r0, double *__restrict__ d0, int n,
-                                  const DevStatus *st)
+                                  int imcon, double Lx, double Ly, double Lz, const DevStatus *st)
 {
     if (st->ierr) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,6 +136,10 @@ __global__ void pc_predict_kernel(double *__restrict__ crd, const double *__rest
     d0[i] = d;
     double p = r + d;
     if (w) p = p + w[i];
+    if (imcon) {
+        const double L = (i % 3 == 0) ? Lx : ((i % 3 == 1) ? Ly : Lz);
+        p = p - L * floor(p / L);   // smbx_wrap_all, simbox_m.f90:128-153
+    }
     crd[i] = p;
 }
 
@@ -251,6 +255,7 @@ int check_status(Ctx &c, bool clear)
     }
     if (e == 1) { set_error("bond length too large (overstretched bond)"); return BD_ERR_BOND; }
     if (e == 2) { set_error("factorisation failed"); return BD_ERR_FACTOR; }
+    if (e == 4) { set_error("negative coordinate in cell-list mode (cell_list_m.f90:193-196)"); return BD_ERR_CELL; }
     return BD_OK;
 }
 
@@ -348,7 +353,7 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
     if (c.opt.integrator == BD_PC) {
         const int nbn = (n + 255) / 256;
         hipLaunchKernelGGL(pc_predict_kernel, dim3(nbn), dim3(256), 0, sm, c.crd, c.drift, dcol, c.pc_r0, c.pc_d0,
-                           n, c.st);
+                           n, c.imcon, c.box[0], c.box[1], c.box[2], c.st);
         BD_TRY(drift_at_crd());
         hipLaunchKernelGGL(pc_correct_kernel, dim3(nbn), dim3(256), 0, sm, c.crd, c.drift, c.pc_r0, c.pc_d0, n,
                            c.st);
@@ -689,7 +694,9 @@ int bd_prof_enable(bd_ctx *h, int on)
     if (!h) { set_error("null context"); return BD_ERR_ARG; }
     Ctx &c = h->c;
     prof_flush(c);
+    if (on < 0 || on > 1 + MAX_S) { set_error("prof: on must be 0, 1 or 1 + block width"); return BD_ERR_ARG; }
     c.prof_on = on != 0;
+    c.prof_width = on > 1 ? on - 1 : 0;
     c.prof_ms = 0.0;
     c.prof_count = 0;
     return BD_OK;

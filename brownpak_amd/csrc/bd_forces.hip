@@ -324,6 +324,95 @@ __global__ __launch_bounds__(256) void nbr_cell_kernel(const double *__restrict_
     save[3 * i + 2] = crd[3 * i + 2];
 }
 
+// Beads of every cell in ascending index order (the fill above places them
+// in atomic order); one thread per cell, insertion sort of a few dozen.
+// Pair lists, and so the force sums, are then the same run to run.
+__global__ void cell_sort_kernel(const int *__restrict__ cell_ptr, int ncell, int *__restrict__ cell_beads,
+                                 int check_flag, const DevStatus *st)
+{
+    if (check_flag && !st->rebuild) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int b = cell_ptr[c], e = cell_ptr[c + 1];
+    for (int p = b + 1; p < e; p++) {
+        const int v = cell_beads[p];
+        int q = p - 1;
+        while (q >= b && cell_beads[q] > v) {
+            cell_beads[q + 1] = cell_beads[q];
+            q--;
+        }
+        cell_beads[q + 1] = v;
+    }
+}
+
+// cl_build cell index (cell_list_m.f90:186-204): cells of side
+// cs = L / floor(L / rc), index int(x / cs) truncated and clamped to nc - 1;
+// a coordinate with x / cs <= -1 (or NaN) is an error there (the
+// reference's cell lists need x >= 0), flagged here as ierr 4.
+__global__ void cl_assign_kernel(const double *__restrict__ crd, int nb, double csx, double csy, double csz,
+                                 int ncx, int ncy, int ncz, int *__restrict__ bead_cell, int *__restrict__ cell_cnt,
+                                 DevStatus *st)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const double cs[3] = {csx, csy, csz};
+    const int nc[3] = {ncx, ncy, ncz};
+    int ic[3];
+    for (int d = 0; d < 3; d++) {
+        const double q = crd[3 * i + d] / cs[d];
+        if (!(q > -1.0)) { atomicExch(&st->ierr, 4); ic[d] = 0; continue; }
+        int k = (int)q;   // truncation toward zero, as Fortran int()
+        if (k > nc[d] - 1) k = nc[d] - 1;
+        ic[d] = k;
+    }
+    const int cell = (ic[2] * ncy + ic[1]) * ncx + ic[0];
+    bead_cell[i] = cell;
+    atomicAdd(&cell_cnt[cell], 1);
+}
+
+// pt_build CL (pairtab_m.fpp:305-385): every pair of beads in one cell, and
+// every pair between a cell and each of its 13 "forward" neighbours
+// (cl_build_cell_nbrs, cell_list_m.f90:206-228), with no distance test and
+// with the repeats a grid of fewer than 3 cells per side produces.  As a
+// full list: for bead i in cell c, the other beads of c, the beads of
+// c + d_k and the beads of c - d_k for the 13 offsets d_k, each time it
+// occurs, exclusions skipped.
+__constant__ int c_cl_dst[13][3] = {{1, 0, 0},  {1, 1, 0},  {-1, 1, 0}, {0, 1, 0},  {0, 0, 1},
+                                    {-1, 0, 1}, {1, 0, 1},  {-1, -1, 1}, {0, -1, 1}, {1, -1, 1},
+                                    {-1, 1, 1}, {0, 1, 1},  {1, 1, 1}};
+
+__global__ __launch_bounds__(256) void cl_nbr_kernel(int nb, int ncx, int ncy, int ncz,
+                                                     const int *__restrict__ bead_cell, const int *__restrict__ cell_ptr,
+                                                     const int *__restrict__ cell_beads, const int *__restrict__ ex_ptr,
+                                                     const int *__restrict__ ex_col, int *__restrict__ nbr_cnt,
+                                                     int *__restrict__ nbr_idx, int cap, DevStatus *st)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const int c = bead_cell[i];
+    const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+    int cnt = 0;
+    auto take = [&](int jc, bool same) {
+        for (int p = cell_ptr[jc]; p < cell_ptr[jc + 1]; p++) {
+            const int j = cell_beads[p];
+            if (same && j == i) continue;
+            if (excluded(ex_ptr, ex_col, i, j)) continue;
+            if (cnt < cap) nbr_idx[(long)cnt * nb + i] = j;
+            cnt++;
+        }
+    };
+    take(c, true);
+    for (int sgn = 1; sgn >= -1; sgn -= 2)
+        for (int k = 0; k < 13; k++) {
+            const int jx = ((cx + sgn * c_cl_dst[k][0]) % ncx + ncx) % ncx;
+            const int jy = ((cy + sgn * c_cl_dst[k][1]) % ncy + ncy) % ncy;
+            const int jz = ((cz + sgn * c_cl_dst[k][2]) % ncz + ncz) % ncz;
+            take((jz * ncy + jy) * ncx + jx, false);
+        }
+    nbr_cnt[i] = cnt;
+    if (cnt > cap) atomicExch(&st->overflow, 1);
+}
+
 __global__ void zero_int_kernel(int *p, int n, int check_flag, const DevStatus *st)
 {
     if (check_flag && !st->rebuild) return;
@@ -651,6 +740,28 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const in
     c.pairs_valid = false;
     // periodic cell grid for the Verlet build (cells of at least rc + skin)
     c.ncell_tot = 0;
+    if (c.mth_ptgen == BD_PT_CL && c.lvdw && nvdw_types > 0) {
+        // cl_set_cell_size (cell_list_m.f90:91-132): nc = floor(L / rc)
+        if (!c.imcon) { set_error("mth_ptgen CL needs a periodic box (imcon 1)"); return BD_ERR_ARG; }
+        int nc[3];
+        for (int d = 0; d < 3; d++) nc[d] = (int)std::floor(c.box[d] / c.rcutoff);
+        if (nc[0] < 2 || nc[1] < 2 || nc[2] < 2) {
+            // one cell along an axis would pair every bead with itself
+            set_error("mth_ptgen CL needs at least 2 cells of rcutoff per box edge");
+            return BD_ERR_ARG;
+        }
+        for (int d = 0; d < 3; d++) {
+            c.ncell[d] = nc[d];
+            c.cl_cs[d] = c.box[d] / nc[d];
+        }
+        c.ncell_tot = nc[0] * nc[1] * nc[2];
+        if (c.cell_cnt) (void)hipFree(c.cell_cnt);
+        if (c.cell_ptr) (void)hipFree(c.cell_ptr);
+        if (!c.cell_beads) BD_HIP(hipMalloc(&c.cell_beads, (size_t)nb * sizeof(int)));
+        if (!c.bead_cell) BD_HIP(hipMalloc(&c.bead_cell, (size_t)nb * sizeof(int)));
+        BD_HIP(hipMalloc(&c.cell_cnt, (size_t)c.ncell_tot * sizeof(int)));
+        BD_HIP(hipMalloc(&c.cell_ptr, (size_t)(c.ncell_tot + 1) * sizeof(int)));
+    }
     if (c.imcon && c.mth_ptgen == BD_PT_VER) {
         double rl = c.rcutoff + c.tskin;
         int nc[3];
@@ -705,10 +816,6 @@ static int build_pairs(Ctx &c, bool sync)
     const int nb = c.nb;
     hipStream_t sm = c.stream;
     BoxArg bx = box_arg(c);
-    if (c.mth_ptgen == BD_PT_CL) {
-        set_error("mth_ptgen CL is not implemented on the device; use DIR or VER");
-        return BD_ERR_ARG;
-    }
     const bool ver = (c.mth_ptgen == BD_PT_VER);
     const double rl = ver ? c.rcutoff + c.tskin : c.rcutoff;
     const double rc2 = rl * rl;
@@ -717,8 +824,22 @@ static int build_pairs(Ctx &c, bool sync)
             hipLaunchKernelGGL(verlet_check_kernel, dim3(1), dim3(1024), 0, sm, c.crd, c.crd_save, nb, bx,
                                c.tskin * c.tskin, c.verlet_first ? 1 : 0, c.st);
         else
-            hipLaunchKernelGGL(set_rebuild_kernel, dim3(1), dim3(1), 0, sm, c.st);  // DIR: every call
-        if (ver && c.ncell_tot > 0) {
+            hipLaunchKernelGGL(set_rebuild_kernel, dim3(1), dim3(1), 0, sm, c.st);  // DIR, CL: every call
+        if (c.mth_ptgen == BD_PT_CL) {
+            const int nbk = (nb + 255) / 256, nck = (c.ncell_tot + 255) / 256;
+            hipLaunchKernelGGL(zero_int_kernel, dim3(nck), dim3(256), 0, sm, c.cell_cnt, c.ncell_tot, 0, c.st);
+            hipLaunchKernelGGL(cl_assign_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, nb, c.cl_cs[0], c.cl_cs[1],
+                               c.cl_cs[2], c.ncell[0], c.ncell[1], c.ncell[2], c.bead_cell, c.cell_cnt, c.st);
+            hipLaunchKernelGGL(cell_scan_kernel, dim3(1), dim3(1024), 0, sm, c.cell_cnt, c.cell_ptr, c.ncell_tot, 0,
+                               c.st);
+            hipLaunchKernelGGL(cell_fill_kernel, dim3(nbk), dim3(256), 0, sm, c.bead_cell, nb, c.cell_ptr, c.cell_cnt,
+                               c.cell_beads, 0, c.st);
+            hipLaunchKernelGGL(cell_sort_kernel, dim3(nck), dim3(256), 0, sm, c.cell_ptr, c.ncell_tot, c.cell_beads, 0,
+                               c.st);
+            hipLaunchKernelGGL(cl_nbr_kernel, dim3(nbk), dim3(256), 0, sm, nb, c.ncell[0], c.ncell[1], c.ncell[2],
+                               c.bead_cell, c.cell_ptr, c.cell_beads, c.ex_ptr, c.ex_col, c.nbr_cnt, c.nbr_idx,
+                               c.pair_cap, c.st);
+        } else if (ver && c.ncell_tot > 0) {
             int nbk = (nb + 255) / 256;
             hipLaunchKernelGGL(zero_int_kernel, dim3((c.ncell_tot + 255) / 256), dim3(256), 0, sm, c.cell_cnt,
                                c.ncell_tot, 1, c.st);
@@ -728,6 +849,8 @@ static int build_pairs(Ctx &c, bool sync)
                                c.st);
             hipLaunchKernelGGL(cell_fill_kernel, dim3(nbk), dim3(256), 0, sm, c.bead_cell, nb, c.cell_ptr, c.cell_cnt,
                                c.cell_beads, 1, c.st);
+            hipLaunchKernelGGL(cell_sort_kernel, dim3((c.ncell_tot + 255) / 256), dim3(256), 0, sm, c.cell_ptr,
+                               c.ncell_tot, c.cell_beads, 1, c.st);
             hipLaunchKernelGGL(nbr_cell_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, nb, bx, rc2, c.ncell[0], c.ncell[1],
                                c.ncell[2], c.bead_cell, c.cell_ptr, c.cell_beads, c.ex_ptr, c.ex_col, c.nbr_cnt,
                                c.nbr_idx, c.pair_cap, c.crd_save, c.st);

@@ -48,7 +48,7 @@ void set_error(const std::string &msg);
 
 // Device-side status words (one struct in device memory per context).
 struct DevStatus {
-    int ierr;          // 1: bond overstretched; 2: factorisation failed
+    int ierr;          // 1: bond overstretched; 2: factorisation failed; 4: cell-list coordinate < 0
     int converged_j;   // Lanczos: first j with err <= tol (0 = not yet)
     int last_j;        // last Lanczos iteration executed
     int rebuild;       // Verlet list rebuild flag for the current step
@@ -173,6 +173,7 @@ struct Ctx {
     bool verlet_first = true;
     int *cell_cnt = nullptr, *cell_ptr = nullptr, *cell_beads = nullptr, *bead_cell = nullptr;
     int ncell[3] = {0, 0, 0};
+    double cl_cs[3] = {0, 0, 0};   // CL cell sides L / nc
     int ncell_tot = 0;
     int *scan_tmp = nullptr;
     int last_rebuilt = 0;
@@ -192,6 +193,7 @@ struct Ctx {
 
     // kernel timing of the tile product (bd_prof_*)
     bool prof_on = false;
+    int prof_width = 0;                // 0: every launch, else launches of this block width
     std::vector<hipEvent_t> prof_ev;   // pairs (start, stop)
     size_t prof_used = 0;
     double prof_ms = 0.0;

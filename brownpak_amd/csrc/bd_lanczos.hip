@@ -584,7 +584,7 @@ __global__ __launch_bounds__(sp::WG_T) void small_split_kernel(const double *__r
     if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
     __syncthreads();
     if (s_done) return;
-    const bool fin = sp::small_problem_split<S>(Ablk, Bblk, R0, nb, NBOUND, 13, part, Cout, ss);
+    const bool fin = sp::small_problem_split<S>(Ablk, Bblk, R0, nb, NBOUND, 13, part, Cout, ss, ss, 1, nb & 1);
     if (fin && threadIdx.x == 0) {
         st->lam_hi = ss->lam_hi;
         st->lam_lo = ss->lam_lo;
@@ -618,7 +618,7 @@ __global__ __launch_bounds__(sp::WG_T) void small_group_kernel(const double *__r
     if (s_done || j > jend || j < 2) return;
     sp::SplitState *ss = ssv + blockIdx.y;
     const bool fin = sp::small_problem_split<S>(Ablk, Bblk, R0, j, NBOUND, 13, part + blockIdx.y * part_stride,
-                                                Cg + blockIdx.y * c_stride, ss);
+                                                Cg + blockIdx.y * c_stride, ss, ssv, LG, ((j0 - 1) / LG) & 1);
     if (fin && threadIdx.x == 0) {
         st->lam_hi = ss->lam_hi;
         st->lam_lo = ss->lam_lo;
@@ -888,7 +888,7 @@ int lanczos_alloc(Ctx &c)
     BD_HIP(hipMalloc(&c.split, LG * sizeof(sp::SplitState)));
     {
         sp::SplitState z[LG];
-        for (int g = 0; g < LG; g++) z[g] = sp::SplitState{0, -1, 0, 0, 0.0, 0.0};
+        for (int g = 0; g < LG; g++) z[g] = sp::SplitState{0, {-1, -1}, 0, 0, 0.0, 0.0};
         BD_HIP(hipMemcpy(c.split, z, sizeof(z), hipMemcpyHostToDevice));
     }
     // combine partials: one set of 2 s norms per block, per group member

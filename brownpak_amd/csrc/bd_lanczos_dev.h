@@ -547,20 +547,50 @@ __device__ SmallOut small_problem(const double *__restrict__ A, const double *__
 // wave 0 tests the spectral probes while wave 1 solves this workgroup's
 // share of the quadrature nodes, with the band of H_j and every node's
 // factor blocks in LDS (so the sequential block sweeps see LDS, not global,
-// latency).  The nodes are placed with the lower bound of the previous
-// iteration (a hint); when the probes find another bound the nodes are
-// solved again.  Each workgroup leaves the weighted sum of its nodes'
-// solutions in part[wg]; the last one to finish sums the parts in
-// workgroup order and forms C = H Z.
+// latency).  The nodes are placed while the probes run, from the largest
+// probe index the previous launch found (the smallest Ritz value only
+// decreases with j) plus one halving of margin: a placement whose range
+// [m, M] contains the spectrum is as accurate as the tight one, so the
+// nodes are solved again only when the probes find the spectrum below the
+// placed range.  Hints are double-buffered by launch parity (read kb[p ^ 1],
+// write kb[p]), so the placement does not depend on the order in which the
+// members of one launch finish.  Each workgroup leaves the weighted sum of
+// its nodes' solutions in part[wg]; the last one to finish sums the parts
+// in workgroup order and forms C = H Z.
 template <int S>
 constexpr int WG_NBM = 208 / S;      // largest H_j block count served (order <= 208)
 template <int S>
 constexpr int WG_NPB = S == 1 ? 16 : (S == 2 ? 10 : (S == 3 ? 8 : 6));   // nodes per workgroup
 constexpr int WG_T = 128;            // threads: probes in wave 0, nodes in wave 1
 
+// Row i of H (block tridiagonal: A_b symmetric, lower triangle stored;
+// H(bS + r, (b-1)S + k) = B_{b-1}(r, k); H(bS + r, (b+1)S + k) = B_b(k, r))
+// times the column z, in the order: previous block, diagonal, next block.
+template <int S>
+__device__ inline double block_row_dot(const double *__restrict__ A, const double *__restrict__ B, int nb, int i,
+                                       const double *__restrict__ z)
+{
+    const int b = i / S, r = i % S;
+    const double *Ab = A + b * S * S;
+    double v = 0.0;
+    if (b > 0) {
+        const double *Bp = B + (b - 1) * S * S;
+#pragma unroll
+        for (int k = 0; k < S; k++) v += Bp[r + k * S] * z[(b - 1) * S + k];
+    }
+#pragma unroll
+    for (int k = 0; k < S; k++) v += (k <= r ? Ab[r + k * S] : Ab[k + r * S]) * z[b * S + k];
+    if (b + 1 < nb) {
+        const double *Bb = B + b * S * S;
+#pragma unroll
+        for (int k = 0; k < S; k++) v += Bb[k + r * S] * z[(b + 1) * S + k];
+    }
+    return v;
+}
+
 struct SplitState {
     int cnt;        // workgroups finished (the last resets it)
-    int kb_hint;    // probe index of the previous lower bound (-1: none)
+    int kb[2];      // probe index of the lower bound, by launch parity (-1: none)
     int nq, err;
     double lam_lo, lam_hi;
 };
@@ -570,7 +600,8 @@ struct SplitState {
 template <int S>
 __device__ bool small_problem_split(const double *__restrict__ Ag, const double *__restrict__ Bg,
                                     const double *__restrict__ R0, int nb, int nbound, int dec_max,
-                                    double *__restrict__ part, double *__restrict__ Cout, SplitState *ss)
+                                    double *__restrict__ part, double *__restrict__ Cout, SplitState *ss,
+                                    const SplitState *hints, int nhints, int parity)
 {
     constexpr int E = NE<S>, NBM = WG_NBM<S>, NPB = WG_NPB<S>;
     __shared__ double sA[NBM * S * S], sB[NBM * S * S], sR[S * S];
@@ -597,8 +628,9 @@ __device__ bool small_problem_split(const double *__restrict__ Ag, const double 
         h = c_qh[min(dec, 13)];
         return dec;
     };
-    const int hint = ss->kb_hint;
-    const int kb_guess = (hint >= 0 && hint < nbound) ? hint : 0;
+    int hint = -1;
+    for (int g = 0; g < nhints; g++) hint = max(hint, hints[g].kb[parity ^ 1]);
+    const int kb_guess = hint >= 0 ? min(hint + 1, nbound - 1) : 0;
     auto solve_nodes = [&](int kb) {
         double m, h;
         int N;
@@ -627,15 +659,23 @@ __device__ bool small_problem_split(const double *__restrict__ Ag, const double 
     if (kb < 0) err = 2;
     double m = 0.0, h = 0.0;
     int N = 0;
-    const int dec = kb < 0 ? 0 : place(kb, m, N, h);
+    // the guessed placement stands when its range reaches below the bound
+    // and its node count is in the table
+    int use = kb;
+    if (kb >= 0 && kb <= kb_guess) {
+        double mg, hg;
+        int Ng;
+        if (place(kb_guess, mg, Ng, hg) <= dec_max) use = kb_guess;
+    }
+    const int dec = kb < 0 ? 0 : place(use, m, N, h);
     if (!err && dec > dec_max) err = 3;
     const int nq = 2 * N + 1;
     const int nact = err ? 1 : (nq + NPB - 1) / NPB;
     if ((int)blockIdx.x >= nact) return false;   // no nodes here: not counted
-    if (!err && kb != kb_guess) {
+    if (!err && use != kb_guess) {
         if (t == 0) s_fail = 0;
         __syncthreads();
-        if (wv == 1) solve_nodes(kb);
+        if (wv == 1) solve_nodes(use);
         __syncthreads();
     }
     if (s_fail && !err) err = 2;
@@ -656,7 +696,7 @@ __device__ bool small_problem_split(const double *__restrict__ Ag, const double 
     __threadfence();
     if (t == 0) {
         ss->cnt = 0;
-        ss->kb_hint = kb;
+        ss->kb[parity] = kb;
         ss->nq = nq;
         ss->err = err;
         ss->lam_lo = m;
@@ -673,12 +713,7 @@ __device__ bool small_problem_split(const double *__restrict__ Ag, const double 
     __syncthreads();
     for (int e = t; e < js * S; e += WG_T) {
         const int i = e % js, c = e / js;
-        double v = 0.0;
-        for (int l = max(0, i - S); l <= min(js - 1, i + S); l++) {
-            double hv = (i >= l) ? h_entry<S>(sA, sB, js, i, l) : h_entry<S>(sA, sB, js, l, i);
-            v += hv * sZ[l + c * js];
-        }
-        Cout[e] = v;
+        Cout[e] = block_row_dot<S>(sA, sB, nb, i, sZ + (long)c * js);
     }
     return true;
 }

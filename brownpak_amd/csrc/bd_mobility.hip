@@ -10,9 +10,8 @@
 // work unit covers a 2-row x 4-column block of tiles; row products are summed
 // over the unit's 4 tile columns in registers before one partial is written
 // per (tile row, super-column), and column products are summed over the unit's
-// tile rows.  A second kernel adds
-// the partials of every output block in a fixed order, so results are
-// deterministic run to run.
+// tile rows.  A second kernel adds the partials of every output block in a
+// fixed order, so results are deterministic run to run.
 #include <type_traits>
 
 #include "bd_internal.h"
@@ -103,11 +102,13 @@ __global__ __launch_bounds__(256) void rpy_tile_kernel(const double *__restrict_
 
 // --------------------------------------------------------------- product
 // Blocks of 256 threads take work units (2 tile rows x 4 tile columns of one
-// chain) in turn, each described by one precomputed record.  Two (FP64) or
-// three (FP32) tiles are in flight at any time: each thread loads 16
-// coalesced elements of a tile into registers, and a register set is refilled
-// with the tile that many slots ahead as soon as its own tile is staged in
-// LDS.  Within a tile the four
+// chain) in turn, each described by one precomputed record.  Three tiles are
+// in flight at any time: each thread loads 16 elements of a tile into
+// registers (FP32: four 16-byte loads; FP64: sixteen 8-byte loads, which
+// measured faster than 16-byte ones there), and a register set is refilled
+// with the tile three slots ahead as soon as its own tile is staged in LDS.
+// Occupancy: three blocks per CU for FP32, two for FP64 (its register sets
+// are twice as wide) and for blocks wider than 4 columns.  Within a tile the four
 // waves split the work by quarters:
 //   row product    y_I[r] += sum_c D(r,c) x_J[c]   (lane r, 16 columns);
 //   column product z_J[c] += sum_r D(r,c) x_I[r]   (lane c, 16 rows);
@@ -116,25 +117,46 @@ __global__ __launch_bounds__(256) void rpy_tile_kernel(const double *__restrict_
 // column sums once per unit column, through a small LDS buffer.
 // rowpart: (ntile * nsc) x TILE x S; colpart: nunits x WSC x TILE x S.
 constexpr int TPT = TILE * TILE / NT;   // tile elements per thread (16)
-// tiles in flight per block (register sets), sized so that three blocks fit
-// a CU without spilling
+// tiles in flight per block (register sets; fewer for wide blocks, whose
+// accumulators take the registers) and blocks per CU
+template <typename E, int S>
+constexpr int PDT = S <= 4 ? 3 : 2;
+template <typename E, int S>
+constexpr int OCC = (sizeof(E) == 8 || S > 4) ? 2 : 3;
+
+// Register slot i of thread tid holds tile element tile_elem<E>(tid, i).
 template <typename E>
-constexpr int PDT = sizeof(E) == 8 ? 2 : 3;
+__device__ inline int tile_elem(int tid, int i)
+{
+    if constexpr (sizeof(E) == 4) return 4 * (tid + (i / 4) * NT) + (i % 4);
+    else return tid + i * NT;
+}
 
 template <typename E>
 __device__ inline void load_tile16(const E *__restrict__ tp, int tid, E (&t)[TPT])
 {
+    if constexpr (sizeof(E) == 4) {
 #pragma unroll
-    for (int i = 0; i < TPT; i++) t[i] = tp[tid + i * NT];
+        for (int i = 0; i < TPT / 4; i++) {
+            const float4 v = reinterpret_cast<const float4 *>(tp)[tid + i * NT];
+            t[4 * i] = v.x;
+            t[4 * i + 1] = v.y;
+            t[4 * i + 2] = v.z;
+            t[4 * i + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < TPT; i++) t[i] = tp[tid + i * NT];
+    }
 }
 
 template <typename E, int S>
-__global__ __launch_bounds__(NT, 3) void symm_tiles_kernel(
+__global__ __launch_bounds__(NT, (OCC<E, S>)) void symm_tiles_kernel(
     const E *__restrict__ mob_all, const double *__restrict__ x_all, int ldx, const int *__restrict__ units,
     int nunits, const int *__restrict__ chain_i, const long *__restrict__ chain_l, double *__restrict__ rowpart,
     double *__restrict__ colpart, int *__restrict__ ticket, const DevStatus *stop, const int *__restrict__ active)
 {
-    constexpr int NS = CHR * WSC, PD = PDT<E>;
+    constexpr int NS = CHR * WSC, PD = PDT<E, S>;
     __shared__ E T[TILE][TILE + 1];
     __shared__ double xJs[WSC][TILE][S];
     __shared__ double xIs[CHR][TILE][S];
@@ -206,7 +228,7 @@ __global__ __launch_bounds__(NT, 3) void symm_tiles_kernel(
                 __syncthreads();   // the previous tile is no longer read
 #pragma unroll
                 for (int i = 0; i < TPT; i++) {
-                    const int e = tid + i * NT;
+                    const int e = tile_elem<E>(tid, i);
                     T[e / TILE][e % TILE] = R[sl % PD][i];
                 }
             }
@@ -515,9 +537,9 @@ int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int
 {
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-    // one block per unit up to three per CU (the occupancy the kernel is
-    // built for); beyond that blocks take units in turn
-    const int per_cu = 3;
+    // one block per unit up to the occupancy the kernel is built for; beyond
+    // that blocks take units in turn
+    const int per_cu = OCC<E, S>;
     int grid = std::max(1, std::min(p.nunits, cus * per_cu));
     if (e0) BD_HIP(hipEventRecord(e0, sm));
     hipLaunchKernelGGL((symm_tiles_kernel<E, S>), dim3(grid), dim3(NT), 0, sm, mob, x, ldx, p.units, p.nunits,
@@ -552,7 +574,7 @@ static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, 
                      const LzEpi *epi)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c.prof_on) {
+    if (c.prof_on && (c.prof_width == 0 || c.prof_width == S)) {
         if (c.prof_used + 2 > c.prof_ev.size()) prof_flush(c);
         if (c.prof_used + 2 > c.prof_ev.size()) {
             hipEvent_t a, b;

@@ -226,7 +226,10 @@ int bd_adjoint_rgsq(bd_ctx *ctx, int nsteps, const double *disp, int ckpt,
  * on, every launch of the mobility tile-product kernel is bracketed by HIP
  * events on the context's stream; bd_prof_get returns the summed kernel
  * time (ms), the launch count and the bytes of stored mobility tiles one
- * launch streams.  bd_prof_enable(ctx, 1) also clears the totals. */
+ * launch streams.  bd_prof_enable(ctx, 1) brackets every launch; on = 1 + w
+ * (w = 1..8) only the launches whose block has w columns (w = nts_mobsam:
+ * the Lanczos products, w = 1: the drift products); any on != 0 clears the
+ * totals. */
 int bd_prof_enable(bd_ctx *ctx, int on);
 int bd_prof_get(bd_ctx *ctx, double *total_ms, long long *launches,
                 double *tile_bytes);

@@ -1263,6 +1263,7 @@ long long orc_bd_run_ex(const orc_system *sys, const orc_bdopts *opt, double *cr
         if (opt->integrator == 1) {
             const double *dcol = disp ? disp + (size_t)step * n : diff + (size_t)(opt->lhdia ? cntr - 1 : 0) * n;
             for (int i = 0; i < n; i++) rpred[i] = crd[i] + drift[i] + dcol[i];
+            if (sys->imcon) orc_wrap_all(sys->box, nb, rpred);
             int e = drift_eval(sys, opt, mob, ex_ptr, ex_col, &vs, pt_ptr, pt_col, ptcap, rpred, frc, dpred, &en);
             if (e == -1) { info->ierr = 1; ret = -1; goto done; }
             if (e) { ret = e; goto done; }

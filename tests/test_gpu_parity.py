@@ -225,6 +225,67 @@ def test_verlet_pairs_bit_exact_over_steps(oracle, nchains, nper):
 
 
 # ------------------------------------------------------------- full steps
+def _pair_multiset(pairs):
+    from collections import Counter
+    return Counter(map(tuple, np.asarray(pairs).reshape(-1, 2).tolist()))
+
+
+def _oracle_multiset(ptr, col):
+    from collections import Counter
+    return Counter((min(i + 1, int(j)), max(i + 1, int(j))) for i in range(len(ptr) - 1)
+                   for j in col[ptr[i]:ptr[i + 1]])
+
+
+@pytest.mark.parametrize("case", ["melt_7cells", "box_2cells"])
+def test_cell_list_pairs_and_forces(oracle, case):
+    """mth_ptgen CL: the table is the reference's (no distance test, pairs
+    repeated on grids of fewer than 3 cells per side), as a multiset; forces
+    over it to 1e-10; then EM steps against the oracle."""
+    _say(f"case: cell list {case}")
+    if case == "melt_7cells":
+        m = _melt(20, 100, 46)
+    else:
+        L = 11.0
+        crd = np.random.default_rng(47).uniform(0.0, L, (60, 3))
+        # spread the beads so that the forces stay moderate
+        crd = np.stack(np.meshgrid(*[np.arange(4) * 2.7 + 0.3] * 3, indexing="ij"), -1).reshape(-1, 3)[:60]
+        bonds = np.concatenate([chain_bonds(20, offset=c * 20) for c in range(3)])
+        m = dict(crd=crd, bonds=bonds, bond_styles=[1], bond_params=[[25.0, 2.0]], vdw_styles=[1],
+                 vdw_params=[[1.0, 2.0, 5.0]], natom_types=1, excluded_atoms=1, rcutoff=5.0, tskin=0.0,
+                 imcon=1, box=(L, L, L))
+    m = dict(m, mth_ptgen="CL", tskin=0.0)
+    nb = len(m["crd"])
+    g = gpu_system(m, lhdia=False)
+    so = oracle_system(oracle, m, "CL")
+    f, e, _, _ = g.ia_calc_forces()
+    tab = oracle.pairs(so, m["crd"], "CL", cap=nb * nb * 16)
+    pairs, rebuilt = g.pair_table()
+    assert _pair_multiset(pairs) == _oracle_multiset(*tab)
+    fr, en, _ = oracle.forces(so, m["crd"], tab)
+    assert rel(f, fr) < TOL
+    assert abs(e[2] - en.energy_tot) <= TOL * abs(en.energy_tot)
+    nsteps = 5
+    xi = np.random.default_rng(48).standard_normal(3 * nb * nsteps)
+    g.bds_run(nsteps, xi)
+    ref, _, _ = oracle.bd_run(so, m["crd"], nsteps, xi, lhdia=False)
+    assert rel(g.get_coordinates(), ref) < TOL
+
+
+def test_cell_list_rejects_negative_coordinates_and_small_boxes():
+    from brownpak_amd import BdError
+    m = dict(_melt(2, 50, 49), mth_ptgen="CL", tskin=0.0)
+    crd = m["crd"].copy()
+    crd[3, 0] = -0.6 * m["box"][0]          # x / cs <= -1 (cs = L / 2)
+    g = gpu_system(dict(m, crd=crd), lhdia=False)
+    with pytest.raises(BdError) as ei:
+        g.ia_calc_forces()
+    assert ei.value.code == -4
+    with pytest.raises(BdError):
+        gpu_system(dict(m, box=(9.0, 9.0, 9.0)), lhdia=False)      # one cell of rc = 5 per side
+    with pytest.raises(BdError):
+        gpu_system(dict(m, imcon=0), lhdia=False)
+
+
 def test_bd_run_cholesky_c1(oracle, chain100):
     _say("case: bd_run chol c1")
     nsteps = 20

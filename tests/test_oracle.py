@@ -20,7 +20,7 @@ import pytest
 
 import lapack_restatement as LR
 from helpers import fene_wca_chain, oracle_system, read_cfg, rel
-from brownpak_amd.configs import periodic_melt, random_walk_chain
+from brownpak_amd.configs import chain_bonds, periodic_melt, random_walk_chain
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -364,6 +364,59 @@ def test_chns10_pairs_forces_and_cell_list(oracle):
     f2, e2, _ = oracle.forces(scl, m["crd"], (ptr, col))
     assert rel(f2, f1) < 1e-13 and abs(e1.energy_tot - e2.energy_tot) <= 1e-13 * abs(e1.energy_tot)
     assert np.abs(f1.sum(0)).max() < 1e-13 * np.abs(f1).max()
+
+
+def _cl_multiset_brute(crd, L, rc, excl):
+    """Cell-list pair multiplicities enumerated pair by pair (not cell by
+    cell, as the oracle does): same cell -> 1; otherwise the number of the
+    13 forward offsets d with cell(i) + d = cell(j) (mod nc) plus those with
+    cell(j) + d = cell(i)."""
+    from collections import Counter
+    dst = [(1, 0, 0), (1, 1, 0), (-1, 1, 0), (0, 1, 0), (0, 0, 1), (-1, 0, 1), (1, 0, 1), (-1, -1, 1),
+           (0, -1, 1), (1, -1, 1), (-1, 1, 1), (0, 1, 1), (1, 1, 1)]
+    nc = np.floor(np.asarray(L) / rc).astype(int)
+    cs = np.asarray(L) / nc
+    cell = np.minimum((crd / cs).astype(int), nc - 1)
+    out = Counter()
+    nb = len(crd)
+    for i in range(nb):
+        for j in range(i + 1, nb):
+            if (i + 1, j + 1) in excl:
+                continue
+            ci, cj = cell[i], cell[j]
+            if (ci == cj).all():
+                m = 1
+            else:
+                m = sum(1 for d in dst if (((ci + d) % nc) == cj).all()) + \
+                    sum(1 for d in dst if (((cj + d) % nc) == ci).all())
+            if m:
+                out[(i + 1, j + 1)] += m
+    return out
+
+
+def test_cell_list_repeats_pairs_on_two_cell_grids(oracle):
+    """A box of 2 cells per side (cell_list_m.f90:134-136): the reference's
+    cell-list table repeats pairs; the oracle's multiplicities equal a
+    pair-by-pair enumeration, and its forces count a repeated pair twice."""
+    from collections import Counter
+    L = 11.0
+    crd = np.random.default_rng(45).uniform(0.0, L, (60, 3))
+    nb = len(crd)
+    bonds = np.concatenate([chain_bonds(20, offset=c * 20) for c in range(3)])
+    so = oracle.System(nb, bonds, [1], [[25.0, 2.0]], vdw_styles=[1], vdw_params=[[1.0, 2.0, 5.0]],
+                       excluded_atoms=1, mth_ptgen=2, rcutoff=5.0, imcon=1, box=(L, L, L))
+    excl = _ring_exclusions(nb, bonds, 1)
+    ptr, col = oracle.pairs(so, crd, "CL", cap=nb * nb * 16)
+    got = Counter((min(i + 1, int(j)), max(i + 1, int(j))) for i in range(nb) for j in col[ptr[i]:ptr[i + 1]])
+    want = _cl_multiset_brute(crd, (L, L, L), 5.0, excl)
+    assert got == want
+    assert max(want.values()) > 1
+    # forces: each listed pair once, repeats included
+    f, _, _ = oracle.forces(so, crd, (ptr, col))
+    sd = oracle.System(nb, bonds, [1], [[25.0, 2.0]], vdw_styles=[1], vdw_params=[[1.0, 2.0, 5.0]],
+                       excluded_atoms=1, mth_ptgen=0, rcutoff=5.0, imcon=1, box=(L, L, L))
+    fd, _, _ = oracle.forces(sd, crd, oracle.pairs(sd, crd, "DIR"))
+    assert np.abs(f - fd).max() > 1.0   # the repeats change the forces
 
 
 def test_verlet_schedule(oracle):
