@@ -142,6 +142,50 @@ def cpu_baseline(budget_s=20.0):
                        f"block Lanczos s=4 tol 1e-6, FP64, oracle/bdoracle.c at 1 thread, {dt:.1f} s")
 
 
+ENSEMBLE_CHAINS = 512
+
+
+def time_ensemble(steps, warmup, precision="fp32"):
+    """The batched dilute ensemble: 512 chains with lengths drawn from
+    U[50, 800] (seeded), FENE/WCA/RPY, block Lanczos s = 4, tol 1e-6, all
+    chains advanced by shared launches (bd_batch_run), device noise.
+    Returns steps/s (every chain advances one step per step) and the tile
+    product's kernel time, from a second pass with HIP events."""
+    import torch
+    from brownpak_amd import BdBatch
+    from brownpak_amd.configs import fene_wca_params, overlap_free_chain
+    rng = np.random.default_rng(3)
+    lengths = [int(x) for x in rng.integers(50, 801, size=ENSEMBLE_CHAINS)]
+    b = BdBatch(lengths, nts_mobsam=S_BLOCK, lanc_mxitr=50, lanc_tol=1e-6, precision=precision)
+    bond, vdw = fene_wca_params()
+    b.set_model(2, bond, lvdw=True, vdw_params=vdw, rcutoff=vdw[2])
+    b.set_coordinates([overlap_free_chain(nb, 10 + c) for c, nb in enumerate(lengths)])
+    b.set_rng(7)
+    b.run(warmup)
+    b.get_coordinates()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(steps)
+    b.get_coordinates()
+    dt = time.perf_counter() - t0
+    info = b.info()
+    b.prof(1)
+    b.run(steps)
+    ms, launches, tile_bytes = b.prof(-1)
+    b.prof(0)
+    info2 = b.info()
+    n = [3 * nb for nb in lengths]
+    return dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, chains=ENSEMBLE_CHAINS,
+                beads=int(sum(lengths)), precision=precision,
+                lanczos_iters_per_chain_update=round(
+                    info["total_lanczos_iters"] / max(info["mob_updates"] * ENSEMBLE_CHAINS, 1), 2),
+                tile_product_avg_us=round(ms / max(launches, 1) * 1e3, 2),
+                tile_product_gbs=round(tile_bytes / (ms / max(launches, 1) * 1e-3) / 1e9, 1),
+                stored_tile_bytes=tile_bytes,
+                algorithmic_tile_bytes=float(sum(4.0 * k * (k + 1) / 2 for k in n)),
+                not_converged=info2["n_not_converged"])
+
+
 def measured_traffic():
     """HBM bytes per launch of symm_tiles_kernel<float, 4> from the two
     rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in profiles/,
@@ -179,6 +223,10 @@ def main():
     ap.add_argument("--states", type=int, default=10, help="seeded states of the accuracy report")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="end with a 256 MiB device copy (counter calibration of profiled runs)")
+    ap.add_argument("--ensemble-steps", type=int, default=20,
+                    help="steps of the 512-chain ensemble reported beside the main line (0: skip)")
+    ap.add_argument("--workload", choices=["chain", "ensemble"], default="chain",
+                    help="ensemble: time only the 512-chain ensemble and print its object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
@@ -188,10 +236,18 @@ def main():
 
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.workload == "ensemble":
+        ens = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK)
+        print(json.dumps({"ensemble": ens}), flush=True)
+        return
     mixed = time_path("fp32", steps, warmup)
     print(f"# mixed: {mixed['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
     fp64 = time_path("fp64", steps, warmup)
     print(f"# fp64: {fp64['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
+    ens = None
+    if args.ensemble_steps > 0:
+        ens = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK)
+        print(f"# ensemble: {ens['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
     acc = None if args.no_error_check else accuracy_states(args.states)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
@@ -246,6 +302,7 @@ def main():
                      "fp64_achieved": round(r64["achieved"], 1),
                      "fp64_avg_launch_us": round(r64["avg_launch_us"], 2)},
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     if cpu:
         out["vs_cpu_fp64"] = round(fp64["steps_per_s"] / cpu["value"], 1)

@@ -26,6 +26,7 @@
 // the result equals the reference's P sqrt(Lambda) P^T E_1 R.
 #include "bd_internal.h"
 #include "bd_lanczos_dev.h"
+#include "bd_pcr_dev.h"
 
 namespace bd {
 
@@ -146,6 +147,12 @@ __global__ __launch_bounds__(RB) void cholqr_kernel(const double *__restrict__ X
 {
     if (lz_done(st)) return;
     __shared__ double sG[S * S];
+    // this thread's first row, loaded before the Gram sum so that the two
+    // memory round trips overlap
+    const int r0 = blockIdx.x * RB + threadIdx.x;
+    double x0[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) x0[k] = X[(long)k * ld + min(r0, rows - 1)];
     sum_partials<S>(gpart, sG);
     double R[S][S];
     bool ok = small_chol<S>(sG, R);
@@ -170,10 +177,10 @@ __global__ __launch_bounds__(RB) void cholqr_kernel(const double *__restrict__ X
     double acc[S * S];
 #pragma unroll
     for (int e = 0; e < S * S; e++) acc[e] = 0.0;
-    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += GBLK * RB) {
+    for (int r = r0; r < rows; r += GBLK * RB) {
         double x[S], q[S];
 #pragma unroll
-        for (int k = 0; k < S; k++) x[k] = X[(long)k * ld + r];
+        for (int k = 0; k < S; k++) x[k] = (r == r0) ? x0[k] : X[(long)k * ld + r];
         // solve q R = x
 #pragma unroll
         for (int c = 0; c < S; c++) {
@@ -392,6 +399,15 @@ __global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict_
         s_done = lz_done(st) ? 1 : 0;
         s_fail = 0;
     }
+    // this thread's first row, loaded before the partial sums so that the
+    // two memory round trips overlap
+    const int r0 = blockIdx.x * RB + threadIdx.x, rr0 = min(r0, rows - 1);
+    double w0[S], v0[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+        w0[k] = W[(long)k * ld + rr0];
+        v0[k] = Vj[(long)k * ld + rr0];
+    }
     // fixed-order sums of the per-block partials (ends with a barrier)
     block_sum_parts(lzpart, 2 * S * S, nblk, s_scr, sAP);
     if (s_done) return;
@@ -435,10 +451,13 @@ __global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict_
     double acc[S * S];
 #pragma unroll
     for (int e = 0; e < S * S; e++) acc[e] = 0.0;
-    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += GBLK * RB) {
+    for (int r = r0; r < rows; r += GBLK * RB) {
         double w[S], v[S], q[S];
 #pragma unroll
-        for (int k = 0; k < S; k++) { w[k] = W[(long)k * ld + r]; v[k] = Vj[(long)k * ld + r]; }
+        for (int k = 0; k < S; k++) {
+            w[k] = (r == r0) ? w0[k] : W[(long)k * ld + r];
+            v[k] = (r == r0) ? v0[k] : Vj[(long)k * ld + r];
+        }
 #pragma unroll
         for (int c = 0; c < S; c++) {
             double t = 0.0;
@@ -594,6 +613,7 @@ __global__ __launch_bounds__(sp::WG_T) void small_split_kernel(const double *__r
 }
 
 constexpr int YR = 32, YG = 8, YC = 32;   // combine: rows per block, column groups, columns per group
+constexpr int YJS = 256;                   // largest j s of the grouped combination (64 blocks of 4)
 
 // ------------------------------------------------------------ grouped tests
 // The small problems of LG consecutive iterations j0 .. j0+LG-1 run as one
@@ -627,6 +647,175 @@ __global__ __launch_bounds__(sp::WG_T) void small_group_kernel(const double *__r
     }
 }
 
+// The small problems of iterations j0..jend (blockIdx.y = j - j0) by
+// parallel cyclic reduction (bd_pcr_dev.h), for nb = j <= 64 blocks.  Every
+// workgroup runs one probe wave (H - m I positive definite?) and three
+// quadrature-node waves, one wave per SIMD.  Bound and nodes come from the
+// hint, the largest probe index the previous launch accepted: the smallest
+// Ritz value only decreases with j, and a node range that reaches below the
+// spectrum is as accurate as a tight one.  If the hinted bound fails, all
+// four waves probe the next halvings together and the nodes are solved again
+// with the first bound that passes; every workgroup reaches the same
+// decision.  Each workgroup writes the weighted sum of its nodes' Z in
+// part[wg]; the last one sums the parts in workgroup order and forms C = H Z.
+template <int S>
+__global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *__restrict__ Ablk,
+                                                                 const double *__restrict__ Bblk,
+                                                                 const double *__restrict__ R0, int j0, int jend,
+                                                                 double *__restrict__ part, long part_stride,
+                                                                 double *__restrict__ Cg, long c_stride,
+                                                                 sp::SplitState *ssv, DevStatus *st)
+{
+    using namespace pcr;
+    constexpr int SS = S * S;
+    constexpr int BUF = (PROBES * 2 + NODES * 3) * SS * ROWS;   // task slots (doubles)
+    __shared__ double s_buf[BUF];
+    __shared__ double sA[ROWS * SS], sB[ROWS * SS], sR[SS];
+    __shared__ double s_red[WAVES];
+    __shared__ int s_done, s_ok[WAVES], s_last;
+    const int j = j0 + (int)blockIdx.y;
+    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    __syncthreads();
+    if (s_done || j > jend || j < 2) return;
+    const int nb = j, js = nb * S, t = threadIdx.x, w = t / 64, lane = t % 64;
+    const int parity = ((j0 - 1) / LG) & 1;
+    sp::SplitState *ss = ssv + blockIdx.y;
+    double *P = part + blockIdx.y * part_stride;
+    for (int e = t; e < nb * SS; e += THREADS) {
+        sA[e] = Ablk[e];
+        sB[e] = Bblk[e];
+    }
+    if (t < SS) sR[t] = R0[t];
+    __syncthreads();
+    const double M = sp::gersh_block<S>(sA, sB, js, s_red);
+    int hint = -1;
+    for (int g = 0; g < LG; g++) hint = max(hint, ssv[g].kb[parity ^ 1]);
+    const int kb_guess = hint >= 0 ? min(hint, NBOUND - 1) : 0;
+    constexpr int DEC_MAX = 13;
+    auto place = [&](int kb, double &m, int &N, double &h) -> int {
+        m = ldexp(M, -(kb + 1));
+        int dec = (int)ceil(log10(M / m));
+        if (dec < 1) dec = 1;
+        if (dec > DEC_MAX) dec = DEC_MAX + 1;
+        N = c_qN[min(dec, 13)];
+        h = c_qh[min(dec, 13)];
+        return dec;
+    };
+    // node batches (NODES nodes each) of the hinted placement; workgroups
+    // without one leave at once
+    int nact_g = 1;
+    {
+        double mg, hg;
+        int Ng;
+        if (place(kb_guess, mg, Ng, hg) <= DEC_MAX) nact_g = (2 * Ng + 1 + NODES - 1) / NODES;
+    }
+    if ((int)blockIdx.x >= nact_g) return;
+    // this wave's slots
+    double *buf = s_buf + (w < PROBES ? w * 2 * SS * ROWS : (PROBES * 2 + (w - PROBES) * 3) * SS * ROWS);
+    double Z[S][S];
+    double wt = 0.0;
+    bool node_on = false;
+    // one round: probes at kb_p (probe waves) and the nodes of batch `batch`
+    // of placement kb_n
+    auto round = [&](int kbp0, int kbp_step, bool nodes, int kb_n, int batch) __attribute__((always_inline)) {
+        double shift = 1.0;
+        bool rhs = false, active = false;
+        if (w < PROBES || !nodes) {
+            const int kbp = kbp0 + w * kbp_step;
+            active = kbp >= 0 && kbp < NBOUND;
+            shift = -ldexp(M, -(kbp + 1));
+        } else {
+            double m, h;
+            int N;
+            const int dec = place(kb_n, m, N, h);
+            const int node = batch * NODES + (w - PROBES);
+            if (dec <= DEC_MAX && node < 2 * N + 1) {
+                const double u0 = 0.25 * (log(m) + log(M));
+                const double v = (node - N) * h;
+                const double u = u0 + sinh(v);
+                wt = (2.0 / M_PI) * h * cosh(v) * exp(u);
+                shift = exp(2.0 * u);
+                rhs = active = true;
+            }
+            node_on = active;
+        }
+        const bool ok = reduce<S>(sA, sB, sR, nb, shift, rhs, buf, Z);
+        if (lane == 0) s_ok[w] = (active && ok) ? 1 : 0;
+        __syncthreads();
+    };
+    round(kb_guess, 1, true, kb_guess, blockIdx.x);
+    int use = kb_guess, kb_found = -1;
+    if (s_ok[0]) {
+        kb_found = kb_guess;
+    } else {
+        // the hinted bound is above the spectrum: four halvings per round
+        for (int kb0 = kb_guess + 1; kb0 < NBOUND && kb_found < 0; kb0 += WAVES) {
+            round(kb0, 1, false, 0, 0);
+            for (int q = 0; q < WAVES; q++)
+                if (s_ok[q]) { kb_found = kb0 + q; break; }
+        }
+        use = kb_found;
+    }
+    int err = 0;
+    double m = 0.0, h = 0.0;
+    int N = 0;
+    if (kb_found < 0) err = 2;
+    else if (place(use, m, N, h) > DEC_MAX) err = 3;
+    const int nq = err ? 1 : 2 * N + 1;
+    const int nact = err ? 1 : (nq + NODES - 1) / NODES;
+    // node batches blockIdx.x, blockIdx.x + nact_g, ... of the placement in
+    // use (one, unless the fallback placed more nodes than the hint did)
+    if (t == 0) s_last = 0;
+    __syncthreads();
+    for (int b = blockIdx.x; b < nact; b += nact_g) {
+        if (!err && !(b == (int)blockIdx.x && use == kb_guess)) round(-1, 0, true, use, b);   // probe wave idle
+        // weighted node solutions summed in wave order into part[b]
+        __syncthreads();
+        if (!err && w >= PROBES && lane < nb) {
+            double *zw = s_buf + (size_t)(w - PROBES) * js * S;
+#pragma unroll
+            for (int a = 0; a < S; a++)
+#pragma unroll
+                for (int c = 0; c < S; c++) zw[(lane * S + a) + c * js] = node_on ? wt * Z[a][c] : 0.0;
+        }
+        __syncthreads();
+        double *pw = P + (long)b * js * S;
+        for (int e = t; e < js * S; e += THREADS) {
+            double acc = 0.0;
+            for (int q = 0; q < NODES; q++) acc += err ? 0.0 : s_buf[(size_t)q * js * S + e];
+            pw[e] = acc;
+        }
+        __threadfence();
+        __syncthreads();
+        // the batch completing the count is this workgroup's last one
+        if (t == 0 && atomicAdd(&ss->cnt, 1) == nact - 1) s_last = 1;
+        __syncthreads();
+    }
+    if (!s_last) return;
+    __threadfence();
+    if (t == 0) {
+        ss->cnt = 0;
+        ss->kb[parity] = kb_found;
+        ss->nq = nq;
+        ss->err = err;
+        ss->lam_lo = m;
+        ss->lam_hi = M;
+        st->lam_hi = M;
+        st->lam_lo = m;
+        st->nq = nq;
+        if (err) atomicExch(&st->ierr, 2);
+    }
+    if (err) return;
+    double *sZ = s_buf;
+    for (int e = t; e < js * S; e += THREADS) sZ[e] = sum_strided(P + e, (long)js * S, nact);
+    __syncthreads();
+    double *Cout = Cg + blockIdx.y * c_stride;
+    for (int e = t; e < js * S; e += THREADS) {
+        const int i = e % js, c = e / js;
+        Cout[e] = sp::block_row_dot<S>(sA, sB, nb, i, sZ + (long)c * js);
+    }
+}
+
 // y_j = V(:, 1:jS) C_j for the group's members (Y is a ring of LG + 1
 // vectors indexed by j mod (LG + 1)); partial norms of y_j - y_{j-1} and
 // y_{j-1}; the last block tests the members in order and records the first
@@ -641,16 +830,46 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
     __shared__ int s_done, s_last;
     __shared__ double s_acc[YG][LG][S][YR];   // row index fastest: conflict-free
     __shared__ double s_nrm[LG][2 * S];
-    __shared__ double s_C[LG][208 * S];
+    __shared__ double s_C[LG][YJS * S];
     __shared__ double s_scr[256];
     const int ng = jend - j0 + 1;               // members (<= LG)
     const int jsmax = jend * S;
     if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
-    for (int g = 0; g < ng; g++) {
-        const int js = (j0 + g) * S;
-        if (j0 + g < 2) continue;
-        for (int e = threadIdx.x; e < js * S; e += blockDim.x)
-            s_C[g][(e % js) * S + e / js] = Cg[g * c_stride + e];
+    {
+        // the members' coefficients (row-major into s_C), all loads of a
+        // thread issued before the first store
+        int off[LG + 1];
+        off[0] = 0;
+#pragma unroll
+        for (int g = 0; g < LG; g++) {
+            const int jg = j0 + g;
+            off[g + 1] = off[g] + ((g < ng && jg >= 2) ? jg * S * S : 0);
+        }
+        const int tot = off[LG];
+        constexpr int U = 16;
+        auto member = [&](int x) __attribute__((always_inline)) {
+            int g = 0;
+#pragma unroll
+            for (int k = 1; k < LG; k++) g += (x >= off[k]) ? 1 : 0;
+            return g;
+        };
+        for (int b0 = threadIdx.x; b0 < tot; b0 += U * (int)blockDim.x) {
+            double v[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int x = min(b0 + q * (int)blockDim.x, tot - 1);
+                const int g = member(x);
+                v[q] = Cg[g * c_stride + (x - off[g])];
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int x = b0 + q * (int)blockDim.x;
+                if (x < tot) {
+                    const int g = member(x), e = x - off[g], js = (j0 + g) * S;
+                    s_C[g][(e % js) * S + e / js] = v[q];
+                }
+            }
+        }
     }
     __syncthreads();
     if (s_done) return;
@@ -775,7 +994,16 @@ __global__ __launch_bounds__(256) void ycomb_kernel(const double *__restrict__ V
     __shared__ double s_C[512 * S];
     __shared__ double s_scr[256];
     if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
-    for (int e = threadIdx.x; e < js * S; e += blockDim.x) s_C[(e % js) * S + e / js] = C[e];
+    for (int b0 = threadIdx.x; b0 < js * S; b0 += 8 * (int)blockDim.x) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = C[min(b0 + q * (int)blockDim.x, js * S - 1)];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int e = b0 + q * (int)blockDim.x;
+            if (e < js * S) s_C[(e % js) * S + e / js] = v[q];
+        }
+    }
     __syncthreads();
     if (s_done) return;
     const int lr = threadIdx.x % YR, g = threadIdx.x / YR;
@@ -952,11 +1180,13 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     hipLaunchKernelGGL(status_reset_kernel, dim3(1), dim3(1), 0, sm, c.st, c.split);
     // small problem split over workgroups when its blocks fit their LDS
     const bool split = S <= 4 && dmax <= sp::WG_NBM<S>;
-    const int nwg = (2 * 125 + 1 + sp::WG_NPB<S> - 1) / sp::WG_NPB<S>;
+    // by parallel cyclic reduction when every H_j has at most 64 blocks
+    const bool use_pcr = S <= 4 && dmax <= pcr::ROWS;
+    const int nwg = use_pcr ? (2 * 125 + 1 + pcr::NODES - 1) / pcr::NODES
+                            : (2 * 125 + 1 + sp::WG_NPB<S> - 1) / sp::WG_NPB<S>;
     // grouped tests: LG iterations' small problems per launch, on the solve's
-    // stream (needs the split small problem)
-    const bool grouped = split;
-    (void)nwg;
+    // stream (needs the split or the reduction small problem)
+    const bool grouped = split || use_pcr;
     {
         // host-side capacity checks of the scratch every launch below indexes
         const size_t ncol = (size_t)S * dmax;
@@ -965,7 +1195,8 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                                          : (size_t)dmax * (3 * S * S + S) * 256;
         const size_t nblk_y = (size_t)(rows + YR - 1) / YR;
         if (need_part > cq || nblk_y * LG * 2 * S > (size_t)(c.npad / YR + 1) * LG * 2 * MAX_S ||
-            (size_t)LG * dmax * S * S > (size_t)LG * ncol * S || (size_t)c.plan.nrowblk > (size_t)(c.npad / TILE + 1)) {
+            (size_t)LG * dmax * S * S > (size_t)LG * ncol * S || (size_t)c.plan.nrowblk > (size_t)(c.npad / TILE + 1) ||
+            (grouped && (size_t)dmax * S > (size_t)YJS)) {
             set_error("internal: Lanczos scratch smaller than its launches need");
             return BD_ERR_STATE;
         }
@@ -1044,8 +1275,14 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             if constexpr (S <= 4) {
                 const long part_stride = (long)nwg * dmax * S * S;
                 const long c_stride = (long)c.dimk_max * S * S;
-                hipLaunchKernelGGL((small_group_kernel<S>), dim3(nwg, j - jg0 + 1), dim3(sp::WG_T), 0, sm, c.Ablk,
-                                   c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split, c.st);
+                if (use_pcr)
+                    hipLaunchKernelGGL((small_pcr_kernel<S>), dim3(nwg, j - jg0 + 1), dim3(pcr::THREADS), 0, sm,
+                                       c.Ablk, c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split,
+                                       c.st);
+                else
+                    hipLaunchKernelGGL((small_group_kernel<S>), dim3(nwg, j - jg0 + 1), dim3(sp::WG_T), 0, sm,
+                                       c.Ablk, c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split,
+                                       c.st);
                 hipLaunchKernelGGL((ycomb_group_kernel<S>), dim3((rows + YR - 1) / YR), dim3(256), 0, sm, c.V, ld,
                                    rows, jg0, j, c.Ccoef, c_stride, c.Y, two_norm ? 1 : 0, c.npart, c.opt.lanc_tol,
                                    c.st);
