@@ -53,6 +53,7 @@ EXPORTS = (
     "bd_set_theta", "bd_run_record", "bd_run_replay", "bd_adjoint_rgsq",
     "bd_batch_create", "bd_batch_destroy", "bd_batch_set_model", "bd_batch_set_coordinates",
     "bd_batch_get_coordinates", "bd_batch_set_rng", "bd_batch_run", "bd_batch_info", "bd_batch_prof",
+    "bd_batch_tile_shape", "bd_batch_set_tile_shape",
 )
 
 
@@ -130,6 +131,8 @@ def load_library(path: str = LIB_PATH):
     L.bd_batch_run.argtypes = [vp, C.c_int, DP, C.c_longlong, C.POINTER(C.c_longlong)]
     L.bd_batch_info.argtypes = [vp, C.POINTER(C.c_longlong), IP, IP, IP]
     L.bd_batch_prof.argtypes = [vp, C.c_int, DP, C.POINTER(C.c_longlong), DP]
+    L.bd_batch_tile_shape.argtypes = [vp, IP, IP, DP]
+    L.bd_batch_set_tile_shape.argtypes = [vp, C.c_int, C.c_int]
     for name in EXPORTS:
         if name not in ("bd_ctx_create", "bd_ctx_destroy", "bd_last_error", "bd_batch_create",
                         "bd_batch_destroy"):
@@ -445,6 +448,17 @@ class BdBatch:
         self._chk(self.L.bd_batch_info(self.h, C.byref(nts), C.byref(upd), C.byref(it), C.byref(nc)))
         return {"nts": nts.value, "mob_updates": upd.value, "total_lanczos_iters": it.value,
                 "n_not_converged": nc.value}
+
+    def tile_shape(self):
+        """((chr, wsc) in use, autotuner timings in us of the shapes
+        2x4, 1x4, 4x4, 2x2)."""
+        a, b = C.c_int(0), C.c_int(0)
+        t = np.zeros(4)
+        self._chk(self.L.bd_batch_tile_shape(self.h, C.byref(a), C.byref(b), _dp(t)))
+        return (a.value, b.value), t
+
+    def set_tile_shape(self, chr_, wsc):
+        self._chk(self.L.bd_batch_set_tile_shape(self.h, int(chr_), int(wsc)))
 
     def prof(self, on=-1):
         ms = C.c_double(0.0)

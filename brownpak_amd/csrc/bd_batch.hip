@@ -42,6 +42,7 @@ struct Batch {
     std::vector<int> nb;
     std::vector<long> row;
     MobPlan plan;
+    double tune_us[4] = {0, 0, 0, 0};   // unit-shape timings of the autotuner
     float *mob32 = nullptr;
     double *mob64 = nullptr;
     BChainDev *chains = nullptr;
@@ -555,6 +556,24 @@ static int b_lanczos(Batch &b, int *iters_out)
     return BD_OK;
 }
 
+template <typename E>
+static int batch_tune_t(Batch &b, const E *mob)
+{
+    switch (b.s) {
+    case 1: return plan_tune<E, 1>(b.plan, b.nb, b.row, mob, b.V, b.W, (int)b.NP, &b.st->ticket, b.stream, b.device, b.tune_us);
+    case 2: return plan_tune<E, 2>(b.plan, b.nb, b.row, mob, b.V, b.W, (int)b.NP, &b.st->ticket, b.stream, b.device, b.tune_us);
+    case 3: return plan_tune<E, 3>(b.plan, b.nb, b.row, mob, b.V, b.W, (int)b.NP, &b.st->ticket, b.stream, b.device, b.tune_us);
+    case 4: return plan_tune<E, 4>(b.plan, b.nb, b.row, mob, b.V, b.W, (int)b.NP, &b.st->ticket, b.stream, b.device, b.tune_us);
+    default: set_error("batched path supports nts_mobsam 1..4"); return BD_ERR_ARG;
+    }
+}
+
+static int batch_tune(Batch &b)
+{
+    if (b.mob32) return batch_tune_t<float>(b, b.mob32);
+    return batch_tune_t<double>(b, b.mob64);
+}
+
 static int b_step(Batch &b, const double *xi_host, long long avail, long long *consumed)
 {
     const int s = b.s;
@@ -712,6 +731,10 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
         if (hipEventCreateWithFlags(&b.ev[k], hipEventDisableTiming) != hipSuccess) return fail("event failed");
     for (int k = 0; k < 2; k++)
         if (hipEventCreateWithFlags(&b.chunk_ev[k], hipEventDisableTiming) != hipSuccess) return fail("event failed");
+    // tile unit shape chosen by timing the Lanczos product on this plan
+    if (b.mob32) ok = hipMemset(b.mob32, 0, elems * sizeof(float)) == hipSuccess;
+    else ok = hipMemset(b.mob64, 0, elems * sizeof(double)) == hipSuccess;
+    if (!ok || batch_tune(b) != BD_OK) return fail("tile shape autotuning failed");
     return h;
 }
 
@@ -823,6 +846,27 @@ int bd_batch_run(bd_batch *h, int nsteps, const double *xi, long long nxi, long 
     BD_HIP(hipMemcpy(cnt, b.counters, sizeof(cnt), hipMemcpyDeviceToHost));
     if (rc == BD_OK && cnt[1] == 1) { set_error("bond length too large (overstretched bond)"); rc = BD_ERR_BOND; }
     return rc;
+}
+
+int bd_batch_tile_shape(bd_batch *h, int *chr, int *wsc, double *times_us)
+{
+    if (!h) { set_error("null batch"); return BD_ERR_ARG; }
+    Batch &b = h->b;
+    if (chr) *chr = b.plan.chr;
+    if (wsc) *wsc = b.plan.wsc;
+    if (times_us)
+        for (int k = 0; k < 4; k++) times_us[k] = b.tune_us[k];
+    return BD_OK;
+}
+
+int bd_batch_set_tile_shape(bd_batch *h, int chr, int wsc)
+{
+    if (!h) { set_error("null batch"); return BD_ERR_ARG; }
+    if (!tile_shape_supported(chr, wsc)) { set_error("unsupported tile unit shape"); return BD_ERR_ARG; }
+    Batch &b = h->b;
+    BD_HIP(hipStreamSynchronize(b.stream));
+    plan_free(b.plan);
+    return plan_build(b.plan, b.nb, b.row, chr, wsc);
 }
 
 int bd_batch_info(bd_batch *h, long long *nts, int *mob_updates, int *total_iters, int *n_not_converged)

@@ -72,6 +72,7 @@ struct SplitState;
 
 struct MobPlan {
     int nchains = 0;
+    int chr = 2, wsc = 4;     // work unit: chr tile rows x wsc tile columns
     long ntiles = 0;
     int nunits = 0, nrowblk = 0;
     int *tiles = nullptr;     // 4 ints per stored tile: chain, I, J, 0
@@ -86,7 +87,13 @@ struct LzEpi {
     const double *Vj, *Vp, *Bp;
     double *part;
 };
-int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off);
+bool tile_shape_supported(int chr, int wsc);
+int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, int chr = 2, int wsc = 4);
+// Times the supported unit shapes on this plan (block width S) and rebuilds
+// it with the fastest; times_us (4 entries) receives the measured times.
+template <typename E, int S>
+int plan_tune(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, const E *mob,
+              const double *x, double *y, int ld, int *ticket, hipStream_t sm, int device, double *times_us);
 void plan_free(MobPlan &p);
 template <typename E>
 int rpy_launch(const MobPlan &p, const double *crd, E *mob, hipStream_t sm);

@@ -832,6 +832,8 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
     __shared__ double s_nrm[LG][2 * S];
     __shared__ double s_C[LG][YJS * S];
     __shared__ double s_scr[256];
+    __shared__ double s_np[LG * 2 * S][YR + 1];
+    __shared__ double s_np2[LG * 2 * S][256 / (LG * 2 * S)];
     const int ng = jend - j0 + 1;               // members (<= LG)
     const int jsmax = jend * S;
     if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
@@ -938,14 +940,34 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
                 }
             }
         }
+        if (lane < YR) {
 #pragma unroll
-        for (int g = 0; g < LG; g++)
+            for (int g = 0; g < LG; g++)
 #pragma unroll
-            for (int e = 0; e < 2 * S; e++) {
-                double v = e < S ? dn[g][e] : on[g][e - S];
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-                if (lane == 0) part[((long)blockIdx.x * LG + g) * 2 * S + e] = v;
-            }
+                for (int e = 0; e < 2 * S; e++) s_np[g * 2 * S + e][lane] = e < S ? dn[g][e] : on[g][e - S];
+        }
+    }
+    __syncthreads();
+    {
+        // the block's partial norms: the YR rows summed in CHK chunks, then
+        // the chunks (fixed order)
+        constexpr int NV = LG * 2 * S, CHK = 256 / NV, RPC = (YR + CHK - 1) / CHK;
+        const int t = threadIdx.x;
+        if (t < NV * CHK) {
+            const int v = t / CHK, c = t % CHK;
+            double a = 0.0;
+#pragma unroll
+            for (int q = 0; q < RPC; q++)
+                if (c * RPC + q < YR) a += s_np[v][c * RPC + q];
+            s_np2[v][c] = a;
+        }
+        __syncthreads();
+        if (t < NV) {
+            double a = 0.0;
+#pragma unroll
+            for (int c = 0; c < CHK; c++) a += s_np2[t][c];
+            part[(long)blockIdx.x * NV + t] = a;
+        }
     }
     __threadfence();
     __syncthreads();

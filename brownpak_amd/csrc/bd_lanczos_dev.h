@@ -18,22 +18,40 @@ static __constant__ double c_qh[14] = {0.205, 0.205, 0.155, 0.125, 0.1, 0.085, 0
 static __device__ inline bool lz_done(const DevStatus *st) { return st->converged_j != 0 || st->ierr != 0; }
 
 // ------------------------------------------------------------ reductions
+// Sums of the S*S per-thread values acc over the RB threads of the block
+// into out (global), in a fixed order: through shared memory, 16 values at
+// a time -- thread t adds threads 16g .. 16g+15 for value t % 16, group
+// g = t / 16, then 16 threads add the 16 group sums.  (Two barriers per 16
+// values; a shuffle tree costs six dependent cross-lane round trips per
+// value instead.)
 template <int S>
 __device__ void block_reduce_store(double (&acc)[S * S], double *out)
 {
-    __shared__ double sh[RB / 64][S * S];
-    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    constexpr int E = S * S, CH = E < 16 ? E : 16, NG = RB / 16;
+    __shared__ double sh[RB][CH + 1];
+    __shared__ double s2[NG][CH];
+    const int t = threadIdx.x;
 #pragma unroll
-    for (int e = 0; e < S * S; e++) {
-        double v = acc[e];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if (lane == 0) sh[wv][e] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < S * S) {
-        double v = 0.0;
-        for (int w = 0; w < RB / 64; w++) v += sh[w][threadIdx.x];
-        out[threadIdx.x] = v;
+    for (int e0 = 0; e0 < E; e0 += CH) {
+#pragma unroll
+        for (int q = 0; q < CH; q++)
+            if (e0 + q < E) sh[t][q] = acc[e0 + q];
+        __syncthreads();
+        if (t < NG * CH) {
+            const int o = t % CH, g = t / CH;
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) v += sh[g * 16 + i][o];
+            s2[g][o] = v;
+        }
+        __syncthreads();
+        if (t < CH && e0 + t < E) {
+            double v = 0.0;
+#pragma unroll
+            for (int g = 0; g < NG; g++) v += s2[g][t];
+            out[e0 + t] = v;
+        }
+        __syncthreads();
     }
 }
 

@@ -12,14 +12,22 @@
 // per (tile row, super-column), and column products are summed over the unit's
 // tile rows.  A second kernel adds the partials of every output block in a
 // fixed order, so results are deterministic run to run.
+#include <map>
+#include <string>
 #include <type_traits>
 
 #include "bd_internal.h"
 
 namespace bd {
 
-constexpr int WSC = 4;   // tile columns per super-column
-constexpr int CHR = 2;   // tile rows per work unit
+// Work unit: CHR tile rows x WSC tile columns (a "super-column" is WSC tile
+// columns).  The default is 2 x 4; 1 x 4, 4 x 4 and 2 x 2 are the other
+// shapes the product is built for (blocks of at most 4 columns), chosen per
+// plan by plan_tune.
+bool tile_shape_supported(int chr, int wsc)
+{
+    return (chr == 2 && wsc == 4) || (chr == 1 && wsc == 4) || (chr == 4 && wsc == 4) || (chr == 2 && wsc == 2);
+}
 constexpr int NT = 256;  // threads per block of the product kernel
 // Work-unit record (16 ints): chain, super-column, Ib, Ie, ntile, nsc,
 // valid-slot mask, 0, tile base (elements, 2 ints), x row offset (2 ints),
@@ -101,37 +109,34 @@ __global__ __launch_bounds__(256) void rpy_tile_kernel(const double *__restrict_
 }
 
 // --------------------------------------------------------------- product
-// Blocks of 256 threads take work units (2 tile rows x 4 tile columns of one
-// chain) in turn, each described by one precomputed record.  Three tiles are
-// in flight at any time: each thread loads 16 elements of a tile into
-// registers (FP32: four 16-byte loads; FP64: sixteen 8-byte loads, which
-// measured faster than 16-byte ones there), and a register set is refilled
-// with the tile three slots ahead as soon as its own tile is staged in LDS.
-// Occupancy: three blocks per CU for FP32, two for FP64 (its register sets
-// are twice as wide) and for blocks wider than 4 columns.  Within a tile the four
-// waves split the work by quarters:
-//   row product    y_I[r] += sum_c D(r,c) x_J[c]   (lane r, 16 columns);
-//   column product z_J[c] += sum_r D(r,c) x_I[r]   (lane c, 16 rows);
-// the padded tile makes both LDS patterns conflict-free and the x reads are
-// broadcasts.  Row sums are combined over the quarters once per unit row,
-// column sums once per unit column, through a small LDS buffer.
-// rowpart: (ntile * nsc) x TILE x S; colpart: nunits x WSC x TILE x S.
+// Blocks of 256 threads take work units (CHR tile rows x WSC tile columns of
+// one chain) in turn, each described by one precomputed record.  Thread t
+// holds a 4 x 4 piece of every tile it touches -- rows 4 (t % 16) .. + 3 of
+// columns t / 16 + 16 i, i = 0..3 -- loaded as four 16-byte (FP32) or eight
+// 16-byte (FP64) pieces, and computes both products of the tile from its
+// registers:
+//   row product    y_I[r] += sum_c D(r, c) x_J[c]   (over its 4 columns),
+//   column product z_J[c] += sum_r D(r, c) x_I[r]   (over its 4 rows),
+// so tiles never pass through shared memory and no barrier separates them.
+// The tiles of a unit are visited column by column (PD tiles in flight: the
+// registers of a tile are refilled with the tile PD slots ahead as soon as
+// its values are converted to FP64).  Column sums are complete after the
+// unit's CHR rows and are summed over the 16 row pieces through shared
+// memory (one partial per unit and tile column); row sums accumulate over
+// the WSC columns and are summed over the 16 column pieces at the end (one
+// partial per tile row and super-column).  A second kernel adds the partials
+// of every output block in a fixed order, so results are deterministic run
+// to run.  rowpart: (ntile * nsc) x TILE x S; colpart: nunits x WSC x TILE x S.
 constexpr int TPT = TILE * TILE / NT;   // tile elements per thread (16)
-// tiles in flight per block (register sets; fewer for wide blocks, whose
-// accumulators take the registers) and blocks per CU
+// tiles in flight per block (register sets) and blocks per CU
 template <typename E, int S>
-constexpr int PDT = S <= 4 ? 3 : 2;
-template <typename E, int S>
-constexpr int OCC = (sizeof(E) == 8 || S > 4) ? 2 : 3;
+constexpr int PDT = (sizeof(E) == 8 || S > 4) ? 2 : 3;
+template <typename E, int S, int CHR>
+constexpr int OCC = (S > 4 || CHR > 2) ? 1 : 2;   // register budget 256 (512 when 1)
 
-// Register slot i of thread tid holds tile element tile_elem<E>(tid, i).
-template <typename E>
-__device__ inline int tile_elem(int tid, int i)
-{
-    if constexpr (sizeof(E) == 4) return 4 * (tid + (i / 4) * NT) + (i % 4);
-    else return tid + i * NT;
-}
-
+// Register slot 4 i + q of thread tid holds tile element (row 4 (tid % 16) + q,
+// column tid / 16 + 16 i), i.e. element 4 (tid + 256 i) + q of the
+// column-major tile.
 template <typename E>
 __device__ inline void load_tile16(const E *__restrict__ tp, int tid, E (&t)[TPT])
 {
@@ -146,22 +151,28 @@ __device__ inline void load_tile16(const E *__restrict__ tp, int tid, E (&t)[TPT
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < TPT; i++) t[i] = tp[tid + i * NT];
+        for (int i = 0; i < TPT / 4; i++) {
+            const double2 a = reinterpret_cast<const double2 *>(tp)[2 * (tid + i * NT)];
+            const double2 b = reinterpret_cast<const double2 *>(tp)[2 * (tid + i * NT) + 1];
+            t[4 * i] = a.x;
+            t[4 * i + 1] = a.y;
+            t[4 * i + 2] = b.x;
+            t[4 * i + 3] = b.y;
+        }
     }
 }
 
-template <typename E, int S>
-__global__ __launch_bounds__(NT, (OCC<E, S>)) void symm_tiles_kernel(
+template <typename E, int S, int CHR, int WSC>
+__global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
     const E *__restrict__ mob_all, const double *__restrict__ x_all, int ldx, const int *__restrict__ units,
     int nunits, const int *__restrict__ chain_i, const long *__restrict__ chain_l, double *__restrict__ rowpart,
     double *__restrict__ colpart, int *__restrict__ ticket, const DevStatus *stop, const int *__restrict__ active)
 {
-    constexpr int NS = CHR * WSC, PD = PDT<E, S>;
-    __shared__ E T[TILE][TILE + 1];
-    __shared__ double xJs[WSC][TILE][S];
-    __shared__ double xIs[CHR][TILE][S];
-    __shared__ double sred[NT / TILE][TILE][S];
-    const int tid = threadIdx.x, lane = tid % TILE, q = tid / TILE;
+    constexpr int NS = CHR * WSC, PD = PDT<E, S>, LDR = TILE * S + 2;
+    __shared__ double xJs[WSC][S][TILE];
+    __shared__ double xIs[CHR][S][TILE];
+    __shared__ double sred[16 * LDR];
+    const int tid = threadIdx.x, cg = tid / 16, rg = tid % 16;
     (void)chain_i;
     (void)chain_l;
     (void)ticket;
@@ -171,7 +182,7 @@ __global__ __launch_bounds__(NT, (OCC<E, S>)) void symm_tiles_kernel(
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int *rec = units + (long)u * UREC;
         const int chain = rec[0], sc = rec[1], Ib = rec[2], Ie = rec[3], ntile = rec[4], nsc = rec[5];
-        const unsigned mask = (unsigned)rec[6];
+        const unsigned mask = (unsigned)rec[6];   // bit ii * WSC + jj: tile (Ib + ii, J0 + jj) stored
         const long tbase = (long)(unsigned)rec[8] | ((long)rec[9] << 32);
         const long xoff = (long)(unsigned)rec[10] | ((long)rec[11] << 32);
         const long rpo = (long)(unsigned)rec[12] | ((long)rec[13] << 32);
@@ -187,13 +198,19 @@ __global__ __launch_bounds__(NT, (OCC<E, S>)) void symm_tiles_kernel(
         const double *x = x_all + xoff;
         const E *mob = mob_all + tbase;
         const int J0 = sc * WSC;
-        // PD tiles in flight: slot sl lives in register set sl % PD, and the
-        // load of slot sl + PD is issued as soon as slot sl is in LDS
+        // slot sl = jj * CHR + ii (column by column)
+        auto present = [&](int sl) __attribute__((always_inline)) {
+            return (mask >> ((sl % CHR) * WSC + sl / CHR)) & 1u;
+        };
+        auto tile_ptr = [&](int sl) __attribute__((always_inline)) {
+            return mob + tile_index(Ib + sl % CHR, J0 + sl / CHR) * (TILE * TILE);
+        };
         E R[PD][TPT];
 #pragma unroll
         for (int sl = 0; sl < PD; sl++)
-            if (mask & (1u << sl)) load_tile16<E>(mob + tile_index(Ib + sl / WSC, J0 + sl % WSC) * (TILE * TILE), tid, R[sl]);
-        // x blocks (zeros past the chain's last tile), unconditional loads
+            if (sl < NS && present(sl)) load_tile16<E>(tile_ptr(sl), tid, R[sl]);
+        // x blocks (zeros past the chain's last tile), unconditional loads,
+        // k-major in shared memory
         __syncthreads();   // the previous unit no longer reads xJs / xIs / sred
         {
             constexpr int NX = (WSC + CHR) * TILE * S;
@@ -201,76 +218,91 @@ __global__ __launch_bounds__(NT, (OCC<E, S>)) void symm_tiles_kernel(
             for (int i = 0; i < (NX + NT - 1) / NT; i++) {
                 const int e = tid + i * NT;
                 if (e < NX) {
-                    const int blk = e / (TILE * S), rem = e % (TILE * S), r = rem / S, k = rem % S;
+                    const int blk = e / (TILE * S), rem = e % (TILE * S), k = rem / TILE, r = rem % TILE;
                     const int B = blk < WSC ? min(J0 + blk, ntile - 1) : min(Ib + blk - WSC, Ie - 1);
                     const bool in = blk < WSC ? (J0 + blk < ntile) : (Ib + blk - WSC < Ie);
                     const double v = x[(long)k * ldx + (long)B * TILE + r];
-                    if (blk < WSC) xJs[blk][r][k] = in ? v : 0.0;
-                    else xIs[blk - WSC][r][k] = in ? v : 0.0;
+                    if (blk < WSC) xJs[blk][k][r] = in ? v : 0.0;
+                    else xIs[blk - WSC][k][r] = in ? v : 0.0;
                 }
             }
         }
-        double racc[CHR][S], cacc[WSC][S];
+        __syncthreads();
+        double racc[CHR][4][S];
 #pragma unroll
         for (int a = 0; a < CHR; a++)
 #pragma unroll
-            for (int k = 0; k < S; k++) racc[a][k] = 0.0;
+            for (int q = 0; q < 4; q++)
 #pragma unroll
-        for (int a = 0; a < WSC; a++)
+                for (int k = 0; k < S; k++) racc[a][q][k] = 0.0;
 #pragma unroll
-            for (int k = 0; k < S; k++) cacc[a][k] = 0.0;
+        for (int jj = 0; jj < WSC; jj++) {
+            double cacc[4][S];
 #pragma unroll
-        for (int sl = 0; sl < NS; sl++) {
-            const bool here = mask & (1u << sl);
-            const int ii = sl / WSC, jj = sl % WSC;
-            const int I = Ib + ii, J = J0 + jj;
-            if (here) {
-                __syncthreads();   // the previous tile is no longer read
+            for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int i = 0; i < TPT; i++) {
-                    const int e = tile_elem<E>(tid, i);
-                    T[e / TILE][e % TILE] = R[sl % PD][i];
+                for (int k = 0; k < S; k++) cacc[i][k] = 0.0;
+#pragma unroll
+            for (int ii = 0; ii < CHR; ii++) {
+                const int sl = jj * CHR + ii;
+                const bool here = present(sl);
+                double d[4][4];   // d[i][q] = D(4 rg + q, cg + 16 i)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) d[i][q] = (double)R[sl % PD][4 * i + q];
+                if (sl + PD < NS && present(sl + PD)) load_tile16<E>(tile_ptr(sl + PD), tid, R[sl % PD]);
+                if (!here) continue;
+                const bool offdiag = (Ib + ii) != (J0 + jj);
+#pragma unroll
+                for (int k = 0; k < S; k++) {
+                    double xj[4], xi[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) xj[i] = xJs[jj][k][cg + 16 * i];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) xi[q] = xIs[ii][k][4 * rg + q];
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) racc[ii][q][k] += d[i][q] * xj[i];
+                    if (offdiag) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+#pragma unroll
+                            for (int q = 0; q < 4; q++) cacc[i][k] += d[i][q] * xi[q];
+                    }
                 }
             }
-            if (sl + PD < NS && (mask & (1u << (sl + PD))))
-                load_tile16<E>(mob + tile_index(Ib + (sl + PD) / WSC, J0 + (sl + PD) % WSC) * (TILE * TILE), tid,
-                               R[sl % PD]);
-            if (!here) continue;
-            __syncthreads();
-#pragma unroll 4
-            for (int cc = 0; cc < TILE / 4; cc++) {
-                const int c = q * (TILE / 4) + cc;
-                const double d = (double)T[c][lane];
-#pragma unroll
-                for (int k = 0; k < S; k++) racc[ii][k] += d * xJs[jj][c][k];
-            }
-            if (I != J) {
-#pragma unroll 4
-                for (int rr = 0; rr < TILE / 4; rr++) {
-                    const int r = q * (TILE / 4) + rr;
-                    const double d = (double)T[lane][r];
-#pragma unroll
-                    for (int k = 0; k < S; k++) cacc[jj][k] += d * xIs[ii][r][k];
-                }
-            }
-        }
-        // combine quarters: one partial per (tile row, super-column) and per
-        // (unit, tile column)
-#pragma unroll
-        for (int a = 0; a < CHR + WSC; a++) {
-            const bool isrow = a < CHR;
-            if (isrow && Ib + a >= Ie) continue;
+            // column partial of tile column J0 + jj: the 16 row pieces summed
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < S; k++) sred[q][lane][k] = isrow ? racc[a][k] : cacc[a - CHR][k];
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int k = 0; k < S; k++) sred[rg * LDR + (cg + 16 * i) * S + k] = cacc[i][k];
             __syncthreads();
-            double *dst = isrow ? rowpart + (rpo + (long)(Ib + a) * nsc + sc) * (TILE * S)
-                                : colpart + ((long)u * WSC + (a - CHR)) * (TILE * S);
+            double *dst = colpart + ((long)u * WSC + jj) * (TILE * S);
             for (int e = tid; e < TILE * S; e += NT) {
-                const int r = e / S, k = e % S;
                 double v = 0.0;
 #pragma unroll
-                for (int qq = 0; qq < NT / TILE; qq++) v += sred[qq][r][k];
+                for (int g = 0; g < 16; g++) v += sred[g * LDR + e];
+                dst[e] = v;
+            }
+        }
+        // row partials of the unit's tile rows: the 16 column pieces summed
+#pragma unroll
+        for (int ii = 0; ii < CHR; ii++) {
+            if (Ib + ii >= Ie) continue;
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int k = 0; k < S; k++) sred[cg * LDR + (4 * rg + q) * S + k] = racc[ii][q][k];
+            __syncthreads();
+            double *dst = rowpart + (rpo + (long)(Ib + ii) * nsc + sc) * (TILE * S);
+            for (int e = tid; e < TILE * S; e += NT) {
+                double v = 0.0;
+#pragma unroll
+                for (int g = 0; g < 16; g++) v += sred[g * LDR + e];
                 dst[e] = v;
             }
         }
@@ -278,12 +310,42 @@ __global__ __launch_bounds__(NT, (OCC<E, S>)) void symm_tiles_kernel(
 }
 
 // Output block G (a tile row of some chain):
-//   y = sum_{sc <= K/WSC} rowpart[K, sc] + sum_{units u of super-column K/WSC} colpart[u, K % WSC]
+//   y = sum_{sc <= K/wsc} rowpart[K, sc] + sum_{units u of super-column K/wsc} colpart[u, K % wsc]
 // rowblk (8 ints per block): chain, K, nsc, u0, u1, rp_off, y row offset.
+// Sums in a fixed order; loads batched eight at a time (clamped addresses,
+// so all eight are issued together).
+template <int S>
+__device__ inline double sum_block_partials(const double *__restrict__ rowpart, const double *__restrict__ colpart,
+                                            const int *__restrict__ rb, int e, int wsc)
+{
+    const int K = rb[1], nsc = rb[2], u0 = rb[3], u1 = rb[4], rpo = rb[5];
+    double v = 0.0;
+    const int nrow = K / wsc + 1;
+    const double *rp = rowpart + ((long)rpo + (long)K * nsc) * (TILE * S) + e;
+    for (int s0 = 0; s0 < nrow; s0 += 8) {
+        double t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = rp[(long)min(s0 + q, nrow - 1) * (TILE * S)];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (s0 + q < nrow) v += t[q];
+    }
+    const double *cp = colpart + (long)(K % wsc) * (TILE * S) + e;
+    for (int u = u0; u < u1; u += 8) {
+        double t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = cp[(long)min(u + q, u1 - 1) * wsc * (TILE * S)];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (u + q < u1) v += t[q];
+    }
+    return v;
+}
+
 template <int S>
 __global__ void symm_reduce_kernel(const double *__restrict__ rowpart, const double *__restrict__ colpart,
                                    const int *__restrict__ rowblk, int nrowblk, double *__restrict__ y, int ldy,
-                                   int *__restrict__ ticket, const int *__restrict__ active)
+                                   int *__restrict__ ticket, const int *__restrict__ active, int wsc)
 {
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g == 0) *ticket = 0;  // re-arm the work counter for the next product
@@ -291,30 +353,9 @@ __global__ void symm_reduce_kernel(const double *__restrict__ rowpart, const dou
     const int G = (int)(g / (TILE * S));
     const int e = (int)(g % (TILE * S));
     const int *rb = rowblk + 8 * G;
-    const int chain = rb[0], K = rb[1], nsc = rb[2], u0 = rb[3], u1 = rb[4], rpo = rb[5], yrow = rb[6];
-    if (active && !active[chain]) return;
+    if (active && !active[rb[0]]) return;
     const int r = e / S, k = e % S;
-    // fixed summation order; loads batched eight at a time
-    double v = 0.0;
-    const int nrow = K / WSC + 1;
-    const double *rp = rowpart + ((long)rpo + (long)K * nsc) * (TILE * S) + e;
-    for (int s0 = 0; s0 < nrow; s0 += 8) {
-        double t[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) t[q] = (s0 + q < nrow) ? rp[(long)(s0 + q) * (TILE * S)] : 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) v += t[q];
-    }
-    const int jj = K % WSC;
-    const double *cp = colpart + (long)jj * (TILE * S) + e;
-    for (int u = u0; u < u1; u += 8) {
-        double t[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) t[q] = (u + q < u1) ? cp[(long)(u + q) * WSC * (TILE * S)] : 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) v += t[q];
-    }
-    y[(long)k * ldy + yrow + r] = v;
+    y[(long)k * ldy + rb[6] + r] = sum_block_partials<S>(rowpart, colpart, rb, e, wsc);
 }
 
 // The same sum for one output block per workgroup (TILE * S threads), fused
@@ -325,45 +366,34 @@ template <int S>
 __global__ __launch_bounds__(TILE * MAX_S) void symm_reduce_lz_kernel(
     const double *__restrict__ rowpart, const double *__restrict__ colpart, const int *__restrict__ rowblk,
     double *__restrict__ W, int ld, const double *__restrict__ Vj, const double *__restrict__ Vp,
-    const double *__restrict__ Bp, double *__restrict__ lzpart, const DevStatus *stop)
+    const double *__restrict__ Bp, double *__restrict__ lzpart, const DevStatus *stop, int wsc)
 {
+    constexpr int NO = 2 * S * S;                 // outputs per block
+    constexpr int NGR = TILE * S / NO;            // row groups of the block sum
+    constexpr int RPG = (TILE + NGR - 1) / NGR;   // rows per group (the last may be short)
     __shared__ int s_done;
     __shared__ double sw[TILE][S];
-    __shared__ double red[TILE][2 * S * S];
+    __shared__ double red[TILE][NO];
+    __shared__ double red2[NGR][NO];
     if (threadIdx.x == 0) s_done = (stop && (stop->converged_j != 0 || stop->ierr != 0)) ? 1 : 0;
     __syncthreads();
     if (s_done) return;
     const int G = blockIdx.x, e = threadIdx.x;
     const int *rb = rowblk + 8 * G;
-    const int K = rb[1], nsc = rb[2], u0 = rb[3], u1 = rb[4], rpo = rb[5], yrow = rb[6];
     const int r = e / S, k = e % S;
-    double v = 0.0;
-    const int nrow = K / WSC + 1;
-    const double *rp = rowpart + ((long)rpo + (long)K * nsc) * (TILE * S) + e;
-    for (int s0 = 0; s0 < nrow; s0 += 8) {
-        double t[8];
+    const long row = rb[6] + r;
+    // this row's vector entries first: their loads overlap the partial sums
+    double vj[S], vp[S];
 #pragma unroll
-        for (int q = 0; q < 8; q++) t[q] = (s0 + q < nrow) ? rp[(long)(s0 + q) * (TILE * S)] : 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) v += t[q];
+    for (int c = 0; c < S; c++) {
+        vj[c] = Vj[(long)c * ld + row];
+        vp[c] = Vp ? Vp[(long)c * ld + row] : 0.0;
     }
-    const int jj = K % WSC;
-    const double *cp = colpart + (long)jj * (TILE * S) + e;
-    for (int u = u0; u < u1; u += 8) {
-        double t[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) t[q] = (u + q < u1) ? cp[(long)(u + q) * WSC * (TILE * S)] : 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) v += t[q];
-    }
-    const long row = yrow + r;
-    double vj[S];
-#pragma unroll
-    for (int c = 0; c < S; c++) vj[c] = Vj[(long)c * ld + row];
+    double v = sum_block_partials<S>(rowpart, colpart, rb, e, wsc);
     if (Vp) {
         double t = 0.0;
 #pragma unroll
-        for (int kp = 0; kp < S; kp++) t += Vp[(long)kp * ld + row] * Bp[k + kp * S];
+        for (int kp = 0; kp < S; kp++) t += vp[kp] * Bp[k + kp * S];
         v = v - t;
     }
     W[(long)k * ld + row] = v;
@@ -375,10 +405,23 @@ __global__ __launch_bounds__(TILE * MAX_S) void symm_reduce_lz_kernel(
         red[r][S * S + c + k * S] = sw[r][c] * v;  // P(c, k)
     }
     __syncthreads();
-    if (e < 2 * S * S) {
+    {
+        // column sums over the 64 rows: NGR groups of RPG rows, then the groups
+        const int o = e % NO, gq = e / NO;
+        if (gq < NGR) {
+            double t = 0.0;
+#pragma unroll
+            for (int q = 0; q < RPG; q++)
+                if (gq * RPG + q < TILE) t += red[gq * RPG + q][o];
+            red2[gq][o] = t;
+        }
+    }
+    __syncthreads();
+    if (e < NO) {
         double t = 0.0;
-        for (int rr = 0; rr < TILE; rr++) t += red[rr][e];
-        lzpart[(long)G * 2 * S * S + e] = t;
+#pragma unroll
+        for (int gq = 0; gq < NGR; gq++) t += red2[gq][e];
+        lzpart[(long)G * NO + e] = t;
     }
 }
 
@@ -403,8 +446,9 @@ __global__ void unpack_kernel(const E *__restrict__ mob, int n, double *__restri
 // Builds the tile list, work units and output-block table of a set of
 // chains with n_c rows each, stored back to back (row offsets row_off_c,
 // tile offsets in the order given).
-int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off)
+int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, int CHR, int WSC)
 {
+    if (!tile_shape_supported(CHR, WSC)) { set_error("unsupported tile unit shape"); return BD_ERR_ARG; }
     const int C = (int)nb.size();
     std::vector<int> tiles, units, rowblk, ci(4 * C);
     std::vector<long> cl(3 * C);
@@ -454,6 +498,8 @@ int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &
         return BD_ERR_ARG;
     }
     p.nchains = C;
+    p.chr = CHR;
+    p.wsc = WSC;
     p.ntiles = toff;
     p.nunits = (int)(units.size() / UREC);
     p.nrowblk = (int)(rowblk.size() / 8);
@@ -539,22 +585,35 @@ int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     // one block per unit up to the occupancy the kernel is built for; beyond
     // that blocks take units in turn
-    const int per_cu = OCC<E, S>;
+    const int per_cu = (S > 4 || p.chr > 2) ? 1 : 2;   // the kernel's occupancy
     int grid = std::max(1, std::min(p.nunits, cus * per_cu));
     if (e0) BD_HIP(hipEventRecord(e0, sm));
-    hipLaunchKernelGGL((symm_tiles_kernel<E, S>), dim3(grid), dim3(NT), 0, sm, mob, x, ldx, p.units, p.nunits,
-                       p.chain_i, p.chain_l, p.rowpart, p.colpart, ticket, stop, active);
+#define BD_TILES(CR, WS)                                                                                         \
+    hipLaunchKernelGGL((symm_tiles_kernel<E, S, CR, WS>), dim3(grid), dim3(NT), 0, sm, mob, x, ldx, p.units,      \
+                       p.nunits, p.chain_i, p.chain_l, p.rowpart, p.colpart, ticket, stop, active)
+    const int shape = p.chr * 16 + p.wsc;
+    if (shape == 2 * 16 + 4) BD_TILES(2, 4);
+    else if constexpr (S <= 4) {
+        if (shape == 1 * 16 + 4) BD_TILES(1, 4);
+        else if (shape == 4 * 16 + 4) BD_TILES(4, 4);
+        else if (shape == 2 * 16 + 2) BD_TILES(2, 2);
+        else { set_error("unsupported tile unit shape"); return BD_ERR_ARG; }
+    } else {
+        set_error("blocks wider than 4 columns use the 2 x 4 tile unit");
+        return BD_ERR_ARG;
+    }
+#undef BD_TILES
     BD_HIP(hipGetLastError());
     if (e1) BD_HIP(hipEventRecord(e1, sm));
     if (epi) {
         hipLaunchKernelGGL((symm_reduce_lz_kernel<S>), dim3(p.nrowblk), dim3(TILE * S), 0, sm, p.rowpart, p.colpart,
-                           p.rowblk, y, ldy, epi->Vj, epi->Vp, epi->Bp, epi->part, stop);
+                           p.rowblk, y, ldy, epi->Vj, epi->Vp, epi->Bp, epi->part, stop, p.wsc);
         BD_HIP(hipGetLastError());
         return BD_OK;
     }
     long tot = (long)p.nrowblk * TILE * S;
     hipLaunchKernelGGL((symm_reduce_kernel<S>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, sm, p.rowpart,
-                       p.colpart, p.rowblk, p.nrowblk, y, ldy, ticket, active);
+                       p.colpart, p.rowblk, p.nrowblk, y, ldy, ticket, active, p.wsc);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }
@@ -567,6 +626,66 @@ BD_INST(float, 1) BD_INST(float, 2) BD_INST(float, 3) BD_INST(float, 4)
 BD_INST(float, 5) BD_INST(float, 6) BD_INST(float, 7) BD_INST(float, 8)
 BD_INST(double, 1) BD_INST(double, 2) BD_INST(double, 3) BD_INST(double, 4)
 BD_INST(double, 5) BD_INST(double, 6) BD_INST(double, 7) BD_INST(double, 8)
+#undef BD_INST
+
+// Autotuner of the work-unit shape: times the product of block width S on
+// the plan's own tile layout for every supported shape (five launches after
+// two warm-up ones, the fastest of the five counts) and rebuilds the plan
+// with the fastest shape.  The time does not depend on the tile values.
+// Choices are remembered per (device, element size, S, chain lengths) for
+// the life of the process, so a plan built twice gets the same shape (and
+// bit-identical results).  times_us[k]: the k-th candidate of kShapes.
+static const int kShapes[4][2] = {{2, 4}, {1, 4}, {4, 4}, {2, 2}};
+
+template <typename E, int S>
+int plan_tune(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, const E *mob,
+              const double *x, double *y, int ld, int *ticket, hipStream_t sm, int device, double *times_us)
+{
+    static std::map<std::string, std::pair<int, std::vector<double>>> cache;
+    std::string key = std::to_string(device) + ":" + std::to_string(sizeof(E)) + ":" + std::to_string(S) + ":";
+    for (int v : nb) key += std::to_string(v) + ",";
+    auto hit = cache.find(key);
+    int best = 0;
+    std::vector<double> t(4, 0.0);
+    if (hit != cache.end()) {
+        best = hit->second.first;
+        t = hit->second.second;
+    } else {
+        hipEvent_t e0, e1;
+        BD_HIP(hipEventCreate(&e0));
+        BD_HIP(hipEventCreate(&e1));
+        for (int k = 0; k < 4; k++) {
+            plan_free(p);
+            BD_TRY(plan_build(p, nb, row_off, kShapes[k][0], kShapes[k][1]));
+            double tmin = 1e30;
+            for (int rep = 0; rep < 7; rep++) {
+                BD_HIP(hipEventRecord(e0, sm));
+                BD_TRY((plan_product<E, S>(p, mob, x, y, ld, ld, ticket, nullptr, nullptr, sm, device, nullptr,
+                                           nullptr)));
+                BD_HIP(hipEventRecord(e1, sm));
+                BD_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                BD_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rep >= 2) tmin = std::min(tmin, 1e3 * (double)ms);
+            }
+            t[k] = tmin;
+            if (k == 0 || tmin < t[best]) best = k;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        cache[key] = {best, t};
+    }
+    plan_free(p);
+    BD_TRY(plan_build(p, nb, row_off, kShapes[best][0], kShapes[best][1]));
+    if (times_us)
+        for (int k = 0; k < 4; k++) times_us[k] = t[k];
+    return BD_OK;
+}
+#define BD_INST(E, S)                                                                                          \
+    template int plan_tune<E, S>(MobPlan &, const std::vector<int> &, const std::vector<long> &, const E *,     \
+                                 const double *, double *, int, int *, hipStream_t, int, double *);
+BD_INST(float, 1) BD_INST(float, 2) BD_INST(float, 3) BD_INST(float, 4)
+BD_INST(double, 1) BD_INST(double, 2) BD_INST(double, 3) BD_INST(double, 4)
 #undef BD_INST
 
 template <typename E, int S>

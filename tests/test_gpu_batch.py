@@ -116,6 +116,7 @@ def test_batch_chain_independence_ensemble():
     tot = sum(3 * nb * s for nb in lengths) * (nsteps // s)
     xi = rng.standard_normal(tot)
     b, chains = _batch(lengths, 900, nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32")
+    b.set_tile_shape(2, 4)   # the single-chain batches below use the same unit shape
     b.run(nsteps, xi)
     out = b.get_coordinates()
     assert all(np.isfinite(o).all() for o in out)
@@ -124,8 +125,49 @@ def test_batch_chain_independence_ensemble():
     per = _split_xi(lengths, s, nsteps // s, xi)
     for c in (0, 7, len(lengths) - 1):
         one, _ = _batch([lengths[c]], 900 + c, nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32")
+        one.set_tile_shape(2, 4)
         one.run(nsteps, per[c])
         assert np.array_equal(one.get_coordinates()[0], out[c]), c
+
+
+@pytest.mark.parametrize("shape", [(2, 4), (1, 4), (4, 4), (2, 2)])
+def test_batch_tile_shapes_match_oracle(oracle, shape):
+    """Every work-unit shape of the tile product gives the oracle's chains
+    (fixed Lanczos iteration count, FP64, 1e-10)."""
+    lengths = [40, 17, 64, 23, 50, 90]
+    s, nsteps, mx = 4, 4, 5
+    _say(f"case: batch tile shape {shape}")
+    tot = sum(3 * nb * s for nb in lengths)
+    xi = np.random.default_rng(17).standard_normal(tot)
+    b, chains = _batch(lengths, 310, nts_mobsam=s, lanc_mxitr=mx, lanc_tol=0.0)
+    b.set_tile_shape(*shape)
+    assert b.tile_shape()[0] == shape
+    b.run(nsteps, xi)
+    out = b.get_coordinates()
+    per = _split_xi(lengths, s, 1, xi)
+    for c, nb in enumerate(lengths):
+        m = fene_wca_chain(nb, 0)
+        m["crd"] = chains[c]
+        so = oracle_system(oracle, m)
+        ref, _, _ = oracle.bd_run(so, chains[c], nsteps, per[c], mob_fctr=oracle.LANC, nts_mobsam=s,
+                                  lanc_mxitr=mx, lanc_tol=0.0)
+        assert rel(out[c], ref) < 1e-10, (shape, c, rel(out[c], ref))
+
+
+def test_batch_autotuner_keeps_the_fastest_shape():
+    from brownpak_amd import BdError
+    rng = np.random.default_rng(5)
+    lengths = [int(x) for x in rng.integers(50, 801, size=32)]
+    b, _ = _batch(lengths, 60, nts_mobsam=4, lanc_mxitr=20, lanc_tol=1e-6, precision="fp32")
+    shape, t = b.tile_shape()
+    shapes = [(2, 4), (1, 4), (4, 4), (2, 2)]
+    assert shape in shapes and np.all(t > 0)
+    assert shape == shapes[int(np.argmin(t))]
+    # a second batch of the same lengths reuses the choice
+    b2, _ = _batch(lengths, 61, nts_mobsam=4, lanc_mxitr=20, lanc_tol=1e-6, precision="fp32")
+    assert b2.tile_shape()[0] == shape
+    with pytest.raises(BdError):
+        b.set_tile_shape(3, 4)
 
 
 def test_batch_overstretch_reports_bond_error():
