@@ -26,6 +26,7 @@ constexpr int BT = 256;            // threads per per-chain block
 constexpr int BNBOUND = 32;        // spectral-bound probes per chain
 constexpr int BDEC = 8;            // node table rows used (kappa <= 1e8)
 constexpr int BNQP = 160;          // node slots per chain (2 * 79 + 1 <= 160)
+constexpr int BJS = 512;           // largest j s of a chain's y combination (the order limit below)
 
 struct BChainDev {
     int nb, n, npad, pad;
@@ -58,7 +59,7 @@ struct Batch {
     // convergence test); run: its Krylov recurrence still advances (also
     // cleared on a breakdown); failed: iteration whose CholeskyQR broke down
     int *active = nullptr, *run = nullptr, *failed = nullptr, *conv = nullptr;
-    int *counters = nullptr;      // [0] active chains, [1] error code
+    int *counters = nullptr;      // [0] active chains, [1] error code, [2] iterations, [3] not converged
     int *host_counters = nullptr; // pinned, 2 slots x 2
     DevStatus *st = nullptr;
     hipEvent_t ev[LZ_NEV] = {};
@@ -73,8 +74,13 @@ struct Batch {
     int cntr = 0;
     long long nts = 0;
     int n_updates = 0, total_iters = 0, n_not_conv = 0;
-    uint64_t seed = 0x5eed, offset = 0;
+    uint64_t seed = 0x5eed;
+    uint64_t *dev_rng = nullptr;  // [0] generator stream position, [1] its copy at a block start
     std::vector<double> stage;    // host staging for injected xi
+    // one mobility block (s steps) with device noise as a HIP graph
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t join_ev = nullptr;
+    double *crd_snap = nullptr;   // coordinates at the start of the block being replayed
     // timing of the product
     bool prof_on = false;
     std::vector<hipEvent_t> prof_ev;
@@ -86,10 +92,11 @@ struct Batch {
 // ------------------------------------------------------------ kernels
 // Gaussian block for all chains: rows beyond a chain's n are zero.
 __global__ void b_gauss_kernel(double *out, long NP, int s, const int *__restrict__ rowblk,
-                               const BChainDev *__restrict__ ch, uint64_t seed, uint64_t offset)
+                               const BChainDev *__restrict__ ch, uint64_t seed, const uint64_t *__restrict__ offp)
 {
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= NP * s) return;
+    const uint64_t offset = *offp;   // the stream position lives on the device (graph replays)
     long rowg = g % NP;
     int c = rowblk[8 * (rowg / TILE)];
     long loc = rowg - ch[c].row;
@@ -111,6 +118,34 @@ __global__ void b_gauss_kernel(double *out, long NP, int s, const int *__restric
     double rr = sqrt(-2.0 * log(u1)), sn, cs;
     sincospi(2.0 * u2, &sn, &cs);
     out[g] = (g & 1) ? rr * sn : rr * cs;
+}
+
+// Advances the generator's stream position (one thread).
+__global__ void b_rng_advance_kernel(uint64_t *offp, uint64_t inc) { *offp += inc; }
+
+// Per-chain Lanczos iteration counts of a solve into counters[2] (sum) and
+// counters[3] (chains that hit the cap), and counters[1] = 5 if a chain's
+// Krylov basis broke down before it converged (the eager path's error).
+__global__ void b_tally_kernel(const int *__restrict__ conv, const int *__restrict__ failed, int C, int dmax,
+                               int *counters)
+{
+    __shared__ int s_it[256], s_nc[256];
+    int it = 0, nc = 0, bad = 0;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+        it += conv[c] ? conv[c] : dmax;
+        nc += conv[c] ? 0 : 1;
+        bad |= (failed[c] && !conv[c]) ? 1 : 0;
+    }
+    s_it[threadIdx.x] = it;
+    s_nc[threadIdx.x] = nc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int a = 0, b = 0;
+        for (int t = 0; t < (int)blockDim.x; t++) { a += s_it[t]; b += s_nc[t]; }
+        counters[2] += a;
+        counters[3] += b;
+    }
+    if (bad) atomicCAS(&counters[1], 0, 5);
 }
 
 // Block-wide sum of an S x S matrix accumulated per thread (BT threads).
@@ -300,6 +335,10 @@ __global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const doub
         return;
     }
     __syncthreads();
+    // C of this chain into shared memory (js S <= BJS S values)
+    __shared__ double sC[BJS * S];
+    for (int e = threadIdx.x; e < js * S; e += BT) sC[e] = Cc[e];
+    __syncthreads();
     const long r0 = ch[c].row;
     const int rows = ch[c].npad;
     double *y = Y + (long)(j % 2) * S * NP + r0;
@@ -312,11 +351,17 @@ __global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const doub
         double acc[S];
 #pragma unroll
         for (int k = 0; k < S; k++) acc[k] = 0.0;
-#pragma unroll 4
-        for (int m = 0; m < js; m++) {
-            double vm = V[(long)m * NP + r0 + r];
+        // sixteen basis columns in flight, added in column order
+        for (int m0 = 0; m0 < js; m0 += 16) {
+            double vm[16];
 #pragma unroll
-            for (int k = 0; k < S; k++) acc[k] += vm * Cc[m + k * js];
+            for (int q = 0; q < 16; q++) vm[q] = V[(long)min(m0 + q, js - 1) * NP + r0 + r];
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+                if (m0 + q < js) {
+#pragma unroll
+                    for (int k = 0; k < S; k++) acc[k] += vm[q] * sC[(m0 + q) + k * js];
+                }
         }
 #pragma unroll
         for (int k = 0; k < S; k++) {
@@ -472,8 +517,11 @@ __global__ void b_finish_kernel(const double *Y, double *out, long NP, int s, in
 }
 
 // ------------------------------------------------------------ host
+// capture: the iterations run to dimk_max without looking at the host
+// (kernels of converged chains are no-ops), nothing waits on the host, and
+// the iteration counts are tallied on the device (counters[2], [3]).
 template <int S>
-static int b_lanczos(Batch &b, int *iters_out)
+static int b_lanczos(Batch &b, int *iters_out, bool capture = false)
 {
     hipStream_t sm = b.stream, ss = b.side;
     const int C = b.C, dmax = b.dimk_max;
@@ -518,7 +566,7 @@ static int b_lanczos(Batch &b, int *iters_out)
                                b.opt.lanc_tol, b.counters);
             BD_HIP(hipGetLastError());
         }
-        if (j % 4 == 0 || j == dmax) {
+        if (!capture && (j % 4 == 0 || j == dmax)) {
             BD_HIP(hipMemcpyAsync(b.host_counters + 2 * slot, b.counters, 2 * sizeof(int), hipMemcpyDeviceToHost, ss));
             BD_HIP(hipEventRecord(b.chunk_ev[slot], ss));
             if (pending >= 0) {
@@ -528,6 +576,16 @@ static int b_lanczos(Batch &b, int *iters_out)
             pending = slot;
             slot ^= 1;
         }
+    }
+    if (capture) {
+        // the side stream rejoins; results of every chain; tallies
+        BD_HIP(hipEventRecord(b.join_ev, ss));
+        BD_HIP(hipStreamWaitEvent(sm, b.join_ev, 0));
+        hipLaunchKernelGGL(b_finish_kernel, dim3((unsigned)((NP * S + 255) / 256)), dim3(256), 0, sm, b.Y, b.diff, NP,
+                           S, dmax, b.conv, b.plan.rowblk);
+        hipLaunchKernelGGL(b_tally_kernel, dim3(1), dim3(256), 0, sm, b.conv, b.failed, C, dmax, b.counters);
+        BD_HIP(hipGetLastError());
+        return BD_OK;
     }
     BD_HIP(hipStreamSynchronize(ss));
     BD_HIP(hipStreamSynchronize(sm));
@@ -574,6 +632,31 @@ static int batch_tune(Batch &b)
     return batch_tune_t<double>(b, b.mob64);
 }
 
+// Step k (1..s) of a mobility block after the Brownian block is ready:
+// forces, drift D.F, the EM update (noise column k - 1 if noise) and the
+// centre-of-mass removal.
+static int b_step_tail(Batch &b, int k, bool noise)
+{
+    const long NP = b.NP;
+    hipStream_t sm = b.stream;
+    const int nbk = (b.NB + 255) / 256;
+    hipLaunchKernelGGL(b_force_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.NB, b.bead_chain, b.bead_local,
+                       b.bead_row, b.chains, b.bond_style, b.dev_bpar, b.lvdw, b.dev_vpar, b.rcutoff * b.rcutoff,
+                       b.excl, b.frc, b.counters);
+    if (b.mob32)
+        BD_TRY((plan_product<float, 1>(b.plan, b.mob32, b.frc, b.dfr, (int)NP, (int)NP, &b.st->ticket, nullptr,
+                                       nullptr, sm, b.device, nullptr, nullptr)));
+    else
+        BD_TRY((plan_product<double, 1>(b.plan, b.mob64, b.frc, b.dfr, (int)NP, (int)NP, &b.st->ticket, nullptr,
+                                        nullptr, sm, b.device, nullptr, nullptr)));
+    const double *dcol = noise ? b.diff + (size_t)(k - 1) * NP : nullptr;
+    hipLaunchKernelGGL(b_update_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.dfr, dcol, b.NB, b.bead_row,
+                       b.opt.tim_stp);
+    hipLaunchKernelGGL(b_com_kernel, dim3(b.C), dim3(256), 0, sm, b.crd, b.chains);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
 static int b_step(Batch &b, const double *xi_host, long long avail, long long *consumed)
 {
     const int s = b.s;
@@ -600,9 +683,9 @@ static int b_step(Batch &b, const double *xi_host, long long avail, long long *c
             *consumed = need;
         } else {
             hipLaunchKernelGGL(b_gauss_kernel, dim3((unsigned)((NP * s + 255) / 256)), dim3(256), 0, sm, b.xi, NP, s,
-                               b.plan.rowblk, b.chains, b.seed, b.offset);
+                               b.plan.rowblk, b.chains, b.seed, b.dev_rng);
+            hipLaunchKernelGGL(b_rng_advance_kernel, dim3(1), dim3(1), 0, sm, b.dev_rng, (uint64_t)NP * s);
             BD_HIP(hipGetLastError());
-            b.offset += (uint64_t)NP * s;
         }
         int rc;
         switch (s) {
@@ -620,24 +703,63 @@ static int b_step(Batch &b, const double *xi_host, long long avail, long long *c
         noise = false;  // bd_solver_m.f90:364
     }
     b.cntr++;
-    const int nbk = (b.NB + 255) / 256;
-    hipLaunchKernelGGL(b_force_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.NB, b.bead_chain, b.bead_local,
-                       b.bead_row, b.chains, b.bond_style, b.dev_bpar, b.lvdw, b.dev_vpar, b.rcutoff * b.rcutoff,
-                       b.excl, b.frc, b.counters);
-    if (b.mob32)
-        BD_TRY((plan_product<float, 1>(b.plan, b.mob32, b.frc, b.dfr, (int)NP, (int)NP, &b.st->ticket, nullptr,
-                                       nullptr, sm, b.device, nullptr, nullptr)));
-    else
-        BD_TRY((plan_product<double, 1>(b.plan, b.mob64, b.frc, b.dfr, (int)NP, (int)NP, &b.st->ticket, nullptr,
-                                        nullptr, sm, b.device, nullptr, nullptr)));
-    const double *dcol = noise ? b.diff + (size_t)(b.cntr - 1) * NP : nullptr;
-    hipLaunchKernelGGL(b_update_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.dfr, dcol, b.NB, b.bead_row,
-                       b.opt.tim_stp);
+    BD_TRY(b_step_tail(b, b.cntr, noise));
     if (b.cntr == s) b.cntr = 0;
-    hipLaunchKernelGGL(b_com_kernel, dim3(b.C), dim3(256), 0, sm, b.crd, b.chains);
-    BD_HIP(hipGetLastError());
     b.nts++;
     return BD_OK;
+}
+
+// One mobility block of s steps with device noise (b.cntr == 0 on entry),
+// device work only -- the sequence the HIP graph holds.
+static int b_block_device(Batch &b)
+{
+    const int s = b.s;
+    const long NP = b.NP;
+    hipStream_t sm = b.stream;
+    if (b.mob32) BD_TRY(rpy_launch<float>(b.plan, b.crd, b.mob32, sm));
+    else BD_TRY(rpy_launch<double>(b.plan, b.crd, b.mob64, sm));
+    hipLaunchKernelGGL(b_gauss_kernel, dim3((unsigned)((NP * s + 255) / 256)), dim3(256), 0, sm, b.xi, NP, s,
+                       b.plan.rowblk, b.chains, b.seed, b.dev_rng);
+    hipLaunchKernelGGL(b_rng_advance_kernel, dim3(1), dim3(1), 0, sm, b.dev_rng, (uint64_t)NP * s);
+    BD_HIP(hipGetLastError());
+    int rc;
+    switch (s) {
+    case 1: rc = b_lanczos<1>(b, nullptr, true); break;
+    case 2: rc = b_lanczos<2>(b, nullptr, true); break;
+    case 3: rc = b_lanczos<3>(b, nullptr, true); break;
+    case 4: rc = b_lanczos<4>(b, nullptr, true); break;
+    default: set_error("batched path supports nts_mobsam 1..4"); return BD_ERR_ARG;
+    }
+    if (rc != BD_OK) return rc;
+    hipLaunchKernelGGL(b_scale_copy_kernel, dim3((unsigned)((NP * s + 255) / 256)), dim3(256), 0, sm, b.diff, b.diff,
+                       NP * s, std::sqrt(2.0 * b.opt.tim_stp));
+    for (int k = 1; k <= s; k++) BD_TRY(b_step_tail(b, k, k == 1 || b.opt.keep_reuse_noise));
+    return BD_OK;
+}
+
+// Captures b_block_device into b.gexec (once per model / plan / seed).
+static int b_graph_build(Batch &b)
+{
+    if (b.gexec) return BD_OK;
+    hipGraph_t g = nullptr;
+    BD_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeRelaxed));
+    int rc = b_block_device(b);
+    hipError_t e = hipStreamEndCapture(b.stream, &g);
+    if (rc != BD_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    BD_HIP(e);
+    e = hipGraphInstantiate(&b.gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    BD_HIP(e);
+    return BD_OK;
+}
+
+static void b_graph_drop(Batch &b)
+{
+    if (b.gexec) (void)hipGraphExecDestroy(b.gexec);
+    b.gexec = nullptr;
 }
 
 }  // namespace bd
@@ -715,7 +837,10 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
     dal(&b.R0, (size_t)nchains * s * s); dal(&b.scr, (size_t)nchains * b.scr_stride);
     dal(&b.Cc, (size_t)nchains * dm * s * s); dal(&b.dev_bpar, MXPARAM); dal(&b.dev_vpar, MXPARAM);
     ial(&b.active, nchains); ial(&b.run, nchains); ial(&b.failed, nchains); ial(&b.conv, nchains);
-    ial(&b.counters, 2); ial(&b.bead_chain, b.NB); ial(&b.bead_local, b.NB);
+    ial(&b.counters, 4); ial(&b.bead_chain, b.NB); ial(&b.bead_local, b.NB);
+    dal(&b.crd_snap, b.NP);
+    if (ok) ok = hipMalloc(&b.dev_rng, 2 * sizeof(uint64_t)) == hipSuccess &&
+                 hipMemset(b.dev_rng, 0, 2 * sizeof(uint64_t)) == hipSuccess;
     if (!ok) return fail("device allocation failed");
     if (hipMalloc(&b.bead_row, b.NB * sizeof(long)) != hipSuccess ||
         hipMalloc(&b.chains, nchains * sizeof(BChainDev)) != hipSuccess ||
@@ -731,6 +856,7 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
         if (hipEventCreateWithFlags(&b.ev[k], hipEventDisableTiming) != hipSuccess) return fail("event failed");
     for (int k = 0; k < 2; k++)
         if (hipEventCreateWithFlags(&b.chunk_ev[k], hipEventDisableTiming) != hipSuccess) return fail("event failed");
+    if (hipEventCreateWithFlags(&b.join_ev, hipEventDisableTiming) != hipSuccess) return fail("event failed");
     // tile unit shape chosen by timing the Lanczos product on this plan
     if (b.mob32) ok = hipMemset(b.mob32, 0, elems * sizeof(float)) == hipSuccess;
     else ok = hipMemset(b.mob64, 0, elems * sizeof(double)) == hipSuccess;
@@ -744,6 +870,10 @@ void bd_batch_destroy(bd_batch *h)
     Batch &b = h->b;
     if (b.stream) (void)hipStreamSynchronize(b.stream);
     if (b.side) (void)hipStreamSynchronize(b.side);
+    b_graph_drop(b);
+    if (b.join_ev) (void)hipEventDestroy(b.join_ev);
+    if (b.crd_snap) (void)hipFree(b.crd_snap);
+    if (b.dev_rng) (void)hipFree(b.dev_rng);
     plan_free(b.plan);
     double **dp[] = {&b.crd, &b.frc, &b.dfr, &b.diff, &b.xi, &b.V, &b.W, &b.Y, &b.A, &b.B, &b.R0, &b.scr,
                      &b.Cc, &b.dev_bpar, &b.dev_vpar, &b.mob64};
@@ -791,6 +921,7 @@ int bd_batch_set_model(bd_batch *h, int bond_style, const double *bond_params, i
     BD_HIP(hipMemcpy(b.dev_bpar, b.bpar, sizeof(b.bpar), hipMemcpyHostToDevice));
     BD_HIP(hipMemcpy(b.dev_vpar, b.vpar, sizeof(b.vpar), hipMemcpyHostToDevice));
     b.have_model = true;
+    b_graph_drop(b);   // model scalars are kernel arguments of the graph
     return BD_OK;
 }
 
@@ -822,8 +953,11 @@ int bd_batch_get_coordinates(bd_batch *h, double *crd)
 int bd_batch_set_rng(bd_batch *h, uint64_t seed, uint64_t offset)
 {
     if (!h) { set_error("null batch"); return BD_ERR_ARG; }
-    h->b.seed = seed;
-    h->b.offset = offset;
+    Batch &b = h->b;
+    BD_HIP(hipStreamSynchronize(b.stream));
+    b.seed = seed;
+    BD_HIP(hipMemcpy(b.dev_rng, &offset, sizeof(uint64_t), hipMemcpyHostToDevice));
+    b_graph_drop(b);   // the seed is a kernel argument of the graph
     return BD_OK;
 }
 
@@ -835,10 +969,47 @@ int bd_batch_run(bd_batch *h, int nsteps, const double *xi, long long nxi, long 
     long long pos = 0;
     hipLaunchKernelGGL(b_com_kernel, dim3(b.C), dim3(256), 0, b.stream, b.crd, b.chains);
     int rc = BD_OK;
-    for (int k = 0; k < nsteps && rc == BD_OK; k++) {
+    // whole mobility blocks with device noise run as one graph launch each;
+    // the rest (injected xi, partial blocks, profiling) step by step
+    const bool graph = !xi && !b.prof_on;
+    for (int k = 0; k < nsteps && rc == BD_OK;) {
+        if (graph && b.cntr == 0 && nsteps - k >= b.s) {
+            rc = b_graph_build(b);
+            if (rc != BD_OK) break;
+            BD_HIP(hipMemcpyAsync(b.crd_snap, b.crd, (size_t)b.NP * sizeof(double), hipMemcpyDeviceToDevice,
+                                  b.stream));
+            BD_HIP(hipMemcpyAsync(b.dev_rng + 1, b.dev_rng, sizeof(uint64_t), hipMemcpyDeviceToDevice, b.stream));
+            BD_HIP(hipGraphLaunch(b.gexec, b.stream));
+            int cnt[4];
+            BD_HIP(hipMemcpyAsync(b.host_counters, b.counters, 4 * sizeof(int), hipMemcpyDeviceToHost, b.stream));
+            BD_HIP(hipStreamSynchronize(b.stream));
+            for (int q = 0; q < 4; q++) cnt[q] = b.host_counters[q];
+            if (cnt[1] >= 2) {
+                // a Lanczos failure: back to the block's start, replayed step by
+                // step so that the error and the state are the eager path's
+                const int zero[4] = {0, 0, 0, 0};
+                BD_HIP(hipMemcpy(b.crd, b.crd_snap, (size_t)b.NP * sizeof(double), hipMemcpyDeviceToDevice));
+                BD_HIP(hipMemcpy(b.dev_rng, b.dev_rng + 1, sizeof(uint64_t), hipMemcpyDeviceToDevice));
+                BD_HIP(hipMemcpy(b.counters + 1, zero + 1, 3 * sizeof(int), hipMemcpyHostToDevice));
+                for (int q = 0; q < b.s && rc == BD_OK; q++, k++) {
+                    long long cons = 0;
+                    rc = b_step(b, nullptr, 0, &cons);
+                }
+                break;
+            }
+            const int zero2[2] = {0, 0};
+            BD_HIP(hipMemcpy(b.counters + 2, zero2, 2 * sizeof(int), hipMemcpyHostToDevice));
+            b.total_iters += cnt[2];
+            b.n_not_conv += cnt[3];
+            b.n_updates++;
+            b.nts += b.s;
+            k += b.s;
+            continue;
+        }
         long long cons = 0;
         rc = b_step(b, xi ? xi + pos : nullptr, xi ? nxi - pos : 0, &cons);
         pos += cons;
+        k++;
     }
     if (used) *used = pos;
     BD_HIP(hipStreamSynchronize(b.stream));
@@ -865,6 +1036,7 @@ int bd_batch_set_tile_shape(bd_batch *h, int chr, int wsc)
     if (!tile_shape_supported(chr, wsc)) { set_error("unsupported tile unit shape"); return BD_ERR_ARG; }
     Batch &b = h->b;
     BD_HIP(hipStreamSynchronize(b.stream));
+    b_graph_drop(b);
     plan_free(b.plan);
     return plan_build(b.plan, b.nb, b.row, chr, wsc);
 }

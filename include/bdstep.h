@@ -253,7 +253,14 @@ int bd_prof_get(bd_ctx *ctx, double *total_ms, long long *launches,
  *   bd_batch_run: nsteps EM steps of every chain.  xi (optional) is the
  *     injected Gaussian stream: at every mobility update it supplies, chain
  *     by chain, an (n_c, s) column-major block (n_c = 3 nb[c]); *used returns
- *     the values consumed.  Without xi a device Philox generator is used.
+ *     the values consumed.  Without xi a device Philox generator is used,
+ *     and every whole mobility block (nts_mobsam steps: assembly, noise,
+ *     block Lanczos to lanc_mxitr with converged chains idle, the steps) is
+ *     one launch of a HIP graph captured at the first such block; the
+ *     result is bit-identical to stepping launch by launch (which the call
+ *     does for injected xi, partial blocks and while profiling).  A Lanczos
+ *     failure inside a graph block restores the block's start and replays
+ *     it step by step, so errors and state are those of the step path.
  *     Errors: BD_ERR_BOND if any bond overstretched during the call.
  *   bd_batch_info: steps taken, mobility updates, summed Lanczos iterations
  *     over chains and updates, and chain-updates that hit lanc_mxitr.
