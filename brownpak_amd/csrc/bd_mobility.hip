@@ -169,6 +169,8 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
     double *__restrict__ colpart, int *__restrict__ ticket, const DevStatus *stop, const int *__restrict__ active)
 {
     constexpr int NS = CHR * WSC, PD = PDT<E, S>, LDR = TILE * S + 2;
+    // x_J[c][k] at xJs[jj][k][(c % 16) * 4 + c / 16] (a thread's 4 columns
+    // c = cg + 16 i are adjacent); x_I[r][k] at xIs[ii][k][r] (its 4 rows are)
     __shared__ double xJs[WSC][S][TILE];
     __shared__ double xIs[CHR][S][TILE];
     __shared__ double sred[16 * LDR];
@@ -205,26 +207,36 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
         auto tile_ptr = [&](int sl) __attribute__((always_inline)) {
             return mob + tile_index(Ib + sl % CHR, J0 + sl / CHR) * (TILE * TILE);
         };
+        // x blocks (zeros past the chain's last tile; unconditional loads at
+        // clamped addresses), issued ahead of the tiles so that they return
+        // first, except where the registers they hold would spill
+        constexpr int NX = (WSC + CHR) * TILE * S, XPT = (NX + NT - 1) / NT;
+        constexpr bool XFIRST = !(sizeof(E) == 8 && S > 2 && CHR * WSC == 8);
+        double xv[XPT];
+        auto load_x = [&]() __attribute__((always_inline)) {
+#pragma unroll
+            for (int i = 0; i < XPT; i++) {
+                const int e = min(tid + i * NT, NX - 1);
+                const int blk = e / (TILE * S), rem = e % (TILE * S), k = rem / TILE, r = rem % TILE;
+                const int B = blk < WSC ? min(J0 + blk, ntile - 1) : min(Ib + blk - WSC, Ie - 1);
+                xv[i] = x[(long)k * ldx + (long)B * TILE + r];
+            }
+        };
+        if constexpr (XFIRST) load_x();
         E R[PD][TPT];
 #pragma unroll
         for (int sl = 0; sl < PD; sl++)
             if (sl < NS && present(sl)) load_tile16<E>(tile_ptr(sl), tid, R[sl]);
-        // x blocks (zeros past the chain's last tile), unconditional loads,
-        // k-major in shared memory
+        if constexpr (!XFIRST) load_x();
         __syncthreads();   // the previous unit no longer reads xJs / xIs / sred
-        {
-            constexpr int NX = (WSC + CHR) * TILE * S;
 #pragma unroll
-            for (int i = 0; i < (NX + NT - 1) / NT; i++) {
-                const int e = tid + i * NT;
-                if (e < NX) {
-                    const int blk = e / (TILE * S), rem = e % (TILE * S), k = rem / TILE, r = rem % TILE;
-                    const int B = blk < WSC ? min(J0 + blk, ntile - 1) : min(Ib + blk - WSC, Ie - 1);
-                    const bool in = blk < WSC ? (J0 + blk < ntile) : (Ib + blk - WSC < Ie);
-                    const double v = x[(long)k * ldx + (long)B * TILE + r];
-                    if (blk < WSC) xJs[blk][k][r] = in ? v : 0.0;
-                    else xIs[blk - WSC][k][r] = in ? v : 0.0;
-                }
+        for (int i = 0; i < XPT; i++) {
+            const int e = tid + i * NT;
+            if (e < NX) {
+                const int blk = e / (TILE * S), rem = e % (TILE * S), k = rem / TILE, r = rem % TILE;
+                const bool in = blk < WSC ? (J0 + blk < ntile) : (Ib + blk - WSC < Ie);
+                if (blk < WSC) xJs[blk][k][(r % 16) * 4 + r / 16] = in ? xv[i] : 0.0;
+                else xIs[blk - WSC][k][r] = in ? xv[i] : 0.0;
             }
         }
         __syncthreads();
@@ -242,6 +254,16 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int k = 0; k < S; k++) cacc[i][k] = 0.0;
+            // this tile column's x at the thread's 4 columns (adjacent slots),
+            // held across the column's tiles unless registers are short
+            constexpr bool HOLD = !(sizeof(E) == 8 && S > 2);
+            double xj[S][4];
+            if constexpr (HOLD) {
+#pragma unroll
+                for (int k = 0; k < S; k++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) xj[k][i] = xJs[jj][k][cg * 4 + i];
+            }
 #pragma unroll
             for (int ii = 0; ii < CHR; ii++) {
                 const int sl = jj * CHR + ii;
@@ -256,15 +278,17 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
                 const bool offdiag = (Ib + ii) != (J0 + jj);
 #pragma unroll
                 for (int k = 0; k < S; k++) {
-                    double xj[4], xi[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) xj[i] = xJs[jj][k][cg + 16 * i];
+                    double xi[4];
 #pragma unroll
                     for (int q = 0; q < 4; q++) xi[q] = xIs[ii][k][4 * rg + q];
+                    if constexpr (!HOLD) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) xj[k][i] = xJs[jj][k][cg * 4 + i];
+                    }
 #pragma unroll
                     for (int q = 0; q < 4; q++)
 #pragma unroll
-                        for (int i = 0; i < 4; i++) racc[ii][q][k] += d[i][q] * xj[i];
+                        for (int i = 0; i < 4; i++) racc[ii][q][k] += d[i][q] * xj[k][i];
                     if (offdiag) {
 #pragma unroll
                         for (int i = 0; i < 4; i++)
