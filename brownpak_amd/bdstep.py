@@ -40,6 +40,9 @@ PT = {"DIR": 0, "VER": 1, "CL": 2}
 MOB_FCTR = {"CHOL": CHOL, "LANC": LANC, "CHOL_LT": CHOL_LT}
 INTEGRATOR = {"EM": 0, "PC": 1}
 MXPARAM = 12
+# work-unit shapes (tile rows, tile columns) of the mobility product, in the
+# order of the batch autotuner's timings (BD_TILE_SHAPES in bdstep.h)
+TILE_SHAPES = ((2, 4), (1, 4), (4, 4), (2, 2), (2, 5), (2, 6), (2, 8))
 
 DP = C.POINTER(C.c_double)
 IP = C.POINTER(C.c_int)
@@ -450,10 +453,9 @@ class BdBatch:
                 "n_not_converged": nc.value}
 
     def tile_shape(self):
-        """((chr, wsc) in use, autotuner timings in us of the shapes
-        2x4, 1x4, 4x4, 2x2)."""
+        """((chr, wsc) in use, autotuner timings in us of TILE_SHAPES)."""
         a, b = C.c_int(0), C.c_int(0)
-        t = np.zeros(4)
+        t = np.zeros(len(TILE_SHAPES))
         self._chk(self.L.bd_batch_tile_shape(self.h, C.byref(a), C.byref(b), _dp(t)))
         return (a.value, b.value), t
 

@@ -43,7 +43,7 @@ struct Batch {
     std::vector<int> nb;
     std::vector<long> row;
     MobPlan plan;
-    double tune_us[4] = {0, 0, 0, 0};   // unit-shape timings of the autotuner
+    double tune_us[BD_TILE_SHAPES] = {};   // unit-shape timings of the autotuner
     float *mob32 = nullptr;
     double *mob64 = nullptr;
     BChainDev *chains = nullptr;
@@ -1026,7 +1026,7 @@ int bd_batch_tile_shape(bd_batch *h, int *chr, int *wsc, double *times_us)
     if (chr) *chr = b.plan.chr;
     if (wsc) *wsc = b.plan.wsc;
     if (times_us)
-        for (int k = 0; k < 4; k++) times_us[k] = b.tune_us[k];
+        for (int k = 0; k < BD_TILE_SHAPES; k++) times_us[k] = b.tune_us[k];
     return BD_OK;
 }
 

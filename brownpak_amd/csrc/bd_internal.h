@@ -76,7 +76,7 @@ struct MobPlan {
     long ntiles = 0;
     int nunits = 0, nrowblk = 0;
     int *tiles = nullptr;     // 4 ints per stored tile: chain, I, J, 0
-    int *units = nullptr;     // 4 ints per unit: chain, super-column, Ib, Ie
+    int *units = nullptr;     // 16 ints per work unit (the record of plan_build)
     int *rowblk = nullptr;    // 8 ints per output block
     int *chain_i = nullptr;
     long *chain_l = nullptr;
@@ -87,10 +87,12 @@ struct LzEpi {
     const double *Vj, *Vp, *Bp;
     double *part;
 };
+extern const int kTileShapes[BD_TILE_SHAPES][2];
 bool tile_shape_supported(int chr, int wsc);
+void plan_pick_shape(int nb, int cus, int &chr, int &wsc);
 int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, int chr = 2, int wsc = 4);
 // Times the supported unit shapes on this plan (block width S) and rebuilds
-// it with the fastest; times_us (4 entries) receives the measured times.
+// it with the fastest; times_us (BD_TILE_SHAPES entries) receives the times.
 template <typename E, int S>
 int plan_tune(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, const E *mob,
               const double *x, double *y, int ld, int *ticket, hipStream_t sm, int device, double *times_us);
@@ -127,6 +129,7 @@ struct Ctx {
 
     // symmetric product plan
     MobPlan plan;
+    MobPlan plan_wide;   // 2 x 4 units for blocks wider than 4 columns (built on first use)
     int nunits = 0;
 
     // Lanczos workspace

@@ -21,12 +21,49 @@
 namespace bd {
 
 // Work unit: CHR tile rows x WSC tile columns (a "super-column" is WSC tile
-// columns).  The default is 2 x 4; 1 x 4, 4 x 4 and 2 x 2 are the other
-// shapes the product is built for (blocks of at most 4 columns), chosen per
-// plan by plan_tune.
+// columns).  2 x 4 is the default and the only shape for blocks wider than 4
+// columns; the others serve blocks of at most 4 columns.  A single chain's
+// plan takes its shape from plan_pick_shape, a batch's from plan_tune.
+const int kTileShapes[BD_TILE_SHAPES][2] = {{2, 4}, {1, 4}, {4, 4}, {2, 2}, {2, 5}, {2, 6}, {2, 8}};
+
 bool tile_shape_supported(int chr, int wsc)
 {
-    return (chr == 2 && wsc == 4) || (chr == 1 && wsc == 4) || (chr == 4 && wsc == 4) || (chr == 2 && wsc == 2);
+    for (const auto &s : kTileShapes)
+        if (s[0] == chr && s[1] == wsc) return true;
+    return false;
+}
+
+// Work units of one chain of nt tile rows (the loop of plan_build).
+static long units_of(int nt, int chr, int wsc)
+{
+    long u = 0;
+    for (int sc = 0; sc * wsc < nt; sc++) u += (nt - sc * wsc + chr - 1) / chr;
+    return u;
+}
+
+// Shape of a single chain's plan, from a model of the product's duration:
+// units run in ceil(units / slots) rounds of slots = cus x blocks per CU, and
+// a round costs its unit's tile count plus a fixed 5 tiles' worth (staging x,
+// the reductions, the latency of the first loads).  The model is a function
+// of the chain length and the CU count only, so a chain gets the same shape,
+// and bit-identical products, in every process.  4 x 4 (one block per CU)
+// is left to explicit choice.
+void plan_pick_shape(int nb, int cus, int &chr, int &wsc)
+{
+    const int nt = (3 * nb + TILE - 1) / TILE;
+    double best = 1e300;
+    chr = 2;
+    wsc = 4;
+    for (const auto &s : kTileShapes) {
+        if (s[0] > 2) continue;
+        const long slots = (long)std::max(cus, 1) * 2;
+        const double cost = (double)((units_of(nt, s[0], s[1]) + slots - 1) / slots) * (5.0 + s[0] * s[1]);
+        if (cost < best) {
+            best = cost;
+            chr = s[0];
+            wsc = s[1];
+        }
+    }
 }
 constexpr int NT = 256;  // threads per block of the product kernel
 // Work-unit record (16 ints): chain, super-column, Ib, Ie, ntile, nsc,
@@ -211,7 +248,7 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
         // clamped addresses), issued ahead of the tiles so that they return
         // first, except where the registers they hold would spill
         constexpr int NX = (WSC + CHR) * TILE * S, XPT = (NX + NT - 1) / NT;
-        constexpr bool XFIRST = !(sizeof(E) == 8 && S > 2 && CHR * WSC == 8);
+        constexpr bool XFIRST = !(sizeof(E) == 8 && S > 2 && CHR == 2 && WSC >= 4);
         double xv[XPT];
         auto load_x = [&]() __attribute__((always_inline)) {
 #pragma unroll
@@ -564,7 +601,10 @@ int mob_alloc(Ctx &c)
     size_t elems = (size_t)c.ntiles_lower * TILE * TILE;
     if (c.opt.precision == BD_FP32) BD_HIP(hipMalloc(&c.mob32, elems * sizeof(float)));
     else BD_HIP(hipMalloc(&c.mob64, elems * sizeof(double)));
-    BD_TRY(plan_build(c.plan, std::vector<int>{c.nb}, std::vector<long>{0}));
+    int cus = 256, chr = 2, wsc = 4;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device);
+    plan_pick_shape(c.nb, cus, chr, wsc);
+    BD_TRY(plan_build(c.plan, std::vector<int>{c.nb}, std::vector<long>{0}, chr, wsc));
     c.nunits = c.plan.nunits;
     return BD_OK;
 }
@@ -575,6 +615,7 @@ void mob_free(Ctx &c)
     if (c.mob32) (void)hipFree(c.mob32);
     c.mob64 = nullptr; c.mob32 = nullptr;
     plan_free(c.plan);
+    plan_free(c.plan_wide);
 }
 
 template <typename E>
@@ -621,6 +662,9 @@ int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int
         if (shape == 1 * 16 + 4) BD_TILES(1, 4);
         else if (shape == 4 * 16 + 4) BD_TILES(4, 4);
         else if (shape == 2 * 16 + 2) BD_TILES(2, 2);
+        else if (shape == 2 * 16 + 5) BD_TILES(2, 5);
+        else if (shape == 2 * 16 + 6) BD_TILES(2, 6);
+        else if (shape == 2 * 16 + 8) BD_TILES(2, 8);
         else { set_error("unsupported tile unit shape"); return BD_ERR_ARG; }
     } else {
         set_error("blocks wider than 4 columns use the 2 x 4 tile unit");
@@ -658,8 +702,7 @@ BD_INST(double, 5) BD_INST(double, 6) BD_INST(double, 7) BD_INST(double, 8)
 // with the fastest shape.  The time does not depend on the tile values.
 // Choices are remembered per (device, element size, S, chain lengths) for
 // the life of the process, so a plan built twice gets the same shape (and
-// bit-identical results).  times_us[k]: the k-th candidate of kShapes.
-static const int kShapes[4][2] = {{2, 4}, {1, 4}, {4, 4}, {2, 2}};
+// bit-identical results).  times_us[k]: the k-th shape of kTileShapes.
 
 template <typename E, int S>
 int plan_tune(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, const E *mob,
@@ -670,7 +713,7 @@ int plan_tune(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &r
     for (int v : nb) key += std::to_string(v) + ",";
     auto hit = cache.find(key);
     int best = 0;
-    std::vector<double> t(4, 0.0);
+    std::vector<double> t(BD_TILE_SHAPES, 0.0);
     if (hit != cache.end()) {
         best = hit->second.first;
         t = hit->second.second;
@@ -678,9 +721,9 @@ int plan_tune(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &r
         hipEvent_t e0, e1;
         BD_HIP(hipEventCreate(&e0));
         BD_HIP(hipEventCreate(&e1));
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < BD_TILE_SHAPES; k++) {
             plan_free(p);
-            BD_TRY(plan_build(p, nb, row_off, kShapes[k][0], kShapes[k][1]));
+            BD_TRY(plan_build(p, nb, row_off, kTileShapes[k][0], kTileShapes[k][1]));
             double tmin = 1e30;
             for (int rep = 0; rep < 7; rep++) {
                 BD_HIP(hipEventRecord(e0, sm));
@@ -700,9 +743,9 @@ int plan_tune(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &r
         cache[key] = {best, t};
     }
     plan_free(p);
-    BD_TRY(plan_build(p, nb, row_off, kShapes[best][0], kShapes[best][1]));
+    BD_TRY(plan_build(p, nb, row_off, kTileShapes[best][0], kTileShapes[best][1]));
     if (times_us)
-        for (int k = 0; k < 4; k++) times_us[k] = t[k];
+        for (int k = 0; k < BD_TILE_SHAPES; k++) times_us[k] = t[k];
     return BD_OK;
 }
 #define BD_INST(E, S)                                                                                          \
@@ -730,7 +773,15 @@ static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, 
         e1 = c.prof_ev[c.prof_used + 1];
         c.prof_used += 2;
     }
-    return plan_product<E, S>(c.plan, mob, x, y, ldx, ldy, &c.st->ticket, stop, nullptr, c.stream, c.device, e0, e1,
+    const MobPlan *pl = &c.plan;
+    if constexpr (S > 4) {
+        // only the 2 x 4 unit is built for these widths
+        if (c.plan.chr != 2 || c.plan.wsc != 4) {
+            if (!c.plan_wide.units) BD_TRY(plan_build(c.plan_wide, std::vector<int>{c.nb}, std::vector<long>{0}, 2, 4));
+            pl = &c.plan_wide;
+        }
+    }
+    return plan_product<E, S>(*pl, mob, x, y, ldx, ldy, &c.st->ticket, stop, nullptr, c.stream, c.device, e0, e1,
                               epi);
 }
 

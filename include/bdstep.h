@@ -267,13 +267,16 @@ int bd_prof_get(bd_ctx *ctx, double *total_ms, long long *launches,
  *   bd_batch_prof: as bd_prof_enable + bd_prof_get for the batch (on < 0
  *     reads without changing the state).
  *   Tile shapes: the mobility product works in units of chr x wsc tiles of
- *     64 x 64; bd_batch_create times the supported shapes (2x4, 1x4, 4x4,
- *     2x2) on the batch's own tile layout and keeps the fastest (the choice
- *     is remembered for the process, per chain-length list).
- *     bd_batch_tile_shape returns the shape in use and the four timings
- *     (us per product, 0 if not timed); bd_batch_set_tile_shape forces a
- *     shape.  Different shapes sum the partial products in different
- *     groupings, so results agree to rounding, not bit for bit. */
+ *     64 x 64; bd_batch_create times the BD_TILE_SHAPES supported shapes
+ *     (in this order: 2x4, 1x4, 4x4, 2x2, 2x5, 2x6, 2x8) on the batch's own
+ *     tile layout and keeps the fastest (the choice is remembered for the
+ *     process, per chain-length list).  bd_batch_tile_shape returns the
+ *     shape in use and the BD_TILE_SHAPES timings (us per product, 0 if not
+ *     timed); bd_batch_set_tile_shape forces a shape.  Different shapes sum
+ *     the partial products in different groupings, so results agree to
+ *     rounding, not bit for bit.  (A single-chain context picks its shape
+ *     from the chain length and the GPU's CU count, without timing.) */
+#define BD_TILE_SHAPES 7
 typedef struct bd_batch bd_batch;
 bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts);
 void bd_batch_destroy(bd_batch *batch);
@@ -287,7 +290,7 @@ int bd_batch_run(bd_batch *batch, int nsteps, const double *xi, long long nxi,
                  long long *used);
 int bd_batch_info(bd_batch *batch, long long *nts, int *mob_updates,
                   int *total_iters, int *n_not_converged);
-int bd_batch_tile_shape(bd_batch *batch, int *chr, int *wsc, double *times_us4);
+int bd_batch_tile_shape(bd_batch *batch, int *chr, int *wsc, double *times_us);
 int bd_batch_set_tile_shape(bd_batch *batch, int chr, int wsc);
 int bd_batch_prof(bd_batch *batch, int on, double *total_ms,
                   long long *launches, double *tile_bytes);

@@ -130,7 +130,7 @@ def test_batch_chain_independence_ensemble():
         assert np.array_equal(one.get_coordinates()[0], out[c]), c
 
 
-@pytest.mark.parametrize("shape", [(2, 4), (1, 4), (4, 4), (2, 2)])
+@pytest.mark.parametrize("shape", [(2, 4), (1, 4), (4, 4), (2, 2), (2, 5), (2, 6), (2, 8)])
 def test_batch_tile_shapes_match_oracle(oracle, shape):
     """Every work-unit shape of the tile product gives the oracle's chains
     (fixed Lanczos iteration count, FP64, 1e-10)."""
@@ -160,7 +160,9 @@ def test_batch_autotuner_keeps_the_fastest_shape():
     lengths = [int(x) for x in rng.integers(50, 801, size=32)]
     b, _ = _batch(lengths, 60, nts_mobsam=4, lanc_mxitr=20, lanc_tol=1e-6, precision="fp32")
     shape, t = b.tile_shape()
-    shapes = [(2, 4), (1, 4), (4, 4), (2, 2)]
+    from brownpak_amd.bdstep import TILE_SHAPES
+    shapes = list(TILE_SHAPES)
+    assert len(t) == len(shapes) == 7
     assert shape in shapes and np.all(t > 0)
     assert shape == shapes[int(np.argmin(t))]
     # a second batch of the same lengths reuses the choice
