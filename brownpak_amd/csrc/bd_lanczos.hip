@@ -785,14 +785,14 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
             for (int q = 0; q < NODES; q++) acc += err ? 0.0 : s_buf[(size_t)q * js * S + e];
             pw[e] = acc;
         }
-        __threadfence();
+        fence_publish();
         __syncthreads();
         // the batch completing the count is this workgroup's last one
         if (t == 0 && atomicAdd(&ss->cnt, 1) == nact - 1) s_last = 1;
         __syncthreads();
     }
     if (!s_last) return;
-    __threadfence();
+    fence_collect();
     if (t == 0) {
         ss->cnt = 0;
         ss->kb[parity] = kb_found;
@@ -969,12 +969,12 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
             part[(long)blockIdx.x * NV + t] = a;
         }
     }
-    __threadfence();
+    fence_publish();
     __syncthreads();
     if (threadIdx.x == 0) s_last = (atomicAdd(&st->ycnt, 1) == (int)gridDim.x - 1) ? 1 : 0;
     __syncthreads();
     if (!s_last) return;
-    __threadfence();
+    fence_collect();
     // part[b][g][e] summed over the blocks b (ends with a barrier)
     block_sum_parts(part, LG * 2 * S, gridDim.x, s_scr, &s_nrm[0][0]);
     if (threadIdx.x != 0) return;
@@ -1077,12 +1077,12 @@ __global__ __launch_bounds__(256) void ycomb_kernel(const double *__restrict__ V
         }
     }
     // last block: sum the partials in block order and test
-    __threadfence();
+    fence_publish();
     __syncthreads();
     if (threadIdx.x == 0) s_last = (atomicAdd(&st->ycnt, 1) == (int)gridDim.x - 1) ? 1 : 0;
     __syncthreads();
     if (!s_last) return;
-    __threadfence();
+    fence_collect();
     block_sum_parts(part, 2 * S, gridDim.x, s_scr, s_nrm);   // ends with a barrier
     if (threadIdx.x != 0) return;
     st->ycnt = 0;

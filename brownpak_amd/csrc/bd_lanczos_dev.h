@@ -17,6 +17,15 @@ static __constant__ double c_qh[14] = {0.205, 0.205, 0.155, 0.125, 0.1, 0.085, 0
 
 static __device__ inline bool lz_done(const DevStatus *st) { return st->converged_j != 0 || st->ierr != 0; }
 
+// The last-block pattern: every workgroup stores its partials, each wave
+// fences (release, device scope), the workgroup synchronises, and one
+// thread bumps a counter; the workgroup that completes the count fences
+// (acquire, device scope) before it reads the others' partials.  A full
+// fence in every workgroup would also make each one invalidate its XCD's
+// L2, throwing away what the other workgroups there are still reading.
+static __device__ inline void fence_publish() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+static __device__ inline void fence_collect() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
 // ------------------------------------------------------------ reductions
 // Sums of the S*S per-thread values acc over the RB threads of the block
 // into out (global), in a fixed order: through shared memory, 16 values at
@@ -706,12 +715,12 @@ __device__ bool small_problem_split(const double *__restrict__ Ag, const double 
         for (int k = 0; k < nmine; k++) acc += sF[(b * E + OFF_F<S> + rc) * NPB + k];
         pw[e] = acc;
     }
-    __threadfence();
+    fence_publish();
     __syncthreads();
     if (t == 0) s_last = (atomicAdd(&ss->cnt, 1) == nact - 1) ? 1 : 0;
     __syncthreads();
     if (!s_last) return false;
-    __threadfence();
+    fence_collect();
     if (t == 0) {
         ss->cnt = 0;
         ss->kb[parity] = kb;
