@@ -658,6 +658,8 @@ __global__ __launch_bounds__(sp::WG_T) void small_group_kernel(const double *__r
 // with the first bound that passes; every workgroup reaches the same
 // decision.  Each workgroup writes the weighted sum of its nodes' Z in
 // part[wg]; the last one sums the parts in workgroup order and forms C = H Z.
+// Grid: x = member, y = node batch, so that the batches that have nodes
+// (the low ones) of every member are dispatched before the idle ones.
 template <int S>
 __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *__restrict__ Ablk,
                                                                  const double *__restrict__ Bblk,
@@ -673,14 +675,14 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
     __shared__ double sA[ROWS * SS], sB[ROWS * SS], sR[SS];
     __shared__ double s_red[WAVES];
     __shared__ int s_done, s_ok[WAVES], s_last;
-    const int j = j0 + (int)blockIdx.y;
+    const int j = j0 + (int)blockIdx.x;
     if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
     __syncthreads();
     if (s_done || j > jend || j < 2) return;
     const int nb = j, js = nb * S, t = threadIdx.x, w = t / 64, lane = t % 64;
     const int parity = ((j0 - 1) / LG) & 1;
-    sp::SplitState *ss = ssv + blockIdx.y;
-    double *P = part + blockIdx.y * part_stride;
+    sp::SplitState *ss = ssv + blockIdx.x;
+    double *P = part + blockIdx.x * part_stride;
     for (int e = t; e < nb * SS; e += THREADS) {
         sA[e] = Ablk[e];
         sB[e] = Bblk[e];
@@ -709,7 +711,7 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
         int Ng;
         if (place(kb_guess, mg, Ng, hg) <= DEC_MAX) nact_g = (2 * Ng + 1 + NODES - 1) / NODES;
     }
-    if ((int)blockIdx.x >= nact_g) return;
+    if ((int)blockIdx.y >= nact_g) return;
     // this wave's slots
     double *buf = s_buf + (w < PROBES ? w * 2 * SS * ROWS : (PROBES * 2 + (w - PROBES) * 3) * SS * ROWS);
     double Z[S][S];
@@ -743,7 +745,7 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
         if (lane == 0) s_ok[w] = (active && ok) ? 1 : 0;
         __syncthreads();
     };
-    round(kb_guess, 1, true, kb_guess, blockIdx.x);
+    round(kb_guess, 1, true, kb_guess, (int)blockIdx.y);
     int use = kb_guess, kb_found = -1;
     if (s_ok[0]) {
         kb_found = kb_guess;
@@ -763,12 +765,12 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
     else if (place(use, m, N, h) > DEC_MAX) err = 3;
     const int nq = err ? 1 : 2 * N + 1;
     const int nact = err ? 1 : (nq + NODES - 1) / NODES;
-    // node batches blockIdx.x, blockIdx.x + nact_g, ... of the placement in
+    // node batches blockIdx.y, blockIdx.y + nact_g, ... of the placement in
     // use (one, unless the fallback placed more nodes than the hint did)
     if (t == 0) s_last = 0;
     __syncthreads();
-    for (int b = blockIdx.x; b < nact; b += nact_g) {
-        if (!err && !(b == (int)blockIdx.x && use == kb_guess)) round(-1, 0, true, use, b);   // probe wave idle
+    for (int b = (int)blockIdx.y; b < nact; b += nact_g) {
+        if (!err && !(b == (int)blockIdx.y && use == kb_guess)) round(-1, 0, true, use, b);   // probe wave idle
         // weighted node solutions summed in wave order into part[b]
         __syncthreads();
         if (!err && w >= PROBES && lane < nb) {
@@ -783,16 +785,15 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
         for (int e = t; e < js * S; e += THREADS) {
             double acc = 0.0;
             for (int q = 0; q < NODES; q++) acc += err ? 0.0 : s_buf[(size_t)q * js * S + e];
-            pw[e] = acc;
+            store_dev(pw + e, acc);
         }
-        fence_publish();
+        stores_complete();
         __syncthreads();
         // the batch completing the count is this workgroup's last one
         if (t == 0 && atomicAdd(&ss->cnt, 1) == nact - 1) s_last = 1;
         __syncthreads();
     }
     if (!s_last) return;
-    fence_collect();
     if (t == 0) {
         ss->cnt = 0;
         ss->kb[parity] = kb_found;
@@ -807,9 +808,9 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
     }
     if (err) return;
     double *sZ = s_buf;
-    for (int e = t; e < js * S; e += THREADS) sZ[e] = sum_strided(P + e, (long)js * S, nact);
+    for (int e = t; e < js * S; e += THREADS) sZ[e] = sum_strided<true>(P + e, (long)js * S, nact);
     __syncthreads();
-    double *Cout = Cg + blockIdx.y * c_stride;
+    double *Cout = Cg + blockIdx.x * c_stride;
     for (int e = t; e < js * S; e += THREADS) {
         const int i = e % js, c = e / js;
         Cout[e] = sp::block_row_dot<S>(sA, sB, nb, i, sZ + (long)c * js);
@@ -837,6 +838,21 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
     const int ng = jend - j0 + 1;               // members (<= LG)
     const int jsmax = jend * S;
     if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    // this thread's entries of V (one column range of at most YC columns:
+    // jsmax <= YJS <= YG * YC) and of y_{j0-1}, loaded before the
+    // coefficients so that the memory round trips overlap
+    static_assert(YG * YC >= YJS, "one pass over the column range");
+    const int lr = threadIdx.x % YR, gq = threadIdx.x / YR;
+    const int r = blockIdx.x * YR + lr;
+    const int rr = min(r, rows - 1);
+    const int per = (jsmax + YG - 1) / YG, m0 = gq * per, m1 = min(jsmax, m0 + per);
+    double vm[YC];
+#pragma unroll
+    for (int q = 0; q < YC; q++) vm[q] = (m0 + q < m1) ? V[(long)(m0 + q) * ld + rr] : 0.0;
+    double prev[S];
+#pragma unroll
+    for (int k = 0; k < S; k++)
+        prev[k] = (threadIdx.x < YR && r < rows && j0 >= 3) ? Y[((long)((j0 - 1) % NR) * S + k) * ld + r] : 0.0;
     {
         // the members' coefficients (row-major into s_C), all loads of a
         // thread issued before the first store
@@ -875,30 +891,21 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
     }
     __syncthreads();
     if (s_done) return;
-    const int lr = threadIdx.x % YR, gq = threadIdx.x / YR;
-    const int r = blockIdx.x * YR + lr;
-    const int rr = min(r, rows - 1);
-    const int per = (jsmax + YG - 1) / YG, m0 = gq * per, m1 = min(jsmax, m0 + per);
     double acc[LG][S];
 #pragma unroll
     for (int g = 0; g < LG; g++)
 #pragma unroll
         for (int k = 0; k < S; k++) acc[g][k] = 0.0;
-    for (int c0 = m0; c0 < m1; c0 += YC) {
-        double vm[YC];
 #pragma unroll
-        for (int q = 0; q < YC; q++) vm[q] = V[(long)min(c0 + q, jsmax - 1) * ld + rr];
+    for (int q = 0; q < YC; q++) {
+        const int m = m0 + q;
+        if (m < m1) {
 #pragma unroll
-        for (int q = 0; q < YC; q++) {
-            const int m = c0 + q;
-            if (m < m1) {
+            for (int g = 0; g < LG; g++)
+                if (g < ng && j0 + g >= 2 && m < (j0 + g) * S) {
 #pragma unroll
-                for (int g = 0; g < LG; g++)
-                    if (g < ng && j0 + g >= 2 && m < (j0 + g) * S) {
-#pragma unroll
-                        for (int k = 0; k < S; k++) acc[g][k] += vm[q] * s_C[g][m * S + k];
-                    }
-            }
+                    for (int k = 0; k < S; k++) acc[g][k] += vm[q] * s_C[g][m * S + k];
+                }
         }
     }
 #pragma unroll
@@ -909,13 +916,6 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         double dn[LG][S], on[LG][S];
-        double prev[S];
-#pragma unroll
-        for (int k = 0; k < S; k++) prev[k] = 0.0;
-        if (lane < YR && r < rows && j0 >= 3) {
-#pragma unroll
-            for (int k = 0; k < S; k++) prev[k] = Y[((long)((j0 - 1) % NR) * S + k) * ld + r];
-        }
 #pragma unroll
         for (int g = 0; g < LG; g++) {
             const int j = j0 + g;
@@ -966,17 +966,16 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
             double a = 0.0;
 #pragma unroll
             for (int c = 0; c < CHK; c++) a += s_np2[t][c];
-            part[(long)blockIdx.x * NV + t] = a;
+            store_dev(part + (long)blockIdx.x * NV + t, a);
         }
     }
-    fence_publish();
+    stores_complete();
     __syncthreads();
     if (threadIdx.x == 0) s_last = (atomicAdd(&st->ycnt, 1) == (int)gridDim.x - 1) ? 1 : 0;
     __syncthreads();
     if (!s_last) return;
-    fence_collect();
     // part[b][g][e] summed over the blocks b (ends with a barrier)
-    block_sum_parts(part, LG * 2 * S, gridDim.x, s_scr, &s_nrm[0][0]);
+    block_sum_parts<true>(part, LG * 2 * S, gridDim.x, s_scr, &s_nrm[0][0]);
     if (threadIdx.x != 0) return;
     st->ycnt = 0;
     st->last_j = jend;
@@ -1073,17 +1072,16 @@ __global__ __launch_bounds__(256) void ycomb_kernel(const double *__restrict__ V
         for (int e = 0; e < 2 * S; e++) {
             double v = e < S ? dn[e] : on[e - S];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == 0) part[blockIdx.x * 2 * S + e] = v;
+            if (lane == 0) store_dev(part + blockIdx.x * 2 * S + e, v);
         }
     }
     // last block: sum the partials in block order and test
-    fence_publish();
+    stores_complete();
     __syncthreads();
     if (threadIdx.x == 0) s_last = (atomicAdd(&st->ycnt, 1) == (int)gridDim.x - 1) ? 1 : 0;
     __syncthreads();
     if (!s_last) return;
-    fence_collect();
-    block_sum_parts(part, 2 * S, gridDim.x, s_scr, s_nrm);   // ends with a barrier
+    block_sum_parts<true>(part, 2 * S, gridDim.x, s_scr, s_nrm);   // ends with a barrier
     if (threadIdx.x != 0) return;
     st->ycnt = 0;
     st->last_j = j;
@@ -1298,7 +1296,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                 const long part_stride = (long)nwg * dmax * S * S;
                 const long c_stride = (long)c.dimk_max * S * S;
                 if (use_pcr)
-                    hipLaunchKernelGGL((small_pcr_kernel<S>), dim3(nwg, j - jg0 + 1), dim3(pcr::THREADS), 0, sm,
+                    hipLaunchKernelGGL((small_pcr_kernel<S>), dim3(j - jg0 + 1, nwg), dim3(pcr::THREADS), 0, sm,
                                        c.Ablk, c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split,
                                        c.st);
                 else

@@ -17,14 +17,22 @@ static __constant__ double c_qh[14] = {0.205, 0.205, 0.155, 0.125, 0.1, 0.085, 0
 
 static __device__ inline bool lz_done(const DevStatus *st) { return st->converged_j != 0 || st->ierr != 0; }
 
-// The last-block pattern: every workgroup stores its partials, each wave
-// fences (release, device scope), the workgroup synchronises, and one
-// thread bumps a counter; the workgroup that completes the count fences
-// (acquire, device scope) before it reads the others' partials.  A full
-// fence in every workgroup would also make each one invalidate its XCD's
-// L2, throwing away what the other workgroups there are still reading.
-static __device__ inline void fence_publish() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
-static __device__ inline void fence_collect() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// The last-block pattern: every workgroup stores its partials with
+// device-scope atomic stores (coherent across the XCDs' L2s without any
+// cache maintenance), each wave waits until its stores have completed, the
+// workgroup synchronises, and one thread bumps a counter; the workgroup that
+// completes the count reads the partials with device-scope atomic loads.
+// (A release fence instead writes back the whole L2 of the XCD, several
+// microseconds per workgroup; an acquire fence invalidates it.)
+static __device__ inline void store_dev(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ inline double load_dev(const double *p)
+{
+    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ inline void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
 
 // ------------------------------------------------------------ reductions
 // Sums of the S*S per-thread values acc over the RB threads of the block
@@ -67,6 +75,7 @@ __device__ void block_reduce_store(double (&acc)[S * S], double *out)
 // p[0] + p[stride] + ... (cnt terms), added in index order with sixteen
 // loads in flight: the same additions as the plain loop, without a memory
 // round trip per term (partials written by other XCDs cost ~0.3 us each).
+template <bool DEV = false>
 __device__ __forceinline__ double sum_strided(const double *__restrict__ p, long stride, int cnt)
 {
     constexpr int U = 16;
@@ -74,7 +83,10 @@ __device__ __forceinline__ double sum_strided(const double *__restrict__ p, long
     for (int b = 0; b < cnt; b += U) {
         double v[U];
 #pragma unroll
-        for (int q = 0; q < U; q++) v[q] = p[(long)min(b + q, cnt - 1) * stride];
+        for (int q = 0; q < U; q++) {
+            const double *a = p + (long)min(b + q, cnt - 1) * stride;
+            v[q] = DEV ? load_dev(a) : *a;
+        }
 #pragma unroll
         for (int q = 0; q < U; q++)
             if (b + q < cnt) t += v[q];
@@ -86,13 +98,15 @@ __device__ __forceinline__ double sum_strided(const double *__restrict__ p, long
 // whole block: G = blockDim.x / E groups, group g summing b = g, g + G, ...;
 // the group sums are added in group order (a fixed order for a given block
 // size).  scr: blockDim.x doubles of shared memory; every thread of the
-// block must call; ends with a barrier.
+// block must call; ends with a barrier.  DEV: partials published with
+// store_dev by other workgroups of the same launch.
+template <bool DEV = false>
 __device__ __forceinline__ void block_sum_parts(const double *__restrict__ part, int E, int cnt,
                                                 double *scr, double *out)
 {
     const int t = threadIdx.x, G = blockDim.x / E, g = t / E, e = t % E;
     double v = 0.0;
-    if (g < G && cnt > g) v = sum_strided(part + (long)g * E + e, (long)G * E, (cnt - g + G - 1) / G);
+    if (g < G && cnt > g) v = sum_strided<DEV>(part + (long)g * E + e, (long)G * E, (cnt - g + G - 1) / G);
     scr[t] = v;
     __syncthreads();
     if (t < E) {
@@ -713,14 +727,13 @@ __device__ bool small_problem_split(const double *__restrict__ Ag, const double 
         const int b = e / (S * S), rc = e % (S * S);
         double acc = 0.0;
         for (int k = 0; k < nmine; k++) acc += sF[(b * E + OFF_F<S> + rc) * NPB + k];
-        pw[e] = acc;
+        store_dev(pw + e, acc);
     }
-    fence_publish();
+    stores_complete();
     __syncthreads();
     if (t == 0) s_last = (atomicAdd(&ss->cnt, 1) == nact - 1) ? 1 : 0;
     __syncthreads();
     if (!s_last) return false;
-    fence_collect();
     if (t == 0) {
         ss->cnt = 0;
         ss->kb[parity] = kb;
@@ -733,7 +746,7 @@ __device__ bool small_problem_split(const double *__restrict__ Ag, const double 
     // Z (block-major, S x S blocks col-major) summed over parts, into sF[0..)
     double *sZ = sF;
     for (int e = t; e < js * S; e += WG_T) {
-        const double acc = sum_strided(part + e, (long)js * S, nact);
+        const double acc = sum_strided<true>(part + e, (long)js * S, nact);
         const int b = e / (S * S), rc = e % (S * S), r = rc % S, c = rc / S;
         sZ[(b * S + r) + c * js] = acc;
     }
