@@ -152,6 +152,11 @@ struct Ctx {
     hipStream_t side = nullptr;    // small-problem stream of the Lanczos solve
     hipEvent_t lz_ev[LZ_NEV] = {};
     hipEvent_t lz_chunk_ev[2] = {};
+    // grouped tests: {iteration tested through, converged_j, ierr} written by
+    // the combine kernel into pinned host memory (lz_flag: host view,
+    // lz_flag_dev: the device's address of the same words)
+    volatile int *lz_flag = nullptr;
+    int *lz_flag_dev = nullptr;
     double *Rq = nullptr, *Rcoef = nullptr;  // reorthogonalisation scratch
 
     // topology

@@ -25,6 +25,8 @@
 // one kernel instead of a serial O(js^3) eigensolve.  In exact arithmetic
 // the result equals the reference's P sqrt(Lambda) P^T E_1 R.
 #include "bd_internal.h"
+
+#include <thread>
 #include "bd_lanczos_dev.h"
 #include "bd_pcr_dev.h"
 
@@ -40,7 +42,7 @@ __global__ __launch_bounds__(RB) void gram_kernel(const double *__restrict__ A, 
     double acc[S * S];
 #pragma unroll
     for (int e = 0; e < S * S; e++) acc[e] = 0.0;
-    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += GBLK * RB) {
+    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += (int)gridDim.x * RB) {
         double a[S], b[S];
 #pragma unroll
         for (int k = 0; k < S; k++) { a[k] = A[(long)k * ld + r]; b[k] = B[(long)k * ld + r]; }
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(RB) void lanczos_a_kernel(double *__restrict__ W, c
     double acc[S * S];
 #pragma unroll
     for (int e = 0; e < S * S; e++) acc[e] = 0.0;
-    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += GBLK * RB) {
+    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += (int)gridDim.x * RB) {
         double w[S], v[S];
 #pragma unroll
         for (int k = 0; k < S; k++) { w[k] = W[(long)k * ld + r]; v[k] = Vj[(long)k * ld + r]; }
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(RB) void lanczos_b_kernel(double *__restrict__ W, c
     double acc[S * S];
 #pragma unroll
     for (int e = 0; e < S * S; e++) acc[e] = 0.0;
-    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += GBLK * RB) {
+    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += (int)gridDim.x * RB) {
         double w[S], v[S];
 #pragma unroll
         for (int k = 0; k < S; k++) { w[k] = W[(long)k * ld + r]; v[k] = Vj[(long)k * ld + r]; }
@@ -177,7 +179,7 @@ __global__ __launch_bounds__(RB) void cholqr_kernel(const double *__restrict__ X
     double acc[S * S];
 #pragma unroll
     for (int e = 0; e < S * S; e++) acc[e] = 0.0;
-    for (int r = r0; r < rows; r += GBLK * RB) {
+    for (int r = r0; r < rows; r += (int)gridDim.x * RB) {
         double x[S], q[S];
 #pragma unroll
         for (int k = 0; k < S; k++) x[k] = (r == r0) ? x0[k] : X[(long)k * ld + r];
@@ -451,7 +453,7 @@ __global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict_
     double acc[S * S];
 #pragma unroll
     for (int e = 0; e < S * S; e++) acc[e] = 0.0;
-    for (int r = r0; r < rows; r += GBLK * RB) {
+    for (int r = r0; r < rows; r += (int)gridDim.x * RB) {
         double w[S], v[S], q[S];
 #pragma unroll
         for (int k = 0; k < S; k++) {
@@ -547,7 +549,7 @@ __global__ __launch_bounds__(RB) void reorth_apply_kernel(const double *__restri
     double acc[S * S];
 #pragma unroll
     for (int e = 0; e < S * S; e++) acc[e] = 0.0;
-    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += GBLK * RB) {
+    for (int r = blockIdx.x * RB + threadIdx.x; r < rows; r += (int)gridDim.x * RB) {
         double w[S];
 #pragma unroll
         for (int k = 0; k < S; k++) w[k] = W[(long)k * ld + r];
@@ -817,6 +819,16 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
     }
 }
 
+// The host's view of a group of tests: converged_j and ierr, then (release,
+// system scope) the last iteration the group covered; the host waits for
+// the latter before it reads the former.
+__device__ inline void report_tests(int *flag, int jend, int conv, int ierr)
+{
+    __hip_atomic_store(flag + 1, conv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(flag + 2, ierr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(flag, jend, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // y_j = V(:, 1:jS) C_j for the group's members (Y is a ring of LG + 1
 // vectors indexed by j mod (LG + 1)); partial norms of y_j - y_{j-1} and
 // y_{j-1}; the last block tests the members in order and records the first
@@ -825,7 +837,8 @@ template <int S>
 __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restrict__ V, int ld, int rows, int j0,
                                                           int jend, const double *__restrict__ Cg, long c_stride,
                                                           double *__restrict__ Y, int two_norm,
-                                                          double *__restrict__ part, double tol, DevStatus *st)
+                                                          double *__restrict__ part, double tol, DevStatus *st,
+                                                          int *flag)
 {
     constexpr int NR = LG + 1;
     __shared__ int s_done, s_last;
@@ -890,7 +903,11 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
         }
     }
     __syncthreads();
-    if (s_done) return;
+    if (s_done) {
+        // nothing to test: report the state reached, as the last block would
+        if (blockIdx.x == 0 && threadIdx.x == 0) report_tests(flag, jend, st->converged_j, st->ierr);
+        return;
+    }
     double acc[LG][S];
 #pragma unroll
     for (int g = 0; g < LG; g++)
@@ -997,6 +1014,7 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
             break;
         }
     }
+    report_tests(flag, jend, st->converged_j, st->ierr);
 }
 
 // y = V(:, 1:js) C, 32 rows per block and the js columns split over eight
@@ -1148,6 +1166,10 @@ int lanczos_alloc(Ctx &c)
     BD_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
     for (int k = 0; k < LZ_NEV; k++) BD_HIP(hipEventCreateWithFlags(&c.lz_ev[k], hipEventDisableTiming));
     for (int k = 0; k < 2; k++) BD_HIP(hipEventCreateWithFlags(&c.lz_chunk_ev[k], hipEventDisableTiming));
+    void *fl = nullptr;
+    BD_HIP(hipHostMalloc(&fl, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    c.lz_flag = static_cast<volatile int *>(fl);
+    BD_HIP(hipHostGetDevicePointer((void **)&c.lz_flag_dev, fl, 0));
     return BD_OK;
 }
 
@@ -1158,6 +1180,9 @@ void lanczos_free(Ctx &c)
         if (c.lz_ev[k]) (void)hipEventDestroy(c.lz_ev[k]);
     for (int k = 0; k < 2; k++)
         if (c.lz_chunk_ev[k]) (void)hipEventDestroy(c.lz_chunk_ev[k]);
+    if (c.lz_flag) (void)hipHostFree((void *)c.lz_flag);
+    c.lz_flag = nullptr;
+    c.lz_flag_dev = nullptr;
     if (c.side) (void)hipStreamDestroy(c.side);
     c.side = nullptr;
     double **ps[] = {&c.V, &c.W, &c.Y, &c.Ablk, &c.Bblk, &c.R0, &c.gpart, &c.Cq, &c.Ccoef, &c.npart,
@@ -1183,6 +1208,21 @@ void lanczos_free(Ctx &c)
 // host looks at the convergence flag once per chunk of iterations, one chunk
 // behind, so it never stalls the device; iterations enqueued past the first
 // converged j find the flag set and return at once.
+// Waits until the grouped tests through iteration jend have reported (see
+// report_tests).  Polls the flag; after a while it waits for the stream
+// instead, after which the flag must be there.
+static int wait_tests(Ctx &c, hipStream_t sm, int jend)
+{
+    for (long it = 0; it < 2000000; it++) {
+        if (__atomic_load_n(&c.lz_flag[0], __ATOMIC_ACQUIRE) >= jend) return BD_OK;
+        if (it > 1000) std::this_thread::yield();
+    }
+    BD_HIP(hipStreamSynchronize(sm));
+    if (__atomic_load_n(&c.lz_flag[0], __ATOMIC_ACQUIRE) >= jend) return BD_OK;
+    set_error("internal: Lanczos tests did not report");
+    return BD_ERR_STATE;
+}
+
 template <int S>
 static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *iters)
 {
@@ -1196,6 +1236,9 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     const bool twopass = !fused && !c.opt.lanc_reorth;
     double *g0 = c.gpart, *g1 = c.gpart + GBLK * MAX_S * MAX_S;
     hipStream_t sm = c.stream, ss = c.side;
+    // grid of every kernel that leaves or sums Gram partials (one partial
+    // per block): a block per RB rows, at most GBLK
+    const int gb = std::min(GBLK, (rows + RB - 1) / RB);
 
     hipLaunchKernelGGL(status_reset_kernel, dim3(1), dim3(1), 0, sm, c.st, c.split);
     // small problem split over workgroups when its blocks fit their LDS
@@ -1222,10 +1265,10 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
         }
     }
     // xi = V_1 R0 (CholeskyQR2)
-    hipLaunchKernelGGL((gram_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, xi, xi, ld, rows, g0, c.st);
-    hipLaunchKernelGGL((cholqr_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, xi, c.V, ld, rows, g0, g1,
+    hipLaunchKernelGGL((gram_kernel<S>), dim3(gb), dim3(RB), 0, sm, xi, xi, ld, rows, g0, c.st);
+    hipLaunchKernelGGL((cholqr_kernel<S>), dim3(gb), dim3(RB), 0, sm, xi, c.V, ld, rows, g0, g1,
                        c.R0 + S * S, (const double *)nullptr, c.st);
-    hipLaunchKernelGGL((cholqr_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.V, c.V, ld, rows, g1,
+    hipLaunchKernelGGL((cholqr_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.V, c.V, ld, rows, g1,
                        (double *)nullptr, c.R0, (const double *)(c.R0 + S * S), c.st);
     BD_HIP(hipGetLastError());
 
@@ -1233,6 +1276,12 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     int pending = -1;   // slot of the status copy of the previous chunk
     int slot = 0;
     bool stop = false;
+    int pend_j = 0;     // grouped: last iteration of the previous group
+    if (grouped) {
+        c.lz_flag[0] = 0;
+        c.lz_flag[1] = 0;
+        c.lz_flag[2] = 0;
+    }
     for (int j = 1; j <= dmax && !stop; j++) {
         double *Vj = c.V + (size_t)(j - 1) * S * ld;
         const double *Vp = (j > 1) ? c.V + (size_t)(j - 2) * S * ld : nullptr;
@@ -1243,10 +1292,10 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             LzEpi epi{Vj, Vp, Bp, c.lzpart};
             BD_TRY(mob_product_lz(c, Vj, c.W, S, ld, ld, &epi));
             double *Vn = c.V + (size_t)j * S * ld;
-            hipLaunchKernelGGL((krylov_k2_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vj, Vn, ld, rows, c.lzpart,
+            hipLaunchKernelGGL((krylov_k2_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.W, Vj, Vn, ld, rows, c.lzpart,
                                c.plan.nrowblk, c.Ablk + (size_t)(j - 1) * S * S, c.R0 + S * S, upd, g0, c.st);
             if (upd)
-                hipLaunchKernelGGL((cholqr_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, Vn, Vn, ld, rows, g0,
+                hipLaunchKernelGGL((cholqr_kernel<S>), dim3(gb), dim3(RB), 0, sm, Vn, Vn, ld, rows, g0,
                                    (double *)nullptr, c.Bblk + (size_t)(j - 1) * S * S,
                                    (const double *)(c.R0 + S * S), c.st);
             BD_HIP(hipGetLastError());
@@ -1260,9 +1309,9 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                                c.Bblk + (size_t)(j - 1) * S * S, upd, c.st);
             BD_HIP(hipGetLastError());
         } else {
-            hipLaunchKernelGGL((lanczos_a_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vj, Vp, Bp, ld, rows, g0,
+            hipLaunchKernelGGL((lanczos_a_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.W, Vj, Vp, Bp, ld, rows, g0,
                                c.st);
-            hipLaunchKernelGGL((lanczos_b_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vj, ld, rows, g0,
+            hipLaunchKernelGGL((lanczos_b_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.W, Vj, ld, rows, g0,
                                c.Ablk + (size_t)(j - 1) * S * S, upd, g1, c.st);
         }
         // A_j and V_1..V_j are final here: hand iteration j to the side stream
@@ -1276,16 +1325,16 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                                    c.Rq, c.st);
                 hipLaunchKernelGGL(reorth_sum_kernel, dim3((ncol * S + 255) / 256), dim3(256), 0, sm, c.Rq,
                                    ncol * S, c.Rcoef, c.st);
-                hipLaunchKernelGGL((reorth_apply_kernel<S>), dim3(GBLK), dim3(RB), (size_t)ncol * S * sizeof(double),
+                hipLaunchKernelGGL((reorth_apply_kernel<S>), dim3(gb), dim3(RB), (size_t)ncol * S * sizeof(double),
                                    sm, c.V, c.W, ld, rows, ncol, c.Rcoef, pass == 1 ? g1 : (double *)nullptr, c.st);
             }
         }
         if (!twopass && !fused && upd) {
             double *Vn = c.V + (size_t)j * S * ld;
             double *Bj = c.Bblk + (size_t)(j - 1) * S * S;
-            hipLaunchKernelGGL((cholqr_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, c.W, Vn, ld, rows, g1, g0,
+            hipLaunchKernelGGL((cholqr_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.W, Vn, ld, rows, g1, g0,
                                c.R0 + S * S /* R1 scratch */, (const double *)nullptr, c.st);
-            hipLaunchKernelGGL((cholqr_kernel<S>), dim3(GBLK), dim3(RB), 0, sm, Vn, Vn, ld, rows, g0,
+            hipLaunchKernelGGL((cholqr_kernel<S>), dim3(gb), dim3(RB), 0, sm, Vn, Vn, ld, rows, g0,
                                (double *)nullptr, Bj, (const double *)(c.R0 + S * S), c.st);
         }
         BD_HIP(hipGetLastError());
@@ -1305,7 +1354,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                                        c.st);
                 hipLaunchKernelGGL((ycomb_group_kernel<S>), dim3((rows + YR - 1) / YR), dim3(256), 0, sm, c.V, ld,
                                    rows, jg0, j, c.Ccoef, c_stride, c.Y, two_norm ? 1 : 0, c.npart, c.opt.lanc_tol,
-                                   c.st);
+                                   c.st, c.lz_flag_dev);
             }
             BD_HIP(hipGetLastError());
         }
@@ -1330,7 +1379,15 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                                ycur, yold, want, two_norm ? 1 : 0, c.npart, j, c.opt.lanc_tol, c.st);
             BD_HIP(hipGetLastError());
         }
-        if (j % chunk == 0 || j == dmax) {
+        if (grouped && (j % chunk == 0 || j == dmax)) {
+            // the previous group's verdict, once its combine kernel reported
+            // (this group's kernels are queued meanwhile)
+            if (pend_j > 0) {
+                BD_TRY(wait_tests(c, sm, pend_j));
+                if (c.lz_flag[2] || c.lz_flag[1]) stop = true;
+            }
+            pend_j = j;
+        } else if (j % chunk == 0 || j == dmax) {
             // snapshot the status after this chunk's tests; read the previous one
             hipStream_t ts = grouped ? sm : ss;
             BD_HIP(hipMemcpyAsync(&c.st_host[slot], c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, ts));

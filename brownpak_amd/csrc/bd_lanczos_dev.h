@@ -7,7 +7,7 @@
 
 namespace bd {
 
-constexpr int GBLK = 120;     // blocks of the Gram/partial-norm reductions
+constexpr int GBLK = 120;     // most blocks of the Gram reductions (one per RB rows below that)
 constexpr int RB = 256;       // threads per reduction block
 
 // Quadrature table: for kappa <= 10^(d), (N, h) with 2N+1 nodes (d = 1..13).
@@ -75,10 +75,9 @@ __device__ void block_reduce_store(double (&acc)[S * S], double *out)
 // p[0] + p[stride] + ... (cnt terms), added in index order with sixteen
 // loads in flight: the same additions as the plain loop, without a memory
 // round trip per term (partials written by other XCDs cost ~0.3 us each).
-template <bool DEV = false>
+template <bool DEV = false, int U = 16>
 __device__ __forceinline__ double sum_strided(const double *__restrict__ p, long stride, int cnt)
 {
-    constexpr int U = 16;
     double t = 0.0;
     for (int b = 0; b < cnt; b += U) {
         double v[U];
@@ -100,13 +99,13 @@ __device__ __forceinline__ double sum_strided(const double *__restrict__ p, long
 // size).  scr: blockDim.x doubles of shared memory; every thread of the
 // block must call; ends with a barrier.  DEV: partials published with
 // store_dev by other workgroups of the same launch.
-template <bool DEV = false>
+template <bool DEV = false, int U = 16>
 __device__ __forceinline__ void block_sum_parts(const double *__restrict__ part, int E, int cnt,
                                                 double *scr, double *out)
 {
     const int t = threadIdx.x, G = blockDim.x / E, g = t / E, e = t % E;
     double v = 0.0;
-    if (g < G && cnt > g) v = sum_strided<DEV>(part + (long)g * E + e, (long)G * E, (cnt - g + G - 1) / G);
+    if (g < G && cnt > g) v = sum_strided<DEV, U>(part + (long)g * E + e, (long)G * E, (cnt - g + G - 1) / G);
     scr[t] = v;
     __syncthreads();
     if (t < E) {
@@ -117,8 +116,9 @@ __device__ __forceinline__ void block_sum_parts(const double *__restrict__ part,
     __syncthreads();
 }
 
-// Sum of GBLK partials of S*S values, fixed order, by the whole block
-// (blockDim.x == RB).  Result in g (shared), valid after the call.
+// Sum of the gridDim.x partials of S*S values left by a kernel of the same
+// grid, fixed order, by the whole block (blockDim.x == RB).  Result in g
+// (shared), valid after the call.
 template <int S>
 __device__ void sum_partials(const double *part, double *g)
 {
@@ -129,7 +129,7 @@ __device__ void sum_partials(const double *part, double *g)
     double v = 0.0;
     if (t < E * STR) {
         const int e = t % E, st = t / E;
-        for (int b = st; b < GBLK; b += STR) v += part[b * E + e];
+        for (int b = st; b < (int)gridDim.x; b += STR) v += part[b * E + e];
     }
     sh[t] = v;
     __syncthreads();

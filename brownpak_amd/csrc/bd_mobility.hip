@@ -373,38 +373,36 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
 // Output block G (a tile row of some chain):
 //   y = sum_{sc <= K/wsc} rowpart[K, sc] + sum_{units u of super-column K/wsc} colpart[u, K % wsc]
 // rowblk (8 ints per block): chain, K, nsc, u0, u1, rp_off, y row offset.
-// Sums in a fixed order; loads batched eight at a time (clamped addresses,
-// so all eight are issued together).
+// Sums in a fixed order (the row partials, then the column partials); the
+// two lists are read as one, PB loads at a time at clamped addresses, so a
+// batch is one memory round trip.
 template <int S>
 __device__ inline double sum_block_partials(const double *__restrict__ rowpart, const double *__restrict__ colpart,
                                             const int *__restrict__ rb, int e, int wsc)
 {
+    constexpr int PB = 32;
     const int K = rb[1], nsc = rb[2], u0 = rb[3], u1 = rb[4], rpo = rb[5];
-    double v = 0.0;
-    const int nrow = K / wsc + 1;
+    const int nrow = K / wsc + 1, ntot = nrow + (u1 - u0);
     const double *rp = rowpart + ((long)rpo + (long)K * nsc) * (TILE * S) + e;
-    for (int s0 = 0; s0 < nrow; s0 += 8) {
-        double t[8];
+    const double *cp = colpart + ((long)u0 * wsc + K % wsc) * (TILE * S) + e;
+    const long cstr = (long)wsc * (TILE * S);
+    double v = 0.0;
+    for (int i0 = 0; i0 < ntot; i0 += PB) {
+        double t[PB];
 #pragma unroll
-        for (int q = 0; q < 8; q++) t[q] = rp[(long)min(s0 + q, nrow - 1) * (TILE * S)];
+        for (int q = 0; q < PB; q++) {
+            const int i = min(i0 + q, ntot - 1);
+            t[q] = i < nrow ? rp[(long)i * (TILE * S)] : cp[(long)(i - nrow) * cstr];
+        }
 #pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (s0 + q < nrow) v += t[q];
-    }
-    const double *cp = colpart + (long)(K % wsc) * (TILE * S) + e;
-    for (int u = u0; u < u1; u += 8) {
-        double t[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) t[q] = cp[(long)min(u + q, u1 - 1) * wsc * (TILE * S)];
-#pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (u + q < u1) v += t[q];
+        for (int q = 0; q < PB; q++)
+            if (i0 + q < ntot) v += t[q];
     }
     return v;
 }
 
 template <int S>
-__global__ void symm_reduce_kernel(const double *__restrict__ rowpart, const double *__restrict__ colpart,
+__global__ __launch_bounds__(256) void symm_reduce_kernel(const double *__restrict__ rowpart, const double *__restrict__ colpart,
                                    const int *__restrict__ rowblk, int nrowblk, double *__restrict__ y, int ldy,
                                    int *__restrict__ ticket, const int *__restrict__ active, int wsc)
 {

@@ -38,52 +38,61 @@ __device__ inline void zero(double (&a)[S][S])
         for (int c = 0; c < S; c++) a[r][c] = 0.0;
 }
 
-// Cholesky D = R^T R of the symmetric D (lower triangle read), then
-// inv = D^{-1} = R^{-1} R^{-T} (full).  Returns false if D is not positive
-// definite (inv is then finite garbage).
+// 1/d from the hardware estimate refined by two Newton-Raphson steps
+// (x <- x + x (1 - d x)), each of which squares the relative error.
+__device__ inline double rcp_nr(double d)
+{
+    double x = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, x, 1.0);
+    x = fma(x, e, x);
+    e = fma(-d, x, 1.0);
+    return fma(x, e, x);
+}
+
+// D = L diag(d) L^T of the symmetric D (lower triangle read; L unit lower),
+// then inv = D^{-1} = M^T diag(1/d) M with M = L^{-1}.  No square roots: the
+// pivots d_j are positive exactly when D is positive definite, which is the
+// verdict returned (inv is finite garbage otherwise).
 template <int S>
 __device__ inline bool chol_inv(const double (&D)[S][S], double (&inv)[S][S])
 {
-    double R[S][S], rd[S];
+    double L[S][S], W[S][S], rd[S];   // W[i][k] = L[i][k] d_k
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < S; j++) {
-        double s = D[j][j];
 #pragma unroll
-        for (int k = 0; k < j; k++) s -= R[k][j] * R[k][j];
-        ok = ok && (s > 0.0);
-        const double d = sqrt(fmax(s, 1e-300));
-        R[j][j] = d;
-        rd[j] = 1.0 / d;
+        for (int i = 0; i < j; i++) {
+            double t = D[j][i];
 #pragma unroll
-        for (int i = j + 1; i < S; i++) {
-            double t = D[i][j];
-#pragma unroll
-            for (int k = 0; k < j; k++) t -= R[k][j] * R[k][i];
-            R[j][i] = t * rd[j];
+            for (int k = 0; k < i; k++) t -= W[j][k] * L[i][k];
+            W[j][i] = t;
+            L[j][i] = t * rd[i];
         }
+        double dj = D[j][j];
+#pragma unroll
+        for (int k = 0; k < j; k++) dj -= W[j][k] * L[j][k];
+        ok = ok && (dj > 0.0);
+        rd[j] = rcp_nr(dj);
     }
-    // U = R^{-1} (upper triangular)
-    double U[S][S];
+    // M = L^{-1}, unit lower triangular (M[i][k], k < i)
+    double M[S][S];
 #pragma unroll
-    for (int j = 0; j < S; j++) {
-        U[j][j] = rd[j];
+    for (int i = 0; i < S; i++)
 #pragma unroll
-        for (int i = j - 1; i >= 0; i--) {
-            double t = 0.0;
+        for (int k = 0; k < i; k++) {
+            double t = -L[i][k];
 #pragma unroll
-            for (int k = i + 1; k <= j; k++) t += R[i][k] * U[k][j];
-            U[i][j] = -t * rd[i];
+            for (int m = k + 1; m < i; m++) t -= L[i][m] * M[m][k];
+            M[i][k] = t;
         }
-    }
-    // inv = U U^T
+    // inv(a, b) = sum_{k >= a} M(k, a) rd_k M(k, b), b <= a, with M(k, k) = 1
 #pragma unroll
     for (int a = 0; a < S; a++)
 #pragma unroll
         for (int b = 0; b <= a; b++) {
-            double t = 0.0;
+            double t = rd[a] * (a == b ? 1.0 : M[a][b]);
 #pragma unroll
-            for (int k = a; k < S; k++) t += U[a][k] * U[b][k];
+            for (int k = a + 1; k < S; k++) t += M[k][a] * rd[k] * M[k][b];
             inv[a][b] = t;
             inv[b][a] = t;
         }
