@@ -396,7 +396,7 @@ __global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict_
                                                        int upd, double *__restrict__ gpart, DevStatus *st)
 {
     __shared__ int s_done, s_fail;
-    __shared__ double sAP[2 * S * S], sAh[S * S], sR[S * S], sI[S], s_scr[RB];
+    __shared__ double sAP[2 * S * S], sAh[S * S], sG[S * S], sR[S * S], sI[S], s_scr[RB];
     if (threadIdx.x == 0) {
         s_done = lz_done(st) ? 1 : 0;
         s_fail = 0;
@@ -415,30 +415,32 @@ __global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict_
     if (s_done) return;
     if (blockIdx.x == 0 && threadIdx.x < S * S) Aout[threadIdx.x] = sAP[threadIdx.x];
     if (!upd) return;
+    const double *A = sAP, *P = sAP + S * S;
+    if (threadIdx.x < S * S) {
+        const int a = threadIdx.x % S, b = threadIdx.x / S;
+        sAh[a + b * S] = (a >= b) ? A[a + b * S] : A[b + a * S];
+    }
+    __syncthreads();
+    if (threadIdx.x < S * S) {
+        // G = W2^T W2 entry (a, b), symmetrised (the three products are
+        // symmetric only up to rounding), one entry per thread
+        const int a = threadIdx.x % S, b = threadIdx.x / S;
+        auto gab = [&](int x, int y) {
+            double t = P[x + y * S];
+            for (int k = 0; k < S; k++) {
+                t -= A[k + x * S] * sAh[k + y * S];     // (A^T Ah)(x,y)
+                t -= sAh[k + x * S] * A[k + y * S];     // (Ah^T A)(x,y)
+                t += sAh[k + x * S] * sAh[k + y * S];   // (Ah^T Ah)(x,y)
+            }
+            return t;
+        };
+        if (a == b) sG[a + b * S] = gab(a, a);
+        else if (a > b) sG[a + b * S] = sG[b + a * S] = 0.5 * (gab(a, b) + gab(b, a));
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const double *A = sAP, *P = sAP + S * S;
-        for (int a = 0; a < S; a++)
-            for (int b = 0; b < S; b++) sAh[a + b * S] = (a >= b) ? A[a + b * S] : A[b + a * S];
-        double G[S * S];
-        for (int a = 0; a < S; a++)
-            for (int b = 0; b < S; b++) {
-                double t = P[a + b * S];
-                for (int k = 0; k < S; k++) {
-                    t -= A[k + a * S] * sAh[k + b * S];     // (A^T Ah)(a,b)
-                    t -= sAh[k + a * S] * A[k + b * S];     // (Ah^T A)(a,b)
-                    t += sAh[k + a * S] * sAh[k + b * S];   // (Ah^T Ah)(a,b)
-                }
-                G[a + b * S] = t;
-            }
-        // symmetrise (the three products are symmetric only up to rounding)
-        for (int a = 0; a < S; a++)
-            for (int b = 0; b < a; b++) {
-                const double t = 0.5 * (G[a + b * S] + G[b + a * S]);
-                G[a + b * S] = t;
-                G[b + a * S] = t;
-            }
         double R[S][S];
-        if (!small_chol<S>(G, R)) s_fail = 1;
+        if (!small_chol<S>(sG, R)) s_fail = 1;
         for (int i = 0; i < S; i++) {
             sI[i] = 1.0 / R[i][i];
             for (int j = 0; j < S; j++) sR[i + j * S] = R[i][j];
@@ -621,8 +623,10 @@ constexpr int YJS = 256;                   // largest j s of the grouped combina
 // The small problems of LG consecutive iterations j0 .. j0+LG-1 run as one
 // launch (blockIdx.y = member, each split over workgroups), and one combine
 // launch forms their y_j and applies the reference's test to each in order.
-// Everything stays on the solve's stream: no cross-stream events, and the
-// Krylov recurrence never waits for a test it does not need.
+// Both run on the side stream behind one event per group, so the Krylov
+// recurrence never waits for a test; the host learns the verdict through a
+// flag in mapped host memory (report_tests) and stops issuing iterations one
+// group after the one that converged.
 constexpr int LG = 4;
 
 template <int S>
@@ -819,14 +823,17 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
     }
 }
 
-// The host's view of a group of tests: converged_j and ierr, then (release,
-// system scope) the last iteration the group covered; the host waits for
-// the latter before it reads the former.
+// The host's view of a group of tests: converged_j and ierr, then the last
+// iteration the group covered; the host waits for the latter before it
+// reads the former.  The host reads nothing else the device wrote, so the
+// three system-scope stores with a wait for completion before the last one
+// are the whole protocol (no cache write-back).
 __device__ inline void report_tests(int *flag, int jend, int conv, int ierr)
 {
     __hip_atomic_store(flag + 1, conv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(flag + 2, ierr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(flag, jend, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    stores_complete();
+    __hip_atomic_store(flag, jend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // y_j = V(:, 1:jS) C_j for the group's members (Y is a ring of LG + 1
@@ -1209,8 +1216,8 @@ void lanczos_free(Ctx &c)
 // behind, so it never stalls the device; iterations enqueued past the first
 // converged j find the flag set and return at once.
 // Waits until the grouped tests through iteration jend have reported (see
-// report_tests).  Polls the flag; after a while it waits for the stream
-// instead, after which the flag must be there.
+// report_tests).  Polls the flag; after a while it waits for the tests'
+// stream instead, after which the flag must be there.
 static int wait_tests(Ctx &c, hipStream_t sm, int jend)
 {
     for (long it = 0; it < 2000000; it++) {
@@ -1247,7 +1254,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     const bool use_pcr = S <= 4 && dmax <= pcr::ROWS;
     const int nwg = use_pcr ? (2 * 125 + 1 + pcr::NODES - 1) / pcr::NODES
                             : (2 * 125 + 1 + sp::WG_NPB<S> - 1) / sp::WG_NPB<S>;
-    // grouped tests: LG iterations' small problems per launch, on the solve's
+    // grouped tests: LG iterations' small problems per launch, on the side
     // stream (needs the split or the reduction small problem)
     const bool grouped = split || use_pcr;
     {
@@ -1339,20 +1346,24 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
         }
         BD_HIP(hipGetLastError());
         if (grouped && (j % LG == 0 || j == dmax)) {
-            // the tests of iterations jg0..j, on this stream
+            // the tests of iterations jg0..j on the side stream, behind the
+            // Krylov kernels of iteration j: the recurrence goes on meanwhile
+            // (it writes only later blocks of V, A and B than the tests read)
             const int jg0 = ((j - 1) / LG) * LG + 1;
+            BD_HIP(hipEventRecord(ev, sm));
+            BD_HIP(hipStreamWaitEvent(ss, ev, 0));
             if constexpr (S <= 4) {
                 const long part_stride = (long)nwg * dmax * S * S;
                 const long c_stride = (long)c.dimk_max * S * S;
                 if (use_pcr)
-                    hipLaunchKernelGGL((small_pcr_kernel<S>), dim3(j - jg0 + 1, nwg), dim3(pcr::THREADS), 0, sm,
+                    hipLaunchKernelGGL((small_pcr_kernel<S>), dim3(j - jg0 + 1, nwg), dim3(pcr::THREADS), 0, ss,
                                        c.Ablk, c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split,
                                        c.st);
                 else
-                    hipLaunchKernelGGL((small_group_kernel<S>), dim3(nwg, j - jg0 + 1), dim3(sp::WG_T), 0, sm,
+                    hipLaunchKernelGGL((small_group_kernel<S>), dim3(nwg, j - jg0 + 1), dim3(sp::WG_T), 0, ss,
                                        c.Ablk, c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split,
                                        c.st);
-                hipLaunchKernelGGL((ycomb_group_kernel<S>), dim3((rows + YR - 1) / YR), dim3(256), 0, sm, c.V, ld,
+                hipLaunchKernelGGL((ycomb_group_kernel<S>), dim3((rows + YR - 1) / YR), dim3(256), 0, ss, c.V, ld,
                                    rows, jg0, j, c.Ccoef, c_stride, c.Y, two_norm ? 1 : 0, c.npart, c.opt.lanc_tol,
                                    c.st, c.lz_flag_dev);
             }
@@ -1383,7 +1394,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             // the previous group's verdict, once its combine kernel reported
             // (this group's kernels are queued meanwhile)
             if (pend_j > 0) {
-                BD_TRY(wait_tests(c, sm, pend_j));
+                BD_TRY(wait_tests(c, ss, pend_j));
                 if (c.lz_flag[2] || c.lz_flag[1]) stop = true;
             }
             pend_j = j;
