@@ -1257,6 +1257,10 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     // grouped tests: LG iterations' small problems per launch, on the side
     // stream (needs the split or the reduction small problem)
     const bool grouped = split || use_pcr;
+    // their stream: the side stream, except while the tile products are
+    // being timed (events on the solve's stream would then also count the
+    // wait for CUs held by the tests)
+    const hipStream_t sg = c.prof_on ? sm : ss;
     {
         // host-side capacity checks of the scratch every launch below indexes
         const size_t ncol = (size_t)S * dmax;
@@ -1350,20 +1354,22 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             // Krylov kernels of iteration j: the recurrence goes on meanwhile
             // (it writes only later blocks of V, A and B than the tests read)
             const int jg0 = ((j - 1) / LG) * LG + 1;
-            BD_HIP(hipEventRecord(ev, sm));
-            BD_HIP(hipStreamWaitEvent(ss, ev, 0));
+            if (sg != sm) {
+                BD_HIP(hipEventRecord(ev, sm));
+                BD_HIP(hipStreamWaitEvent(sg, ev, 0));
+            }
             if constexpr (S <= 4) {
                 const long part_stride = (long)nwg * dmax * S * S;
                 const long c_stride = (long)c.dimk_max * S * S;
                 if (use_pcr)
-                    hipLaunchKernelGGL((small_pcr_kernel<S>), dim3(j - jg0 + 1, nwg), dim3(pcr::THREADS), 0, ss,
+                    hipLaunchKernelGGL((small_pcr_kernel<S>), dim3(j - jg0 + 1, nwg), dim3(pcr::THREADS), 0, sg,
                                        c.Ablk, c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split,
                                        c.st);
                 else
-                    hipLaunchKernelGGL((small_group_kernel<S>), dim3(nwg, j - jg0 + 1), dim3(sp::WG_T), 0, ss,
+                    hipLaunchKernelGGL((small_group_kernel<S>), dim3(nwg, j - jg0 + 1), dim3(sp::WG_T), 0, sg,
                                        c.Ablk, c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split,
                                        c.st);
-                hipLaunchKernelGGL((ycomb_group_kernel<S>), dim3((rows + YR - 1) / YR), dim3(256), 0, ss, c.V, ld,
+                hipLaunchKernelGGL((ycomb_group_kernel<S>), dim3((rows + YR - 1) / YR), dim3(256), 0, sg, c.V, ld,
                                    rows, jg0, j, c.Ccoef, c_stride, c.Y, two_norm ? 1 : 0, c.npart, c.opt.lanc_tol,
                                    c.st, c.lz_flag_dev);
             }
@@ -1394,7 +1400,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             // the previous group's verdict, once its combine kernel reported
             // (this group's kernels are queued meanwhile)
             if (pend_j > 0) {
-                BD_TRY(wait_tests(c, ss, pend_j));
+                BD_TRY(wait_tests(c, sg, pend_j));
                 if (c.lz_flag[2] || c.lz_flag[1]) stop = true;
             }
             pend_j = j;
