@@ -56,7 +56,7 @@ EXPORTS = (
     "bd_set_theta", "bd_run_record", "bd_run_replay", "bd_adjoint_rgsq",
     "bd_batch_create", "bd_batch_destroy", "bd_batch_set_model", "bd_batch_set_coordinates",
     "bd_batch_get_coordinates", "bd_batch_set_rng", "bd_batch_run", "bd_batch_info", "bd_batch_prof",
-    "bd_batch_tile_shape", "bd_batch_set_tile_shape",
+    "bd_batch_tile_shape", "bd_batch_set_tile_shape", "bd_plan_layout",
 )
 
 
@@ -84,6 +84,23 @@ class BdError(RuntimeError):
 
 
 _lib = None
+
+
+def plan_layout(nbeads: int, cus: int = 256):
+    """The single-chain work decomposition of the mobility product (host
+    only): ((chr, wsc), units (n, 16) in execution order, rowblk (nt, 8))."""
+    L = load_library()
+    a, b, nu, nr = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    rc = L.bd_plan_layout(int(nbeads), int(cus), C.byref(a), C.byref(b), C.byref(nu), None, 0, C.byref(nr), None, 0)
+    if rc != 0:
+        raise BdError(rc, L.bd_last_error().decode())
+    units = np.zeros((nu.value, 16), dtype=np.int32)
+    rowblk = np.zeros((nr.value, 8), dtype=np.int32)
+    rc = L.bd_plan_layout(int(nbeads), int(cus), C.byref(a), C.byref(b), C.byref(nu),
+                          units.ctypes.data_as(IP), units.size, C.byref(nr), rowblk.ctypes.data_as(IP), rowblk.size)
+    if rc != 0:
+        raise BdError(rc, L.bd_last_error().decode())
+    return (a.value, b.value), units, rowblk
 
 
 def load_library(path: str = LIB_PATH):
@@ -136,6 +153,7 @@ def load_library(path: str = LIB_PATH):
     L.bd_batch_prof.argtypes = [vp, C.c_int, DP, C.POINTER(C.c_longlong), DP]
     L.bd_batch_tile_shape.argtypes = [vp, IP, IP, DP]
     L.bd_batch_set_tile_shape.argtypes = [vp, C.c_int, C.c_int]
+    L.bd_plan_layout.argtypes = [C.c_int, C.c_int, IP, IP, IP, IP, C.c_int, IP, IP, C.c_int]
     for name in EXPORTS:
         if name not in ("bd_ctx_create", "bd_ctx_destroy", "bd_last_error", "bd_batch_create",
                         "bd_batch_destroy"):

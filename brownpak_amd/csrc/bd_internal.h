@@ -89,8 +89,9 @@ struct LzEpi {
 };
 extern const int kTileShapes[BD_TILE_SHAPES][2];
 bool tile_shape_supported(int chr, int wsc);
-void plan_pick_shape(int nb, int cus, int &chr, int &wsc);
-int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, int chr = 2, int wsc = 4);
+void plan_pick(int nb, int cus, int &chr, int &wsc, std::vector<int> &widths);
+int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, int chr = 2, int wsc = 4,
+               const std::vector<std::vector<int>> *widths = nullptr, int slots = 0);
 // Times the supported unit shapes on this plan (block width S) and rebuilds
 // it with the fastest; times_us (BD_TILE_SHAPES entries) receives the times.
 template <typename E, int S>

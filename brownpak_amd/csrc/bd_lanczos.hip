@@ -147,15 +147,17 @@ __global__ __launch_bounds__(RB) void cholqr_kernel(const double *__restrict__ X
                                                     double *__restrict__ gpart_out, double *__restrict__ Rsave,
                                                     const double *__restrict__ rprev, DevStatus *st)
 {
-    if (lz_done(st)) return;
     __shared__ double sG[S * S];
+    __shared__ int s_done;
     // this thread's first row, loaded before the Gram sum so that the two
-    // memory round trips overlap
+    // memory round trips overlap, and the status with them
+    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
     const int r0 = blockIdx.x * RB + threadIdx.x;
     double x0[S];
 #pragma unroll
     for (int k = 0; k < S; k++) x0[k] = X[(long)k * ld + min(r0, rows - 1)];
-    sum_partials<S>(gpart, sG);
+    sum_partials<S>(gpart, sG);   // ends with a barrier
+    if (s_done) return;
     double R[S][S];
     bool ok = small_chol<S>(sG, R);
     if (!ok) {
@@ -682,19 +684,20 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
     __shared__ double s_red[WAVES];
     __shared__ int s_done, s_ok[WAVES], s_last;
     const int j = j0 + (int)blockIdx.x;
+    // the blocks of H_j are loaded together with the status
     if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    const int nbl = (j >= 2 && j <= jend) ? j : 0;
+    for (int e = threadIdx.x; e < nbl * SS; e += THREADS) {
+        sA[e] = Ablk[e];
+        sB[e] = Bblk[e];
+    }
+    if (threadIdx.x < SS) sR[threadIdx.x] = R0[threadIdx.x];
     __syncthreads();
     if (s_done || j > jend || j < 2) return;
     const int nb = j, js = nb * S, t = threadIdx.x, w = t / 64, lane = t % 64;
     const int parity = ((j0 - 1) / LG) & 1;
     sp::SplitState *ss = ssv + blockIdx.x;
     double *P = part + blockIdx.x * part_stride;
-    for (int e = t; e < nb * SS; e += THREADS) {
-        sA[e] = Ablk[e];
-        sB[e] = Bblk[e];
-    }
-    if (t < SS) sR[t] = R0[t];
-    __syncthreads();
     const double M = sp::gersh_block<S>(sA, sB, js, s_red);
     int hint = -1;
     for (int g = 0; g < LG; g++) hint = max(hint, ssv[g].kb[parity ^ 1]);

@@ -23,7 +23,8 @@ namespace bd {
 // Work unit: CHR tile rows x WSC tile columns (a "super-column" is WSC tile
 // columns).  2 x 4 is the default and the only shape for blocks wider than 4
 // columns; the others serve blocks of at most 4 columns.  A single chain's
-// plan takes its shape from plan_pick_shape, a batch's from plan_tune.
+// plan takes its shape and super-column widths from plan_pick, a batch's
+// shape from plan_tune.
 const int kTileShapes[BD_TILE_SHAPES][2] = {{2, 4}, {1, 4}, {4, 4}, {2, 2}, {2, 5}, {2, 6}, {2, 8}};
 
 bool tile_shape_supported(int chr, int wsc)
@@ -33,42 +34,12 @@ bool tile_shape_supported(int chr, int wsc)
     return false;
 }
 
-// Work units of one chain of nt tile rows (the loop of plan_build).
-static long units_of(int nt, int chr, int wsc)
-{
-    long u = 0;
-    for (int sc = 0; sc * wsc < nt; sc++) u += (nt - sc * wsc + chr - 1) / chr;
-    return u;
-}
-
-// Shape of a single chain's plan, from a model of the product's duration:
-// units run in ceil(units / slots) rounds of slots = cus x blocks per CU, and
-// a round costs its unit's tile count plus a fixed 5 tiles' worth (staging x,
-// the reductions, the latency of the first loads).  The model is a function
-// of the chain length and the CU count only, so a chain gets the same shape,
-// and bit-identical products, in every process.  4 x 4 (one block per CU)
-// is left to explicit choice.
-void plan_pick_shape(int nb, int cus, int &chr, int &wsc)
-{
-    const int nt = (3 * nb + TILE - 1) / TILE;
-    double best = 1e300;
-    chr = 2;
-    wsc = 4;
-    for (const auto &s : kTileShapes) {
-        if (s[0] > 2) continue;
-        const long slots = (long)std::max(cus, 1) * 2;
-        const double cost = (double)((units_of(nt, s[0], s[1]) + slots - 1) / slots) * (5.0 + s[0] * s[1]);
-        if (cost < best) {
-            best = cost;
-            chr = s[0];
-            wsc = s[1];
-        }
-    }
-}
 constexpr int NT = 256;  // threads per block of the product kernel
 // Work-unit record (16 ints): chain, super-column, Ib, Ie, ntile, nsc,
-// valid-slot mask, 0, tile base (elements, 2 ints), x row offset (2 ints),
-// row-partial offset (2 ints), 0, 0.
+// valid-slot mask, first column J0 of the super-column, tile base (elements,
+// 2 ints), x row offset (2 ints), row-partial offset (2 ints), canonical
+// index (the unit's place in super-column order, which indexes its column
+// partials), width of the super-column (<= WSC).
 constexpr int UREC = 16;
 
 __host__ __device__ inline long tile_index(int I, int J) { return (long)I * (I + 1) / 2 + J; }
@@ -215,8 +186,10 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
     (void)chain_i;
     (void)chain_l;
     (void)ticket;
-    // inside a Lanczos solve: nothing to do once it has converged
-    if (stop && (stop->converged_j != 0 || stop->ierr != 0)) return;
+    // inside a Lanczos solve: nothing to do once it has converged (the
+    // status words are read together with the first unit's record, and
+    // tested once both have arrived)
+    const int st_conv = stop ? stop->converged_j : 0, st_err = stop ? stop->ierr : 0;
     // units are dealt out statically; one record load describes a unit
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int *rec = units + (long)u * UREC;
@@ -225,6 +198,8 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
         const long tbase = (long)(unsigned)rec[8] | ((long)rec[9] << 32);
         const long xoff = (long)(unsigned)rec[10] | ((long)rec[11] << 32);
         const long rpo = (long)(unsigned)rec[12] | ((long)rec[13] << 32);
+        const int J0 = rec[7], cid = rec[14], wdt = rec[15];
+        if (st_conv != 0 || st_err != 0) return;
         if (active) {
             // one thread reads the flag (another stream may clear it), so the
             // whole block takes the same branch
@@ -236,7 +211,6 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
         }
         const double *x = x_all + xoff;
         const E *mob = mob_all + tbase;
-        const int J0 = sc * WSC;
         // slot sl = jj * CHR + ii (column by column)
         auto present = [&](int sl) __attribute__((always_inline)) {
             return (mask >> ((sl % CHR) * WSC + sl / CHR)) & 1u;
@@ -335,13 +309,15 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
                 }
             }
             // column partial of tile column J0 + jj: the 16 row pieces summed
+            // (columns past the super-column's width are never read)
+            if (jj >= wdt) continue;
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int k = 0; k < S; k++) sred[rg * LDR + (cg + 16 * i) * S + k] = cacc[i][k];
             __syncthreads();
-            double *dst = colpart + ((long)u * WSC + jj) * (TILE * S);
+            double *dst = colpart + ((long)cid * WSC + jj) * (TILE * S);
             for (int e = tid; e < TILE * S; e += NT) {
                 double v = 0.0;
 #pragma unroll
@@ -370,21 +346,31 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
     }
 }
 
-// Output block G (a tile row of some chain):
-//   y = sum_{sc <= K/wsc} rowpart[K, sc] + sum_{units u of super-column K/wsc} colpart[u, K % wsc]
-// rowblk (8 ints per block): chain, K, nsc, u0, u1, rp_off, y row offset.
+// Output block G (a tile row K of some chain, in super-column sc(K) at
+// offset off(K)):
+//   y = sum_{sc <= sc(K)} rowpart[K, sc] + sum_{units u of sc(K)} colpart[u, off(K)]
+// rowblk (8 ints per block): chain, K, nsc, u0, u1 (canonical indices of
+// sc(K)'s units), rp_off, y row offset, sc(K) | off(K) << 16.
 // Sums in a fixed order (the row partials, then the column partials); the
 // two lists are read as one, PB loads at a time at clamped addresses, so a
 // batch is one memory round trip.
+struct BlockRec {
+    int K, nsc, u0, u1, rpo, yrow, sc, off;   // sc, off: K's super-column and its place in it
+};
+__device__ inline BlockRec block_rec(const int *__restrict__ rb)
+{
+    return {rb[1], rb[2], rb[3], rb[4], rb[5], rb[6], rb[7] & 0xffff, rb[7] >> 16};
+}
+
 template <int S>
 __device__ inline double sum_block_partials(const double *__restrict__ rowpart, const double *__restrict__ colpart,
-                                            const int *__restrict__ rb, int e, int wsc)
+                                            const BlockRec &br, int e, int wsc)
 {
     constexpr int PB = 32;
-    const int K = rb[1], nsc = rb[2], u0 = rb[3], u1 = rb[4], rpo = rb[5];
-    const int nrow = K / wsc + 1, ntot = nrow + (u1 - u0);
+    const int K = br.K, nsc = br.nsc, u0 = br.u0, u1 = br.u1, rpo = br.rpo;
+    const int nrow = br.sc + 1, ntot = nrow + (u1 - u0);
     const double *rp = rowpart + ((long)rpo + (long)K * nsc) * (TILE * S) + e;
-    const double *cp = colpart + ((long)u0 * wsc + K % wsc) * (TILE * S) + e;
+    const double *cp = colpart + ((long)u0 * wsc + br.off) * (TILE * S) + e;
     const long cstr = (long)wsc * (TILE * S);
     double v = 0.0;
     for (int i0 = 0; i0 < ntot; i0 += PB) {
@@ -414,7 +400,8 @@ __global__ __launch_bounds__(256) void symm_reduce_kernel(const double *__restri
     const int *rb = rowblk + 8 * G;
     if (active && !active[rb[0]]) return;
     const int r = e / S, k = e % S;
-    y[(long)k * ldy + rb[6] + r] = sum_block_partials<S>(rowpart, colpart, rb, e, wsc);
+    const BlockRec br = block_rec(rb);
+    y[(long)k * ldy + br.yrow + r] = sum_block_partials<S>(rowpart, colpart, br, e, wsc);
 }
 
 // The same sum for one output block per workgroup (TILE * S threads), fused
@@ -434,13 +421,14 @@ __global__ __launch_bounds__(TILE * MAX_S) void symm_reduce_lz_kernel(
     __shared__ double sw[TILE][S];
     __shared__ double red[TILE][NO];
     __shared__ double red2[NGR][NO];
+    const int G = blockIdx.x, e = threadIdx.x;
+    // the block's record and the status are read together
+    const BlockRec br = block_rec(rowblk + 8 * G);
     if (threadIdx.x == 0) s_done = (stop && (stop->converged_j != 0 || stop->ierr != 0)) ? 1 : 0;
     __syncthreads();
     if (s_done) return;
-    const int G = blockIdx.x, e = threadIdx.x;
-    const int *rb = rowblk + 8 * G;
     const int r = e / S, k = e % S;
-    const long row = rb[6] + r;
+    const long row = br.yrow + r;
     // this row's vector entries first: their loads overlap the partial sums
     double vj[S], vp[S];
 #pragma unroll
@@ -448,7 +436,7 @@ __global__ __launch_bounds__(TILE * MAX_S) void symm_reduce_lz_kernel(
         vj[c] = Vj[(long)c * ld + row];
         vp[c] = Vp ? Vp[(long)c * ld + row] : 0.0;
     }
-    double v = sum_block_partials<S>(rowpart, colpart, rb, e, wsc);
+    double v = sum_block_partials<S>(rowpart, colpart, br, e, wsc);
     if (Vp) {
         double t = 0.0;
 #pragma unroll
@@ -502,18 +490,142 @@ __global__ void unpack_kernel(const E *__restrict__ mob, int n, double *__restri
 }
 
 // ---------------------------------------------------------------- host side
+// Tiles of the unit (rows Ib .. Ie-1, columns J0 .. J0+w-1) that are stored.
+static int unit_tiles(int Ib, int Ie, int J0, int w)
+{
+    int t = 0;
+    for (int I = Ib; I < Ie; I++) t += std::max(0, std::min(I, J0 + w - 1) - J0 + 1);
+    return t;
+}
+
+// Super-column widths of one chain: `first` columns wide for the first `a`
+// super-columns, then `rest` wide (the last one takes what is left).
+static std::vector<int> make_widths(int nt, int first, int a, int rest)
+{
+    std::vector<int> w;
+    for (int J0 = 0; J0 < nt;) {
+        const int x = std::min((int)w.size() < a ? first : rest, nt - J0);
+        w.push_back(x);
+        J0 += x;
+    }
+    return w;
+}
+
+// Execution order of a single round of units on `slots` block slots of
+// which block b and block b + slots/2 share a CU: the largest units run
+// alone, the others in pairs of large with small.  Returns positions ->
+// unit (canonical) indices; identity when the units do not fit one round.
+static std::vector<int> pair_order(const std::vector<int> &tiles, int slots)
+{
+    const int U = (int)tiles.size(), M = slots / 2;
+    std::vector<int> idx(U);
+    for (int i = 0; i < U; i++) idx[i] = i;
+    if (U <= M || U > 2 * M) return idx;
+    std::vector<int> srt = idx;
+    std::stable_sort(srt.begin(), srt.end(), [&](int x, int y) { return tiles[x] > tiles[y]; });
+    const int k = U - M;   // CUs that get two units
+    std::vector<int> pos(U);
+    for (int i = 0; i < M - k; i++) pos[k + i] = srt[i];   // singles
+    for (int j = 0; j < k; j++) {
+        pos[j] = srt[M - k + j];        // larger of the pair
+        pos[M + j] = srt[U - 1 - j];    // smaller of the pair
+    }
+    return pos;
+}
+
+// The largest per-CU cost of a single chain's units under pair_order, in
+// tile-equivalents: 4 for a CU's start-up, 0.8 per tile on it (measured
+// on MI355X: one unit of 10 tiles alone about 13 us, two on one CU about
+// 19 us); more than one round costs the rounds times a full CU.
+static double plan_cost(const std::vector<int> &tiles, int slots)
+{
+    const int U = (int)tiles.size(), M = slots / 2;
+    int tmax = 0;
+    for (int t : tiles) tmax = std::max(tmax, t);
+    if (U > 2 * M) return (double)((U + 2 * M - 1) / (2 * M)) * (4.0 + 0.8 * 2 * tmax);
+    const std::vector<int> pos = pair_order(tiles, slots);
+    double worst = 0.0;
+    for (int c = 0; c < std::min(U, M); c++) {
+        int t = tiles[pos[c]];
+        if (c + M < U) t += tiles[pos[c + M]];
+        worst = std::max(worst, 4.0 + 0.8 * t);
+    }
+    return worst;
+}
+
+static std::vector<int> chain_unit_tiles(int nt, int chr, const std::vector<int> &w)
+{
+    std::vector<int> t;
+    for (int J0 = 0, sc = 0; sc < (int)w.size(); J0 += w[sc], sc++)
+        for (int Ib = J0; Ib < nt; Ib += chr) t.push_back(unit_tiles(Ib, std::min(Ib + chr, nt), J0, w[sc]));
+    return t;
+}
+
+// Shape and super-column widths of a single chain's plan: among the unit
+// templates (4 x 4 aside, which runs one block per CU) and, for each, the
+// uniform widths and every "a super-columns of the full width, then
+// narrower ones" mix, the one of least plan_cost on slots = 2 x CUs.  A
+// function of the chain length and the CU count only, so a chain gets the
+// same plan, and bit-identical products, in every process.
+void plan_pick(int nb, int cus, int &chr, int &wsc, std::vector<int> &widths)
+{
+    const int nt = (3 * nb + TILE - 1) / TILE, slots = 2 * std::max(cus, 1);
+    double best = 1e300;
+    chr = 2;
+    wsc = 4;
+    widths = make_widths(nt, 4, 0, 4);
+    for (const auto &shp : kTileShapes) {
+        if (shp[0] > 2) continue;
+        const int c = shp[0], W = shp[1];
+        for (int rest = W; rest >= 2; rest--) {
+            for (int a = 0; a * W < nt + W; a++) {
+                if (rest == W && a > 0) break;   // uniform: one candidate
+                const std::vector<int> w = make_widths(nt, W, a, rest);
+                const double cost = plan_cost(chain_unit_tiles(nt, c, w), slots);
+                if (cost < best - 1e-9) {
+                    best = cost;
+                    chr = c;
+                    wsc = W;
+                    widths = w;
+                }
+            }
+        }
+    }
+}
+
 // Builds the tile list, work units and output-block table of a set of
 // chains with n_c rows each, stored back to back (row offsets row_off_c,
-// tile offsets in the order given).
-int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, int CHR, int WSC)
+// tile offsets in the order given).  widths: per chain, the super-column
+// widths (each <= WSC; uniform WSC when null).  slots > 0: order a single
+// round of units by pair_order for that many block slots.
+// The host half of plan_build: the records only.
+struct PlanLayout {
+    std::vector<int> tiles, units, rowblk, ci;
+    std::vector<long> cl;
+    long ntiles = 0, nrowpart = 0;
+};
+
+static int plan_layout(PlanLayout &L, const std::vector<int> &nb, const std::vector<long> &row_off, int CHR,
+                       int WSC, const std::vector<std::vector<int>> *widths, int slots)
 {
     if (!tile_shape_supported(CHR, WSC)) { set_error("unsupported tile unit shape"); return BD_ERR_ARG; }
     const int C = (int)nb.size();
-    std::vector<int> tiles, units, rowblk, ci(4 * C);
-    std::vector<long> cl(3 * C);
+    if (widths && (int)widths->size() != C) { set_error("internal: widths per chain"); return BD_ERR_ARG; }
+    std::vector<int> &tiles = L.tiles, &units = L.units, &rowblk = L.rowblk, &ci = L.ci, ntl;
+    std::vector<long> &cl = L.cl;
+    ci.assign(4 * C, 0);
+    cl.assign(3 * C, 0);
     long toff = 0, rpo = 0;
     for (int ch = 0; ch < C; ch++) {
-        const int n = 3 * nb[ch], nt = (n + TILE - 1) / TILE, nsc = (nt + WSC - 1) / WSC;
+        const int n = 3 * nb[ch], nt = (n + TILE - 1) / TILE;
+        const std::vector<int> w = widths ? (*widths)[ch] : make_widths(nt, WSC, 0, WSC);
+        int sum = 0;
+        for (int x : w) {
+            if (x < 1 || x > WSC) { set_error("internal: super-column wider than the unit"); return BD_ERR_ARG; }
+            sum += x;
+        }
+        if (sum != nt) { set_error("internal: super-column widths do not cover the chain"); return BD_ERR_ARG; }
+        const int nsc = (int)w.size();
         ci[4 * ch] = nb[ch];
         ci[4 * ch + 1] = n;
         ci[4 * ch + 2] = nt;
@@ -525,29 +637,33 @@ int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &
             for (int J = 0; J <= I; J++) {
                 tiles.push_back(ch); tiles.push_back(I); tiles.push_back(J); tiles.push_back(0);
             }
-        std::vector<int> scu(nsc + 1, 0);
+        std::vector<int> scu(nsc + 1, 0), scJ0(nsc + 1, 0);
         for (int sc = 0; sc < nsc; sc++) {
             scu[sc] = (int)(units.size() / UREC);
-            for (int Ib = sc * WSC; Ib < nt; Ib += CHR) {
-                const int Ie = std::min(Ib + CHR, nt), J0 = sc * WSC;
+            const int J0 = scJ0[sc];
+            scJ0[sc + 1] = J0 + w[sc];
+            for (int Ib = J0; Ib < nt; Ib += CHR) {
+                const int Ie = std::min(Ib + CHR, nt);
                 int mask = 0;
                 for (int sl = 0; sl < CHR * WSC; sl++) {
                     const int I = Ib + sl / WSC, J = J0 + sl % WSC;
-                    if (I < Ie && J <= I && J < nt) mask |= 1 << sl;
+                    if (I < Ie && J <= I && J < J0 + w[sc]) mask |= 1 << sl;
                 }
                 const long tb = toff * (TILE * TILE), xo = row_off[ch];
-                const int rec[UREC] = {ch, sc, Ib, Ie, nt, nsc, mask, 0,
+                const int cid = (int)(units.size() / UREC);
+                const int rec[UREC] = {ch, sc, Ib, Ie, nt, nsc, mask, J0,
                                        (int)(tb & 0xffffffffL), (int)(tb >> 32), (int)(xo & 0xffffffffL),
-                                       (int)(xo >> 32), (int)(rpo & 0xffffffffL), (int)(rpo >> 32), 0, 0};
+                                       (int)(xo >> 32), (int)(rpo & 0xffffffffL), (int)(rpo >> 32), cid, w[sc]};
                 units.insert(units.end(), rec, rec + UREC);
+                ntl.push_back(unit_tiles(Ib, Ie, J0, w[sc]));
             }
         }
         scu[nsc] = (int)(units.size() / UREC);
-        for (int K = 0; K < nt; K++) {
-            int sc = K / WSC;
+        for (int K = 0, sc = 0; K < nt; K++) {
+            while (K >= scJ0[sc + 1]) sc++;
             rowblk.push_back(ch); rowblk.push_back(K); rowblk.push_back(nsc); rowblk.push_back(scu[sc]);
             rowblk.push_back(scu[sc + 1]); rowblk.push_back((int)rpo); rowblk.push_back((int)(row_off[ch] + (long)K * TILE));
-            rowblk.push_back(0);
+            rowblk.push_back(sc | ((K - scJ0[sc]) << 16));
         }
         toff += (long)nt * (nt + 1) / 2;
         rpo += (long)nt * nsc;
@@ -556,10 +672,32 @@ int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &
         set_error("batch too large for 32-bit block offsets");
         return BD_ERR_ARG;
     }
-    p.nchains = C;
+    const int U = (int)(units.size() / UREC);
+    if (slots > 0) {
+        const std::vector<int> pos = pair_order(ntl, slots);
+        std::vector<int> ord(units.size());
+        for (int i = 0; i < U; i++)
+            std::copy(units.begin() + (long)pos[i] * UREC, units.begin() + (long)(pos[i] + 1) * UREC,
+                      ord.begin() + (long)i * UREC);
+        units.swap(ord);
+    }
+    L.ntiles = toff;
+    L.nrowpart = rpo;
+    return BD_OK;
+}
+
+int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, int CHR, int WSC,
+               const std::vector<std::vector<int>> *widths, int slots)
+{
+    PlanLayout L;
+    BD_TRY(plan_layout(L, nb, row_off, CHR, WSC, widths, slots));
+    const std::vector<int> &tiles = L.tiles, &units = L.units, &rowblk = L.rowblk, &ci = L.ci;
+    const std::vector<long> &cl = L.cl;
+    const long rpo = L.nrowpart;
+    p.nchains = (int)nb.size();
     p.chr = CHR;
     p.wsc = WSC;
-    p.ntiles = toff;
+    p.ntiles = L.ntiles;
     p.nunits = (int)(units.size() / UREC);
     p.nrowblk = (int)(rowblk.size() / 8);
     auto up = [](auto **dst, const auto &v) -> int {
@@ -577,6 +715,31 @@ int plan_build(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &
     BD_HIP(hipMalloc(&p.colpart, (size_t)std::max(p.nunits, 1) * WSC * TILE * MAX_S * sizeof(double)));
     return BD_OK;
 }
+
+}  // namespace bd
+
+// Work decomposition of the single-chain plan a context of nbeads beads
+// would use on a GPU with cus CUs (host only: no device call).
+extern "C" int bd_plan_layout(int nbeads, int cus, int *chr, int *wsc, int *nunits, int *units, int units_cap,
+                              int *nrowblk, int *rowblk, int rowblk_cap)
+{
+    using namespace bd;
+    if (nbeads < 1 || cus < 1 || !chr || !wsc || !nunits || !nrowblk) {
+        set_error("bd_plan_layout: bad arguments");
+        return BD_ERR_ARG;
+    }
+    std::vector<std::vector<int>> w(1);
+    plan_pick(nbeads, cus, *chr, *wsc, w[0]);
+    PlanLayout L;
+    BD_TRY(plan_layout(L, std::vector<int>{nbeads}, std::vector<long>{0}, *chr, *wsc, &w, 2 * cus));
+    *nunits = (int)(L.units.size() / UREC);
+    *nrowblk = (int)(L.rowblk.size() / 8);
+    if (units && units_cap >= (int)L.units.size()) std::copy(L.units.begin(), L.units.end(), units);
+    if (rowblk && rowblk_cap >= (int)L.rowblk.size()) std::copy(L.rowblk.begin(), L.rowblk.end(), rowblk);
+    return BD_OK;
+}
+
+namespace bd {
 
 void plan_free(MobPlan &p)
 {
@@ -601,8 +764,9 @@ int mob_alloc(Ctx &c)
     else BD_HIP(hipMalloc(&c.mob64, elems * sizeof(double)));
     int cus = 256, chr = 2, wsc = 4;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device);
-    plan_pick_shape(c.nb, cus, chr, wsc);
-    BD_TRY(plan_build(c.plan, std::vector<int>{c.nb}, std::vector<long>{0}, chr, wsc));
+    std::vector<std::vector<int>> widths(1);
+    plan_pick(c.nb, cus, chr, wsc, widths[0]);
+    BD_TRY(plan_build(c.plan, std::vector<int>{c.nb}, std::vector<long>{0}, chr, wsc, &widths, 2 * cus));
     c.nunits = c.plan.nunits;
     return BD_OK;
 }

@@ -295,6 +295,22 @@ int bd_batch_set_tile_shape(bd_batch *batch, int chr, int wsc);
 int bd_batch_prof(bd_batch *batch, int on, double *total_ms,
                   long long *launches, double *tile_bytes);
 
+/* Work decomposition of the mobility product (no counterpart in the
+ * reference: introspection for tests and tuning; host only, no device call).
+ * The single-chain plan a context of nbeads beads uses on a GPU with cus
+ * CUs: the unit template chr x wsc (tile rows x tile columns), the unit
+ * records in execution order (16 ints each: chain, super-column, first and
+ * past-last tile row, tile rows of the chain, super-columns of the chain,
+ * stored-tile mask (bit ii * wsc + jj), first tile column, tile base (2
+ * ints), x row offset (2 ints), row-partial offset (2 ints), canonical
+ * index, super-column width) and the output-block records (8 ints each:
+ * chain, tile row K, super-columns, first and past-last canonical unit of
+ * K's super-column, row-partial offset, y row offset, K's super-column |
+ * K's offset in it << 16).  The arrays are filled when their capacity (in
+ * ints) suffices; the counts always. */
+int bd_plan_layout(int nbeads, int cus, int *chr, int *wsc, int *nunits, int *units, int units_cap,
+                   int *nrowblk, int *rowblk, int rowblk_cap);
+
 #ifdef __cplusplus
 }
 #endif
