@@ -87,6 +87,19 @@ struct LzEpi {
     const double *Vj, *Vp, *Bp;
     double *part;
 };
+// Lanczos prologue of a product: the second CholeskyQR pass of the block
+// the product multiplies.  x holds Q; the product uses V = Q R2^{-1} with
+// R2^T R2 the sum of the ng Gram partials of Q, writes V (same layout as
+// x) and B = R2 R1 (R1 the first pass's factor); a failed factorisation
+// sets st->ierr = 2.  gpart == nullptr: x is used as it is.
+struct XPre {
+    const double *gpart = nullptr;
+    int ng = 0;
+    const double *R1 = nullptr;
+    double *Bout = nullptr;
+    double *V = nullptr;
+    DevStatus *st = nullptr;
+};
 extern const int kTileShapes[BD_TILE_SHAPES][2];
 bool tile_shape_supported(int chr, int wsc);
 void plan_pick(int nb, int cus, int &chr, int &wsc, std::vector<int> &widths);
@@ -103,7 +116,7 @@ int rpy_launch(const MobPlan &p, const double *crd, E *mob, hipStream_t sm);
 template <typename E, int S>
 int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int ldx, int ldy, int *ticket,
                  const DevStatus *stop, const int *active, hipStream_t sm, int device, hipEvent_t e0, hipEvent_t e1,
-                 const LzEpi *epi = nullptr);
+                 const LzEpi *epi = nullptr, const XPre *pre = nullptr);
 
 struct Ctx {
     int nb = 0, n = 0, npad = 0, ntile = 0, ntiles_lower = 0;
@@ -147,13 +160,14 @@ struct Ctx {
     int *bound_ok = nullptr;     // NBOUND flags
     sp::SplitState *split = nullptr;   // state of the split small problem
     double *lzpart = nullptr;          // per-block partials of the product epilogue
+    double *Qbuf = nullptr;            // next Krylov block before the second CholeskyQR pass (npad x s)
     double *npart = nullptr;     // norm partials
     DevStatus *st = nullptr;
     DevStatus *st_host = nullptr;  // pinned mirror, 2 slots
     hipStream_t side = nullptr;    // small-problem stream of the Lanczos solve
     hipEvent_t lz_ev[LZ_NEV] = {};
     hipEvent_t lz_chunk_ev[2] = {};
-    // grouped tests: {iteration tested through, converged_j, ierr} written by
+    // grouped tests: {iteration tested through, converged_j, ierr, nodes} written by
     // the combine kernel into pinned host memory (lz_flag: host view,
     // lz_flag_dev: the device's address of the same words)
     volatile int *lz_flag = nullptr;
@@ -224,7 +238,8 @@ int rpy_assemble(Ctx &c);
 // y(npad, s) = D x(npad, s) on the context stream; x, y device, ld = npad.
 int mob_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy);
 // Same product inside a Lanczos solve: skipped once the solve has converged.
-int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy, const LzEpi *epi = nullptr);
+int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy, const LzEpi *epi = nullptr,
+                   const XPre *pre = nullptr);
 // y = alpha * (D x + add) with optional add vector (s = 1); used for the drift.
 int mob_unpack(Ctx &c, double *dense, int ld);
 

@@ -7,7 +7,8 @@
 //   W = D V_j                                  (symmetric tile product)
 //   W -= V_{j-1} B_{j-1}^T ; A_j = V_j^T W     (lanczos_a)
 //   W -= V_j symL(A_j)                          (lanczos_b)
-//   W = V_{j+1} B_j                             (CholeskyQR2, FP64)
+//   W = V_{j+1} B_j                             (CholeskyQR2, FP64; the second
+//                                                 pass inside the next product)
 //   y_j = V_{1:j} sqrt(H_j) E_1 R_0             (small problem, FP64)
 //   err_j = ||y_j - y_{j-1}|| / ||y_{j-1}||     (2-norm for s = 1, max column
 //                                                 1-norm for s > 1)
@@ -389,7 +390,9 @@ __global__ __launch_bounds__(KT) void krylov_fused_kernel(const double *__restri
 // W2 = W1 - V_j symL(A) without another pass over the rows,
 //   W2^T W2 = P - A^T Ah - Ah^T A + Ah^T Ah     (V_j^T V_j = I, Ah = symL(A)),
 // factors it (R1) and writes Q = W2 R1^{-1} with the partial Gram matrix of
-// Q; the second CholeskyQR pass (cholqr_kernel) completes V_{j+1} and B_j.
+// Q.  The second CholeskyQR pass (V_{j+1} = Q R2^{-1}, B_j = R2 R1) is done
+// by the next iteration's product as it stages Q (XPre), which saves a
+// launch per iteration; cholqr_kernel does it where no product follows.
 template <int S>
 __global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict__ W, const double *__restrict__ Vj,
                                                        double *__restrict__ Q, int ld, int rows,
@@ -667,7 +670,8 @@ __global__ __launch_bounds__(sp::WG_T) void small_group_kernel(const double *__r
 // decision.  Each workgroup writes the weighted sum of its nodes' Z in
 // part[wg]; the last one sums the parts in workgroup order and forms C = H Z.
 // Grid: x = member, y = node batch, so that the batches that have nodes
-// (the low ones) of every member are dispatched before the idle ones.
+// (the low ones) of every member are dispatched before the idle ones; the
+// host sizes y from the node count the previous group reported.
 template <int S>
 __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *__restrict__ Ablk,
                                                                  const double *__restrict__ Bblk,
@@ -720,6 +724,9 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
         int Ng;
         if (place(kb_guess, mg, Ng, hg) <= DEC_MAX) nact_g = (2 * Ng + 1 + NODES - 1) / NODES;
     }
+    // the grid may hold fewer batches than the placement (sized from the
+    // previous group's node count): then every workgroup takes several
+    nact_g = min(nact_g, (int)gridDim.y);
     if ((int)blockIdx.y >= nact_g) return;
     // this wave's slots
     double *buf = s_buf + (w < PROBES ? w * 2 * SS * ROWS : (PROBES * 2 + (w - PROBES) * 3) * SS * ROWS);
@@ -826,15 +833,16 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
     }
 }
 
-// The host's view of a group of tests: converged_j and ierr, then the last
-// iteration the group covered; the host waits for the latter before it
-// reads the former.  The host reads nothing else the device wrote, so the
+// The host's view of a group of tests: converged_j, ierr and the small
+// problem's node count, then the last iteration the group covered; the host
+// waits for the latter before it reads the former.  The host reads nothing else the device wrote, so the
 // three system-scope stores with a wait for completion before the last one
 // are the whole protocol (no cache write-back).
-__device__ inline void report_tests(int *flag, int jend, int conv, int ierr)
+__device__ inline void report_tests(int *flag, int jend, int conv, int ierr, int nq)
 {
     __hip_atomic_store(flag + 1, conv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(flag + 2, ierr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(flag + 3, nq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     stores_complete();
     __hip_atomic_store(flag, jend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
@@ -915,7 +923,7 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
     __syncthreads();
     if (s_done) {
         // nothing to test: report the state reached, as the last block would
-        if (blockIdx.x == 0 && threadIdx.x == 0) report_tests(flag, jend, st->converged_j, st->ierr);
+        if (blockIdx.x == 0 && threadIdx.x == 0) report_tests(flag, jend, st->converged_j, st->ierr, st->nq);
         return;
     }
     double acc[LG][S];
@@ -1024,7 +1032,7 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
             break;
         }
     }
-    report_tests(flag, jend, st->converged_j, st->ierr);
+    report_tests(flag, jend, st->converged_j, st->ierr, st->nq);
 }
 
 // y = V(:, 1:js) C, 32 rows per block and the js columns split over eight
@@ -1161,6 +1169,7 @@ int lanczos_alloc(Ctx &c)
     BD_HIP(hipMalloc(&c.Ccoef, (size_t)LG * ncol * s * sizeof(double)));
     BD_HIP(hipMalloc(&c.bound_ok, NBOUND * sizeof(int)));
     BD_HIP(hipMalloc(&c.lzpart, (size_t)(c.npad / TILE + 1) * 2 * MAX_S * MAX_S * sizeof(double)));
+    BD_HIP(hipMalloc(&c.Qbuf, (size_t)c.npad * MAX_S * sizeof(double)));
     BD_HIP(hipMalloc(&c.split, LG * sizeof(sp::SplitState)));
     {
         sp::SplitState z[LG];
@@ -1207,6 +1216,8 @@ void lanczos_free(Ctx &c)
     c.split = nullptr;
     if (c.lzpart) (void)hipFree(c.lzpart);
     c.lzpart = nullptr;
+    if (c.Qbuf) (void)hipFree(c.Qbuf);
+    c.Qbuf = nullptr;
     if (c.chol) (void)hipFree(c.chol);
     c.chol = nullptr;
 }
@@ -1291,10 +1302,12 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     int slot = 0;
     bool stop = false;
     int pend_j = 0;     // grouped: last iteration of the previous group
+    int pcr_batches = 0;   // node batches of the last reported placement (0: unknown)
     if (grouped) {
         c.lz_flag[0] = 0;
         c.lz_flag[1] = 0;
         c.lz_flag[2] = 0;
+        c.lz_flag[3] = 0;
     }
     for (int j = 1; j <= dmax && !stop; j++) {
         double *Vj = c.V + (size_t)(j - 1) * S * ld;
@@ -1302,16 +1315,23 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
         const double *Bp = (j > 1) ? c.Bblk + (size_t)(j - 2) * S * S : nullptr;
         const int upd = (j < dmax) ? 1 : 0;
         if (twopass) {
-            // product + epilogue, then one kernel to Q and the last CholeskyQR pass
+            // product (which, from j = 2 on, first completes V_j from Q by
+            // the second CholeskyQR pass and writes B_{j-1}) + epilogue, then
+            // one kernel to A_j, R1 and Q of the next block
             LzEpi epi{Vj, Vp, Bp, c.lzpart};
-            BD_TRY(mob_product_lz(c, Vj, c.W, S, ld, ld, &epi));
-            double *Vn = c.V + (size_t)j * S * ld;
-            hipLaunchKernelGGL((krylov_k2_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.W, Vj, Vn, ld, rows, c.lzpart,
-                               c.plan.nrowblk, c.Ablk + (size_t)(j - 1) * S * S, c.R0 + S * S, upd, g0, c.st);
-            if (upd)
-                hipLaunchKernelGGL((cholqr_kernel<S>), dim3(gb), dim3(RB), 0, sm, Vn, Vn, ld, rows, g0,
-                                   (double *)nullptr, c.Bblk + (size_t)(j - 1) * S * S,
-                                   (const double *)(c.R0 + S * S), c.st);
+            XPre pre;
+            if (j > 1) {
+                pre.gpart = g0;
+                pre.ng = gb;
+                pre.R1 = c.R0 + S * S;
+                pre.Bout = c.Bblk + (size_t)(j - 2) * S * S;
+                pre.V = Vj;
+                pre.st = c.st;
+            }
+            BD_TRY(mob_product_lz(c, j > 1 ? c.Qbuf : Vj, c.W, S, ld, ld, &epi, j > 1 ? &pre : nullptr));
+            hipLaunchKernelGGL((krylov_k2_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.W, Vj, c.Qbuf, ld, rows,
+                               c.lzpart, c.plan.nrowblk, c.Ablk + (size_t)(j - 1) * S * S, c.R0 + S * S, upd, g0,
+                               c.st);
             BD_HIP(hipGetLastError());
         } else {
             BD_TRY(mob_product_lz(c, Vj, c.W, S, ld, ld));
@@ -1364,8 +1384,11 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             if constexpr (S <= 4) {
                 const long part_stride = (long)nwg * dmax * S * S;
                 const long c_stride = (long)c.dimk_max * S * S;
+                // node batches: those of the previous group's placement and one
+                // more (a placement of more nodes is covered by the kernel's loop)
+                const int nby = pcr_batches > 0 ? std::min(nwg, pcr_batches + 1) : nwg;
                 if (use_pcr)
-                    hipLaunchKernelGGL((small_pcr_kernel<S>), dim3(j - jg0 + 1, nwg), dim3(pcr::THREADS), 0, sg,
+                    hipLaunchKernelGGL((small_pcr_kernel<S>), dim3(j - jg0 + 1, nby), dim3(pcr::THREADS), 0, sg,
                                        c.Ablk, c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split,
                                        c.st);
                 else
@@ -1405,6 +1428,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             if (pend_j > 0) {
                 BD_TRY(wait_tests(c, sg, pend_j));
                 if (c.lz_flag[2] || c.lz_flag[1]) stop = true;
+                if (c.lz_flag[3] > 0) pcr_batches = (c.lz_flag[3] + pcr::NODES - 1) / pcr::NODES;
             }
             pend_j = j;
         } else if (j % chunk == 0 || j == dmax) {

@@ -116,11 +116,10 @@ __device__ __forceinline__ void block_sum_parts(const double *__restrict__ part,
     __syncthreads();
 }
 
-// Sum of the gridDim.x partials of S*S values left by a kernel of the same
-// grid, fixed order, by the whole block (blockDim.x == RB).  Result in g
-// (shared), valid after the call.
+// Sum of n partials of S*S values, fixed order, by the whole block
+// (blockDim.x == RB).  Result in g (shared), valid after the call.
 template <int S>
-__device__ void sum_partials(const double *part, double *g)
+__device__ void sum_partials_n(const double *part, int n, double *g)
 {
     constexpr int E = S * S;
     constexpr int STR = RB / E;          // stripes over the partial index
@@ -129,7 +128,7 @@ __device__ void sum_partials(const double *part, double *g)
     double v = 0.0;
     if (t < E * STR) {
         const int e = t % E, st = t / E;
-        for (int b = st; b < (int)gridDim.x; b += STR) v += part[b * E + e];
+        for (int b = st; b < n; b += STR) v += part[b * E + e];
     }
     sh[t] = v;
     __syncthreads();
@@ -139,6 +138,13 @@ __device__ void sum_partials(const double *part, double *g)
         g[t] = a;
     }
     __syncthreads();
+}
+
+// The gridDim.x partials left by a kernel of the same grid.
+template <int S>
+__device__ void sum_partials(const double *part, double *g)
+{
+    sum_partials_n<S>(part, (int)gridDim.x, g);
 }
 
 // Cholesky of an S x S SPD matrix g (col-major): g = R^T R, R upper.

@@ -17,6 +17,7 @@
 #include <type_traits>
 
 #include "bd_internal.h"
+#include "bd_lanczos_dev.h"
 
 namespace bd {
 
@@ -174,7 +175,8 @@ template <typename E, int S, int CHR, int WSC>
 __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
     const E *__restrict__ mob_all, const double *__restrict__ x_all, int ldx, const int *__restrict__ units,
     int nunits, const int *__restrict__ chain_i, const long *__restrict__ chain_l, double *__restrict__ rowpart,
-    double *__restrict__ colpart, int *__restrict__ ticket, const DevStatus *stop, const int *__restrict__ active)
+    double *__restrict__ colpart, int *__restrict__ ticket, const DevStatus *stop, const int *__restrict__ active,
+    XPre pre)
 {
     constexpr int NS = CHR * WSC, PD = PDT<E, S>, LDR = TILE * S + 2;
     // x_J[c][k] at xJs[jj][k][(c % 16) * 4 + c / 16] (a thread's 4 columns
@@ -190,6 +192,37 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
     // status words are read together with the first unit's record, and
     // tested once both have arrived)
     const int st_conv = stop ? stop->converged_j : 0, st_err = stop ? stop->ierr : 0;
+    // second CholeskyQR pass of the block (XPre): R2 from the Gram partials,
+    // by every block (the same fixed-order sum and Cholesky as cholqr_kernel;
+    // done before the unit loop: inside it, with the first tiles' registers
+    // live, the FP64 variants spill)
+    __shared__ double sR2[S * S];
+    __shared__ int s_fail;
+    if (pre.gpart && st_conv == 0 && st_err == 0) {
+        __shared__ double sG[S * S];
+        sum_partials_n<S>(pre.gpart, pre.ng, sG);   // ends with a barrier
+        if (tid == 0) {
+            double R[S][S];
+            const bool ok = small_chol<S>(sG, R);
+            s_fail = ok ? 0 : 1;
+            for (int i = 0; i < S; i++)
+                for (int j = 0; j < S; j++) sR2[i + j * S] = R[i][j];
+            if (ok && blockIdx.x == 0) {
+                // B = R2 R1 (both upper triangular)
+                for (int i = 0; i < S; i++)
+                    for (int j = 0; j < S; j++) {
+                        double t = 0.0;
+                        for (int k = 0; k < S; k++) t += R[i][k] * pre.R1[k + j * S];
+                        pre.Bout[i + j * S] = t;
+                    }
+            }
+        }
+        __syncthreads();
+        if (s_fail) {
+            if (blockIdx.x == 0 && tid == 0) atomicExch(&pre.st->ierr, 2);
+            return;
+        }
+    }
     // units are dealt out statically; one record load describes a unit
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int *rec = units + (long)u * UREC;
@@ -251,6 +284,33 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
             }
         }
         __syncthreads();
+        if (pre.gpart) {
+            // x rows q -> q R2^{-1} in place (the forward solve of cholqr_kernel)
+            for (int rr = tid; rr < (WSC + CHR) * TILE; rr += NT) {
+                const int blk = rr / TILE, r = rr % TILE;
+                double *xs = blk < WSC ? &xJs[blk][0][(r % 16) * 4 + r / 16] : &xIs[blk - WSC][0][r];
+                double q[S];
+#pragma unroll
+                for (int c = 0; c < S; c++) {
+                    double t = xs[c * TILE];
+#pragma unroll
+                    for (int k = 0; k < c; k++) t -= q[k] * sR2[k + c * S];
+                    q[c] = t / sR2[c + c * S];
+                }
+#pragma unroll
+                for (int c = 0; c < S; c++) xs[c * TILE] = q[c];
+            }
+            __syncthreads();
+            // the first super-column's units hold every row block once as
+            // their row blocks: they write V
+            if (sc == 0) {
+                double *V = pre.V + xoff;
+                for (int e = tid; e < CHR * TILE * S; e += NT) {
+                    const int ii = e / (TILE * S), rem = e % (TILE * S), k = rem / TILE, r = rem % TILE;
+                    if (Ib + ii < Ie) V[(long)k * ldx + (long)(Ib + ii) * TILE + r] = xIs[ii][k][r];
+                }
+            }
+        }
         double racc[CHR][4][S];
 #pragma unroll
         for (int a = 0; a < CHR; a++)
@@ -806,8 +866,9 @@ int rpy_assemble(Ctx &c) { return rpy_assemble_at(c, c.crd); }
 template <typename E, int S>
 int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int ldx, int ldy, int *ticket,
                  const DevStatus *stop, const int *active, hipStream_t sm, int device, hipEvent_t e0, hipEvent_t e1,
-                 const LzEpi *epi)
+                 const LzEpi *epi, const XPre *pre)
 {
+    const XPre xp = pre ? *pre : XPre{};
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     // one block per unit up to the occupancy the kernel is built for; beyond
@@ -817,7 +878,7 @@ int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int
     if (e0) BD_HIP(hipEventRecord(e0, sm));
 #define BD_TILES(CR, WS)                                                                                         \
     hipLaunchKernelGGL((symm_tiles_kernel<E, S, CR, WS>), dim3(grid), dim3(NT), 0, sm, mob, x, ldx, p.units,      \
-                       p.nunits, p.chain_i, p.chain_l, p.rowpart, p.colpart, ticket, stop, active)
+                       p.nunits, p.chain_i, p.chain_l, p.rowpart, p.colpart, ticket, stop, active, xp)
     const int shape = p.chr * 16 + p.wsc;
     if (shape == 2 * 16 + 4) BD_TILES(2, 4);
     else if constexpr (S <= 4) {
@@ -851,7 +912,7 @@ int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int
 #define BD_INST(E, S)                                                                                          \
     template int plan_product<E, S>(const MobPlan &, const E *, const double *, double *, int, int, int *,     \
                                     const DevStatus *, const int *, hipStream_t, int, hipEvent_t, hipEvent_t,  \
-                                    const LzEpi *);
+                                    const LzEpi *, const XPre *);
 BD_INST(float, 1) BD_INST(float, 2) BD_INST(float, 3) BD_INST(float, 4)
 BD_INST(float, 5) BD_INST(float, 6) BD_INST(float, 7) BD_INST(float, 8)
 BD_INST(double, 1) BD_INST(double, 2) BD_INST(double, 3) BD_INST(double, 4)
@@ -919,7 +980,7 @@ BD_INST(double, 1) BD_INST(double, 2) BD_INST(double, 3) BD_INST(double, 4)
 
 template <typename E, int S>
 static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, int ldy, const DevStatus *stop,
-                     const LzEpi *epi)
+                     const LzEpi *epi, const XPre *pre)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c.prof_on && (c.prof_width == 0 || c.prof_width == S)) {
@@ -944,22 +1005,22 @@ static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, 
         }
     }
     return plan_product<E, S>(*pl, mob, x, y, ldx, ldy, &c.st->ticket, stop, nullptr, c.stream, c.device, e0, e1,
-                              epi);
+                              epi, pre);
 }
 
 template <typename E>
 static int product_dispatch(Ctx &c, const E *mob, const double *x, double *y, int s, int ldx, int ldy,
-                            const DevStatus *stop, const LzEpi *epi = nullptr)
+                            const DevStatus *stop, const LzEpi *epi = nullptr, const XPre *pre = nullptr)
 {
     switch (s) {
-    case 1: return product_t<E, 1>(c, mob, x, y, ldx, ldy, stop, epi);
-    case 2: return product_t<E, 2>(c, mob, x, y, ldx, ldy, stop, epi);
-    case 3: return product_t<E, 3>(c, mob, x, y, ldx, ldy, stop, epi);
-    case 4: return product_t<E, 4>(c, mob, x, y, ldx, ldy, stop, epi);
-    case 5: return product_t<E, 5>(c, mob, x, y, ldx, ldy, stop, epi);
-    case 6: return product_t<E, 6>(c, mob, x, y, ldx, ldy, stop, epi);
-    case 7: return product_t<E, 7>(c, mob, x, y, ldx, ldy, stop, epi);
-    case 8: return product_t<E, 8>(c, mob, x, y, ldx, ldy, stop, epi);
+    case 1: return product_t<E, 1>(c, mob, x, y, ldx, ldy, stop, epi, pre);
+    case 2: return product_t<E, 2>(c, mob, x, y, ldx, ldy, stop, epi, pre);
+    case 3: return product_t<E, 3>(c, mob, x, y, ldx, ldy, stop, epi, pre);
+    case 4: return product_t<E, 4>(c, mob, x, y, ldx, ldy, stop, epi, pre);
+    case 5: return product_t<E, 5>(c, mob, x, y, ldx, ldy, stop, epi, pre);
+    case 6: return product_t<E, 6>(c, mob, x, y, ldx, ldy, stop, epi, pre);
+    case 7: return product_t<E, 7>(c, mob, x, y, ldx, ldy, stop, epi, pre);
+    case 8: return product_t<E, 8>(c, mob, x, y, ldx, ldy, stop, epi, pre);
     default: set_error("block size s must be in 1..8"); return BD_ERR_ARG;
     }
 }
@@ -971,11 +1032,11 @@ int mob_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy)
     return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy, nullptr);
 }
 
-int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy, const LzEpi *epi)
+int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy, const LzEpi *epi, const XPre *pre)
 {
     if (!c.mob_valid) { set_error("mobility not assembled"); return BD_ERR_STATE; }
-    if (c.mob32) return product_dispatch<float>(c, c.mob32, x, y, s, ldx, ldy, c.st, epi);
-    return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy, c.st, epi);
+    if (c.mob32) return product_dispatch<float>(c, c.mob32, x, y, s, ldx, ldy, c.st, epi, pre);
+    return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy, c.st, epi, pre);
 }
 
 // Sums the recorded (start, stop) pairs into prof_ms and recycles them.
