@@ -181,6 +181,59 @@ __global__ __launch_bounds__(1024) void to_origin_kernel(double *crd, int nb, do
         for (int d = 0; d < 3; d++) crd[3 * i + d] = crd[3 * i + d] - com[d];
 }
 
+// The tail of an EM step of an isolated molecule in one launch: drift
+// (drift_kernel), r <- r + drift + w (update_kernel) and the centre of mass
+// to the origin (to_origin_kernel), with the same arithmetic and the same
+// summation order as the three kernels, so the coordinates are the same to
+// the bit.  One block of 1024 threads.
+__global__ __launch_bounds__(1024) void em_tail_kernel(double *__restrict__ crd, const double *__restrict__ v,
+                                                       const double *__restrict__ diff, int nb, int flow_style,
+                                                       double rate, double dt, double *__restrict__ drift,
+                                                       double *com_acc, DevStatus *st)
+{
+    if (st->ierr) return;
+    if (threadIdx.x == 0) st->steps_done += 1;
+    __shared__ double sh[3][16];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        double f[3] = {0.0, 0.0, 0.0};
+        switch (flow_style) {  // bd_solver_m.f90:420-441
+        case 1: f[0] = rate * crd[3 * i + 1]; break;
+        case 2: f[0] = rate * crd[3 * i]; f[1] = -rate * crd[3 * i + 1]; break;
+        case 3:
+            f[0] = rate * crd[3 * i];
+            f[1] = -0.5 * rate * crd[3 * i + 1];
+            f[2] = -0.5 * rate * crd[3 * i + 2];
+            break;
+        default: break;
+        }
+        for (int d = 0; d < 3; d++) {
+            const double dr = dt * (v[3 * i + d] + f[d]);
+            drift[3 * i + d] = dr;
+            double r = crd[3 * i + d] + dr;
+            if (diff) r = r + diff[3 * i + d];
+            crd[3 * i + d] = r;
+            s[d] += r;
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        double x = s[d];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (threadIdx.x % 64 == 0) sh[d][threadIdx.x / 64] = x;
+    }
+    __syncthreads();
+    __shared__ double com[3];
+    if (threadIdx.x < 3) {
+        double x = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / 64); w++) x += sh[threadIdx.x][w];
+        com[threadIdx.x] = x / nb;
+        if (com_acc) com_acc[threadIdx.x] += x / nb;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        for (int d = 0; d < 3; d++) crd[3 * i + d] = crd[3 * i + d] - com[d];
+}
+
 __global__ __launch_bounds__(1024) void rgsq_kernel(const double *crd, int nb, double *out)
 {
     __shared__ double sh[4][16];
@@ -333,18 +386,23 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
         c.cntr_mobsam++;
     }
     // ---- calc_drift (bd_solver_m.f90:407-454): drift = dt (U(r) + D F(r))
-    auto drift_at_crd = [&]() -> int {
+    // (an EM step of an isolated molecule leaves the drift to its tail kernel)
+    const bool fused_tail = c.opt.integrator != BD_PC && c.imcon == 0;
+    const double *vdrift = nullptr;
+    auto drift_at_crd = [&](bool launch) -> int {
         BD_TRY(compute_forces(c, false, false));
         const double *v = c.frc;
         if (hi) {
             BD_TRY(mob_product(c, c.frc, c.W, 1, c.npad, c.npad));
             v = c.W;
         }
-        hipLaunchKernelGGL(drift_kernel, dim3(nbk), dim3(256), 0, sm, v, c.crd, nb, c.opt.flow_style,
-                           c.opt.flow_rate, c.opt.tim_stp, c.drift, c.st);
+        vdrift = v;
+        if (launch)
+            hipLaunchKernelGGL(drift_kernel, dim3(nbk), dim3(256), 0, sm, v, c.crd, nb, c.opt.flow_style,
+                               c.opt.flow_rate, c.opt.tim_stp, c.drift, c.st);
         return BD_OK;
     };
-    BD_TRY(drift_at_crd());
+    BD_TRY(drift_at_crd(!fused_tail));
     // Brownian displacement w of this step
     const double *dcol = nullptr;
     if (disp_dev) dcol = disp_dev;
@@ -354,7 +412,7 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
         const int nbn = (n + 255) / 256;
         hipLaunchKernelGGL(pc_predict_kernel, dim3(nbn), dim3(256), 0, sm, c.crd, c.drift, dcol, c.pc_r0, c.pc_d0,
                            n, c.imcon, c.box[0], c.box[1], c.box[2], c.st);
-        BD_TRY(drift_at_crd());
+        BD_TRY(drift_at_crd(true));
         hipLaunchKernelGGL(pc_correct_kernel, dim3(nbn), dim3(256), 0, sm, c.crd, c.drift, c.pc_r0, c.pc_d0, n,
                            c.st);
     }
@@ -363,11 +421,18 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
         if (dcol) BD_HIP(hipMemcpyAsync(rec_dev, dcol, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, sm));
         else BD_HIP(hipMemsetAsync(rec_dev, 0, (size_t)n * sizeof(double), sm));
     }
-    hipLaunchKernelGGL(update_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, c.drift, dcol, nb, c.imcon, c.box[0],
-                       c.box[1], c.box[2], c.st);
+    if (fused_tail) {
+        // drift, update and the bds_run tail (bd_solver_m.f90:176-186) in one launch
+        hipLaunchKernelGGL(em_tail_kernel, dim3(1), dim3(1024), 0, sm, c.crd, vdrift, dcol, nb, c.opt.flow_style,
+                           c.opt.flow_rate, c.opt.tim_stp, c.drift, c.scal + 16, c.st);
+    } else {
+        hipLaunchKernelGGL(update_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, c.drift, dcol, nb, c.imcon, c.box[0],
+                           c.box[1], c.box[2], c.st);
+        // ---- bds_run tail (bd_solver_m.f90:176-186)
+        if (c.imcon == 0)
+            hipLaunchKernelGGL(to_origin_kernel, dim3(1), dim3(1024), 0, sm, c.crd, nb, c.scal + 16, c.st);
+    }
     if (hi && c.cntr_mobsam == s) c.cntr_mobsam = 0;
-    // ---- bds_run tail (bd_solver_m.f90:176-186)
-    if (c.imcon == 0) hipLaunchKernelGGL(to_origin_kernel, dim3(1), dim3(1024), 0, sm, c.crd, nb, c.scal + 16, c.st);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }
