@@ -194,7 +194,68 @@ __global__ __launch_bounds__(1024) void em_tail_kernel(double *__restrict__ crd,
     if (st->ierr) return;
     if (threadIdx.x == 0) st->steps_done += 1;
     __shared__ double sh[3][16];
+    __shared__ double com[3];
     double s[3] = {0.0, 0.0, 0.0};
+    constexpr int RPT = 2;   // beads per thread kept in registers (the same order of sums)
+    if (nb <= RPT * (int)blockDim.x) {
+        double c0[RPT][3], vv[RPT][3], dd[RPT][3];
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int i = min((int)threadIdx.x + q * (int)blockDim.x, nb - 1);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                c0[q][d] = crd[3 * i + d];
+                vv[q][d] = v[3 * i + d];
+                dd[q][d] = diff ? diff[3 * i + d] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int i = threadIdx.x + q * blockDim.x;
+            if (i >= nb) continue;
+            double f[3] = {0.0, 0.0, 0.0};
+            switch (flow_style) {  // bd_solver_m.f90:420-441
+            case 1: f[0] = rate * c0[q][1]; break;
+            case 2: f[0] = rate * c0[q][0]; f[1] = -rate * c0[q][1]; break;
+            case 3:
+                f[0] = rate * c0[q][0];
+                f[1] = -0.5 * rate * c0[q][1];
+                f[2] = -0.5 * rate * c0[q][2];
+                break;
+            default: break;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double dr = dt * (vv[q][d] + f[d]);
+                drift[3 * i + d] = dr;
+                // separately rounded adds, as update_kernel's (which reads dr from memory)
+                double r = __dadd_rn(c0[q][d], dr);
+                if (diff) r = __dadd_rn(r, dd[q][d]);
+                c0[q][d] = r;
+                s[d] += r;
+            }
+        }
+        for (int d = 0; d < 3; d++) {
+            double x = s[d];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+            if (threadIdx.x % 64 == 0) sh[d][threadIdx.x / 64] = x;
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            double x = 0.0;
+            for (int w = 0; w < (int)(blockDim.x / 64); w++) x += sh[threadIdx.x][w];
+            com[threadIdx.x] = x / nb;
+            if (com_acc) com_acc[threadIdx.x] += x / nb;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int i = threadIdx.x + q * blockDim.x;
+            if (i < nb)
+                for (int d = 0; d < 3; d++) crd[3 * i + d] = c0[q][d] - com[d];
+        }
+        return;
+    }
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
         double f[3] = {0.0, 0.0, 0.0};
         switch (flow_style) {  // bd_solver_m.f90:420-441
@@ -210,8 +271,8 @@ __global__ __launch_bounds__(1024) void em_tail_kernel(double *__restrict__ crd,
         for (int d = 0; d < 3; d++) {
             const double dr = dt * (v[3 * i + d] + f[d]);
             drift[3 * i + d] = dr;
-            double r = crd[3 * i + d] + dr;
-            if (diff) r = r + diff[3 * i + d];
+            double r = __dadd_rn(crd[3 * i + d], dr);
+            if (diff) r = __dadd_rn(r, diff[3 * i + d]);
             crd[3 * i + d] = r;
             s[d] += r;
         }
@@ -222,7 +283,6 @@ __global__ __launch_bounds__(1024) void em_tail_kernel(double *__restrict__ crd,
         if (threadIdx.x % 64 == 0) sh[d][threadIdx.x / 64] = x;
     }
     __syncthreads();
-    __shared__ double com[3];
     if (threadIdx.x < 3) {
         double x = 0.0;
         for (int w = 0; w < (int)(blockDim.x / 64); w++) x += sh[threadIdx.x][w];

@@ -22,6 +22,9 @@ namespace bd {
 
 constexpr int NBR_CAP0 = 64;        // initial neighbours per bead (full list)
 constexpr int FB = 256;             // threads per force block
+constexpr int FW = FB / 64;         // beads per force_gather_kernel block (a wavefront each)
+constexpr int N2_WAVES = 8;         // beads per nbr_n2_kernel block (a wavefront each)
+constexpr int N2_CHUNK = 2048;      // coordinates staged in LDS per pass of nbr_n2_kernel
 
 __device__ inline bool excluded(const int *ex_ptr, const int *ex_col, int i, int j);
 
@@ -52,9 +55,12 @@ __device__ void block_acc_store(double (&a)[NACC], double *out)
 }
 
 // ------------------------------------------------------------ forces
-// Per-bead gather of bond and pair terms.  Energies/stress/bond lengths are
-// counted once per term: bonds from their first bead, pairs from the lower
-// index.
+// Per-bead gather of bond and pair terms, one wavefront per bead: lane l
+// takes the bead's bonds l, l + 64, ... and then its neighbours l, l + 64,
+// ... (list order), and the wavefront adds the lanes' sums in a fixed
+// butterfly order (a handful of terms per lane instead of a chain of
+// dependent loads per bead).  Energies/stress/bond lengths are counted once
+// per term: bonds from their first bead, pairs from the lower index.
 __global__ __launch_bounds__(FB) void force_gather_kernel(
     const double *__restrict__ crd, int nb, BoxArg bx, const int *__restrict__ bond_ptr,
     const int *__restrict__ bond_idx, const int *__restrict__ bond_ij, const int *__restrict__ bond_type,
@@ -65,14 +71,15 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
     const double *__restrict__ vparams, double *__restrict__ frc, int want_energy, double *__restrict__ epart,
     DevStatus *st)
 {
-    const int i = blockIdx.x * FB + threadIdx.x;
+    const int lane = threadIdx.x % 64;
+    const int i = blockIdx.x * FW + threadIdx.x / 64;
     double acc[NACC];
     for (int e = 0; e < NACC; e++) acc[e] = 0.0;
     acc[12] = 1.0e300;
     acc[13] = 0.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
     if (i < nb) {
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int k = bond_ptr[i]; k < bond_ptr[i + 1]; k++) {
+        for (int k = bond_ptr[i] + lane; k < bond_ptr[i + 1]; k += 64) {
             int b = bond_idx[k];
             int bi = bond_ij[2 * b], bj = bond_ij[2 * b + 1];
             double x, y, z;
@@ -100,7 +107,7 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
             const int cnt = nbr_cnt[i];
             const bool full = cnt > nbr_cap;  // list overflowed: scan all beads
             const int kend = full ? nb : cnt;
-            for (int k = 0; k < kend; k++) {
+            for (int k = lane; k < kend; k += 64) {
                 int j;
                 if (full) {
                     j = k;
@@ -128,6 +135,15 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
                 }
             }
         }
+    }
+    // the wavefront's sum (every lane ends with the same value: each step
+    // adds the same two operands on both lanes of a pair)
+    for (int off = 32; off > 0; off >>= 1) {
+        fx += __shfl_xor(fx, off, 64);
+        fy += __shfl_xor(fy, off, 64);
+        fz += __shfl_xor(fz, off, 64);
+    }
+    if (i < nb && lane == 0) {
         frc[3 * i] = fx;
         frc[3 * i + 1] = fy;
         frc[3 * i + 2] = fz;
@@ -138,14 +154,21 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
 __global__ void energy_final_kernel(const double *__restrict__ epart, int nblk, int nbonds, int imcon, double vol,
                                     double *__restrict__ out)
 {
+    // thread e combines quantity e over the blocks in block order
+    __shared__ double sa[NACC];
+    const int e = threadIdx.x;
+    if (e < NACC) {
+        double a = epart[e];
+        for (int b = 1; b < nblk; b++) {
+            const double v = epart[b * NACC + e];
+            a = (e == 12) ? fmin(a, v) : (e == 13) ? fmax(a, v) : a + v;
+        }
+        sa[e] = a;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     double a[NACC];
-    for (int e = 0; e < NACC; e++) a[e] = epart[e];
-    for (int b = 1; b < nblk; b++)
-        for (int e = 0; e < NACC; e++) {
-            double v = epart[b * NACC + e];
-            a[e] = (e == 12) ? fmin(a[e], v) : (e == 13) ? fmax(a[e], v) : a[e] + v;
-        }
+    for (int q = 0; q < NACC; q++) a[q] = sa[q];
     out[0] = a[0];
     out[1] = a[1];
     out[2] = a[0] + a[1];
@@ -191,31 +214,65 @@ __global__ __launch_bounds__(1024) void verlet_check_kernel(const double *__rest
 
 // All-pairs neighbour build: one wavefront per bead scans every j != i.
 // Keeps j with r^2 < rc2 that is not excluded.  Entries are in ascending j.
-__global__ __launch_bounds__(256) void nbr_n2_kernel(const double *__restrict__ crd, int nb, BoxArg bx, double rc2,
-                                                     const int *__restrict__ ex_ptr, const int *__restrict__ ex_col,
-                                                     int *__restrict__ nbr_cnt, int *__restrict__ nbr_idx, int cap,
-                                                     int check_flag, double *__restrict__ save, DevStatus *st)
+// The block's N2_WAVES beads scan coordinates staged in LDS (as x, y, z
+// arrays, N2_CHUNK beads per pass), the same differences and predicate as
+// pair_vec / rsq3.  check_flag 0 (a build on every call): the kernel also
+// reports the rebuild (st->rebuild = 1).
+__global__ __launch_bounds__(64 * N2_WAVES) void nbr_n2_kernel(const double *__restrict__ crd, int nb, BoxArg bx,
+                                                               double rc2, const int *__restrict__ ex_ptr,
+                                                               const int *__restrict__ ex_col,
+                                                               int *__restrict__ nbr_cnt, int *__restrict__ nbr_idx,
+                                                               int cap, int check_flag, double *__restrict__ save,
+                                                               DevStatus *st)
 {
     if (check_flag && !st->rebuild) return;
+    if (!check_flag && blockIdx.x == 0 && threadIdx.x == 0) st->rebuild = 1;
+    constexpr int PER = 3 * N2_CHUNK / (64 * N2_WAVES);   // staged values per thread and pass
+    static_assert(PER * 64 * N2_WAVES == 3 * N2_CHUNK, "staging split");
+    __shared__ double sc[3][N2_CHUNK];
     const int lane = threadIdx.x % 64;
-    const int i = blockIdx.x * 4 + threadIdx.x / 64;
-    if (i >= nb) return;
+    const int i = blockIdx.x * N2_WAVES + threadIdx.x / 64;
+    const bool live = i < nb;   // uniform over the wavefront
+    const int ii = live ? i : 0;
+    const double xi = crd[3 * ii], yi = crd[3 * ii + 1], zi = crd[3 * ii + 2];
     int cnt = 0;
-    for (int j0 = 0; j0 < nb; j0 += 64) {
-        int j = j0 + lane;
-        bool keep = false;
-        if (j < nb && j != i) {
-            double x, y, z;
-            pair_vec(crd, i, j, bx, x, y, z);
-            if (rsq3(x, y, z) < rc2) keep = !excluded(ex_ptr, ex_col, i, j);
+    for (int c0 = 0; c0 < nb; c0 += N2_CHUNK) {
+        const int cn = min(N2_CHUNK, nb - c0);
+        __syncthreads();   // the previous pass no longer reads sc
+        double v[PER];   // all loads in flight before the first store
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int t = threadIdx.x + q * 64 * N2_WAVES;
+            v[q] = t < 3 * cn ? crd[3L * c0 + t] : 0.0;
         }
-        unsigned long long m = __ballot(keep);
-        if (keep) {
-            int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
-            if (pos < cap) nbr_idx[(long)pos * nb + i] = j;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int t = threadIdx.x + q * 64 * N2_WAVES;
+            if (t < 3 * cn) sc[t % 3][t / 3] = v[q];
         }
-        cnt += __popcll(m);
+        __syncthreads();
+        if (!live) continue;
+        for (int j0 = 0; j0 < cn; j0 += 64) {
+            const int jl = j0 + lane, j = c0 + jl;
+            bool keep = false;
+            if (jl < cn && j != i) {
+                double x = __dsub_rn(sc[0][jl], xi), y = __dsub_rn(sc[1][jl], yi), z = __dsub_rn(sc[2][jl], zi);
+                if (bx.imcon) {
+                    x = img(x, bx.L[0]);
+                    y = img(y, bx.L[1]);
+                    z = img(z, bx.L[2]);
+                }
+                if (rsq3(x, y, z) < rc2) keep = !excluded(ex_ptr, ex_col, i, j);
+            }
+            unsigned long long m = __ballot(keep);
+            if (keep) {
+                int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+                if (pos < cap) nbr_idx[(long)pos * nb + i] = j;
+            }
+            cnt += __popcll(m);
+        }
     }
+    if (!live) return;
     if (lane == 0) {
         nbr_cnt[i] = cnt;
         if (cnt > cap) atomicExch(&st->overflow, 1);
@@ -730,7 +787,7 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const in
         BD_HIP(hipMalloc(&c.nbr_idx, (size_t)c.pair_cap * std::max(nb, 1) * sizeof(int)));
     }
     if (!c.crd_save) BD_HIP(hipMalloc(&c.crd_save, (size_t)3 * std::max(nb, 1) * sizeof(double)));
-    int nblk = (nb + FB - 1) / FB;
+    int nblk = (nb + FW - 1) / FW;   // force_gather_kernel blocks
     if (c.red_cap < nblk * NACC) {
         if (c.red) (void)hipFree(c.red);
         c.red_cap = std::max(nblk, 1) * NACC;
@@ -820,11 +877,13 @@ static int build_pairs(Ctx &c, bool sync)
     const double rl = ver ? c.rcutoff + c.tskin : c.rcutoff;
     const double rc2 = rl * rl;
     for (int attempt = 0; attempt < 8; attempt++) {
+        // DIR builds on every call and nbr_n2_kernel reports that itself
+        const bool n2_dir = !ver && c.mth_ptgen != BD_PT_CL;
         if (ver)
             hipLaunchKernelGGL(verlet_check_kernel, dim3(1), dim3(1024), 0, sm, c.crd, c.crd_save, nb, bx,
                                c.tskin * c.tskin, c.verlet_first ? 1 : 0, c.st);
-        else
-            hipLaunchKernelGGL(set_rebuild_kernel, dim3(1), dim3(1), 0, sm, c.st);  // DIR, CL: every call
+        else if (!n2_dir)
+            hipLaunchKernelGGL(set_rebuild_kernel, dim3(1), dim3(1), 0, sm, c.st);  // CL: every call
         if (c.mth_ptgen == BD_PT_CL) {
             const int nbk = (nb + 255) / 256, nck = (c.ncell_tot + 255) / 256;
             hipLaunchKernelGGL(zero_int_kernel, dim3(nck), dim3(256), 0, sm, c.cell_cnt, c.ncell_tot, 0, c.st);
@@ -855,8 +914,10 @@ static int build_pairs(Ctx &c, bool sync)
                                c.ncell[2], c.bead_cell, c.cell_ptr, c.cell_beads, c.ex_ptr, c.ex_col, c.nbr_cnt,
                                c.nbr_idx, c.pair_cap, c.crd_save, c.st);
         } else {
-            hipLaunchKernelGGL(nbr_n2_kernel, dim3((nb + 3) / 4), dim3(256), 0, sm, c.crd, nb, bx, rc2, c.ex_ptr,
-                               c.ex_col, c.nbr_cnt, c.nbr_idx, c.pair_cap, 1, ver ? c.crd_save : nullptr, c.st);
+            hipLaunchKernelGGL(nbr_n2_kernel, dim3((nb + N2_WAVES - 1) / N2_WAVES), dim3(64 * N2_WAVES), 0, sm, c.crd,
+                               nb, bx, rc2, c.ex_ptr,
+                               c.ex_col, c.nbr_cnt, c.nbr_idx, c.pair_cap, n2_dir ? 0 : 1, ver ? c.crd_save : nullptr,
+                               c.st);
         }
         if (ver) hipLaunchKernelGGL(count_rebuild_kernel, dim3(1), dim3(1), 0, sm, c.st, &c.st->nrebuild);
         BD_HIP(hipGetLastError());
@@ -895,7 +956,11 @@ int compute_forces(Ctx &c, bool want_energy, bool sync)
     const bool ver = (c.mth_ptgen == BD_PT_VER);
     const double rl = ver ? c.rcutoff + c.tskin : c.rcutoff;
     BoxArg bx = box_arg(c);
-    int nblk = (nb + FB - 1) / FB;
+    int nblk = (nb + FW - 1) / FW;
+    if ((long)std::max(nblk, 1) * NACC > c.red_cap) {
+        set_error("internal: force partial buffer smaller than the launch");
+        return BD_ERR_STATE;
+    }
     hipLaunchKernelGGL(force_gather_kernel, dim3(std::max(nblk, 1)), dim3(FB), 0, c.stream, c.crd, nb, bx,
                        c.bead_bond_ptr, c.bead_bond_idx, c.bond_ij, c.bond_type, c.bond_styles, c.bond_params,
                        pairs ? 1 : 0, c.nbr_cnt, c.nbr_idx, c.pair_cap, ver ? c.crd_save : c.crd, rl * rl, c.ex_ptr,
