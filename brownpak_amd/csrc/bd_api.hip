@@ -191,13 +191,13 @@ __global__ __launch_bounds__(1024) void em_tail_kernel(double *__restrict__ crd,
                                                        double rate, double dt, double *__restrict__ drift,
                                                        double *com_acc, DevStatus *st)
 {
-    if (st->ierr) return;
-    if (threadIdx.x == 0) st->steps_done += 1;
     __shared__ double sh[3][16];
     __shared__ double com[3];
     double s[3] = {0.0, 0.0, 0.0};
     constexpr int RPT = 2;   // beads per thread kept in registers (the same order of sums)
     if (nb <= RPT * (int)blockDim.x) {
+        // the beads' loads go out together with the status word's
+        const int ierr = st->ierr;
         double c0[RPT][3], vv[RPT][3], dd[RPT][3];
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
@@ -209,6 +209,8 @@ __global__ __launch_bounds__(1024) void em_tail_kernel(double *__restrict__ crd,
                 dd[q][d] = diff ? diff[3 * i + d] : 0.0;
             }
         }
+        if (ierr) return;
+        if (threadIdx.x == 0) st->steps_done += 1;
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
             const int i = threadIdx.x + q * blockDim.x;
@@ -256,6 +258,8 @@ __global__ __launch_bounds__(1024) void em_tail_kernel(double *__restrict__ crd,
         }
         return;
     }
+    if (st->ierr) return;
+    if (threadIdx.x == 0) st->steps_done += 1;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
         double f[3] = {0.0, 0.0, 0.0};
         switch (flow_style) {  // bd_solver_m.f90:420-441

@@ -79,7 +79,11 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
     acc[13] = 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (i < nb) {
-        for (int k = bond_ptr[i] + lane; k < bond_ptr[i + 1]; k += 64) {
+        // the bead's bond range and neighbour count in one round trip
+        const int kb0 = bond_ptr[i], kb1 = bond_ptr[i + 1];
+        const int cnt = use_pairs ? nbr_cnt[i] : 0;
+        const int ati = use_pairs ? atype[i] : 0;
+        for (int k = kb0 + lane; k < kb1; k += 64) {
             int b = bond_idx[k];
             int bi = bond_ij[2 * b], bj = bond_ij[2 * b + 1];
             double x, y, z;
@@ -103,8 +107,6 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
             }
         }
         if (use_pairs) {
-            const int ati = atype[i];
-            const int cnt = nbr_cnt[i];
             const bool full = cnt > nbr_cap;  // list overflowed: scan all beads
             const int kend = full ? nb : cnt;
             for (int k = lane; k < kend; k += 64) {
@@ -235,6 +237,7 @@ __global__ __launch_bounds__(64 * N2_WAVES) void nbr_n2_kernel(const double *__r
     const bool live = i < nb;   // uniform over the wavefront
     const int ii = live ? i : 0;
     const double xi = crd[3 * ii], yi = crd[3 * ii + 1], zi = crd[3 * ii + 2];
+    const int ex0 = ex_ptr[ii], ex1 = ex_ptr[ii + 1];   // the bead's exclusions (ascending)
     int cnt = 0;
     for (int c0 = 0; c0 < nb; c0 += N2_CHUNK) {
         const int cn = min(N2_CHUNK, nb - c0);
@@ -262,7 +265,11 @@ __global__ __launch_bounds__(64 * N2_WAVES) void nbr_n2_kernel(const double *__r
                     y = img(y, bx.L[1]);
                     z = img(z, bx.L[2]);
                 }
-                if (rsq3(x, y, z) < rc2) keep = !excluded(ex_ptr, ex_col, i, j);
+                if (rsq3(x, y, z) < rc2) {
+                    keep = true;
+                    for (int k = ex0; k < ex1; k++)
+                        if (ex_col[k] == j) keep = false;
+                }
             }
             unsigned long long m = __ballot(keep);
             if (keep) {
