@@ -172,6 +172,9 @@ struct Ctx {
     // lz_flag_dev: the device's address of the same words)
     volatile int *lz_flag = nullptr;
     int *lz_flag_dev = nullptr;
+    // iteration at which the last Lanczos solve converged (0: none yet); the
+    // next solve tests the iterations around it one by one
+    int lz_hint = 0;
     double *Rq = nullptr, *Rcoef = nullptr;  // reorthogonalisation scratch
 
     // topology

@@ -630,8 +630,10 @@ constexpr int YJS = 256;                   // largest j s of the grouped combina
 // launch forms their y_j and applies the reference's test to each in order.
 // Both run on the side stream behind one event per group, so the Krylov
 // recurrence never waits for a test; the host learns the verdict through a
-// flag in mapped host memory (report_tests) and stops issuing iterations one
-// group after the one that converged.
+// flag in mapped host memory (report_tests) and stops issuing iterations
+// once it has read a converged one.  Groups hold up to LG consecutive
+// iterations; parity (the group's count mod 2) selects which of the two
+// node-placement hint slots a group reads and which it writes.
 constexpr int LG = 4;
 
 template <int S>
@@ -640,7 +642,7 @@ __global__ __launch_bounds__(sp::WG_T) void small_group_kernel(const double *__r
                                                                const double *__restrict__ R0, int j0, int jend,
                                                                double *__restrict__ part, long part_stride,
                                                                double *__restrict__ Cg, long c_stride,
-                                                               sp::SplitState *ssv, DevStatus *st)
+                                                               sp::SplitState *ssv, int parity, DevStatus *st)
 {
     const int j = j0 + (int)blockIdx.y;
     __shared__ int s_done;
@@ -649,7 +651,7 @@ __global__ __launch_bounds__(sp::WG_T) void small_group_kernel(const double *__r
     if (s_done || j > jend || j < 2) return;
     sp::SplitState *ss = ssv + blockIdx.y;
     const bool fin = sp::small_problem_split<S>(Ablk, Bblk, R0, j, NBOUND, 13, part + blockIdx.y * part_stride,
-                                                Cg + blockIdx.y * c_stride, ss, ssv, LG, ((j0 - 1) / LG) & 1);
+                                                Cg + blockIdx.y * c_stride, ss, ssv, LG, parity);
     if (fin && threadIdx.x == 0) {
         st->lam_hi = ss->lam_hi;
         st->lam_lo = ss->lam_lo;
@@ -678,7 +680,7 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
                                                                  const double *__restrict__ R0, int j0, int jend,
                                                                  double *__restrict__ part, long part_stride,
                                                                  double *__restrict__ Cg, long c_stride,
-                                                                 sp::SplitState *ssv, DevStatus *st)
+                                                                 sp::SplitState *ssv, int parity, DevStatus *st)
 {
     using namespace pcr;
     constexpr int SS = S * S;
@@ -699,7 +701,6 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
     __syncthreads();
     if (s_done || j > jend || j < 2) return;
     const int nb = j, js = nb * S, t = threadIdx.x, w = t / 64, lane = t % 64;
-    const int parity = ((j0 - 1) / LG) & 1;
     sp::SplitState *ss = ssv + blockIdx.x;
     double *P = part + blockIdx.x * part_stride;
     const double M = sp::gersh_block<S>(sA, sB, js, s_red);
@@ -1301,8 +1302,20 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     int pending = -1;   // slot of the status copy of the previous chunk
     int slot = 0;
     bool stop = false;
-    int pend_j = 0;     // grouped: last iteration of the previous group
     int pcr_batches = 0;   // node batches of the last reported placement (0: unknown)
+    // grouped tests: a group ends after LG iterations, and every iteration
+    // from hint - 1 to hint + 2 is a group of its own (hint: the iteration at
+    // which this context's previous solve converged), so that a solve that
+    // converges there is seen without waiting for the rest of a group.  The
+    // host takes a group's verdict once it is LG iterations old (waiting if
+    // needed: the device then still has LG iterations queued) and reads newer
+    // ones whenever they have already arrived.
+    const int hint = c.lz_hint;
+    auto in_window = [&](int jj) { return hint > 0 && jj >= hint - 1 && jj <= hint + 2; };
+    int jg_start = 1;      // first iteration of the open group
+    int ngroups = 0;       // groups launched (their parity alternates)
+    int pend[64];          // last iterations of the groups whose verdict is not read yet
+    int npend = 0;
     if (grouped) {
         c.lz_flag[0] = 0;
         c.lz_flag[1] = 0;
@@ -1372,11 +1385,15 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                                (double *)nullptr, Bj, (const double *)(c.R0 + S * S), c.st);
         }
         BD_HIP(hipGetLastError());
-        if (grouped && (j % LG == 0 || j == dmax)) {
+        const bool group_end = grouped && (j == dmax || j - jg_start + 1 >= LG || in_window(j) || in_window(j + 1));
+        if (group_end) {
             // the tests of iterations jg0..j on the side stream, behind the
             // Krylov kernels of iteration j: the recurrence goes on meanwhile
             // (it writes only later blocks of V, A and B than the tests read)
-            const int jg0 = ((j - 1) / LG) * LG + 1;
+            const int jg0 = jg_start;
+            const int parity = ngroups & 1;
+            jg_start = j + 1;
+            ngroups++;
             if (sg != sm) {
                 BD_HIP(hipEventRecord(ev, sm));
                 BD_HIP(hipStreamWaitEvent(sg, ev, 0));
@@ -1390,11 +1407,11 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                 if (use_pcr)
                     hipLaunchKernelGGL((small_pcr_kernel<S>), dim3(j - jg0 + 1, nby), dim3(pcr::THREADS), 0, sg,
                                        c.Ablk, c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split,
-                                       c.st);
+                                       parity, c.st);
                 else
                     hipLaunchKernelGGL((small_group_kernel<S>), dim3(nwg, j - jg0 + 1), dim3(sp::WG_T), 0, sg,
                                        c.Ablk, c.Bblk, c.R0, jg0, j, c.Cq, part_stride, c.Ccoef, c_stride, c.split,
-                                       c.st);
+                                       parity, c.st);
                 hipLaunchKernelGGL((ycomb_group_kernel<S>), dim3((rows + YR - 1) / YR), dim3(256), 0, sg, c.V, ld,
                                    rows, jg0, j, c.Ccoef, c_stride, c.Y, two_norm ? 1 : 0, c.npart, c.opt.lanc_tol,
                                    c.st, c.lz_flag_dev);
@@ -1422,15 +1439,25 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                                ycur, yold, want, two_norm ? 1 : 0, c.npart, j, c.opt.lanc_tol, c.st);
             BD_HIP(hipGetLastError());
         }
-        if (grouped && (j % chunk == 0 || j == dmax)) {
-            // the previous group's verdict, once its combine kernel reported
-            // (this group's kernels are queued meanwhile)
-            if (pend_j > 0) {
-                BD_TRY(wait_tests(c, sg, pend_j));
+        if (grouped) {
+            if (group_end) {
+                if (npend == 64) {   // cannot happen: verdicts older than LG iterations are taken each iteration
+                    set_error("internal: too many Lanczos test groups pending");
+                    return BD_ERR_STATE;
+                }
+                pend[npend++] = j;
+            }
+            // verdicts in group order: those LG iterations old (waiting for
+            // them), then any newer one that has already arrived
+            while (npend > 0 && !stop) {
+                const int e = pend[0];
+                if (j - e >= LG) BD_TRY(wait_tests(c, sg, e));
+                else if (__atomic_load_n(&c.lz_flag[0], __ATOMIC_ACQUIRE) < e) break;
+                for (int q = 1; q < npend; q++) pend[q - 1] = pend[q];
+                npend--;
                 if (c.lz_flag[2] || c.lz_flag[1]) stop = true;
                 if (c.lz_flag[3] > 0) pcr_batches = (c.lz_flag[3] + pcr::NODES - 1) / pcr::NODES;
             }
-            pend_j = j;
         } else if (j % chunk == 0 || j == dmax) {
             // snapshot the status after this chunk's tests; read the previous one
             hipStream_t ts = grouped ? sm : ss;
@@ -1456,6 +1483,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
         return BD_ERR_FACTOR;
     }
     int jf = c.st_host->converged_j ? c.st_host->converged_j : c.st_host->last_j;
+    c.lz_hint = c.st_host->converged_j;
     *lconv = c.st_host->converged_j ? 1 : 0;
     *iters = c.st_host->converged_j ? c.st_host->converged_j : dmax;
     const double *yres = c.Y + (size_t)(jf % (grouped ? LG + 1 : 2)) * S * ld;
