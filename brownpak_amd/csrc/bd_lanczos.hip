@@ -1304,14 +1304,14 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     bool stop = false;
     int pcr_batches = 0;   // node batches of the last reported placement (0: unknown)
     // grouped tests: a group ends after LG iterations, and every iteration
-    // from hint - 1 to hint + 2 is a group of its own (hint: the iteration at
+    // from hint - 1 to hint + 1 is a group of its own (hint: the iteration at
     // which this context's previous solve converged), so that a solve that
     // converges there is seen without waiting for the rest of a group.  The
     // host takes a group's verdict once it is LG iterations old (waiting if
     // needed: the device then still has LG iterations queued) and reads newer
     // ones whenever they have already arrived.
     const int hint = c.lz_hint;
-    auto in_window = [&](int jj) { return hint > 0 && jj >= hint - 1 && jj <= hint + 2; };
+    auto in_window = [&](int jj) { return hint > 0 && jj >= hint - 1 && jj <= hint + 1; };
     int jg_start = 1;      // first iteration of the open group
     int ngroups = 0;       // groups launched (their parity alternates)
     int pend[64];          // last iterations of the groups whose verdict is not read yet
