@@ -131,6 +131,36 @@ def test_lanczos_reference_converged(oracle, chain100, chain2000, nb, s):
     assert rel(out, ref) < 1e-5
 
 
+@pytest.mark.parametrize("nb", [100, 2000])
+def test_lanczos_repeated_solves(oracle, chain100, chain2000, nb):
+    """Several solves in one context: from the second on, the tests are
+    grouped around the previous solve's converged iteration.  Solves that
+    converge at, well before (a block of near-rigid translations) and after
+    that iteration must each match the oracle's iteration and result."""
+    _say(f"case: lanczos repeated solves nb={nb}")
+    m = chain100 if nb == 100 else chain2000
+    s = 4
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6)
+    g.calc_rpy_tensor()
+    D = oracle.rpy(m["crd"])
+    rng = np.random.default_rng(nb + 11)
+    smooth = np.zeros((3 * nb, s))
+    for k in range(s):
+        smooth[k % 3::3, k] = 1.0 + 0.01 * rng.standard_normal(nb)
+    cases = [rng.standard_normal((3 * nb, s)), rng.standard_normal((3 * nb, s)), smooth,
+             rng.standard_normal((3 * nb, s))]
+    iters = []
+    for xi in cases:
+        out, lc, it = g.brn_calc_bdw(xi)
+        ref, lc2, it2, _ = oracle.brn_calc_bdw(oracle.LANC, D, xi, lanc_mxitr=50, lanc_tol=1e-6)
+        _say(f"  iters gpu {it} oracle {it2} rel {rel(out, ref):.2e}")
+        assert lc and lc2 and abs(it - it2) <= 1
+        assert rel(out, ref) < 1e-5
+        iters.append(it)
+    # the smooth block converges well before the random ones
+    assert iters[2] < iters[1] - 1
+
+
 def test_lanczos_not_converged_flag(chain100):
     g = gpu_system(chain100, mob_fctr="LANC", nts_mobsam=4, lanc_mxitr=1, lanc_tol=1e-12)
     g.calc_rpy_tensor()
