@@ -424,16 +424,18 @@ __device__ inline BlockRec block_rec(const int *__restrict__ rb)
 
 template <int S>
 __device__ inline double sum_block_partials(const double *__restrict__ rowpart, const double *__restrict__ colpart,
-                                            const BlockRec &br, int e, int wsc)
+                                            const BlockRec &br, int e, int wsc, int half = -1)
 {
     constexpr int PB = 32;
     const int K = br.K, nsc = br.nsc, u0 = br.u0, u1 = br.u1, rpo = br.rpo;
-    const int nrow = br.sc + 1, ntot = nrow + (u1 - u0);
+    const int nrow = br.sc + 1, nall = nrow + (u1 - u0);
+    // half 0 / 1: the first / second half of the list only (half -1: all)
+    const int ib = half == 1 ? (nall + 1) / 2 : 0, ntot = half == 0 ? (nall + 1) / 2 : nall;
     const double *rp = rowpart + ((long)rpo + (long)K * nsc) * (TILE * S) + e;
     const double *cp = colpart + ((long)u0 * wsc + br.off) * (TILE * S) + e;
     const long cstr = (long)wsc * (TILE * S);
     double v = 0.0;
-    for (int i0 = 0; i0 < ntot; i0 += PB) {
+    for (int i0 = ib; i0 < ntot; i0 += PB) {
         double t[PB];
 #pragma unroll
         for (int q = 0; q < PB; q++) {
@@ -464,6 +466,12 @@ __global__ __launch_bounds__(256) void symm_reduce_kernel(const double *__restri
     y[(long)k * ldy + br.yrow + r] = sum_block_partials<S>(rowpart, colpart, br, e, wsc);
 }
 
+// Thread sets of symm_reduce_lz_kernel: two (one per half of an output
+// block's partial list, added first half + second half) where 2 TILE S
+// threads fit the kernel's bound, else one.
+template <int S>
+constexpr int LZ_HALVES = (2 * TILE * S <= TILE * MAX_S) ? 2 : 1;
+
 // The same sum for one output block per workgroup (TILE * S threads), fused
 // with the first step of a Lanczos iteration:
 //   W1 = W - V_{j-1} B_{j-1}^T  (stored over W),
@@ -477,11 +485,14 @@ __global__ __launch_bounds__(TILE * MAX_S) void symm_reduce_lz_kernel(
     constexpr int NO = 2 * S * S;                 // outputs per block
     constexpr int NGR = TILE * S / NO;            // row groups of the block sum
     constexpr int RPG = (TILE + NGR - 1) / NGR;   // rows per group (the last may be short)
+    constexpr int HB = LZ_HALVES<S>;              // thread sets, one per half of the partial list
     __shared__ int s_done;
     __shared__ double sw[TILE][S];
     __shared__ double red[TILE][NO];
     __shared__ double red2[NGR][NO];
-    const int G = blockIdx.x, e = threadIdx.x;
+    __shared__ double s_hv[HB > 1 ? TILE * S : 1];
+    const int G = blockIdx.x, e = threadIdx.x % (TILE * S), hf = threadIdx.x / (TILE * S);
+    const bool act = hf == 0;                     // the set that finishes the block
     // the block's record and the status are read together
     const BlockRec br = block_rec(rowblk + 8 * G);
     if (threadIdx.x == 0) s_done = (stop && (stop->converged_j != 0 || stop->ierr != 0)) ? 1 : 0;
@@ -493,26 +504,36 @@ __global__ __launch_bounds__(TILE * MAX_S) void symm_reduce_lz_kernel(
     double vj[S], vp[S];
 #pragma unroll
     for (int c = 0; c < S; c++) {
-        vj[c] = Vj[(long)c * ld + row];
-        vp[c] = Vp ? Vp[(long)c * ld + row] : 0.0;
+        vj[c] = act ? Vj[(long)c * ld + row] : 0.0;
+        vp[c] = (act && Vp) ? Vp[(long)c * ld + row] : 0.0;
     }
-    double v = sum_block_partials<S>(rowpart, colpart, br, e, wsc);
-    if (Vp) {
-        double t = 0.0;
+    double v = sum_block_partials<S>(rowpart, colpart, br, e, wsc, HB > 1 ? hf : -1);
+    if constexpr (HB > 1) {
+        // first half + second half
+        if (!act) s_hv[e] = v;
+        __syncthreads();
+        if (act) v = v + s_hv[e];
+    }
+    if (act) {
+        if (Vp) {
+            double t = 0.0;
 #pragma unroll
-        for (int kp = 0; kp < S; kp++) t += vp[kp] * Bp[k + kp * S];
-        v = v - t;
+            for (int kp = 0; kp < S; kp++) t += vp[kp] * Bp[k + kp * S];
+            v = v - t;
+        }
+        W[(long)k * ld + row] = v;
+        sw[r][k] = v;
     }
-    W[(long)k * ld + row] = v;
-    sw[r][k] = v;
     __syncthreads();
+    if (act) {
 #pragma unroll
-    for (int c = 0; c < S; c++) {
-        red[r][c + k * S] = vj[c] * v;             // A(c, k)
-        red[r][S * S + c + k * S] = sw[r][c] * v;  // P(c, k)
+        for (int c = 0; c < S; c++) {
+            red[r][c + k * S] = vj[c] * v;             // A(c, k)
+            red[r][S * S + c + k * S] = sw[r][c] * v;  // P(c, k)
+        }
     }
     __syncthreads();
-    {
+    if (act) {
         // column sums over the 64 rows: NGR groups of RPG rows, then the groups
         const int o = e % NO, gq = e / NO;
         if (gq < NGR) {
@@ -524,7 +545,7 @@ __global__ __launch_bounds__(TILE * MAX_S) void symm_reduce_lz_kernel(
         }
     }
     __syncthreads();
-    if (e < NO) {
+    if (act && e < NO) {
         double t = 0.0;
 #pragma unroll
         for (int gq = 0; gq < NGR; gq++) t += red2[gq][e];
@@ -897,7 +918,8 @@ int plan_product(const MobPlan &p, const E *mob, const double *x, double *y, int
     BD_HIP(hipGetLastError());
     if (e1) BD_HIP(hipEventRecord(e1, sm));
     if (epi) {
-        hipLaunchKernelGGL((symm_reduce_lz_kernel<S>), dim3(p.nrowblk), dim3(TILE * S), 0, sm, p.rowpart, p.colpart,
+        hipLaunchKernelGGL((symm_reduce_lz_kernel<S>), dim3(p.nrowblk), dim3(TILE * S * LZ_HALVES<S>), 0, sm, p.rowpart,
+                           p.colpart,
                            p.rowblk, y, ldy, epi->Vj, epi->Vp, epi->Bp, epi->part, stop, p.wsc);
         BD_HIP(hipGetLastError());
         return BD_OK;
