@@ -472,8 +472,8 @@ __global__ __launch_bounds__(256) void symm_reduce_kernel(const double *__restri
 template <int S>
 constexpr int LZ_HALVES = (2 * TILE * S <= TILE * MAX_S) ? 2 : 1;
 
-// The same sum for one output block per workgroup (TILE * S threads), fused
-// with the first step of a Lanczos iteration:
+// The same sum for one output block per workgroup (LZ_HALVES<S> sets of
+// TILE * S threads), fused with the first step of a Lanczos iteration:
 //   W1 = W - V_{j-1} B_{j-1}^T  (stored over W),
 //   partials of A = V_j^T W1 and P = W1^T W1 (2 S^2 per block, into lzpart).
 template <int S>
