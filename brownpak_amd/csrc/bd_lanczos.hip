@@ -1447,11 +1447,13 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                 }
                 pend[npend++] = j;
             }
-            // verdicts in group order: those LG iterations old (waiting for
-            // them), then any newer one that has already arrived
+            // verdicts in group order: those LG iterations old (3 for the
+            // single iterations around the hint, whose verdict is the likely
+            // stop; the device then still has two iterations queued), waiting
+            // for them, then any newer one that has already arrived
             while (npend > 0 && !stop) {
                 const int e = pend[0];
-                if (j - e >= LG) BD_TRY(wait_tests(c, sg, e));
+                if (j - e >= (in_window(e) ? 3 : LG)) BD_TRY(wait_tests(c, sg, e));
                 else if (__atomic_load_n(&c.lz_flag[0], __ATOMIC_ACQUIRE) < e) break;
                 for (int q = 1; q < npend; q++) pend[q - 1] = pend[q];
                 npend--;
