@@ -1,13 +1,19 @@
 #!/usr/bin/env python3
 """BD steps/s of the MI355X Brownian-dynamics step at N = 2000 RPY + block
-Lanczos (s = 4, tol 1e-6), mixed precision (FP32 mobility storage, FP64
+Lanczos (s = 4, tol 1e-6), mixed precision (FP32 mobility products, FP64
 accumulation and Krylov small problem) and FP64.
 
 One "step" is one EM step of the bds_run loop (bd_solver_m.f90:167-238):
-every s = 4 steps the RPY mobility is assembled and the block-Lanczos B.xi
-computed; every step forces, the drift D.F and the update run.  The timed
-region covers a whole number of s-step blocks; the Gaussian vectors come
-from the device Philox generator (synthetic, resident in HBM).
+every s = 4 steps the RPY mobility is (re)defined at the current positions
+and the block-Lanczos B.xi computed; every step forces, the drift D.F and
+the update run.  Each precision is timed with the mobility products
+matrix-free (RPY blocks evaluated inside the product) and from stored tiles
+(the lower triangle of D in HBM, as the reference stores mob(n,n)); the
+line's value is the faster mixed form.  Exactly --steps steps are timed,
+after --warmup steps and, if needed, the steps that bring the run to the
+start of an s-step block (reported as warmup_alignment_steps).  The
+Gaussian vectors come from the device Philox generator (synthetic, resident
+in HBM).
 
 Prints one JSON line (see --help).  --gpus > 1 is refused: this build runs on
 one GPU.
@@ -28,15 +34,21 @@ sys.path.insert(0, ROOT)
 N_BEADS = 2000
 S_BLOCK = 4
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+# algorithmic flops of one ordered bead pair of a matrix-free D.X product with
+# s columns: forming the RPY block (~40) and the 3 x 3 by 3 x s product (18 s)
+MF_FLOPS_PAIR = 40 + 18 * S_BLOCK
 
 
-def build_system(precision, seed=2000, reorth=False):
+def build_system(precision, seed=2000, reorth=False, storage="auto"):
     from brownpak_amd import BdStep
     from brownpak_amd.configs import chain_bonds, fene_wca_params, random_walk_chain
     crd = random_walk_chain(N_BEADS, seed)
     bond, vdw = fene_wca_params()
     g = BdStep(N_BEADS, lhdia=True, mob_fctr="LANC", nts_mobsam=S_BLOCK, lanc_mxitr=50,
-               lanc_tol=1e-6, tim_stp=1e-3, precision=precision, lanc_reorth=reorth)
+               lanc_tol=1e-6, tim_stp=1e-3, precision=precision, lanc_reorth=reorth,
+               mob_storage=storage)
     g.set_box(0, (0.0, 0.0, 0.0))
     g.ia_setup(chain_bonds(N_BEADS), [2], [bond], vdw_styles=[1], vdw_params=[vdw],
                excluded_atoms=0, mth_ptgen="DIR", rcutoff=vdw[2])
@@ -45,14 +57,14 @@ def build_system(precision, seed=2000, reorth=False):
     return g, crd
 
 
-def time_path(precision, steps, warmup):
+def time_path(precision, steps, warmup, storage):
     """Steps/s of the plain run, then the same number of steps again with
-    HIP events around every Lanczos tile-product launch (block width s; the
+    HIP events around every Lanczos product launch (block width s; the
     events add dependency points to the stream, so the timed run goes
     without them)."""
     import torch
-    g, _ = build_system(precision)
-    g.bds_run(warmup)
+    g, _ = build_system(precision, storage=storage)
+    g.bds_run(warmup + (-warmup) % S_BLOCK)
     g.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -74,7 +86,7 @@ def time_path(precision, steps, warmup):
                 prof_lanczos_iters=info2["total_lanczos_iters"] - info["total_lanczos_iters"])
 
 
-def accuracy_states(nstates=10):
+def accuracy_states(nstates=10, storage="auto"):
     """Per seeded state k (random walk seed k, xi seed k): the relative
     2-norm error of the Brownian block B.xi (all s columns) and of one EM
     displacement, mixed precision against FP64 block Lanczos (the reference
@@ -94,7 +106,7 @@ def accuracy_states(nstates=10):
         xi = np.random.default_rng(k).standard_normal((n, S_BLOCK))
         bxi, disp, its = {}, {}, {}
         for prec in ("fp64", "fp32"):
-            g, _ = build_system(prec, seed=k)
+            g, _ = build_system(prec, seed=k, storage=storage)
             g.calc_rpy_tensor()
             bxi[prec], _, its[prec] = g.brn_calc_bdw(xi)
             if prec == "fp64":
@@ -198,11 +210,14 @@ def time_ensemble(steps, warmup, precision="fp32"):
                 not_converged=info2["n_not_converged"])
 
 
-def measured_traffic():
-    """HBM bytes per launch of symm_tiles_kernel<float, 4> from the two
-    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in profiles/,
-    already scaled by the copy-kernel calibration; None until measured."""
-    path = os.path.join(ROOT, "profiles", "symm_tiles_n2000_s4_hbm_traffic.md")
+def measured_traffic(kind):
+    """HBM bytes per launch of the FP32 Lanczos product kernel (kind "tiles":
+    symm_tiles_kernel<float, 4>; "mf": mf_product_kernel<float, 4, 1>) from
+    the two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in
+    profiles/, already scaled by the copy-kernel calibration; None until
+    measured."""
+    name = {"tiles": "symm_tiles_n2000_s4_hbm_traffic.md", "mf": "mf_product_n2000_s4_hbm_traffic.md"}[kind]
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             for line in f:
@@ -243,8 +258,9 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
-    steps = max(S_BLOCK, (args.steps // S_BLOCK) * S_BLOCK)
-    warmup = max(S_BLOCK, (args.warmup // S_BLOCK) * S_BLOCK)
+    steps = max(1, args.steps)
+    warmup = max(0, args.warmup)
+    align = (-warmup) % S_BLOCK
 
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
@@ -252,20 +268,24 @@ def main():
         ens = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK)
         print(json.dumps({"ensemble": ens}), flush=True)
         return
-    mixed = time_path("fp32", steps, warmup)
-    print(f"# mixed: {mixed['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
-    fp64 = time_path("fp64", steps, warmup)
-    print(f"# fp64: {fp64['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
+    runs = {}
+    for prec in ("fp32", "fp64"):
+        for storage in ("matfree", "tiles"):
+            runs[prec, storage] = time_path(prec, steps, warmup, storage)
+            print(f"# {prec} {storage}: {runs[prec, storage]['steps_per_s']:.1f} steps/s", file=sys.stderr,
+                  flush=True)
+    best = {p: max(("matfree", "tiles"), key=lambda st: runs[p, st]["steps_per_s"]) for p in ("fp32", "fp64")}
+    mixed, fp64 = runs["fp32", best["fp32"]], runs["fp64", best["fp64"]]
     ens = None
     if args.ensemble_steps > 0:
         ens = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK)
         print(f"# ensemble: {ens['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
-    acc = None if args.no_error_check else accuracy_states(args.states)
+    acc = None if args.no_error_check else accuracy_states(args.states, best["fp32"])
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
     n = 3 * N_BEADS
 
-    def roof(run, elem):
+    def roof_tiles(run, elem):
         # algorithmic bytes of the Lanczos tile-product launches (block width
         # s = 4) of the profiled run: each working launch streams the lower
         # triangle of D (n(n+1)/2 entries of elem bytes) and reads and writes
@@ -280,8 +300,47 @@ def main():
                     bytes_per_launch=n_lz * per / launches, bytes_per_working_launch=per,
                     launches=launches, empty_launches=launches - n_lz)
 
-    rm, r64 = roof(mixed, 4), roof(fp64, 8)
-    achieved = rm["achieved"]
+    def roof_mf(run):
+        # algorithmic flops of the matrix-free Lanczos product launches: N(N-1)
+        # ordered bead pairs x MF_FLOPS_PAIR per working launch (empty
+        # launches after convergence count in the time only)
+        n_lz = run["prof_lanczos_iters"]
+        per = float(N_BEADS) * (N_BEADS - 1) * MF_FLOPS_PAIR
+        t = run["kernel_ms"] * 1e-3
+        launches = max(run["launches"], 1)
+        return dict(achieved=n_lz * per / t / 1e12, avg_launch_us=t / launches * 1e6,
+                    flops_per_launch=n_lz * per / launches, flops_per_working_launch=per,
+                    launches=launches, empty_launches=launches - n_lz)
+
+    rt32, rt64 = roof_tiles(runs["fp32", "tiles"], 4), roof_tiles(runs["fp64", "tiles"], 8)
+    rm32, rm64 = roof_mf(runs["fp32", "matfree"]), roof_mf(runs["fp64", "matfree"])
+    if best["fp32"] == "matfree":
+        roofline = {"bound": "mfma", "achieved": round(rm32["achieved"], 2), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(rm32["achieved"] / FP32_PEAK_TFLOPS, 4),
+                    "traffic": measured_traffic("mf"),
+                    "kernel": "mf_product_kernel<float, 4, 1>",
+                    "note": "FLOP-bound on the vector ALUs (FP32, no packed FP32 instructions: their "
+                            "ceiling is half the listed FP32 peak); 'mfma' names the compute bound",
+                    "avg_launch_us": round(rm32["avg_launch_us"], 2),
+                    "flops_per_working_launch": rm32["flops_per_working_launch"],
+                    "launches": rm32["launches"], "empty_launches": rm32["empty_launches"],
+                    "fp64_kernel": "mf_product_kernel<double, 4, 2>",
+                    "fp64_achieved": round(rm64["achieved"], 2), "fp64_peak": FP64_PEAK_TFLOPS,
+                    "fp64_frac": round(rm64["achieved"] / FP64_PEAK_TFLOPS, 4),
+                    "fp64_avg_launch_us": round(rm64["avg_launch_us"], 2)}
+    else:
+        roofline = {"bound": "hbm", "achieved": round(rt32["achieved"], 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(rt32["achieved"] / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic("tiles"), "kernel": "symm_tiles_kernel<float, 4>",
+                    "avg_launch_us": round(rt32["avg_launch_us"], 2)}
+    roof_tiles_entry = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                        "kernel": "symm_tiles_kernel<float, 4>", "achieved": round(rt32["achieved"], 1),
+                        "frac": round(rt32["achieved"] / HBM_PEAK_GBS, 4),
+                        "avg_launch_us": round(rt32["avg_launch_us"], 2),
+                        "bytes_per_working_launch": round(rt32["bytes_per_working_launch"]),
+                        "traffic": measured_traffic("tiles"),
+                        "fp64_achieved": round(rt64["achieved"], 1),
+                        "fp64_avg_launch_us": round(rt64["avg_launch_us"], 2)}
     out = {
         "metric": "BD steps/sec at N=2000 RPY+Lanczos, mixed vs FP64 at <=1e-6 rel err; HBM GB/s",
         "value": round(mixed["steps_per_s"], 2),
@@ -289,30 +348,26 @@ def main():
         "n_gpus": 1,
         "steps": steps,
         "warmup": warmup,
+        "warmup_alignment_steps": align,
         "ms_per_step": round(mixed["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 mobility storage / f64 accumulate + Krylov",
+        "dtype": "f32 mobility products / f64 accumulate + Krylov",
         "data": "synthetic (seeded random-walk chain, device Philox noise)",
         "config": {"workload": "single chain N=2000, FENE/WCA, RPY, block Lanczos s=4 tol 1e-6 "
                                "(lanc_mxitr 50), dt 1e-3, EM",
                    "nbeads": N_BEADS, "block": S_BLOCK, "parallelism": "single GPU"},
+        "mobility": {"mixed": best["fp32"], "fp64": best["fp64"]},
         "fp64_steps_per_s": round(fp64["steps_per_s"], 2),
         "fp64_ms_per_step": round(fp64["ms_per_step"], 4),
         "mixed_over_fp64": round(mixed["steps_per_s"] / fp64["steps_per_s"], 3),
+        "steps_per_s_by_storage": {f"{p}_{st}": round(r["steps_per_s"], 2) for (p, st), r in runs.items()},
+        "mixed_over_fp64_tiles": round(mixed["steps_per_s"] / runs["fp64", "tiles"]["steps_per_s"], 3),
         "accuracy": acc,
         "lanczos_iters_per_update": round(mixed["lanczos_iters"] / max(mixed["mob_updates"], 1), 2),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(),
-                     "kernel": "symm_tiles_kernel<float, 4>",
-                     "avg_launch_us": round(rm["avg_launch_us"], 2),
-                     "bytes_per_launch": round(rm["bytes_per_launch"]),
-                     "bytes_per_working_launch": round(rm["bytes_per_working_launch"]),
-                     "launches": rm["launches"], "empty_launches": rm["empty_launches"],
-                     "fp64_achieved": round(r64["achieved"], 1),
-                     "fp64_avg_launch_us": round(r64["avg_launch_us"], 2)},
+        "roofline": roofline,
+        "roofline_tiles": roof_tiles_entry,
         "cpu_baseline": cpu,
         "ensemble": ens,
     }

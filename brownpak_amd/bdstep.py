@@ -39,6 +39,7 @@ FP64, FP32 = 0, 1
 PT = {"DIR": 0, "VER": 1, "CL": 2}
 MOB_FCTR = {"CHOL": CHOL, "LANC": LANC, "CHOL_LT": CHOL_LT}
 INTEGRATOR = {"EM": 0, "PC": 1}
+MOB_STORAGE = {"auto": 0, "tiles": 1, "matfree": 2}
 MXPARAM = 12
 # work-unit shapes (tile rows, tile columns) of the mobility product, in the
 # order of the batch autotuner's timings (BD_TILE_SHAPES in bdstep.h)
@@ -66,6 +67,7 @@ class BdOpts(C.Structure):
         ("lanc_mxitr", C.c_int), ("lanc_tol", C.c_double), ("tim_stp", C.c_double),
         ("precision", C.c_int), ("keep_reuse_noise", C.c_int), ("flow_style", C.c_int),
         ("flow_rate", C.c_double), ("lanc_reorth", C.c_int), ("integrator", C.c_int),
+        ("mob_storage", C.c_int),
     ]
 
 
@@ -191,7 +193,7 @@ class BdStep:
 
     def __init__(self, nbeads, *, lhdia=True, mob_fctr="LANC", nts_mobsam=1, lanc_mxitr=50,
                  lanc_tol=1e-6, tim_stp=1e-3, precision="fp64", keep_reuse_noise=False,
-                 flow_style=0, flow_rate=0.0, lanc_reorth=False, integrator="EM"):
+                 flow_style=0, flow_rate=0.0, lanc_reorth=False, integrator="EM", mob_storage="auto"):
         self.L = load_library()
         self.nbeads = int(nbeads)
         self.n = 3 * self.nbeads
@@ -201,7 +203,8 @@ class BdStep:
         self.opts = BdOpts(int(bool(lhdia)), mf, int(nts_mobsam), int(lanc_mxitr), float(lanc_tol),
                            float(tim_stp), prec, int(bool(keep_reuse_noise)), int(flow_style),
                            float(flow_rate), int(bool(lanc_reorth)),
-                           INTEGRATOR[integrator] if isinstance(integrator, str) else int(integrator))
+                           INTEGRATOR[integrator] if isinstance(integrator, str) else int(integrator),
+                           MOB_STORAGE[mob_storage] if isinstance(mob_storage, str) else int(mob_storage))
         self.h = self.L.bd_ctx_create(self.nbeads, C.byref(self.opts))
         if not self.h:
             raise BdError(BD_ERR_ARG, self.L.bd_last_error().decode())

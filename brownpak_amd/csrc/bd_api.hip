@@ -519,6 +519,10 @@ bd_ctx *bd_ctx_create(int nbeads, const bd_opts *opts)
     if (opts->mob_fctr < BD_CHOL || opts->mob_fctr > BD_CHOL_LT) { set_error("unknown mob_fctr"); return nullptr; }
     if (opts->precision != BD_FP64 && opts->precision != BD_FP32) { set_error("unknown precision"); return nullptr; }
     if (opts->integrator != BD_EM && opts->integrator != BD_PC) { set_error("unknown integrator"); return nullptr; }
+    if (opts->mob_storage < BD_MOB_AUTO || opts->mob_storage > BD_MOB_MATFREE) {
+        set_error("unknown mob_storage");
+        return nullptr;
+    }
     bd_ctx *h = new bd_ctx();
     Ctx &c = h->c;
     c.nb = nbeads;
@@ -558,7 +562,8 @@ bd_ctx *bd_ctx_create(int nbeads, const bd_opts *opts)
         return fail(0);
     }
     if (opts->lhdia) {
-        if (mob_alloc(c) != BD_OK) return fail(0);
+        c.mf = mob_use_matfree(*opts, nbeads);
+        if ((c.mf ? mf_alloc(c) : mob_alloc(c)) != BD_OK) return fail(0);
         if (opts->mob_fctr == BD_LANC && lanczos_alloc(c) != BD_OK) return fail(0);
         if (opts->mob_fctr != BD_LANC && !c.W) {
             if (!dalloc(&c.W, (size_t)c.npad * s)) { set_error("device allocation failed"); return fail(0); }
@@ -576,6 +581,7 @@ void bd_ctx_destroy(bd_ctx *h)
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     for (auto e : c.prof_ev) (void)hipEventDestroy(e);
     mob_free(c);
+    mf_free(c);
     lanczos_free(c);
     forces_free(c);
     double **dp[] = {&c.crd, &c.frc, &c.drift, &c.diff, &c.xi_dev, &c.scal, &c.pc_r0, &c.pc_d0};

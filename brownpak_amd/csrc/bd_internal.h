@@ -141,6 +141,16 @@ struct Ctx {
     double *chol = nullptr;      // dense n x n work for the Cholesky factor
     int chol_ld = 0;
 
+    // matrix-free products (bd_matfree.hip): no stored D; records of the
+    // coordinates D stands for (mf_nbp beads, 4 doubles each, padded) and of
+    // the block being multiplied (mf_rows x s row-major, FP32 or FP64), and
+    // the per-workgroup partial sums (mf_jg x nb x 3 MAX_S FP64)
+    bool mf = false;
+    int mf_nbp = 0, mf_rows = 0, mf_jg = 1;
+    double *mf_crd = nullptr;
+    void *mf_x = nullptr;
+    double *mf_part = nullptr;
+
     // symmetric product plan
     MobPlan plan;
     MobPlan plan_wide;   // 2 x 4 units for blocks wider than 4 columns (built on first use)
@@ -233,6 +243,9 @@ struct Ctx {
     long long prof_count = 0;
 };
 void prof_flush(Ctx &c);
+// Events bracketing the next product launch of block width w when the
+// tile-product timing is on (both null otherwise).
+int prof_pair(Ctx &c, int w, hipEvent_t *e0, hipEvent_t *e1);
 
 // --- mobility (bd_mobility.hip)
 int mob_alloc(Ctx &c);
@@ -245,6 +258,22 @@ int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy, 
                    const XPre *pre = nullptr);
 // y = alpha * (D x + add) with optional add vector (s = 1); used for the drift.
 int mob_unpack(Ctx &c, double *dense, int ld);
+
+// Output blocks of 64 rows of a product (the Krylov epilogue's partial count).
+inline int mob_rowblocks(const Ctx &c) { return c.mf ? c.ntile : c.plan.nrowblk; }
+
+// --- matrix-free mobility (bd_matfree.hip)
+bool mob_use_matfree(const bd_opts &o, int nbeads);
+int mf_alloc(Ctx &c);
+void mf_free(Ctx &c);
+int mf_assemble(Ctx &c, const double *crd);
+int mf_pack(Ctx &c, const double *x, int s, int ldx);
+int mf_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy, const DevStatus *stop);
+// Lanczos product of X (packed already when packed) with the epilogue of
+// symm_reduce_lz_kernel written into W; pre: the second CholeskyQR pass.
+int mf_product_lz(Ctx &c, const double *X, double *W, int s, int ld, bool packed, const LzEpi &epi,
+                  const XPre *pre);
+int mf_dense(Ctx &c, double *dense, int ld);
 
 // --- Brownian (bd_lanczos.hip)
 int lanczos_alloc(Ctx &c);

@@ -398,7 +398,8 @@ __global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict_
                                                        double *__restrict__ Q, int ld, int rows,
                                                        const double *__restrict__ lzpart, int nblk,
                                                        double *__restrict__ Aout, double *__restrict__ R1save,
-                                                       int upd, double *__restrict__ gpart, DevStatus *st)
+                                                       int upd, double *__restrict__ gpart, DevStatus *st,
+                                                       float *__restrict__ qrec32, double *__restrict__ qrec64)
 {
     __shared__ int s_done, s_fail;
     __shared__ double sAP[2 * S * S], sAh[S * S], sG[S * S], sR[S * S], sI[S], s_scr[RB];
@@ -483,6 +484,14 @@ __global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict_
         }
 #pragma unroll
         for (int k = 0; k < S; k++) Q[(long)k * ld + r] = q[k];
+        // row-major copy of Q for a matrix-free product (bd_matfree.hip)
+        if (qrec32) {
+#pragma unroll
+            for (int k = 0; k < S; k++) qrec32[(long)r * S + k] = (float)q[k];
+        } else if (qrec64) {
+#pragma unroll
+            for (int k = 0; k < S; k++) qrec64[(long)r * S + k] = q[k];
+        }
 #pragma unroll
         for (int cj = 0; cj < S; cj++)
 #pragma unroll
@@ -1284,7 +1293,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
                                          : (size_t)dmax * (3 * S * S + S) * 256;
         const size_t nblk_y = (size_t)(rows + YR - 1) / YR;
         if (need_part > cq || nblk_y * LG * 2 * S > (size_t)(c.npad / YR + 1) * LG * 2 * MAX_S ||
-            (size_t)LG * dmax * S * S > (size_t)LG * ncol * S || (size_t)c.plan.nrowblk > (size_t)(c.npad / TILE + 1) ||
+            (size_t)LG * dmax * S * S > (size_t)LG * ncol * S || (size_t)mob_rowblocks(c) > (size_t)(c.npad / TILE + 1) ||
             (grouped && (size_t)dmax * S > (size_t)YJS)) {
             set_error("internal: Lanczos scratch smaller than its launches need");
             return BD_ERR_STATE;
@@ -1343,8 +1352,9 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             }
             BD_TRY(mob_product_lz(c, j > 1 ? c.Qbuf : Vj, c.W, S, ld, ld, &epi, j > 1 ? &pre : nullptr));
             hipLaunchKernelGGL((krylov_k2_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.W, Vj, c.Qbuf, ld, rows,
-                               c.lzpart, c.plan.nrowblk, c.Ablk + (size_t)(j - 1) * S * S, c.R0 + S * S, upd, g0,
-                               c.st);
+                               c.lzpart, mob_rowblocks(c), c.Ablk + (size_t)(j - 1) * S * S, c.R0 + S * S, upd, g0,
+                               c.st, (c.mf && c.opt.precision == BD_FP32) ? (float *)c.mf_x : nullptr,
+                               (c.mf && c.opt.precision != BD_FP32) ? (double *)c.mf_x : nullptr);
             BD_HIP(hipGetLastError());
         } else {
             BD_TRY(mob_product_lz(c, Vj, c.W, S, ld, ld));

@@ -1,0 +1,580 @@
+// bd_matfree.hip -- matrix-free RPY products Y = D(r) X (no stored mobility).
+//
+// Replaces calc_rpy_tensor (bd_solver_m.f90:458-522) followed by the
+// dsymv/dsymm('L') products (bd_solver_m.f90:446, brown_m.f90:232, 386)
+// with one pass that evaluates every 3 x 3 bead-pair block of D in
+// registers, where it is used, and never writes D:
+//   D_ij X_j = c1 X_j + c2 r (r . X_j),   r = r_j - r_i,
+//   r >= 2:  c1 = 3/(4r) + 1/(2r^3),   c2 = 3/(4r^3) - 3/(2r^5)
+//   r <  2:  c1 = 1 - 9r/32,           c2 = 3/(32r)
+//   i == j:  c1 = 1, c2 = 0
+// (bd_solver_m.f90:484-504 with a = 1; c2 multiplies r r^T, not r^ r^T).
+//
+// Work decomposition.  A wavefront owns 64 row beads (R x 64 with R rows per
+// lane) and walks a range of column beads j; all its lanes take the same j,
+// so the record of bead j -- coordinates (FP64) and its 3 x S block of X --
+// is read with scalar loads into SGPRs and feeds the vector instructions as
+// a scalar operand: no LDS and no vector memory traffic in the inner loop.
+// The column range is split over the 4 waves of a workgroup (summed in LDS,
+// fixed order) and over JG workgroups (partials summed by the next kernel in
+// a fixed order), so results repeat bit for bit from run to run.
+//
+// Precision.  FP64: everything in FP64 (1/r from v_rsq_f64 refined by two
+// Newton steps).  FP32: the coordinate differences are formed in FP64 and
+// rounded (so near pairs keep their relative accuracy however far the chain
+// is from the origin), the block and its product in FP32, and the products
+// are summed in FP32 over 8 column beads, then in FP64.
+//
+// Records are padded to a multiple of 8 beads with beads far away whose X
+// block is zero, so the inner loop has no tail.
+#include "bd_internal.h"
+#include "bd_lanczos_dev.h"
+
+namespace bd {
+
+constexpr int MF_CH = 8;        // column beads per FP32 partial sum (and record padding)
+constexpr int MF_WAVES = 4;     // waves per workgroup (column-range split)
+constexpr int MF_JG_MAX = 64;   // largest workgroup split of the column range
+
+#define AS4 __attribute__((address_space(4)))
+
+int mf_rows_per_lane(const Ctx &c) { return c.opt.precision == BD_FP32 ? 1 : 2; }
+
+// Workgroups along the column range: about 2048 waves in total, at least
+// one 8-bead chunk per wave.
+int mf_jg(const Ctx &c)
+{
+    const int R = mf_rows_per_lane(c);
+    const int nib = (c.nb + 64 * R - 1) / (64 * R);
+    const int nch = (c.nb + MF_CH - 1) / MF_CH;
+    int jg = 512 / nib;
+    jg = std::min(jg, nch / MF_WAVES);
+    return std::max(1, std::min(jg, MF_JG_MAX));
+}
+
+// Coordinates (3, N) -> records of 4 doubles (x, y, z, 0), padded to nbp
+// beads with points 1e6 apart from everything.
+__global__ void mf_crd_kernel(const double *__restrict__ crd, int nb, int nbp, double *__restrict__ rec)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nbp) return;
+    double x, y, z;
+    if (j < nb) {
+        x = crd[3 * j];
+        y = crd[3 * j + 1];
+        z = crd[3 * j + 2];
+    } else {
+        x = 1.0e6 * (j - nb + 2);
+        y = 1.0e6;
+        z = 1.0e6;
+    }
+    rec[4 * j] = x;
+    rec[4 * j + 1] = y;
+    rec[4 * j + 2] = z;
+    rec[4 * j + 3] = 0.0;
+}
+
+// X (rows x S column-major, ld) -> row-major records xr[r * S + k] in T,
+// rows r < nrows (rows at or beyond n are zero).
+template <typename T>
+__global__ void mf_pack_kernel(const double *__restrict__ x, int ld, int n, int nrows, int S, T *__restrict__ xr)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nrows * S) return;
+    const int r = (int)(e / S), k = (int)(e % S);
+    xr[e] = r < n ? (T)x[(long)k * ld + r] : (T)0;
+}
+
+template <typename T>
+__device__ inline T mf_rsqrt(T v);
+template <>
+__device__ inline float mf_rsqrt<float>(float v)
+{
+    return __builtin_amdgcn_rsqf(v);
+}
+template <>
+__device__ inline double mf_rsqrt<double>(double v)
+{
+    double y = __builtin_amdgcn_rsq(v);
+    double e = fma(-v * y, y, 1.0);   // two Newton steps: y <- y + y (1 - v y^2) / 2
+    y = fma(0.5 * y, e, y);
+    e = fma(-v * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// c1, c2 of the pair (see the header), including the overlap branch and
+// the diagonal block.
+template <typename T>
+__device__ inline void mf_coef(T r2, bool self, T &c1, T &c2)
+{
+    const T y = mf_rsqrt<T>(r2), y2 = y * y;
+    if (r2 >= (T)4) {
+        c1 = y * ((T)0.75 + (T)0.5 * y2);
+        c2 = y * y2 * ((T)0.75 - (T)1.5 * y2);
+    } else {
+        const T r = r2 * y;
+        c1 = (T)1 - (T)(9.0 / 32.0) * r;
+        c2 = (T)(3.0 / 32.0) * y;
+    }
+    if (self) {
+        c1 = (T)1;
+        c2 = (T)0;
+    }
+}
+
+template <typename T, int S>
+struct MfRec {
+    double x, y, z;
+    T v[3][S];
+};
+
+template <typename T, int S>
+__device__ inline MfRec<T, S> mf_load(const double AS4 *crec, const T AS4 *xrec, int j)
+{
+    MfRec<T, S> r;
+    r.x = crec[4 * j];
+    r.y = crec[4 * j + 1];
+    r.z = crec[4 * j + 2];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int k = 0; k < S; k++) r.v[a][k] = xrec[(3 * j + a) * S + k];
+    return r;
+}
+
+// part[g][i][a S + k] = sum over the column beads j of workgroup column g of
+// (D_ij X_j)(a, k), FP64.  grid (ceil(nb / (64 R)), JG), 256 threads.
+template <typename T, int S, int R>
+__global__ __launch_bounds__(256) void mf_product_kernel(const double AS4 *crec, const T AS4 *xrec, int nb, int nch,
+                                                         double *__restrict__ part, const DevStatus *stop)
+{
+    __shared__ double red[MF_WAVES][64][3 * S + 1];
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = (stop && lz_done(stop)) ? 1 : 0;
+    __syncthreads();
+    if (s_done) return;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double xi[R], yi[R], zi[R];
+    int ii[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        ii[r] = (blockIdx.x * R + r) * 64 + lane;
+        const int ic = min(ii[r], nb - 1);
+        xi[r] = crec[4 * ic];
+        yi[r] = crec[4 * ic + 1];
+        zi[r] = crec[4 * ic + 2];
+    }
+    const int nparts = (int)gridDim.y * MF_WAVES, p = (int)blockIdx.y * MF_WAVES + w;
+    const int cb = (int)((long)nch * p / nparts), ce = (int)((long)nch * (p + 1) / nparts);
+    double acc[R][3][S];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int k = 0; k < S; k++) acc[r][a][k] = 0.0;
+    // the pair term (D_ij X_j)(a, k) added to acc[r][a][k]
+    auto pair = [&](const MfRec<T, S> &cur, int j, auto &acc_) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const T dx = (T)(cur.x - xi[r]), dy = (T)(cur.y - yi[r]), dz = (T)(cur.z - zi[r]);
+            const T r2 = dx * dx + dy * dy + dz * dz;
+            T c1, c2;
+            if (__builtin_amdgcn_ballot_w64(r2 < (T)4) == 0) {
+                // every lane far from bead j: the r >= 2 branch only
+                const T y = mf_rsqrt<T>(r2), y2 = y * y;
+                c1 = y * ((T)0.75 + (T)0.5 * y2);
+                c2 = y * y2 * ((T)0.75 - (T)1.5 * y2);
+            } else {
+                mf_coef<T>(r2, j == ii[r], c1, c2);
+            }
+#pragma unroll
+            for (int k = 0; k < S; k++) {
+                const T t = c2 * (dx * cur.v[0][k] + dy * cur.v[1][k] + dz * cur.v[2][k]);
+                acc_[r][0][k] = fma(c1, cur.v[0][k], acc_[r][0][k]);
+                acc_[r][0][k] = fma(t, dx, acc_[r][0][k]);
+                acc_[r][1][k] = fma(c1, cur.v[1][k], acc_[r][1][k]);
+                acc_[r][1][k] = fma(t, dy, acc_[r][1][k]);
+                acc_[r][2][k] = fma(c1, cur.v[2][k], acc_[r][2][k]);
+                acc_[r][2][k] = fma(t, dz, acc_[r][2][k]);
+            }
+        }
+    };
+    // records small enough are double-buffered in SGPRs: the next one loads
+    // while the current one is used
+    constexpr bool PF = (6 + 3 * S * (int)sizeof(T) / 4) <= 36;
+    if constexpr (sizeof(T) == 8) {
+        // FP64: straight into the accumulators
+        const int jb = MF_CH * cb, je = MF_CH * ce;
+        if (jb < je) {
+            MfRec<T, S> cur = mf_load<T, S>(crec, xrec, jb);
+            for (int j = jb; j < je; j++) {
+                if constexpr (PF) {
+                    const MfRec<T, S> nxt = mf_load<T, S>(crec, xrec, j + 1 < je ? j + 1 : j);
+                    pair(cur, j, acc);
+                    cur = nxt;
+                } else {
+                    pair(mf_load<T, S>(crec, xrec, j), j, acc);
+                }
+            }
+        }
+    } else if (cb < ce) {
+        // FP32: partial sums over 8 column beads, then added in FP64
+        for (int ch = cb; ch < ce; ch++) {
+            T cacc[R][3][S];
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int k = 0; k < S; k++) cacc[r][a][k] = (T)0;
+            if constexpr (PF) {
+                MfRec<T, S> cur = mf_load<T, S>(crec, xrec, MF_CH * ch);
+#pragma unroll 2
+                for (int q = 0; q < MF_CH; q++) {
+                    const int j = MF_CH * ch + q;
+                    const MfRec<T, S> nxt = mf_load<T, S>(crec, xrec, q + 1 < MF_CH ? j + 1 : j);
+                    pair(cur, j, cacc);
+                    cur = nxt;
+                }
+            } else {
+                for (int q = 0; q < MF_CH; q++) {
+                    const int j = MF_CH * ch + q;
+                    pair(mf_load<T, S>(crec, xrec, j), j, cacc);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int k = 0; k < S; k++) acc[r][a][k] += (double)cacc[r][a][k];
+        }
+    }
+    // the four waves' sums, added in wave order
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int k = 0; k < S; k++) red[w][lane][a * S + k] = acc[r][a][k];
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 3 * S; e += blockDim.x) {
+            const int l = e / (3 * S), col = e % (3 * S);
+            const int ib = (blockIdx.x * R + r) * 64 + l;
+            double v = red[0][l][col];
+#pragma unroll
+            for (int q = 1; q < MF_WAVES; q++) v += red[q][l][col];
+            if (ib < nb) part[((long)blockIdx.y * nb + ib) * (3 * S) + col] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// y(rows, S) = sum over g of the partials (fixed order); rows >= n are zero.
+__global__ void mf_reduce_kernel(const double *__restrict__ part, int jg, int nb, int n, int rows, int S,
+                                 double *__restrict__ y, int ldy, const DevStatus *stop)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)rows * S) return;
+    if (stop && lz_done(stop)) return;
+    const int row = (int)(e % rows), k = (int)(e / rows);
+    double v = 0.0;
+    if (row < n) {
+        const long o = (long)(row / 3) * (3 * S) + (row % 3) * S + k, str = (long)nb * 3 * S;
+        v = sum_strided(part + o, str, jg);
+    }
+    y[(long)k * ldy + row] = v;
+}
+
+// One 64-row block of a Lanczos iteration (grid npad / 64, 256 threads):
+//   DX = sum of the partials (D applied to X, X = V_1, or Q from the second
+//   iteration on);
+//   with pre: R2 from the Gram partials of Q (every block, fixed order),
+//     V_j = Q R2^{-1} (written), W = DX R2^{-1} (= D V_j by linearity),
+//     B_{j-1} = R2 R1 (block 0 writes it);  else V_j = X, W = DX;
+//   W1 = W - V_{j-1} B_{j-1}^T (stored over W);
+//   partials of A = V_j^T W1 and P = W1^T W1 (2 S^2 per block, into lzpart),
+// the same outputs as the stored-tile product's epilogue.
+template <int S>
+__global__ __launch_bounds__(256) void mf_reduce_lz_kernel(const double *__restrict__ part, int jg, int nb, int n,
+                                                           const double *__restrict__ X, int ld, XPre pre,
+                                                           const double *__restrict__ Vp, const double *__restrict__ Bp,
+                                                           double *__restrict__ W, double *__restrict__ lzpart,
+                                                           DevStatus *st)
+{
+    constexpr int NO = 2 * S * S;
+    __shared__ double sdx[TILE][S + 1], sx[TILE][S + 1], sw[TILE][S + 1], sv[TILE][S + 1];
+    __shared__ double sG[S * S], sR2[S * S], sB[S * S];
+    __shared__ int s_done, s_fail;
+    const int tid = threadIdx.x, G = blockIdx.x;
+    if (tid == 0) s_done = lz_done(st) ? 1 : 0;
+    __syncthreads();
+    if (s_done) return;
+    const bool xpre = pre.gpart != nullptr;
+    if (xpre) {
+        sum_partials_n<S>(pre.gpart, pre.ng, sG);   // ends with a barrier
+        if (tid == 0) {
+            double R[S][S];
+            const bool ok = small_chol<S>(sG, R);
+            s_fail = ok ? 0 : 1;
+            for (int i = 0; i < S; i++)
+                for (int j = 0; j < S; j++) sR2[i + j * S] = R[i][j];
+            if (ok) {
+                // B = R2 R1 (both upper triangular)
+                for (int i = 0; i < S; i++)
+                    for (int j = 0; j < S; j++) {
+                        double t = 0.0;
+                        for (int k = 0; k < S; k++) t += R[i][k] * pre.R1[k + j * S];
+                        sB[i + j * S] = t;
+                        if (G == 0) pre.Bout[i + j * S] = t;
+                    }
+            }
+        }
+        __syncthreads();
+        if (s_fail) {
+            if (G == 0 && tid == 0) atomicCAS(&st->ierr, 0, 2);
+            return;
+        }
+    } else if (Vp && tid < S * S) {
+        sB[tid] = Bp[tid];
+    }
+    for (int e = tid; e < TILE * S; e += blockDim.x) {
+        const int r = e / S, k = e % S, row = G * TILE + r;
+        double v = 0.0;
+        if (row < n) {
+            const long o = (long)(row / 3) * (3 * S) + (row % 3) * S + k, str = (long)nb * 3 * S;
+            v = sum_strided(part + o, str, jg);
+        }
+        sdx[r][k] = v;
+        sx[r][k] = X[(long)k * ld + row];
+    }
+    __syncthreads();
+    for (int e = tid; e < TILE * S; e += blockDim.x) {
+        const int r = e / S, k = e % S, row = G * TILE + r;
+        double w, vj;
+        if (xpre) {
+            // forward solves of the row against R2, columns 0..k
+            double tw[S], tv[S];
+#pragma unroll
+            for (int c = 0; c < S; c++) {
+                if (c > k) break;
+                double a = sdx[r][c], b = sx[r][c];
+                for (int m = 0; m < c; m++) {
+                    a -= tw[m] * sR2[m + c * S];
+                    b -= tv[m] * sR2[m + c * S];
+                }
+                tw[c] = a / sR2[c + c * S];
+                tv[c] = b / sR2[c + c * S];
+            }
+            w = tw[k];
+            vj = tv[k];
+            pre.V[(long)k * ld + row] = vj;
+        } else {
+            w = sdx[r][k];
+            vj = sx[r][k];
+        }
+        if (Vp) {
+            double t = 0.0;
+#pragma unroll
+            for (int kp = 0; kp < S; kp++) t += Vp[(long)kp * ld + row] * sB[k + kp * S];
+            w = w - t;
+        }
+        W[(long)k * ld + row] = w;
+        sw[r][k] = w;
+        sv[r][k] = vj;
+    }
+    __syncthreads();
+    for (int o = tid; o < NO; o += blockDim.x) {
+        const bool isA = o < S * S;
+        const int oo = isA ? o : o - S * S, c = oo % S, k = oo / S;
+        double t = 0.0;
+        for (int r = 0; r < TILE; r++) t += (isA ? sv[r][c] : sw[r][c]) * sw[r][k];
+        lzpart[(long)G * NO + o] = t;
+    }
+}
+
+// Dense n x n RPY matrix (column-major, unit diagonal) from the records,
+// with the arithmetic of the tile assembly (calc_rpy_tensor).
+__global__ void mf_dense_kernel(const double *__restrict__ crec, int nb, double *__restrict__ dense, int ld)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = 3L * nb;
+    if (e >= n * n) return;
+    const int col = (int)(e / n), row = (int)(e % n);
+    const int bi = row / 3, bj = col / 3, da = row % 3, db = col % 3;
+    double v;
+    if (bi == bj) {
+        v = da == db ? 1.0 : 0.0;
+    } else {
+        const double rx = crec[4 * bj] - crec[4 * bi], ry = crec[4 * bj + 1] - crec[4 * bi + 1],
+                     rz = crec[4 * bj + 2] - crec[4 * bi + 2];
+        const double rijm = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(rx, rx), __dmul_rn(ry, ry)), __dmul_rn(rz, rz)));
+        const double irijm = 1.0 / rijm, irijm2 = irijm * irijm;
+        double c1, c2, cons;
+        if (rijm >= 2.0) {
+            c1 = 1.0 + (2.0 / 3.0) * irijm2;
+            c2 = 1.0 - 2.0 * irijm2;
+            cons = 0.75 * irijm;
+        } else {
+            c1 = 1.0 - 9.0 * rijm / 32.0;
+            c2 = 3.0 * rijm / 32.0;
+            cons = 1.0;
+        }
+        const double ra = da == 0 ? rx : (da == 1 ? ry : rz), rb = db == 0 ? rx : (db == 1 ? ry : rz);
+        v = cons * ((da == db ? c1 : 0.0) + c2 * ra * rb * irijm2);
+    }
+    dense[(long)col * ld + row] = v;
+}
+
+// ---------------------------------------------------------------- host
+// BD_MOB_AUTO: matrix-free from the bead count at which it was measured to
+// beat the stored tiles (both precisions, one MI355X).
+bool mob_use_matfree(const bd_opts &o, int nbeads)
+{
+    if (o.mob_storage == BD_MOB_TILES) return false;
+    if (o.mob_storage == BD_MOB_MATFREE) return true;
+    if (o.mob_fctr != BD_LANC) return false;   // the Cholesky path factors a stored matrix
+    return nbeads >= 64;
+}
+
+int mf_alloc(Ctx &c)
+{
+    c.mf_nbp = ((c.nb + MF_CH - 1) / MF_CH) * MF_CH;
+    c.mf_rows = std::max(c.npad, 3 * c.mf_nbp);
+    c.mf_jg = mf_jg(c);
+    const size_t esz = c.opt.precision == BD_FP32 ? sizeof(float) : sizeof(double);
+    BD_HIP(hipMalloc(&c.mf_crd, (size_t)c.mf_nbp * 4 * sizeof(double)));
+    BD_HIP(hipMalloc(&c.mf_x, (size_t)c.mf_rows * MAX_S * esz));
+    BD_HIP(hipMemset(c.mf_x, 0, (size_t)c.mf_rows * MAX_S * esz));
+    BD_HIP(hipMalloc(&c.mf_part, (size_t)c.mf_jg * c.nb * 3 * MAX_S * sizeof(double)));
+    return BD_OK;
+}
+
+void mf_free(Ctx &c)
+{
+    if (c.mf_crd) (void)hipFree(c.mf_crd);
+    if (c.mf_x) (void)hipFree(c.mf_x);
+    if (c.mf_part) (void)hipFree(c.mf_part);
+    c.mf_crd = nullptr;
+    c.mf_x = nullptr;
+    c.mf_part = nullptr;
+}
+
+int mf_assemble(Ctx &c, const double *crd)
+{
+    hipLaunchKernelGGL(mf_crd_kernel, dim3((c.mf_nbp + 255) / 256), dim3(256), 0, c.stream, crd, c.nb, c.mf_nbp,
+                       c.mf_crd);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+int mf_pack(Ctx &c, const double *x, int s, int ldx)
+{
+    const long tot = (long)c.mf_rows * s;
+    if (c.opt.precision == BD_FP32)
+        hipLaunchKernelGGL(mf_pack_kernel<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, x, ldx,
+                           c.n, c.mf_rows, s, (float *)c.mf_x);
+    else
+        hipLaunchKernelGGL(mf_pack_kernel<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, x,
+                           ldx, c.n, c.mf_rows, s, (double *)c.mf_x);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+template <typename T, int S>
+static int mf_launch_t(Ctx &c, const DevStatus *stop)
+{
+    const int R = mf_rows_per_lane(c);
+    const int nch = c.mf_nbp / MF_CH;
+    const dim3 grid((c.nb + 64 * R - 1) / (64 * R), c.mf_jg);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BD_TRY(prof_pair(c, S, &e0, &e1));
+    if (e0) BD_HIP(hipEventRecord(e0, c.stream));
+    const double AS4 *crec = (const double AS4 *)c.mf_crd;
+    const T AS4 *xrec = (const T AS4 *)c.mf_x;
+    if (R == 1)
+        hipLaunchKernelGGL((mf_product_kernel<T, S, 1>), grid, dim3(256), 0, c.stream, crec, xrec, c.nb, nch,
+                           c.mf_part, stop);
+    else
+        hipLaunchKernelGGL((mf_product_kernel<T, S, 2>), grid, dim3(256), 0, c.stream, crec, xrec, c.nb, nch,
+                           c.mf_part, stop);
+    BD_HIP(hipGetLastError());
+    if (e1) BD_HIP(hipEventRecord(e1, c.stream));
+    return BD_OK;
+}
+
+template <typename T>
+static int mf_launch(Ctx &c, int s, const DevStatus *stop)
+{
+    switch (s) {
+    case 1: return mf_launch_t<T, 1>(c, stop);
+    case 2: return mf_launch_t<T, 2>(c, stop);
+    case 3: return mf_launch_t<T, 3>(c, stop);
+    case 4: return mf_launch_t<T, 4>(c, stop);
+    case 5: return mf_launch_t<T, 5>(c, stop);
+    case 6: return mf_launch_t<T, 6>(c, stop);
+    case 7: return mf_launch_t<T, 7>(c, stop);
+    case 8: return mf_launch_t<T, 8>(c, stop);
+    default: set_error("block size s must be in 1..8"); return BD_ERR_ARG;
+    }
+}
+
+// Product of the records already packed in c.mf_x (s columns) into the partials.
+int mf_product_packed(Ctx &c, int s, const DevStatus *stop)
+{
+    if (c.opt.precision == BD_FP32) return mf_launch<float>(c, s, stop);
+    return mf_launch<double>(c, s, stop);
+}
+
+int mf_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy, const DevStatus *stop)
+{
+    BD_TRY(mf_pack(c, x, s, ldx));
+    BD_TRY(mf_product_packed(c, s, stop));
+    const long tot = (long)c.npad * s;
+    hipLaunchKernelGGL(mf_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, c.mf_part,
+                       c.mf_jg, c.nb, c.n, c.npad, s, y, ldy, stop);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+template <int S>
+static int mf_reduce_lz_t(Ctx &c, const double *X, double *W, int ld, const LzEpi &epi, const XPre *pre)
+{
+    const XPre xp = pre ? *pre : XPre{};
+    hipLaunchKernelGGL((mf_reduce_lz_kernel<S>), dim3(c.npad / TILE), dim3(256), 0, c.stream, c.mf_part, c.mf_jg,
+                       c.nb, c.n, X, ld, xp, epi.Vp, epi.Bp, W, epi.part, c.st);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+int mf_product_lz(Ctx &c, const double *X, double *W, int s, int ld, bool packed, const LzEpi &epi,
+                  const XPre *pre)
+{
+    if (!packed) BD_TRY(mf_pack(c, X, s, ld));
+    BD_TRY(mf_product_packed(c, s, c.st));
+    switch (s) {
+    case 1: return mf_reduce_lz_t<1>(c, X, W, ld, epi, pre);
+    case 2: return mf_reduce_lz_t<2>(c, X, W, ld, epi, pre);
+    case 3: return mf_reduce_lz_t<3>(c, X, W, ld, epi, pre);
+    case 4: return mf_reduce_lz_t<4>(c, X, W, ld, epi, pre);
+    case 5: return mf_reduce_lz_t<5>(c, X, W, ld, epi, pre);
+    case 6: return mf_reduce_lz_t<6>(c, X, W, ld, epi, pre);
+    case 7: return mf_reduce_lz_t<7>(c, X, W, ld, epi, pre);
+    case 8: return mf_reduce_lz_t<8>(c, X, W, ld, epi, pre);
+    default: set_error("block size s must be in 1..8"); return BD_ERR_ARG;
+    }
+}
+
+int mf_dense(Ctx &c, double *dense, int ld)
+{
+    const long tot = (long)c.n * c.n;
+    if (tot == 0) return BD_OK;
+    hipLaunchKernelGGL(mf_dense_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, c.mf_crd, c.nb,
+                       dense, ld);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+}  // namespace bd

@@ -875,6 +875,11 @@ template int rpy_launch<double>(const MobPlan &, const double *, double *, hipSt
 
 int rpy_assemble_at(Ctx &c, const double *crd)
 {
+    if (c.mf) {
+        BD_TRY(mf_assemble(c, crd));
+        c.mob_valid = true;
+        return BD_OK;
+    }
     if (c.mob32) BD_TRY(rpy_launch<float>(c.plan, crd, c.mob32, c.stream));
     else BD_TRY(rpy_launch<double>(c.plan, crd, c.mob64, c.stream));
     c.mob_valid = true;
@@ -1000,24 +1005,30 @@ BD_INST(float, 1) BD_INST(float, 2) BD_INST(float, 3) BD_INST(float, 4)
 BD_INST(double, 1) BD_INST(double, 2) BD_INST(double, 3) BD_INST(double, 4)
 #undef BD_INST
 
+int prof_pair(Ctx &c, int w, hipEvent_t *e0, hipEvent_t *e1)
+{
+    *e0 = *e1 = nullptr;
+    if (!c.prof_on || (c.prof_width != 0 && c.prof_width != w)) return BD_OK;
+    if (c.prof_used + 2 > c.prof_ev.size()) prof_flush(c);
+    if (c.prof_used + 2 > c.prof_ev.size()) {
+        hipEvent_t a, b;
+        BD_HIP(hipEventCreate(&a));
+        BD_HIP(hipEventCreate(&b));
+        c.prof_ev.push_back(a);
+        c.prof_ev.push_back(b);
+    }
+    *e0 = c.prof_ev[c.prof_used];
+    *e1 = c.prof_ev[c.prof_used + 1];
+    c.prof_used += 2;
+    return BD_OK;
+}
+
 template <typename E, int S>
 static int product_t(Ctx &c, const E *mob, const double *x, double *y, int ldx, int ldy, const DevStatus *stop,
                      const LzEpi *epi, const XPre *pre)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c.prof_on && (c.prof_width == 0 || c.prof_width == S)) {
-        if (c.prof_used + 2 > c.prof_ev.size()) prof_flush(c);
-        if (c.prof_used + 2 > c.prof_ev.size()) {
-            hipEvent_t a, b;
-            BD_HIP(hipEventCreate(&a));
-            BD_HIP(hipEventCreate(&b));
-            c.prof_ev.push_back(a);
-            c.prof_ev.push_back(b);
-        }
-        e0 = c.prof_ev[c.prof_used];
-        e1 = c.prof_ev[c.prof_used + 1];
-        c.prof_used += 2;
-    }
+    BD_TRY(prof_pair(c, S, &e0, &e1));
     const MobPlan *pl = &c.plan;
     if constexpr (S > 4) {
         // only the 2 x 4 unit is built for these widths
@@ -1050,6 +1061,7 @@ static int product_dispatch(Ctx &c, const E *mob, const double *x, double *y, in
 int mob_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy)
 {
     if (!c.mob_valid) { set_error("mobility not assembled"); return BD_ERR_STATE; }
+    if (c.mf) return mf_product(c, x, y, s, ldx, ldy, nullptr);
     if (c.mob32) return product_dispatch<float>(c, c.mob32, x, y, s, ldx, ldy, nullptr);
     return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy, nullptr);
 }
@@ -1057,6 +1069,10 @@ int mob_product(Ctx &c, const double *x, double *y, int s, int ldx, int ldy)
 int mob_product_lz(Ctx &c, const double *x, double *y, int s, int ldx, int ldy, const LzEpi *epi, const XPre *pre)
 {
     if (!c.mob_valid) { set_error("mobility not assembled"); return BD_ERR_STATE; }
+    if (c.mf) {
+        if (epi) return mf_product_lz(c, x, y, s, ldx, pre != nullptr, *epi, pre);
+        return mf_product(c, x, y, s, ldx, ldy, c.st);
+    }
     if (c.mob32) return product_dispatch<float>(c, c.mob32, x, y, s, ldx, ldy, c.st, epi, pre);
     return product_dispatch<double>(c, c.mob64, x, y, s, ldx, ldy, c.st, epi, pre);
 }
@@ -1086,6 +1102,7 @@ void prof_flush(Ctx &c)
 
 int mob_unpack(Ctx &c, double *dense, int ld)
 {
+    if (c.mf) return mf_dense(c, dense, ld);
     long tot = (long)c.n * c.n;
     if (tot == 0) return BD_OK;
     unsigned grid = (unsigned)((tot + 255) / 256);

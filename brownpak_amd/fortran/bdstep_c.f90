@@ -15,6 +15,7 @@ integer(c_int), parameter :: BD_CHOL = 0, BD_LANC = 1, BD_CHOL_LT = 2
 integer(c_int), parameter :: BD_FP64 = 0, BD_FP32 = 1
 integer(c_int), parameter :: BD_PT_DIR = 0, BD_PT_VER = 1, BD_PT_CL = 2
 integer(c_int), parameter :: BD_EM = 0, BD_PC = 1
+integer(c_int), parameter :: BD_MOB_AUTO = 0, BD_MOB_TILES = 1, BD_MOB_MATFREE = 2
 
 type, bind(C) :: bd_opts_t
     integer(c_int) :: lhdia = 1
@@ -29,6 +30,7 @@ type, bind(C) :: bd_opts_t
     real(c_double) :: flow_rate = 0.0_c_double
     integer(c_int) :: lanc_reorth = 0
     integer(c_int) :: integrator = 0
+    integer(c_int) :: mob_storage = BD_MOB_AUTO
 end type
 
 type, bind(C) :: bd_runinfo_t

@@ -81,6 +81,15 @@ enum {
  * The reference's semi-implicit 'SE' (NITSOL) is not provided. */
 enum { BD_EM = 0, BD_PC = 1 };
 
+/* mob_storage: how the mobility products D.x (dsymv/dsymm('L'),
+ * bd_solver_m.f90:446, brown_m.f90:232, 386) get D.  BD_MOB_TILES stores the
+ * lower triangle of calc_rpy_tensor's matrix (bd_solver_m.f90:458-522) in
+ * 64 x 64 tiles and streams it once per product; BD_MOB_MATFREE stores only
+ * the coordinates and evaluates every 3 x 3 RPY block inside the product
+ * (same formula, no n x n storage); BD_MOB_AUTO takes the one measured to be
+ * faster for the bead count and precision. */
+enum { BD_MOB_AUTO = 0, BD_MOB_TILES = 1, BD_MOB_MATFREE = 2 };
+
 /* mth_ptgen (pairtab_m.fpp:23-24) */
 enum { BD_PT_DIR = 0, BD_PT_VER = 1, BD_PT_CL = 2 };
 
@@ -102,6 +111,7 @@ typedef struct {
                              loses orthogonality; see brown_m.f90:402-435)     */
     int integrator;       /* BD_EM (integrate_em, bd_solver_m.f90:216-238) or
                              BD_PC: predictor-corrector, not in the reference  */
+    int mob_storage;      /* BD_MOB_AUTO / BD_MOB_TILES / BD_MOB_MATFREE      */
 } bd_opts;
 
 /* Context.  Replaces bds_init/brn_init (bd_solver_m.f90:50-104,

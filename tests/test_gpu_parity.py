@@ -23,6 +23,8 @@ from brownpak_amd.configs import chain_bonds, periodic_melt, random_walk_chain
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 TOL = 1e-10
+# mobility products from stored tiles and matrix-free (evaluated in place)
+STORAGES = ["tiles", "matfree"]
 
 
 def _say(*a):
@@ -51,9 +53,10 @@ def test_rpy_assembly_n100(oracle, chain100, prec):
     assert np.array_equal(D, D.T)
 
 
-def test_rpy_assembly_and_product_n2000(oracle, chain2000):
-    _say("case: rpy n2000")
-    g = gpu_system(chain2000, mob_fctr="LANC", nts_mobsam=4)
+@pytest.mark.parametrize("storage", STORAGES)
+def test_rpy_assembly_and_product_n2000(oracle, chain2000, storage):
+    _say(f"case: rpy n2000 {storage}")
+    g = gpu_system(chain2000, mob_fctr="LANC", nts_mobsam=4, mob_storage=storage)
     g.calc_rpy_tensor()
     D = g.get_mobility()
     Dref = oracle.rpy(chain2000["crd"])
@@ -65,9 +68,10 @@ def test_rpy_assembly_and_product_n2000(oracle, chain2000):
     assert rel(y1, Dref @ x[:, 0]) < 1e-13
 
 
+@pytest.mark.parametrize("storage", STORAGES)
 @pytest.mark.parametrize("s", [1, 2, 3, 4, 5, 8])
-def test_mob_symv_block_sizes(oracle, chain100, s):
-    g = gpu_system(chain100, mob_fctr="CHOL", nts_mobsam=s)
+def test_mob_symv_block_sizes(oracle, chain100, s, storage):
+    g = gpu_system(chain100, mob_fctr="CHOL", nts_mobsam=s, mob_storage=storage)
     g.calc_rpy_tensor()
     Dref = oracle.rpy(chain100["crd"])
     x = np.random.default_rng(s).standard_normal((300, s))
@@ -87,11 +91,12 @@ def test_cholesky_bdw(oracle, chain100, method):
 
 
 # ------------------------------------------------------------- Lanczos
+@pytest.mark.parametrize("storage", STORAGES)
 @pytest.mark.parametrize("nb,s,mx", [(100, 1, 12), (100, 4, 7), (100, 3, 7), (2000, 4, 7)])
-def test_lanczos_fixed_iterations(oracle, chain100, chain2000, nb, s, mx):
-    _say(f"case: lanczos fixed nb={nb} s={s}")
+def test_lanczos_fixed_iterations(oracle, chain100, chain2000, nb, s, mx, storage):
+    _say(f"case: lanczos fixed nb={nb} s={s} {storage}")
     m = chain100 if nb == 100 else chain2000
-    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=s, lanc_mxitr=mx, lanc_tol=0.0)
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=s, lanc_mxitr=mx, lanc_tol=0.0, mob_storage=storage)
     g.calc_rpy_tensor()
     xi = np.random.default_rng(nb + s).standard_normal((3 * nb, s))
     out, lc, it = g.brn_calc_bdw(xi)
@@ -101,11 +106,13 @@ def test_lanczos_fixed_iterations(oracle, chain100, chain2000, nb, s, mx):
     assert rel(out, ref) < TOL
 
 
+@pytest.mark.parametrize("storage", STORAGES)
 @pytest.mark.parametrize("nb,s", [(100, 1), (100, 4), (2000, 4)])
-def test_lanczos_reorth_converged(oracle, chain100, chain2000, nb, s):
-    _say(f"case: lanczos reorth nb={nb} s={s}")
+def test_lanczos_reorth_converged(oracle, chain100, chain2000, nb, s, storage):
+    _say(f"case: lanczos reorth nb={nb} s={s} {storage}")
     m = chain100 if nb == 100 else chain2000
-    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, lanc_reorth=True)
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, lanc_reorth=True,
+                   mob_storage=storage)
     g.calc_rpy_tensor()
     xi = np.random.default_rng(nb * s).standard_normal((3 * nb, s))
     out, lc, it = g.brn_calc_bdw(xi)
@@ -116,11 +123,12 @@ def test_lanczos_reorth_converged(oracle, chain100, chain2000, nb, s):
     assert rel(out, ref) < TOL
 
 
+@pytest.mark.parametrize("storage", STORAGES)
 @pytest.mark.parametrize("nb,s", [(100, 1), (100, 4), (2000, 4)])
-def test_lanczos_reference_converged(oracle, chain100, chain2000, nb, s):
-    _say(f"case: lanczos reference nb={nb} s={s}")
+def test_lanczos_reference_converged(oracle, chain100, chain2000, nb, s, storage):
+    _say(f"case: lanczos reference nb={nb} s={s} {storage}")
     m = chain100 if nb == 100 else chain2000
-    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6)
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, mob_storage=storage)
     g.calc_rpy_tensor()
     xi = np.random.default_rng(nb + 7 * s).standard_normal((3 * nb, s))
     out, lc, it = g.brn_calc_bdw(xi)
@@ -168,13 +176,14 @@ def test_lanczos_not_converged_flag(chain100):
     assert not lc and it == 3 and np.all(np.isfinite(out))
 
 
-def test_mixed_precision_error_n2000(chain2000):
-    _say("case: mixed precision n2000")
+@pytest.mark.parametrize("storage", STORAGES)
+def test_mixed_precision_error_n2000(chain2000, storage):
+    _say(f"case: mixed precision n2000 {storage}")
     xi = np.random.default_rng(5).standard_normal((6000, 4))
     outs = {}
     for prec in ("fp64", "fp32"):
         g = gpu_system(chain2000, mob_fctr="LANC", nts_mobsam=4, lanc_mxitr=50, lanc_tol=1e-6,
-                       precision=prec, lanc_reorth=True)
+                       precision=prec, lanc_reorth=True, mob_storage=storage)
         g.calc_rpy_tensor()
         outs[prec] = g.brn_calc_bdw(xi)
         f = np.random.default_rng(6).standard_normal(6000)
@@ -332,11 +341,13 @@ def test_bd_run_cholesky_c1(oracle, chain100):
     assert rel(np.array(ri["molecule_com"]), np.array(info.molecule_com)) < 1e-8
 
 
+@pytest.mark.parametrize("storage", STORAGES)
 @pytest.mark.parametrize("keep", [False, True])
-def test_bd_run_block_lanczos_reorth(oracle, chain100, keep):
+def test_bd_run_block_lanczos_reorth(oracle, chain100, keep, storage):
     nsteps, s = 8, 4
     xi = np.random.default_rng(42).standard_normal(300 * s * 2)
-    g = gpu_system(chain100, mob_fctr="LANC", nts_mobsam=s, lanc_reorth=True, keep_reuse_noise=keep)
+    g = gpu_system(chain100, mob_fctr="LANC", nts_mobsam=s, lanc_reorth=True, keep_reuse_noise=keep,
+                   mob_storage=storage)
     g.bds_run(nsteps, xi)
     out = g.get_coordinates()
     so = oracle_system(oracle, chain100)
