@@ -226,7 +226,7 @@ __global__ __launch_bounds__(BT) void b_qr0_kernel(const double *xi, double *V, 
     const int c = blockIdx.x;
     const long r0 = ch[c].row;
     if (!b_cholqr2<S>(xi + r0, V + r0, NP, ch[c].npad, R0 + (long)c * S * S) && threadIdx.x == 0)
-        atomicExch(&counters[1], 2);
+        atomicCAS(&counters[1], 0, 2);
 }
 
 // One Krylov step for every active chain (one block per chain):
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const doub
     const sp::SmallOut o =
         sp::small_problem<S>(Ab, Bb, R0 + (long)c * S * S, j, BNBOUND, BDEC, scr + c * scr_stride, BNQP, Cc);
     if (o.err) {
-        if (threadIdx.x == 0) atomicExch(&counters[1], o.err);
+        if (threadIdx.x == 0) atomicCAS(&counters[1], 0, o.err);
         return;
     }
     __syncthreads();
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(256) void b_force_kernel(const double *__restrict__
         double x = __dsub_rn(cr[3 * j], cr[3 * i]), y = __dsub_rn(cr[3 * j + 1], cr[3 * i + 1]),
                z = __dsub_rn(cr[3 * j + 2], cr[3 * i + 2]);
         double r = sqrt(rsq3(x, y, z)), en, fr;
-        if (bond_force(r, bsty, bp, en, fr)) atomicExch(&counters[1], 1);
+        if (bond_force(r, bsty, bp, en, fr)) atomicCAS(&counters[1], 0, 1);
         f[0] += fr * x / r;
         f[1] += fr * y / r;
         f[2] += fr * z / r;
@@ -449,12 +449,14 @@ __global__ __launch_bounds__(256) void b_force_kernel(const double *__restrict__
     frc[o] = f[0]; frc[o + 1] = f[1]; frc[o + 2] = f[2];
 }
 
-// r += dt * (D F) + diff (diff may be null); one thread per bead.
+// r += dt * (D F) + diff (diff may be null); one thread per bead.  Nothing
+// moves once an error is flagged (counters[1]): bds_run returns before the
+// update of the failing step (bd_solver_m.f90:174).
 __global__ void b_update_kernel(double *crd, const double *dfr, const double *diff, int NB, const long *bead_row,
-                                double dt)
+                                double dt, const int *counters)
 {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= NB) return;
+    if (g >= NB || counters[1] != 0) return;
     long o = bead_row[g];
     for (int d = 0; d < 3; d++) {
         double r = crd[o + d] + dt * dfr[o + d];
@@ -464,8 +466,9 @@ __global__ void b_update_kernel(double *crd, const double *dfr, const double *di
 }
 
 // centre of mass to the origin, one block per chain (simbox_m.f90:182-202)
-__global__ __launch_bounds__(256) void b_com_kernel(double *crd, const BChainDev *ch)
+__global__ __launch_bounds__(256) void b_com_kernel(double *crd, const BChainDev *ch, const int *counters)
 {
+    if (counters[1] != 0) return;
     const int c = blockIdx.x;
     double *cr = crd + ch[c].row;
     const int nb = ch[c].nb;
@@ -651,8 +654,8 @@ static int b_step_tail(Batch &b, int k, bool noise)
                                         nullptr, sm, b.device, nullptr, nullptr)));
     const double *dcol = noise ? b.diff + (size_t)(k - 1) * NP : nullptr;
     hipLaunchKernelGGL(b_update_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.dfr, dcol, b.NB, b.bead_row,
-                       b.opt.tim_stp);
-    hipLaunchKernelGGL(b_com_kernel, dim3(b.C), dim3(256), 0, sm, b.crd, b.chains);
+                       b.opt.tim_stp, b.counters);
+    hipLaunchKernelGGL(b_com_kernel, dim3(b.C), dim3(256), 0, sm, b.crd, b.chains, b.counters);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }
@@ -705,6 +708,11 @@ static int b_step(Batch &b, const double *xi_host, long long avail, long long *c
     b.cntr++;
     BD_TRY(b_step_tail(b, b.cntr, noise));
     if (b.cntr == s) b.cntr = 0;
+    // a bond error of this step ends the run here, as in bds_run
+    BD_HIP(hipMemcpyAsync(b.host_counters + 1, b.counters + 1, sizeof(int), hipMemcpyDeviceToHost, sm));
+    BD_HIP(hipStreamSynchronize(sm));
+    if (b.host_counters[1] == 1) { set_error("bond length too large (overstretched bond)"); return BD_ERR_BOND; }
+    if (b.host_counters[1] != 0) { set_error("batched Lanczos failed (matrix not SPD?)"); return BD_ERR_FACTOR; }
     b.nts++;
     return BD_OK;
 }
@@ -967,7 +975,9 @@ int bd_batch_run(bd_batch *h, int nsteps, const double *xi, long long nxi, long 
     Batch &b = h->b;
     if (!b.have_model) { set_error("model not set"); return BD_ERR_STATE; }
     long long pos = 0;
-    hipLaunchKernelGGL(b_com_kernel, dim3(b.C), dim3(256), 0, b.stream, b.crd, b.chains);
+    // a fresh call: no error pending (a failed call leaves its code until here)
+    BD_HIP(hipMemsetAsync(b.counters + 1, 0, sizeof(int), b.stream));
+    hipLaunchKernelGGL(b_com_kernel, dim3(b.C), dim3(256), 0, b.stream, b.crd, b.chains, b.counters);
     int rc = BD_OK;
     // whole mobility blocks with device noise run as one graph launch each;
     // the rest (injected xi, partial blocks, profiling) step by step
@@ -984,9 +994,10 @@ int bd_batch_run(bd_batch *h, int nsteps, const double *xi, long long nxi, long 
             BD_HIP(hipMemcpyAsync(b.host_counters, b.counters, 4 * sizeof(int), hipMemcpyDeviceToHost, b.stream));
             BD_HIP(hipStreamSynchronize(b.stream));
             for (int q = 0; q < 4; q++) cnt[q] = b.host_counters[q];
-            if (cnt[1] >= 2) {
-                // a Lanczos failure: back to the block's start, replayed step by
-                // step so that the error and the state are the eager path's
+            if (cnt[1] != 0) {
+                // an overstretched bond or a Lanczos failure: back to the
+                // block's start, replayed step by step so that the error, the
+                // step count and the state are the eager path's
                 const int zero[4] = {0, 0, 0, 0};
                 BD_HIP(hipMemcpy(b.crd, b.crd_snap, (size_t)b.NP * sizeof(double), hipMemcpyDeviceToDevice));
                 BD_HIP(hipMemcpy(b.dev_rng, b.dev_rng + 1, sizeof(uint64_t), hipMemcpyDeviceToDevice));

@@ -59,6 +59,7 @@ struct DevStatus {
     int steps_done;    // EM steps completed since the last bd_run started
     int nrebuild;      // Verlet list rebuilds
     int ycnt;          // blocks of the Lanczos combine kernel done (last one finalises)
+    int fail_j;        // Lanczos: smallest Krylov block whose CholeskyQR failed (0: none)
     double err;        // last Lanczos relative change
     double lam_lo, lam_hi;  // spectral bounds of the last small problem
 };
@@ -93,6 +94,7 @@ struct LzEpi {
 // x) and B = R2 R1 (R1 the first pass's factor); a failed factorisation
 // sets st->ierr = 2.  gpart == nullptr: x is used as it is.
 struct XPre {
+    int blk = 0;   // index j of the block V_j being completed (for fail_j)
     const double *gpart = nullptr;
     int ng = 0;
     const double *R1 = nullptr;

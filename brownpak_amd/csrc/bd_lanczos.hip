@@ -146,7 +146,7 @@ template <int S>
 __global__ __launch_bounds__(RB) void cholqr_kernel(const double *__restrict__ X, double *__restrict__ Q,
                                                     int ld, int rows, const double *__restrict__ gpart,
                                                     double *__restrict__ gpart_out, double *__restrict__ Rsave,
-                                                    const double *__restrict__ rprev, DevStatus *st)
+                                                    const double *__restrict__ rprev, DevStatus *st, int blk)
 {
     __shared__ double sG[S * S];
     __shared__ int s_done;
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(RB) void cholqr_kernel(const double *__restrict__ X
     double R[S][S];
     bool ok = small_chol<S>(sG, R);
     if (!ok) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(&st->ierr, 2);
+        if (blockIdx.x == 0 && threadIdx.x == 0) lz_factor_failed(st, blk);
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(KT) void krylov_fused_kernel(const double *__restri
                                                           const double *__restrict__ Vp, const double *__restrict__ Bp,
                                                           double *__restrict__ Vn, int ld, int rows,
                                                           double *__restrict__ Aout, double *__restrict__ Bout,
-                                                          int upd, DevStatus *st)
+                                                          int upd, DevStatus *st, int blk)
 {
     // the S x S matrices live in LDS and are read as broadcasts, which keeps
     // the registers for the rows
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(KT) void krylov_fused_kernel(const double *__restri
     }
     __syncthreads();
     if (s_fail) {
-        if (threadIdx.x == 0) atomicExch(&st->ierr, 2);
+        if (threadIdx.x == 0) lz_factor_failed(st, blk);
         return;
     }
 #pragma unroll
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(KT) void krylov_fused_kernel(const double *__restri
     }
     __syncthreads();
     if (s_fail) {
-        if (threadIdx.x == 0) atomicExch(&st->ierr, 2);
+        if (threadIdx.x == 0) lz_factor_failed(st, blk);
         return;
     }
 #pragma unroll
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict_
                                                        const double *__restrict__ lzpart, int nblk,
                                                        double *__restrict__ Aout, double *__restrict__ R1save,
                                                        int upd, double *__restrict__ gpart, DevStatus *st,
-                                                       float *__restrict__ qrec32, double *__restrict__ qrec64)
+                                                       float *__restrict__ qrec32, double *__restrict__ qrec64, int blk)
 {
     __shared__ int s_done, s_fail;
     __shared__ double sAP[2 * S * S], sAh[S * S], sG[S * S], sR[S * S], sI[S], s_scr[RB];
@@ -454,7 +454,7 @@ __global__ __launch_bounds__(RB) void krylov_k2_kernel(const double *__restrict_
     }
     __syncthreads();
     if (s_fail) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(&st->ierr, 2);
+        if (blockIdx.x == 0 && threadIdx.x == 0) lz_factor_failed(st, blk);
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x < S * S) R1save[threadIdx.x] = sR[threadIdx.x];
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(RB) void reorth_coef_kernel(const double *__restric
 __global__ void reorth_sum_kernel(const double *__restrict__ part, int ne, double *__restrict__ coef,
                                   const DevStatus *st)
 {
-    if (st->converged_j != 0 || st->ierr != 0) return;
+    if (lz_done(st)) return;
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     double v = 0.0;
@@ -597,7 +597,7 @@ __global__ __launch_bounds__(256) void small_kernel(const double *__restrict__ A
                                                     double *__restrict__ Cout, DevStatus *st)
 {
     __shared__ int s_done;
-    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    if (threadIdx.x == 0) s_done = (lz_tests_done(st) || !lz_valid(st, nb)) ? 1 : 0;
     __syncthreads();
     if (s_done) return;
     const sp::SmallOut o = sp::small_problem<S>(Ablk, Bblk, R0, nb, NBOUND, 13, scr, 256, Cout);
@@ -618,7 +618,7 @@ __global__ __launch_bounds__(sp::WG_T) void small_split_kernel(const double *__r
                                                                sp::SplitState *ss, DevStatus *st)
 {
     __shared__ int s_done;
-    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    if (threadIdx.x == 0) s_done = (lz_tests_done(st) || !lz_valid(st, nb)) ? 1 : 0;
     __syncthreads();
     if (s_done) return;
     const bool fin = sp::small_problem_split<S>(Ablk, Bblk, R0, nb, NBOUND, 13, part, Cout, ss, ss, 1, nb & 1);
@@ -655,7 +655,7 @@ __global__ __launch_bounds__(sp::WG_T) void small_group_kernel(const double *__r
 {
     const int j = j0 + (int)blockIdx.y;
     __shared__ int s_done;
-    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    if (threadIdx.x == 0) s_done = lz_tests_done(st) ? 1 : 0;
     __syncthreads();
     if (s_done || j > jend || j < 2) return;
     sp::SplitState *ss = ssv + blockIdx.y;
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(sp::WG_T) void small_group_kernel(const double *__r
         st->lam_hi = ss->lam_hi;
         st->lam_lo = ss->lam_lo;
         st->nq = ss->nq;
-        if (ss->err) atomicExch(&st->ierr, 2);
+        if (ss->err && lz_valid(st, j)) atomicExch(&st->ierr, 2);
     }
 }
 
@@ -700,7 +700,7 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
     __shared__ int s_done, s_ok[WAVES], s_last;
     const int j = j0 + (int)blockIdx.x;
     // the blocks of H_j are loaded together with the status
-    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    if (threadIdx.x == 0) s_done = lz_tests_done(st) ? 1 : 0;
     const int nbl = (j >= 2 && j <= jend) ? j : 0;
     for (int e = threadIdx.x; e < nbl * SS; e += THREADS) {
         sA[e] = Ablk[e];
@@ -830,7 +830,7 @@ __global__ __launch_bounds__(pcr::THREADS) void small_pcr_kernel(const double *_
         st->lam_hi = M;
         st->lam_lo = m;
         st->nq = nq;
-        if (err) atomicExch(&st->ierr, 2);
+        if (err && lz_valid(st, j)) atomicExch(&st->ierr, 2);
     }
     if (err) return;
     double *sZ = s_buf;
@@ -878,7 +878,7 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
     __shared__ double s_np2[LG * 2 * S][256 / (LG * 2 * S)];
     const int ng = jend - j0 + 1;               // members (<= LG)
     const int jsmax = jend * S;
-    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    if (threadIdx.x == 0) s_done = lz_tests_done(st) ? 1 : 0;
     // this thread's entries of V (one column range of at most YC columns:
     // jsmax <= YJS <= YG * YC) and of y_{j0-1}, loaded before the
     // coefficients so that the memory round trips overlap
@@ -1026,6 +1026,11 @@ __global__ __launch_bounds__(256) void ycomb_group_kernel(const double *__restri
     st->last_j = jend;
     for (int g = 0; g < ng; g++) {
         const int j = j0 + g;
+        if (!lz_valid(st, j)) {
+            // the recurrence could not form V_j: no converged iteration before it
+            st->ierr = 2;
+            break;
+        }
         if (j <= 2) continue;
         double num = 0.0, den = 0.0;
         if (two_norm) {
@@ -1060,7 +1065,7 @@ __global__ __launch_bounds__(256) void ycomb_kernel(const double *__restrict__ V
     __shared__ double s_nrm[2 * S];
     __shared__ double s_C[512 * S];
     __shared__ double s_scr[256];
-    if (threadIdx.x == 0) s_done = lz_done(st) ? 1 : 0;
+    if (threadIdx.x == 0) s_done = lz_tests_done(st) ? 1 : 0;
     for (int b0 = threadIdx.x; b0 < js * S; b0 += 8 * (int)blockDim.x) {
         double v[8];
 #pragma unroll
@@ -1131,6 +1136,10 @@ __global__ __launch_bounds__(256) void ycomb_kernel(const double *__restrict__ V
     if (threadIdx.x != 0) return;
     st->ycnt = 0;
     st->last_j = j;
+    if (!lz_valid(st, j)) {
+        st->ierr = 2;
+        return;
+    }
     if (!want_err) return;
     double num = 0.0, den = 0.0;
     if (two_norm) {
@@ -1147,6 +1156,7 @@ __global__ __launch_bounds__(256) void ycomb_kernel(const double *__restrict__ V
 
 __global__ void status_reset_kernel(DevStatus *st, sp::SplitState *ss)
 {
+    st->fail_j = 0;
     st->converged_j = 0;
     st->last_j = 0;
     st->err = 0.0;
@@ -1302,9 +1312,9 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     // xi = V_1 R0 (CholeskyQR2)
     hipLaunchKernelGGL((gram_kernel<S>), dim3(gb), dim3(RB), 0, sm, xi, xi, ld, rows, g0, c.st);
     hipLaunchKernelGGL((cholqr_kernel<S>), dim3(gb), dim3(RB), 0, sm, xi, c.V, ld, rows, g0, g1,
-                       c.R0 + S * S, (const double *)nullptr, c.st);
+                       c.R0 + S * S, (const double *)nullptr, c.st, 1);
     hipLaunchKernelGGL((cholqr_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.V, c.V, ld, rows, g1,
-                       (double *)nullptr, c.R0, (const double *)(c.R0 + S * S), c.st);
+                       (double *)nullptr, c.R0, (const double *)(c.R0 + S * S), c.st, 1);
     BD_HIP(hipGetLastError());
 
     const int chunk = LG;
@@ -1343,6 +1353,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             LzEpi epi{Vj, Vp, Bp, c.lzpart};
             XPre pre;
             if (j > 1) {
+                pre.blk = j;
                 pre.gpart = g0;
                 pre.ng = gb;
                 pre.R1 = c.R0 + S * S;
@@ -1354,7 +1365,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             hipLaunchKernelGGL((krylov_k2_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.W, Vj, c.Qbuf, ld, rows,
                                c.lzpart, mob_rowblocks(c), c.Ablk + (size_t)(j - 1) * S * S, c.R0 + S * S, upd, g0,
                                c.st, (c.mf && c.opt.precision == BD_FP32) ? (float *)c.mf_x : nullptr,
-                               (c.mf && c.opt.precision != BD_FP32) ? (double *)c.mf_x : nullptr);
+                               (c.mf && c.opt.precision != BD_FP32) ? (double *)c.mf_x : nullptr, j + 1);
             BD_HIP(hipGetLastError());
         } else {
             BD_TRY(mob_product_lz(c, Vj, c.W, S, ld, ld));
@@ -1363,7 +1374,7 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
         } else if (fused) {
             hipLaunchKernelGGL((krylov_fused_kernel<S>), dim3(1), dim3(KT), 0, sm, c.W, Vj, Vp, Bp,
                                c.V + (size_t)j * S * ld, ld, rows, c.Ablk + (size_t)(j - 1) * S * S,
-                               c.Bblk + (size_t)(j - 1) * S * S, upd, c.st);
+                               c.Bblk + (size_t)(j - 1) * S * S, upd, c.st, j + 1);
             BD_HIP(hipGetLastError());
         } else {
             hipLaunchKernelGGL((lanczos_a_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.W, Vj, Vp, Bp, ld, rows, g0,
@@ -1390,9 +1401,9 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
             double *Vn = c.V + (size_t)j * S * ld;
             double *Bj = c.Bblk + (size_t)(j - 1) * S * S;
             hipLaunchKernelGGL((cholqr_kernel<S>), dim3(gb), dim3(RB), 0, sm, c.W, Vn, ld, rows, g1, g0,
-                               c.R0 + S * S /* R1 scratch */, (const double *)nullptr, c.st);
+                               c.R0 + S * S /* R1 scratch */, (const double *)nullptr, c.st, j + 1);
             hipLaunchKernelGGL((cholqr_kernel<S>), dim3(gb), dim3(RB), 0, sm, Vn, Vn, ld, rows, g0,
-                               (double *)nullptr, Bj, (const double *)(c.R0 + S * S), c.st);
+                               (double *)nullptr, Bj, (const double *)(c.R0 + S * S), c.st, j + 1);
         }
         BD_HIP(hipGetLastError());
         const bool group_end = grouped && (j == dmax || j - jg_start + 1 >= LG || in_window(j) || in_window(j + 1));
@@ -1490,7 +1501,10 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
         set_error("bond length too large (overstretched bond)");
         return BD_ERR_BOND;
     }
-    if (c.st_host->ierr) {
+    // a CholeskyQR failure beyond the converged iteration belongs to an
+    // iteration the reference never runs (the tests stop at convergence and
+    // report failures only before it)
+    if (c.st_host->ierr || (c.st_host->converged_j == 0 && c.st_host->fail_j != 0)) {
         set_error("Lanczos small problem or CholeskyQR failed (matrix not SPD?)");
         return BD_ERR_FACTOR;
     }

@@ -15,7 +15,30 @@ static __constant__ int c_qN[14] = {20, 20, 27, 34, 42, 50, 57, 66, 79, 87, 97, 
 static __constant__ double c_qh[14] = {0.205, 0.205, 0.155, 0.125, 0.1, 0.085, 0.075,
                                 0.065, 0.055, 0.05, 0.045, 0.04, 0.04, 0.035};
 
-static __device__ inline bool lz_done(const DevStatus *st) { return st->converged_j != 0 || st->ierr != 0; }
+// Main-stream kernels of a Lanczos solve stop once it has converged, failed,
+// or a block of its Krylov recurrence could not be formed; the tests on the
+// side stream go on through the iterations before that block.
+static __device__ inline bool lz_done(const DevStatus *st)
+{
+    return st->converged_j != 0 || st->ierr != 0 || st->fail_j != 0;
+}
+static __device__ inline bool lz_tests_done(const DevStatus *st) { return st->converged_j != 0 || st->ierr != 0; }
+// Iteration j of a solve is valid (its blocks V_1..V_j exist).
+static __device__ inline bool lz_valid(const DevStatus *st, int j) { return st->fail_j == 0 || j < st->fail_j; }
+// A CholeskyQR of the Krylov recurrence failed while forming block blk.  The
+// recurrence runs ahead of the convergence tests; the tests go on through
+// iteration blk - 1, and the solve fails only if none of them converged
+// (a converged iteration stands, as in brown_m.f90:497-506, which stops
+// there and never forms the later blocks).
+static __device__ inline void lz_factor_failed(DevStatus *st, int blk)
+{
+    int old = st->fail_j;
+    while (old == 0 || blk < old) {
+        const int prev = atomicCAS(&st->fail_j, old, blk);
+        if (prev == old) break;
+        old = prev;
+    }
+}
 
 // The last-block pattern: every workgroup stores its partials with
 // device-scope atomic stores (coherent across the XCDs' L2s without any

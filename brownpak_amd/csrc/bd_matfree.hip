@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void mf_reduce_lz_kernel(const double *__restr
         }
         __syncthreads();
         if (s_fail) {
-            if (G == 0 && tid == 0) atomicCAS(&st->ierr, 0, 2);
+            if (G == 0 && tid == 0) lz_factor_failed(st, pre.blk);
             return;
         }
     } else if (Vp && tid < S * S) {
@@ -429,13 +429,18 @@ __global__ void mf_dense_kernel(const double *__restrict__ crec, int nb, double 
 }
 
 // ---------------------------------------------------------------- host
-// BD_MOB_AUTO: matrix-free from the bead count at which it was measured to
-// beat the stored tiles (both precisions, one MI355X).
+// BD_MOB_AUTO.  FP64: matrix-free from the bead count at which it was
+// measured to beat the stored FP64 tiles.  FP32: stored tiles (FP32 storage,
+// FP64 arithmetic), whose D.X error (~2e-8 relative, one rounding per entry)
+// is a quarter of the FP32 matrix-free product's (~8e-8, several roundings
+// per pair); the vector ALUs run FP32 and FP64 at one rate here (no packed
+// FP32), so the FP32 matrix-free product is no faster than the FP64 one.
 bool mob_use_matfree(const bd_opts &o, int nbeads)
 {
     if (o.mob_storage == BD_MOB_TILES) return false;
     if (o.mob_storage == BD_MOB_MATFREE) return true;
     if (o.mob_fctr != BD_LANC) return false;   // the Cholesky path factors a stored matrix
+    if (o.precision == BD_FP32) return false;
     return nbeads >= 64;
 }
 

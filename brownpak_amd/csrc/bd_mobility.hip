@@ -191,7 +191,7 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
     // inside a Lanczos solve: nothing to do once it has converged (the
     // status words are read together with the first unit's record, and
     // tested once both have arrived)
-    const int st_conv = stop ? stop->converged_j : 0, st_err = stop ? stop->ierr : 0;
+    const int st_conv = stop ? stop->converged_j : 0, st_err = stop ? (stop->ierr | stop->fail_j) : 0;
     // second CholeskyQR pass of the block (XPre): R2 from the Gram partials,
     // by every block (the same fixed-order sum and Cholesky as cholqr_kernel;
     // done before the unit loop: inside it, with the first tiles' registers
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(NT, (OCC<E, S, CHR>)) void symm_tiles_kernel(
         }
         __syncthreads();
         if (s_fail) {
-            if (blockIdx.x == 0 && tid == 0) atomicExch(&pre.st->ierr, 2);
+            if (blockIdx.x == 0 && tid == 0) lz_factor_failed(pre.st, pre.blk);
             return;
         }
     }
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(TILE * MAX_S) void symm_reduce_lz_kernel(
     const bool act = hf == 0;                     // the set that finishes the block
     // the block's record and the status are read together
     const BlockRec br = block_rec(rowblk + 8 * G);
-    if (threadIdx.x == 0) s_done = (stop && (stop->converged_j != 0 || stop->ierr != 0)) ? 1 : 0;
+    if (threadIdx.x == 0) s_done = (stop && lz_done(stop)) ? 1 : 0;
     __syncthreads();
     if (s_done) return;
     const int r = e / S, k = e % S;

@@ -87,3 +87,61 @@ def test_autograd_function(c5):
     g.set_theta(*THETA0)
     _, grad = g.adjoint_rgsq(disp, ckpt=CKPT)
     assert np.allclose(theta.grad.numpy(), grad, rtol=1e-12, atol=0)
+
+
+# ---------------------------------------------------------------- FP32
+# The configuration as specified: FP32 mobility products (drift D.F and the
+# adjoint's D.mu), FP64 Lanczos tridiagonal, FP64 coordinates and adjoint
+# accumulation.  Bounds: the FP32 products carry ~1e-7 relative error, which
+# the 200-step trajectory carries into the loss (< 1e-5 relative against the
+# oracle's FP64 replay) and the gradient (< 1e-4 against the oracle's FP64
+# central differences).  Central differences of the FP32 forward itself
+# need a step large enough that FP32 rounding of the loss (~1e-7) does not
+# dominate: h = 1e-3 gives about 1e-4 relative from rounding and 1e-6 from
+# truncation; bound 1e-3.
+H32 = 1e-3
+
+
+@pytest.fixture(scope="module")
+def c5_fp32():
+    m = fene_wca_chain(NB, 256)
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=S, lanc_mxitr=50, lanc_tol=1e-6, lanc_reorth=True,
+                   precision="fp32")
+    g.set_rng(2025)
+    disp = g.run_record(NSTEPS)
+    return m, g, disp
+
+
+def test_adjoint_fp32_matches_oracle_fp64_gradient(oracle, c5_fp32):
+    m, g, disp = c5_fp32
+    g.set_coordinates(m["crd"])
+    g.reset_run()
+    g.set_theta(*THETA0)
+    loss, grad = g.adjoint_rgsq(disp, ckpt=CKPT)
+    lref = _oracle_loss(oracle, m, disp, THETA0)
+    fd = np.zeros(3)
+    for a in range(3):
+        tp, tm = THETA0.copy(), THETA0.copy()
+        tp[a] += H
+        tm[a] -= H
+        fd[a] = (_oracle_loss(oracle, m, disp, tp) - _oracle_loss(oracle, m, disp, tm)) / (2 * H)
+    print(f"fp32 loss rel {abs(loss - lref) / lref:.2e}; adjoint {grad} oracle fd {fd} rel {rel(grad, fd):.2e}",
+          flush=True)
+    assert abs(loss - lref) < 1e-5 * lref
+    assert rel(grad, fd) < 1e-4
+
+
+def test_adjoint_fp32_matches_fp32_finite_differences(c5_fp32):
+    m, g, disp = c5_fp32
+    g.set_coordinates(m["crd"])
+    g.reset_run()
+    g.set_theta(*THETA0)
+    loss, grad = g.adjoint_rgsq(disp, ckpt=CKPT)
+    fd = np.zeros(3)
+    for a in range(3):
+        tp, tm = THETA0.copy(), THETA0.copy()
+        tp[a] += H32
+        tm[a] -= H32
+        fd[a] = (_gpu_loss(m, g, disp, tp) - _gpu_loss(m, g, disp, tm)) / (2 * H32)
+    print(f"fp32 adjoint {grad} fp32 fd {fd} rel {rel(grad, fd):.2e}", flush=True)
+    assert rel(grad, fd) < 1e-3

@@ -24,12 +24,14 @@ def _say(*a):
     print(*a, flush=True)
 
 
-def _batch(lengths, seed, **opts):
+def _batch(lengths, seed, overlap_free=False, **opts):
     from brownpak_amd import BdBatch
+    from brownpak_amd.configs import overlap_free_chain
     b = BdBatch(lengths, **opts)
     bond, vdw = fene_wca_params()
     b.set_model(2, bond, lvdw=True, vdw_params=vdw, rcutoff=vdw[2], excluded_atoms=0)
-    chains = [random_walk_chain(int(nb), seed + c) for c, nb in enumerate(lengths)]
+    gen = overlap_free_chain if overlap_free else random_walk_chain
+    chains = [gen(int(nb), seed + c) for c, nb in enumerate(lengths)]
     b.set_coordinates(chains)
     return b, chains
 
@@ -197,16 +199,82 @@ def test_batch_graph_blocks_match_step_by_step():
     assert infos[0]["nts"] == 14 and infos[0]["mob_updates"] == 4
 
 
-def test_batch_overstretch_reports_bond_error():
+@pytest.mark.parametrize("path", ["graph", "eager"])
+def test_batch_overstretch_reports_bond_error(path):
+    """A bond beyond rmax at the first step: BD_ERR_BOND, no step counted,
+    and every chain left where the failing step found it (bds_run returns
+    before the update, bd_solver_m.f90:174; the centre of mass was moved to
+    the origin once before the loop, bd_solver_m.f90:161)."""
     from brownpak_amd import BdError
-    lengths = [20, 30]
-    b, chains = _batch(lengths, 77, nts_mobsam=1, lanc_mxitr=20, lanc_tol=1e-6)
+    lengths = [20, 30, 25]
+    b, chains = _batch(lengths, 77, nts_mobsam=4, lanc_mxitr=20, lanc_tol=1e-6)
     chains[1][15:] += np.array([6.0, 0.0, 0.0])
     b.set_coordinates(chains)
-    xi = np.random.default_rng(5).standard_normal(3 * sum(lengths) * 2)
     with pytest.raises(BdError) as ei:
-        b.run(2, xi)
+        if path == "graph":
+            b.set_rng(5, 0)
+            b.run(8)
+        else:
+            b.run(8, np.random.default_rng(5).standard_normal(3 * sum(lengths) * 4 * 2))
     assert ei.value.code == 1
+    assert b.info()["nts"] == 0
+    for o, c in zip(b.get_coordinates(), chains):
+        assert np.allclose(o, c - c.mean(axis=0), rtol=0, atol=1e-12)
+
+
+C3_CHAINS = 512
+
+
+def test_batch_c3_full_size(oracle):
+    """The dilute-ensemble configuration at its full size: 512 chains with
+    lengths from U[50, 800] (seed 3), FENE/WCA/RPY, block Lanczos s = 4,
+    tol 1e-6, FP32 tiles.  (1) Device noise, whole mobility blocks as HIP
+    graph launches, 8 steps: every chain finite, centre of mass at the
+    origin, every solve converged.  (2) Injected xi, one mobility block:
+    six sampled chains bit-identical to the same chain run alone, and three
+    chains against the oracle at a fixed Lanczos iteration count (FP32
+    products: 1e-6)."""
+    rng = np.random.default_rng(3)
+    lengths = [int(x) for x in rng.integers(50, 801, size=C3_CHAINS)]
+    s = 4
+    _say(f"case: batch C3 full size, {sum(lengths)} beads")
+    b, chains = _batch(lengths, 1000, overlap_free=True, nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6,
+                       precision="fp32")
+    b.set_rng(11, 0)
+    b.run(8)
+    out = b.get_coordinates()
+    info = b.info()
+    assert info["nts"] == 8 and info["mob_updates"] == 2 and info["n_not_converged"] == 0
+    for o in out:
+        assert np.isfinite(o).all()
+        assert np.abs(o.mean(axis=0)).max() < 1e-9
+    # (2) injected xi, chain independence and oracle parity at fixed iterations
+    shape = b.tile_shape()[0]
+    tot = sum(3 * nb * s for nb in lengths)
+    xi = rng.standard_normal(tot)
+    per = _split_xi(lengths, s, 1, xi)
+    for tol, mx in ((1e-6, 50), (0.0, 5)):
+        bb, _ = _batch(lengths, 1000, overlap_free=True, nts_mobsam=s, lanc_mxitr=mx, lanc_tol=tol,
+                       precision="fp32")
+        bb.set_tile_shape(*shape)
+        bb.run(s, xi)
+        res = bb.get_coordinates()
+        if tol > 0:
+            for c in (0, 101, 202, 303, 404, C3_CHAINS - 1):
+                one, _ = _batch([lengths[c]], 1000 + c, overlap_free=True, nts_mobsam=s, lanc_mxitr=mx,
+                                lanc_tol=tol, precision="fp32")
+                one.set_tile_shape(*shape)
+                one.run(s, per[c])
+                assert np.array_equal(one.get_coordinates()[0], res[c]), c
+        else:
+            for c in (5, 250, 500):
+                m = fene_wca_chain(lengths[c], 0)
+                m["crd"] = chains[c]
+                so = oracle_system(oracle, m)
+                ref, _, _ = oracle.bd_run(so, chains[c], s, per[c], mob_fctr=oracle.LANC, nts_mobsam=s,
+                                          lanc_mxitr=mx, lanc_tol=0.0)
+                _say(f"  chain {c} (N={lengths[c]}): rel {rel(res[c], ref):.2e}")
+                assert rel(res[c], ref) < 1e-6, c
 
 
 def test_batch_rejects_unsupported_options():

@@ -169,6 +169,47 @@ def test_lanczos_repeated_solves(oracle, chain100, chain2000, nb):
     assert iters[2] < iters[1] - 1
 
 
+@pytest.mark.parametrize("storage", STORAGES)
+@pytest.mark.parametrize("nb", [16, 20])
+def test_lanczos_converged_before_krylov_exhaustion(oracle, nb, storage):
+    """Short chains at s = 4: the Krylov space is complete after ceil(3N/4)
+    blocks, and the recurrence issued ahead of the tests tries to form
+    blocks beyond it.  A CholeskyQR failure there belongs to an iteration
+    after convergence and must not void the converged result (the reference
+    stops at convergence, brown_m.f90:497-506).  Several solves in one
+    context, against the oracle as the converged-reference bar."""
+    _say(f"case: exhaustion nb={nb} {storage}")
+    m = fene_wca_chain(nb, nb)
+    D = oracle.rpy(m["crd"])
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=4, lanc_mxitr=50, lanc_tol=1e-6, mob_storage=storage)
+    g.calc_rpy_tensor()
+    rng = np.random.default_rng(nb)
+    for _ in range(4):
+        xi = rng.standard_normal((3 * nb, 4))
+        out, lc, it = g.brn_calc_bdw(xi)
+        ref, lc2, it2, ierr = oracle.brn_calc_bdw(oracle.LANC, D, xi, lanc_mxitr=50, lanc_tol=1e-6)
+        _say(f"  iters gpu {it} oracle {it2} rel {rel(out, ref):.2e}")
+        assert lc and lc2 and ierr == 0 and abs(it - it2) <= 1
+        assert rel(out, ref) < 1e-5
+
+
+def test_lanczos_exhausted_before_convergence_is_an_error(oracle):
+    """N = 10, s = 4: the reference converges only at iteration 9, one past
+    the complete Krylov space (8 blocks), through a pivoted Householder QR of
+    a rank-deficient block; CholeskyQR cannot form that block, so the device
+    solve reports BD_ERR_FACTOR (such chains take mob_fctr CHOL)."""
+    from brownpak_amd import BdError
+    m = fene_wca_chain(10, 10)
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=4, lanc_mxitr=50, lanc_tol=1e-6)
+    g.calc_rpy_tensor()
+    xi = np.random.default_rng(10).standard_normal((30, 4))
+    _, lc2, it2, _ = oracle.brn_calc_bdw(oracle.LANC, oracle.rpy(m["crd"]), xi, lanc_mxitr=50, lanc_tol=1e-6)
+    assert lc2 and it2 == 9
+    with pytest.raises(BdError) as ei:
+        g.brn_calc_bdw(xi)
+    assert ei.value.code == 2
+
+
 def test_lanczos_not_converged_flag(chain100):
     g = gpu_system(chain100, mob_fctr="LANC", nts_mobsam=4, lanc_mxitr=1, lanc_tol=1e-12)
     g.calc_rpy_tensor()
@@ -192,6 +233,57 @@ def test_mixed_precision_error_n2000(chain2000, storage):
     e_df = rel(outs["fp32_df"], outs["fp64_df"])
     _say(f"  mixed rel err: B.xi {e_b:.2e}  D.F {e_df:.2e}")
     assert e_b < 1e-6 and e_df < 1e-6
+
+
+@pytest.mark.parametrize("storage", ["tiles", "matfree"])
+def test_mixed_headline_configuration(oracle, chain2000, storage):
+    """The configuration the throughput figure is quoted on: N = 2000,
+    block Lanczos s = 4, tol 1e-6, no reorthogonalisation, FP32 mobility
+    products (stored FP32 tiles with FP64 arithmetic, or FP32 matrix-free),
+    several solves in one context so that the tests are grouped around the
+    previous solve's converged iteration.  Against FP64 (same storage) and
+    the oracle (FP64 reference algorithm).  Bounds:
+      * tiles: stopping iteration within 1 of FP64's, B.xi within 2e-6 of
+        FP64, one-step displacement within 2e-6;
+      * matfree (FP32 arithmetic in the product, ~8e-8 per product, about
+        four times the tiles'; measured to stop 1-3 iterations after FP64,
+        the reason the automatic choice keeps stored tiles for FP32):
+        within 3 iterations, 4e-6 and 4e-6;
+      * against the oracle one more iteration (the FP64 device path itself
+        stops within 1 of it, test_lanczos_reference_converged) and 1e-5
+        (the converged-reference bar)."""
+    _say(f"case: mixed headline {storage}")
+    itol, btol = (1, 2e-6) if storage == "tiles" else (3, 4e-6)
+    g32 = gpu_system(chain2000, mob_fctr="LANC", nts_mobsam=4, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32",
+                     mob_storage=storage)
+    g64 = gpu_system(chain2000, mob_fctr="LANC", nts_mobsam=4, lanc_mxitr=50, lanc_tol=1e-6, precision="fp64",
+                     mob_storage=storage)
+    g32.calc_rpy_tensor()
+    g64.calc_rpy_tensor()
+    D = oracle.rpy(chain2000["crd"])
+    rng = np.random.default_rng(2001)
+    for q in range(3):
+        xi = rng.standard_normal((6000, 4))
+        o32, lc32, it32 = g32.brn_calc_bdw(xi)
+        o64, lc64, it64 = g64.brn_calc_bdw(xi)
+        ref, lco, ito, _ = oracle.brn_calc_bdw(oracle.LANC, D, xi, lanc_mxitr=50, lanc_tol=1e-6)
+        _say(f"  solve {q}: iters fp32 {it32} fp64 {it64} oracle {ito}; fp32 vs fp64 {rel(o32, o64):.2e}, "
+             f"vs oracle {rel(o32, ref):.2e}")
+        assert lc32 and lc64 and lco
+        assert abs(it32 - it64) <= itol and abs(it32 - ito) <= itol + 1
+        assert rel(o32, o64) < btol and rel(o32, ref) < 1e-5
+    # one EM step from the same state and noise
+    xi = rng.standard_normal((6000, 4))
+    disp = {}
+    for prec in ("fp32", "fp64"):
+        g = gpu_system(chain2000, mob_fctr="LANC", nts_mobsam=4, lanc_mxitr=50, lanc_tol=1e-6, precision=prec,
+                       mob_storage=storage)
+        c0 = g.get_coordinates()
+        c0 = c0 - c0.mean(axis=0)
+        g.bds_run(1, xi.T.ravel())
+        disp[prec] = g.get_coordinates() - c0
+    _say(f"  displacement fp32 vs fp64 {rel(disp['fp32'], disp['fp64']):.2e}")
+    assert rel(disp["fp32"], disp["fp64"]) < btol
 
 
 # ------------------------------------------------------------- forces
@@ -400,6 +492,64 @@ def test_bd_run_predictor_corrector(oracle, chain100, case):
     # and it is not the EM trajectory
     em, _, _ = oracle.bd_run(so, m["crd"], nsteps, xi, **okw)
     assert rel(em, ref) > 1e3 * rel(out, ref)
+
+
+@pytest.mark.parametrize("flow", [1, 2, 3])
+def test_bd_run_flow_styles(oracle, chain100, flow):
+    """Flow fields of calc_drift (shear, planar, uniaxial; bd_solver_m.f90:
+    420-441) in EM steps with Cholesky noise, against the oracle."""
+    nsteps = 6
+    xi = np.random.default_rng(60 + flow).standard_normal(300 * nsteps)
+    g = gpu_system(chain100, mob_fctr="CHOL", nts_mobsam=1, flow_style=flow, flow_rate=5.0)
+    g.bds_run(nsteps, xi)
+    so = oracle_system(oracle, chain100)
+    ref, _, _ = oracle.bd_run(so, chain100["crd"], nsteps, xi, mob_fctr=oracle.CHOL, nts_mobsam=1,
+                              flow_style=flow, flow_rate=5.0)
+    still, _, _ = oracle.bd_run(so, chain100["crd"], nsteps, xi, mob_fctr=oracle.CHOL, nts_mobsam=1)
+    assert rel(g.get_coordinates(), ref) < TOL
+    assert rel(still, ref) > 1e-6   # the flow moved the chain
+
+
+def test_marko_siggia_chain(oracle):
+    """Bond style 4 (Marko-Siggia, ia_bond_m.f90:334-362, restated as
+    written) on a chain: forces and energies, then EM steps with Lanczos
+    noise, against the oracle."""
+    m = fene_wca_chain(60, 61)
+    m = dict(m, bond_styles=[4], bond_params=[[10.0, 5.0]])
+    g = gpu_system(m, mob_fctr="LANC", nts_mobsam=2, lanc_reorth=True)
+    f, e, _, _ = g.ia_calc_forces()
+    so = oracle_system(oracle, m)
+    fr, en, ierr = oracle.forces(so, m["crd"], oracle.pairs(so, m["crd"], "DIR"))
+    assert ierr == 0 and rel(f, fr) < TOL
+    assert abs(e[0] - en.energy_bond) <= TOL * abs(en.energy_bond)
+    nsteps = 4
+    xi = np.random.default_rng(62).standard_normal(180 * 2 * 2)
+    g.bds_run(nsteps, xi)
+    ref, _, _ = oracle.bd_run(so, m["crd"], nsteps, xi, mob_fctr=oracle.LANC, nts_mobsam=2, lanc_reorth=True)
+    assert rel(g.get_coordinates(), ref) < 1e-9
+
+
+def test_device_gaussian_stream(chain100):
+    """The device generator (bd_set_rng) against its CPU restatement
+    (tests/philox_ref.py, pinned by the Philox-4x32-10 known-answer
+    vectors): the recorded Brownian displacements of a free-draining run
+    with dt = 1/2 are the stream itself, sqrt(2 dt) = 1, step by step
+    (n values per step, the position kept even)."""
+    from philox_ref import gaussian_stream
+    from brownpak_amd import BdStep
+    nb, nsteps = 7, 3
+    n = 3 * nb
+    for seed, off in ((12345, 0), (2 ** 40 + 17, 1000)):
+        h = BdStep(nb, lhdia=False, tim_stp=0.5)
+        h.set_box(0, (0, 0, 0))
+        h.ia_setup([], [], None, lvdw=False)
+        h.set_coordinates(np.zeros((nb, 3)))
+        h.set_rng(seed, off)
+        rec = h.run_record(nsteps)
+        h.close()
+        for q in range(nsteps):
+            want = gaussian_stream(seed, off + q * (n + (n & 1)), n)
+            assert np.allclose(rec[q].ravel(), want, rtol=1e-13, atol=1e-13), (seed, q)
 
 
 def test_bond_overstretch_reports_ierr(chain100):
