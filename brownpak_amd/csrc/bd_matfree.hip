@@ -42,15 +42,15 @@ int mf_rows_per_lane(const Ctx &c) { return c.opt.precision == BD_FP32 ? 1 : 2; 
 
 // Workgroups along the column range: about 2048 waves in total, at least
 // one 8-bead chunk per wave.
-int mf_jg(const Ctx &c)
+static int mf_jg_for(int nb, int R)
 {
-    const int R = mf_rows_per_lane(c);
-    const int nib = (c.nb + 64 * R - 1) / (64 * R);
-    const int nch = (c.nb + MF_CH - 1) / MF_CH;
+    const int nib = (nb + 64 * R - 1) / (64 * R);
+    const int nch = (nb + MF_CH - 1) / MF_CH;
     int jg = 512 / nib;
     jg = std::min(jg, nch / MF_WAVES);
     return std::max(1, std::min(jg, MF_JG_MAX));
 }
+int mf_jg(const Ctx &c) { return mf_jg_for(c.nb, mf_rows_per_lane(c)); }
 
 // Coordinates (3, N) -> records of 4 doubles (x, y, z, 0), padded to nbp
 // beads with points 1e6 apart from everything.
@@ -582,4 +582,113 @@ int mf_dense(Ctx &c, double *dense, int ld)
     return BD_OK;
 }
 
+// ---------------------------------------------------------------- context-free product
+// y(n, s) = D(crd) x(n, s) on device buffers with scratch scr (mf_raw_scratch
+// doubles): the coordinate records, the packed block, then the partials.
+// The column split jg is capped so that the partials fit MF_RAW_PART
+// doubles, the same for every caller, so that results do not depend on
+// where the scratch lives.
+constexpr long MF_RAW_PART = 524288;   // 4 MiB of partial sums
+
+static int mf_raw_jg(int nb, int s, int precision)
+{
+    const int jg = mf_jg_for(nb, precision == BD_FP32 ? 1 : 2);
+    const long cap = MF_RAW_PART / ((long)nb * 3 * s);
+    return (int)std::max(1L, std::min((long)jg, cap));
+}
+
+static long mf_raw_scratch(int nb, int s, int precision)
+{
+    const long nbp = (long)((nb + MF_CH - 1) / MF_CH) * MF_CH;
+    const long esz = precision == BD_FP32 ? 4 : 8;
+    return 4 * nbp + (3 * nbp * s * esz + 7) / 8 + (long)mf_raw_jg(nb, s, precision) * nb * 3 * s;
+}
+
+template <typename T, int S>
+static void mf_raw_product(int nb, int nbp, int jg, const double *crec, const T *xrec, double *part,
+                           hipStream_t st)
+{
+    const int R = sizeof(T) == 4 ? 1 : 2;
+    const dim3 grid((nb + 64 * R - 1) / (64 * R), jg);
+    if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL((mf_product_kernel<T, S, 1>), grid, dim3(256), 0, st, (const double AS4 *)crec,
+                           (const T AS4 *)xrec, nb, nbp / MF_CH, part, (const DevStatus *)nullptr);
+    else
+        hipLaunchKernelGGL((mf_product_kernel<T, S, 2>), grid, dim3(256), 0, st, (const double AS4 *)crec,
+                           (const T AS4 *)xrec, nb, nbp / MF_CH, part, (const DevStatus *)nullptr);
+}
+
+template <typename T>
+static void mf_raw_product_s(int s, int nb, int nbp, int jg, const double *crec, const T *xrec, double *part,
+                             hipStream_t st)
+{
+    switch (s) {
+    case 1: mf_raw_product<T, 1>(nb, nbp, jg, crec, xrec, part, st); break;
+    case 2: mf_raw_product<T, 2>(nb, nbp, jg, crec, xrec, part, st); break;
+    case 3: mf_raw_product<T, 3>(nb, nbp, jg, crec, xrec, part, st); break;
+    case 4: mf_raw_product<T, 4>(nb, nbp, jg, crec, xrec, part, st); break;
+    case 5: mf_raw_product<T, 5>(nb, nbp, jg, crec, xrec, part, st); break;
+    case 6: mf_raw_product<T, 6>(nb, nbp, jg, crec, xrec, part, st); break;
+    case 7: mf_raw_product<T, 7>(nb, nbp, jg, crec, xrec, part, st); break;
+    default: mf_raw_product<T, 8>(nb, nbp, jg, crec, xrec, part, st); break;
+    }
+}
+
+static int mf_raw_check(int nbeads, const double *crd, const double *x, int ldx, int s, const double *y, int ldy,
+                        int precision)
+{
+    if (nbeads < 1 || !crd || !x || !y || s < 1 || s > MAX_S || ldx < 3 * nbeads || ldy < 3 * nbeads ||
+        (precision != BD_FP64 && precision != BD_FP32)) {
+        set_error("bd_rpy_apply: bad arguments");
+        return BD_ERR_ARG;
+    }
+    return BD_OK;
+}
+
+// Launches the whole product on stream st with scratch scr.
+static int mf_raw_apply(int nb, const double *crd, const double *x, int ldx, int s, double *y, int ldy,
+                        int precision, double *scr, hipStream_t st)
+{
+    const int nbp = ((nb + MF_CH - 1) / MF_CH) * MF_CH, rows = 3 * nbp, n = 3 * nb;
+    const int jg = mf_raw_jg(nb, s, precision);
+    const long esz = precision == BD_FP32 ? 4 : 8;
+    double *crec = scr;
+    void *xrec = scr + 4L * nbp;
+    double *part = scr + 4L * nbp + (3L * nbp * s * esz + 7) / 8;
+    hipLaunchKernelGGL(mf_crd_kernel, dim3((nbp + 255) / 256), dim3(256), 0, st, crd, nb, nbp, crec);
+    const long tot = (long)rows * s;
+    if (precision == BD_FP32) {
+        hipLaunchKernelGGL(mf_pack_kernel<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, x, ldx, n,
+                           rows, s, (float *)xrec);
+        mf_raw_product_s<float>(s, nb, nbp, jg, crec, (const float *)xrec, part, st);
+    } else {
+        hipLaunchKernelGGL(mf_pack_kernel<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, x, ldx,
+                           n, rows, s, (double *)xrec);
+        mf_raw_product_s<double>(s, nb, nbp, jg, crec, (const double *)xrec, part, st);
+    }
+    const long ytot = (long)n * s;
+    hipLaunchKernelGGL(mf_reduce_kernel, dim3((unsigned)((ytot + 255) / 256)), dim3(256), 0, st, part, jg, nb, n, n,
+                       s, y, ldy, (const DevStatus *)nullptr);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
 }  // namespace bd
+
+using namespace bd;
+
+extern "C" int bd_rpy_apply(int nbeads, const double *crd, const double *x, int ldx, int s, double *y, int ldy,
+                            int precision, void *stream)
+{
+    BD_TRY(mf_raw_check(nbeads, crd, x, ldx, s, y, ldy, precision));
+    hipStream_t st = (hipStream_t)stream;
+    double *scr = nullptr;
+    BD_HIP(hipMalloc(&scr, (size_t)mf_raw_scratch(nbeads, s, precision) * sizeof(double)));
+    int rc = mf_raw_apply(nbeads, crd, x, ldx, s, y, ldy, precision, scr, st);
+    if (hipStreamSynchronize(st) != hipSuccess && rc == BD_OK) {
+        set_error("bd_rpy_apply: stream synchronisation failed");
+        rc = BD_ERR_HIP;
+    }
+    (void)hipFree(scr);
+    return rc;
+}

@@ -13,6 +13,7 @@
 // tile rows.  A second kernel adds the partials of every output block in a
 // fixed order, so results are deterministic run to run.
 #include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -958,7 +959,11 @@ template <typename E, int S>
 int plan_tune(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &row_off, const E *mob,
               const double *x, double *y, int ld, int *ticket, hipStream_t sm, int device, double *times_us)
 {
+    // process-wide, shared by every batch (contexts on other threads too):
+    // the lock is held through the timing so one thread times a key once
     static std::map<std::string, std::pair<int, std::vector<double>>> cache;
+    static std::mutex cache_mu;
+    std::lock_guard<std::mutex> lock(cache_mu);
     std::string key = std::to_string(device) + ":" + std::to_string(sizeof(E)) + ":" + std::to_string(S) + ":";
     for (int v : nb) key += std::to_string(v) + ",";
     auto hit = cache.find(key);

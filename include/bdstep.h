@@ -305,6 +305,21 @@ int bd_batch_set_tile_shape(bd_batch *batch, int chr, int wsc);
 int bd_batch_prof(bd_batch *batch, int on, double *total_ms,
                   long long *launches, double *tile_bytes);
 
+/* Matrix-free RPY product on device buffers, without a context:
+ *   y(n, s) := D(crd) x(n, s),  n = 3 nbeads,
+ * with D the RPY mobility of calc_rpy_tensor (bd_solver_m.f90:458-522,
+ * unit diagonal) applied as the dsymm('L') of brown_m.f90:386 (or the
+ * dsymv of bd_solver_m.f90:446 for s = 1) would apply the stored matrix,
+ * but evaluated block by block inside the product (D is never formed).
+ * crd (3, N), x (ldx, s) and y (ldy, s) are FP64 device pointers
+ * (column-major, ldx and ldy >= n); s in 1..8; precision BD_FP64 or BD_FP32
+ * (FP32 arithmetic in the product, FP64 sums); stream is a hipStream_t
+ * (NULL: the default stream).  The work is ordered on stream; the scratch
+ * (coordinate records, the packed block, partial sums) is allocated and
+ * freed by the call, which therefore returns after the stream has drained. */
+int bd_rpy_apply(int nbeads, const double *crd, const double *x, int ldx, int s, double *y, int ldy,
+                 int precision, void *stream);
+
 /* Work decomposition of the mobility product (no counterpart in the
  * reference: introspection for tests and tuning; host only, no device call).
  * The single-chain plan a context of nbeads beads uses on a GPU with cus
