@@ -58,6 +58,7 @@ EXPORTS = (
     "bd_batch_create", "bd_batch_destroy", "bd_batch_set_model", "bd_batch_set_coordinates",
     "bd_batch_get_coordinates", "bd_batch_set_rng", "bd_batch_run", "bd_batch_info", "bd_batch_prof",
     "bd_batch_tile_shape", "bd_batch_set_tile_shape", "bd_plan_layout", "bd_rpy_apply",
+    "bd_rpy_apply_noalloc",
 )
 
 
@@ -128,6 +129,7 @@ def load_library(path: str = LIB_PATH):
     L.bd_get_mobility.argtypes = [vp, DP]
     L.bd_mob_symv.argtypes = [vp, DP, C.c_int, DP]
     L.bd_rpy_apply.argtypes = [C.c_int, vp, vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, vp]
+    L.bd_rpy_apply_noalloc.argtypes = [C.c_int, vp, vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, vp]
     L.bd_brownian.argtypes = [vp, DP, C.c_int, C.c_int, DP, IP, IP]
     L.bd_forces.argtypes = [vp, DP, DP, DP, DP]
     L.bd_pair_table.argtypes = [vp, IP, C.c_int, IP, IP]

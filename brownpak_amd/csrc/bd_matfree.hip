@@ -27,6 +27,9 @@
 //
 // Records are padded to a multiple of 8 beads with beads far away whose X
 // block is zero, so the inner loop has no tail.
+#include <mutex>
+#include <string>
+
 #include "bd_internal.h"
 #include "bd_lanczos_dev.h"
 
@@ -673,9 +676,61 @@ static int mf_raw_apply(int nb, const double *crd, const double *x, int ldx, int
     return BD_OK;
 }
 
+// ---------------------------------------------------------------- no allocation
+// Scratch of bd_rpy_apply_noalloc: one array per GPU, in the code object,
+// for up to MF_NOALLOC_MAX_BEADS beads and s <= 8 (the records, the packed
+// block, MF_RAW_PART doubles of partial sums): 753664 doubles, 5.75 MiB.
+// Host code never uses the array's name as an address: the device address
+// comes from hipGetSymbolAddress, once per device.
+constexpr int MF_NOALLOC_MAX_BEADS = 8192;
+constexpr long MF_NOALLOC_DOUBLES = 4L * MF_NOALLOC_MAX_BEADS + 3L * MF_NOALLOC_MAX_BEADS * MAX_S + MF_RAW_PART;
+constexpr int MF_MAX_DEVICES = 64;
+
+static __device__ __align__(256) double scratch[MF_NOALLOC_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[MF_MAX_DEVICES] = {};
+
+// The current device's address of scratch (resolved on its first use).
+static double *noalloc_scratch(int dev)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("bd_rpy_apply_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace bd
 
 using namespace bd;
+
+extern "C" int bd_rpy_apply_noalloc(int nbeads, const double *crd, const double *x, int ldx, int s, double *y,
+                                    int ldy, int precision, void *stream)
+{
+    BD_TRY(mf_raw_check(nbeads, crd, x, ldx, s, y, ldy, precision));
+    if (nbeads > MF_NOALLOC_MAX_BEADS) {
+        set_error("bd_rpy_apply_noalloc: nbeads exceeds " + std::to_string(MF_NOALLOC_MAX_BEADS) +
+                  " (the size of its static scratch); use bd_rpy_apply");
+        return BD_ERR_ARG;
+    }
+    if (mf_raw_scratch(nbeads, s, precision) > MF_NOALLOC_DOUBLES) {
+        set_error("internal: bd_rpy_apply_noalloc scratch too small");
+        return BD_ERR_STATE;
+    }
+    int dev = 0;
+    BD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MF_MAX_DEVICES) { set_error("bd_rpy_apply_noalloc: device index out of range"); return BD_ERR_ARG; }
+    double *scr = noalloc_scratch(dev);
+    if (!scr) return BD_ERR_HIP;
+    return mf_raw_apply(nbeads, crd, x, ldx, s, y, ldy, precision, scr, (hipStream_t)stream);
+}
 
 extern "C" int bd_rpy_apply(int nbeads, const double *crd, const double *x, int ldx, int s, double *y, int ldy,
                             int precision, void *stream)

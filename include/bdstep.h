@@ -320,6 +320,18 @@ int bd_batch_prof(bd_batch *batch, int on, double *total_ms,
 int bd_rpy_apply(int nbeads, const double *crd, const double *x, int ldx, int s, double *y, int ldy,
                  int precision, void *stream);
 
+/* bd_rpy_apply without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and the same results, bit for bit; the scratch is a static
+ * device array of the library (5.75 MiB on every GPU the library runs on,
+ * held until the process ends), so nbeads is limited to 8192 (BD_ERR_ARG
+ * beyond).  The call only enqueues work on stream and returns.  Every call
+ * on one GPU shares that GPU's scratch: calls that overlap on one GPU (other
+ * streams or threads) race and must be serialised by the caller. */
+int bd_rpy_apply_noalloc(int nbeads, const double *crd, const double *x, int ldx, int s, double *y, int ldy,
+                         int precision, void *stream);
+
 /* Work decomposition of the mobility product (no counterpart in the
  * reference: introspection for tests and tuning; host only, no device call).
  * The single-chain plan a context of nbeads beads uses on a GPU with cus
