@@ -86,7 +86,7 @@ def time_path(precision, steps, warmup, storage):
                 prof_lanczos_iters=info2["total_lanczos_iters"] - info["total_lanczos_iters"])
 
 
-def accuracy_states(nstates=10, storage="auto"):
+def accuracy_states(nstates=10, storage=None):
     """Per seeded state k (random walk seed k, xi seed k): the relative
     2-norm error of the Brownian block B.xi (all s columns) and of one EM
     displacement, mixed precision against FP64 block Lanczos (the reference
@@ -106,7 +106,7 @@ def accuracy_states(nstates=10, storage="auto"):
         xi = np.random.default_rng(k).standard_normal((n, S_BLOCK))
         bxi, disp, its = {}, {}, {}
         for prec in ("fp64", "fp32"):
-            g, _ = build_system(prec, seed=k, storage=storage)
+            g, _ = build_system(prec, seed=k, storage=(storage or {}).get(prec, "auto"))
             g.calc_rpy_tensor()
             bxi[prec], _, its[prec] = g.brn_calc_bdw(xi)
             if prec == "fp64":
@@ -210,7 +210,7 @@ def time_ensemble(steps, warmup, precision="fp32"):
                 not_converged=info2["n_not_converged"])
 
 
-def measured_traffic(kind):
+def measured_traffic(kind, prec="fp32"):
     """HBM bytes per launch of the FP32 Lanczos product kernel (kind "tiles":
     symm_tiles_kernel<float, 4>; "mf": mf_product_kernel<float, 4, 1>) from
     the two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in
@@ -221,7 +221,7 @@ def measured_traffic(kind):
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch_fp32:"):
+                if line.startswith(f"traffic_bytes_per_launch_{prec}:"):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -274,13 +274,22 @@ def main():
             runs[prec, storage] = time_path(prec, steps, warmup, storage)
             print(f"# {prec} {storage}: {runs[prec, storage]['steps_per_s']:.1f} steps/s", file=sys.stderr,
                   flush=True)
-    best = {p: max(("matfree", "tiles"), key=lambda st: runs[p, st]["steps_per_s"]) for p in ("fp32", "fp64")}
+    # the library's automatic storage: FP64 matrix-free, FP32 (mixed) stored
+    # tiles (FP32 storage, FP64 arithmetic: the accurate mixed form)
+    best = {"fp32": "tiles", "fp64": max(("matfree", "tiles"), key=lambda st: runs["fp64", st]["steps_per_s"])}
     mixed, fp64 = runs["fp32", best["fp32"]], runs["fp64", best["fp64"]]
     ens = None
     if args.ensemble_steps > 0:
         ens = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK)
         print(f"# ensemble: {ens['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
-    acc = None if args.no_error_check else accuracy_states(args.states, best["fp32"])
+    acc = None if args.no_error_check else accuracy_states(args.states, best)
+    # the headline: the fastest form whose one-step displacement stays within
+    # 1e-6 of FP64 (FP64 itself by definition; the mixed form when its
+    # measured maximum over the seeded states is within the bar)
+    mixed_ok = acc is not None and acc["disp_mixed_vs_fp64"]["max"] <= 1e-6
+    head = ("fp32", best["fp32"]) if mixed_ok and mixed["steps_per_s"] > fp64["steps_per_s"] else \
+        ("fp64", best["fp64"])
+    hrun = runs[head]
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
     n = 3 * N_BEADS
@@ -314,20 +323,19 @@ def main():
 
     rt32, rt64 = roof_tiles(runs["fp32", "tiles"], 4), roof_tiles(runs["fp64", "tiles"], 8)
     rm32, rm64 = roof_mf(runs["fp32", "matfree"]), roof_mf(runs["fp64", "matfree"])
-    if best["fp32"] == "matfree":
-        roofline = {"bound": "mfma", "achieved": round(rm32["achieved"], 2), "peak": FP32_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(rm32["achieved"] / FP32_PEAK_TFLOPS, 4),
-                    "traffic": measured_traffic("mf"),
-                    "kernel": "mf_product_kernel<float, 4, 1>",
-                    "note": "FLOP-bound on the vector ALUs (FP32, no packed FP32 instructions: their "
-                            "ceiling is half the listed FP32 peak); 'mfma' names the compute bound",
-                    "avg_launch_us": round(rm32["avg_launch_us"], 2),
-                    "flops_per_working_launch": rm32["flops_per_working_launch"],
-                    "launches": rm32["launches"], "empty_launches": rm32["empty_launches"],
-                    "fp64_kernel": "mf_product_kernel<double, 4, 2>",
-                    "fp64_achieved": round(rm64["achieved"], 2), "fp64_peak": FP64_PEAK_TFLOPS,
-                    "fp64_frac": round(rm64["achieved"] / FP64_PEAK_TFLOPS, 4),
-                    "fp64_avg_launch_us": round(rm64["avg_launch_us"], 2)}
+    if head == ("fp64", "matfree"):
+        roofline = {"bound": "mfma", "achieved": round(rm64["achieved"], 2), "peak": FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(rm64["achieved"] / FP64_PEAK_TFLOPS, 4),
+                    "traffic": measured_traffic("mf", "fp64"),
+                    "kernel": "mf_product_kernel<double, 4, 2>",
+                    "note": "FLOP-bound on the FP64 vector ALUs (no MFMA in the product; FP64 vector and "
+                            "FP64 matrix peaks are both 78.6 TFLOP/s)",
+                    "avg_launch_us": round(rm64["avg_launch_us"], 2),
+                    "flops_per_working_launch": rm64["flops_per_working_launch"],
+                    "launches": rm64["launches"], "empty_launches": rm64["empty_launches"],
+                    "fp32_kernel": "mf_product_kernel<float, 4, 1>",
+                    "fp32_achieved": round(rm32["achieved"], 2), "fp32_peak": FP32_PEAK_TFLOPS,
+                    "fp32_avg_launch_us": round(rm32["avg_launch_us"], 2)}
     else:
         roofline = {"bound": "hbm", "achieved": round(rt32["achieved"], 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(rt32["achieved"] / HBM_PEAK_GBS, 4),
@@ -343,29 +351,31 @@ def main():
                         "fp64_avg_launch_us": round(rt64["avg_launch_us"], 2)}
     out = {
         "metric": "BD steps/sec at N=2000 RPY+Lanczos, mixed vs FP64 at <=1e-6 rel err; HBM GB/s",
-        "value": round(mixed["steps_per_s"], 2),
+        "value": round(hrun["steps_per_s"], 2),
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": warmup,
         "warmup_alignment_steps": align,
-        "ms_per_step": round(mixed["ms_per_step"], 4),
+        "ms_per_step": round(hrun["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 mobility products / f64 accumulate + Krylov",
+        "dtype": "f64" if head[0] == "fp64" else "f32 mobility storage / f64 arithmetic",
+        "value_path": {"precision": head[0], "mobility": head[1]},
         "data": "synthetic (seeded random-walk chain, device Philox noise)",
         "config": {"workload": "single chain N=2000, FENE/WCA, RPY, block Lanczos s=4 tol 1e-6 "
                                "(lanc_mxitr 50), dt 1e-3, EM",
                    "nbeads": N_BEADS, "block": S_BLOCK, "parallelism": "single GPU"},
         "mobility": {"mixed": best["fp32"], "fp64": best["fp64"]},
+        "mixed_steps_per_s": round(mixed["steps_per_s"], 2),
         "fp64_steps_per_s": round(fp64["steps_per_s"], 2),
         "fp64_ms_per_step": round(fp64["ms_per_step"], 4),
         "mixed_over_fp64": round(mixed["steps_per_s"] / fp64["steps_per_s"], 3),
         "steps_per_s_by_storage": {f"{p}_{st}": round(r["steps_per_s"], 2) for (p, st), r in runs.items()},
         "mixed_over_fp64_tiles": round(mixed["steps_per_s"] / runs["fp64", "tiles"]["steps_per_s"], 3),
         "accuracy": acc,
-        "lanczos_iters_per_update": round(mixed["lanczos_iters"] / max(mixed["mob_updates"], 1), 2),
+        "lanczos_iters_per_update": round(hrun["lanczos_iters"] / max(hrun["mob_updates"], 1), 2),
         "roofline": roofline,
         "roofline_tiles": roof_tiles_entry,
         "cpu_baseline": cpu,

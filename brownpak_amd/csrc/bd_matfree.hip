@@ -347,8 +347,9 @@ __global__ __launch_bounds__(256) void mf_reduce_lz_kernel(const double *__restr
         const int r = e / S, k = e % S, row = G * TILE + r;
         double v = 0.0;
         if (row < n) {
+            // all the partials of the element in flight at once (jg <= 64)
             const long o = (long)(row / 3) * (3 * S) + (row % 3) * S + k, str = (long)nb * 3 * S;
-            v = sum_strided(part + o, str, jg);
+            v = sum_strided<false, 32>(part + o, str, jg);
         }
         sdx[r][k] = v;
         sx[r][k] = X[(long)k * ld + row];
@@ -389,11 +390,27 @@ __global__ __launch_bounds__(256) void mf_reduce_lz_kernel(const double *__restr
         sv[r][k] = vj;
     }
     __syncthreads();
-    for (int o = tid; o < NO; o += blockDim.x) {
+    // the 64-row column sums: RG row groups of 64 / RG rows per thread, then
+    // the groups in order (fixed order)
+    constexpr int RG = (NO <= 32) ? 8 : (NO <= 64 ? 4 : 2);
+    __shared__ double sred[RG][NO];
+    for (int e = tid; e < RG * NO; e += blockDim.x) {
+        const int o = e % NO, gq = e / NO;
         const bool isA = o < S * S;
         const int oo = isA ? o : o - S * S, c = oo % S, k = oo / S;
         double t = 0.0;
-        for (int r = 0; r < TILE; r++) t += (isA ? sv[r][c] : sw[r][c]) * sw[r][k];
+#pragma unroll
+        for (int q = 0; q < TILE / RG; q++) {
+            const int r = gq * (TILE / RG) + q;
+            t += (isA ? sv[r][c] : sw[r][c]) * sw[r][k];
+        }
+        sred[gq][o] = t;
+    }
+    __syncthreads();
+    for (int o = tid; o < NO; o += blockDim.x) {
+        double t = sred[0][o];
+#pragma unroll
+        for (int gq = 1; gq < RG; gq++) t += sred[gq][o];
         lzpart[(long)G * NO + o] = t;
     }
 }
