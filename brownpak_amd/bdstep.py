@@ -58,7 +58,15 @@ EXPORTS = (
     "bd_batch_create", "bd_batch_destroy", "bd_batch_set_model", "bd_batch_set_coordinates",
     "bd_batch_get_coordinates", "bd_batch_set_rng", "bd_batch_run", "bd_batch_info", "bd_batch_prof",
     "bd_batch_tile_shape", "bd_batch_set_tile_shape", "bd_plan_layout", "bd_rpy_apply",
-    "bd_rpy_apply_noalloc",
+    "bd_rpy_apply_noalloc", "bd_get_rng", "bd_resume_run", "bd_stats_enable", "bd_stats_get",
+)
+
+# bd_stats_get's output, in stats_write's column order (stats_m.f90:235-290)
+STATS_FIELDS = (
+    ("bndlen", 1), ("bndlen_min", 1), ("bndlen_max", 1), ("energy_bond", 1), ("energy_angle", 1),
+    ("energy_dihedral", 1), ("energy_vdw", 1), ("energy_tether", 1), ("energy_external", 1),
+    ("energy_tot", 1), ("stress", 9), ("rgsq", 1), ("reedsq", 1), ("span", 3), ("reev", 3),
+    ("molecule_com", 3),
 )
 
 
@@ -135,6 +143,10 @@ def load_library(path: str = LIB_PATH):
     L.bd_pair_table.argtypes = [vp, IP, C.c_int, IP, IP]
     L.bd_run.argtypes = [vp, C.c_int, DP, C.c_longlong, C.POINTER(C.c_longlong)]
     L.bd_set_rng.argtypes = [vp, C.c_uint64, C.c_uint64]
+    L.bd_get_rng.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.bd_resume_run.argtypes = [vp, DP]
+    L.bd_stats_enable.argtypes = [vp, C.c_int, C.c_int, C.c_int, IP]
+    L.bd_stats_get.argtypes = [vp, DP]
     L.bd_get_runinfo.argtypes = [vp, C.POINTER(BdRuninfo)]
     L.bd_reset_run.argtypes = [vp]
     L.bd_rgsq.argtypes = [vp, DP]
@@ -330,6 +342,38 @@ class BdStep:
 
     def set_rng(self, seed, offset=0):
         self._chk(self.L.bd_set_rng(self.h, int(seed), int(offset)))
+
+    def get_rng(self):
+        """(seed, offset): the generator's whole state."""
+        sd, off = C.c_uint64(0), C.c_uint64(0)
+        self._chk(self.L.bd_get_rng(self.h, C.byref(sd), C.byref(off)))
+        return sd.value, off.value
+
+    def resume_run(self, molecule_com=(0.0, 0.0, 0.0)):
+        """Continue a run from a revive: molecule_com restored, no second
+        centring of the coordinates just set."""
+        a = _f64(molecule_com)
+        assert a.size == 3
+        self._chk(self.L.bd_resume_run(self.h, _dp(a)))
+
+    def stats_init(self, binsize=(1, 1), molecules=None):
+        """stats_init (stats_m.f90:53-105): accumulate on the device from now
+        on.  molecules: (nmol, 2) rows (atom count, first atom 1-based),
+        needed for a periodic box (compute_ic1)."""
+        mol = np.zeros((0, 2), np.int32) if molecules is None else _i32(np.reshape(molecules, (-1, 2)))
+        self._chk(self.L.bd_stats_enable(self.h, int(binsize[0]), int(binsize[1]), mol.shape[0],
+                                         _ip(mol) if mol.size else None))
+
+    def stats(self):
+        """The accumulators stats_write prints, by name (STATS_FIELDS)."""
+        out = np.zeros(30)
+        self._chk(self.L.bd_stats_get(self.h, _dp(out)))
+        d, k = {}, 0
+        for name, w in STATS_FIELDS:
+            d[name] = out[k] if w == 1 else out[k:k + w].copy()
+            k += w
+        d["stress"] = d["stress"].reshape(3, 3, order="F")
+        return d
 
     def runinfo(self):
         ri = BdRuninfo()

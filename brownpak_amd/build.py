@@ -63,7 +63,7 @@ def build_fortran_driver(verbose: bool = False) -> str:
     mod = os.path.join(OUT, "fmod")
     os.makedirs(mod, exist_ok=True)
     objs = []
-    for src in ("bdstep_c.f90", "bdrun.f90"):
+    for src in ("bdstep_c.f90", "bdrun_io.f90", "bdrun.f90"):
         obj = os.path.join(mod, src + ".o")
         cmd = [FLANG, "-O2", "-c", os.path.join(FDIR, src), "-o", obj, "-module-dir", mod, "-I", mod]
         r = subprocess.run(cmd, capture_output=True, text=True)

@@ -453,8 +453,8 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
     // (an EM step of an isolated molecule leaves the drift to its tail kernel)
     const bool fused_tail = c.opt.integrator != BD_PC && c.imcon == 0;
     const double *vdrift = nullptr;
-    auto drift_at_crd = [&](bool launch) -> int {
-        BD_TRY(compute_forces(c, false, false));
+    auto drift_at_crd = [&](bool launch, bool energies) -> int {
+        BD_TRY(compute_forces(c, energies, false));
         const double *v = c.frc;
         if (hi) {
             BD_TRY(mob_product(c, c.frc, c.W, 1, c.npad, c.npad));
@@ -466,7 +466,9 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
                                c.opt.flow_rate, c.opt.tim_stp, c.drift, c.st);
         return BD_OK;
     };
-    BD_TRY(drift_at_crd(!fused_tail));
+    // the statistics take the energies of this evaluation (calc_drift at the
+    // start-of-step coordinates, bd_solver_m.f90:407-454)
+    BD_TRY(drift_at_crd(!fused_tail, c.stats_on));
     // Brownian displacement w of this step
     const double *dcol = nullptr;
     if (disp_dev) dcol = disp_dev;
@@ -476,7 +478,7 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
         const int nbn = (n + 255) / 256;
         hipLaunchKernelGGL(pc_predict_kernel, dim3(nbn), dim3(256), 0, sm, c.crd, c.drift, dcol, c.pc_r0, c.pc_d0,
                            n, c.imcon, c.box[0], c.box[1], c.box[2], c.st);
-        BD_TRY(drift_at_crd(true));
+        BD_TRY(drift_at_crd(true, false));
         hipLaunchKernelGGL(pc_correct_kernel, dim3(nbn), dim3(256), 0, sm, c.crd, c.drift, c.pc_r0, c.pc_d0, n,
                            c.st);
     }
@@ -496,6 +498,8 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
         if (c.imcon == 0)
             hipLaunchKernelGGL(to_origin_kernel, dim3(1), dim3(1024), 0, sm, c.crd, nb, c.scal + 16, c.st);
     }
+    // stats_accumulate (bd_solver_m.f90:189) on the end-of-step coordinates
+    if (c.stats_on) BD_TRY(stats_step(c));
     if (hi && c.cntr_mobsam == s) c.cntr_mobsam = 0;
     BD_HIP(hipGetLastError());
     return BD_OK;
@@ -582,6 +586,7 @@ void bd_ctx_destroy(bd_ctx *h)
     for (auto e : c.prof_ev) (void)hipEventDestroy(e);
     mob_free(c);
     mf_free(c);
+    stats_free(c);
     lanczos_free(c);
     forces_free(c);
     double **dp[] = {&c.crd, &c.frc, &c.drift, &c.diff, &c.xi_dev, &c.scal, &c.pc_r0, &c.pc_d0};
@@ -634,6 +639,7 @@ int bd_set_coordinates(bd_ctx *h, const double *crd)
     BD_HIP(hipMemcpyAsync(c.crd, crd, (size_t)c.n * sizeof(double), hipMemcpyHostToDevice, c.stream));
     BD_HIP(hipStreamSynchronize(c.stream));
     c.mob_valid = false;
+    c.origin_pending = true;
     return BD_OK;
 }
 
@@ -751,6 +757,25 @@ int bd_pair_table(bd_ctx *h, int *pairs2, int cap, int *npairs, int *rebuilt)
     return BD_OK;
 }
 
+int bd_get_rng(bd_ctx *h, uint64_t *seed, uint64_t *offset)
+{
+    if (!h || !seed || !offset) { set_error("null argument"); return BD_ERR_ARG; }
+    *seed = h->c.rng_seed;
+    *offset = h->c.rng_offset;
+    return BD_OK;
+}
+
+int bd_resume_run(bd_ctx *h, const double *com)
+{
+    if (!h || !com) { set_error("null argument"); return BD_ERR_ARG; }
+    Ctx &c = h->c;
+    BD_HIP(hipMemcpyAsync(c.scal + 16, com, 3 * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    BD_HIP(hipStreamSynchronize(c.stream));
+    for (int d = 0; d < 3; d++) c.info.molecule_com[d] = com[d];
+    c.origin_pending = false;
+    return BD_OK;
+}
+
 int bd_set_rng(bd_ctx *h, uint64_t seed, uint64_t offset)
 {
     if (!h) { set_error("null context"); return BD_ERR_ARG; }
@@ -767,10 +792,12 @@ int bd_run(bd_ctx *h, int nsteps, const double *xi, long long nxi, long long *us
     long long pos = 0;
     const long long nts0 = c.info.nts;
     BD_HIP(hipMemsetAsync(&c.st->steps_done, 0, sizeof(int), c.stream));
-    if (c.imcon == 0) {
-        // isolated molecule: COM to the origin before the loop (bd_solver_m.f90:161)
+    if (c.imcon == 0 && c.origin_pending) {
+        // isolated molecule: COM to the origin before the loop (bd_solver_m.f90:161),
+        // once per run: later calls continue the same bds_run loop
         hipLaunchKernelGGL(to_origin_kernel, dim3(1), dim3(1024), 0, c.stream, c.crd, c.nb, (double *)nullptr, c.st);
     }
+    c.origin_pending = false;
     int rc = BD_OK;
     for (int k = 0; k < nsteps && rc == BD_OK; k++) {
         long long cons = 0;
@@ -807,6 +834,7 @@ int bd_reset_run(bd_ctx *h)
     c.cntr_mobsam = 0;
     c.info = bd_runinfo{};
     c.verlet_first = true;
+    c.origin_pending = true;
     BD_HIP(hipMemsetAsync(c.scal + 16, 0, 3 * sizeof(double), c.stream));
     BD_HIP(hipMemsetAsync(c.st, 0, sizeof(DevStatus), c.stream));
     BD_HIP(hipStreamSynchronize(c.stream));

@@ -216,6 +216,9 @@ struct Ctx {
     int *nbr_idx = nullptr;      // pair_cap (full list, both directions)
     double *crd_save = nullptr;  // Verlet reference positions
     bool verlet_first = true;
+    // bds_run's pre-loop centring (bd_solver_m.f90:161) is still due: set by
+    // bd_set_coordinates and bd_reset_run, cleared by the first bd_run
+    bool origin_pending = true;
     int *cell_cnt = nullptr, *cell_ptr = nullptr, *cell_beads = nullptr, *bead_cell = nullptr;
     int ncell[3] = {0, 0, 0};
     double cl_cs[3] = {0, 0, 0};   // CL cell sides L / nc
@@ -235,6 +238,13 @@ struct Ctx {
     bd_runinfo info{};
     uint64_t rng_seed = 0x5eed, rng_offset = 0;
     double *xi_dev = nullptr;    // staging for injected xi (n x s)
+
+    // run statistics on the device (bd_stats.hip)
+    bool stats_on = false;
+    double *stats = nullptr;       // accumulators and the step's structure
+    int *stats_mol = nullptr;      // (natm, first bead) per molecule
+    double *stats_part = nullptr;  // per-molecule structure of the step
+    int stats_nmol = 0, stats_bnsiz = 0, stats_bnsiz_stress = 0;
 
     // kernel timing of the tile product (bd_prof_*)
     bool prof_on = false;
@@ -297,6 +307,10 @@ int grow_pairs(Ctx &c);
 int adj_force(Ctx &c, const double *u, double *out, double *tpart, int tie_rc);
 int set_theta(Ctx &c, double eps, double sigma, double kb, int tie_rc);
 int pair_table_host(Ctx &c, std::vector<int> &pairs);
+
+// --- statistics (bd_stats.hip)
+int stats_step(Ctx &c);
+void stats_free(Ctx &c);
 
 // --- step pieces (bd_api.hip)
 int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed, const double *disp_dev,

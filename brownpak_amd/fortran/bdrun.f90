@@ -2,43 +2,59 @@
 !>
 !> Keeps the reference's entry and call structure: command-line arguments
 !> fn_control= and job_tag= (src/main.f90:47-71), the control.txt keys of
-!> ctrlpar_t (src/control_m.f90:92-150), the .cfg sections read by
-!> read_config (src/config_io_m.f90:247-544), and the bds_run loop
-!> (src/bd_solver_m.f90:128-212) with integrate_em / calc_diffusion /
-!> calc_drift executed on the GPU through libbdstep (bdstep_c).
-!> Only bdintg = EM is available on the device; the Gaussian stream comes
-!> from the device Philox generator seeded from job_tag.
+!> ctrlpar_t (src/control_m.f90:92-150), run set-up (src/setup_m.f90:39-125:
+!> read_config or, with lrevive, read_dump; stats_init; the trajectory;
+!> the random seed) and the bds_run loop (src/bd_solver_m.f90:128-212):
+!> centre an isolated molecule, dump the starting configuration, then steps
+!> until nts > nts_sim, with statistics accumulated every step, a log line
+!> every nts_log steps, a revive dump every nts_dump, a statistics record
+!> and a trajectory frame every nts_samp, and a final dump.  integrate_em /
+!> calc_diffusion / calc_drift run on the GPU through libbdstep (bdstep_c);
+!> the driver calls bd_run for the steps between two of those events.
+!> bdintg EM and PC (predictor-corrector) are available on the device.
+!>
+!> The Gaussian stream is the device Philox generator.  Its seed comes from
+!> random_seed.txt<job_tag> with read_seed T (init_stream,
+!> src/random_m.f90:57-86), else from job_tag; write_seed T writes it there.
+!> Every revive dump ends with the generator's (seed, offset), and a revived
+!> run continues that stream without centring the molecule again, so a run
+!> split by a revive at a multiple of nts_mobsam takes the same steps as an
+!> uninterrupted one.  An interval key of 0 (nts_log,
+!> nts_dump, nts_samp) disables that event.
 program bdrun
 use iso_c_binding
 use bdstep_c
+use bdrun_io
 implicit none
 
 integer, parameter :: rp = c_double
-integer, parameter :: mxparam = 12
 character(len=512) :: arg, fn_control, job_tag
 ! control keys (ctrlpar_t)
 real(rp) :: rcutoff = 0, tskin = 0, lanc_tol = 0, tim_stp = 0
 character(len=8) :: mth_ptgen = 'DIR', mob_fctr = 'CHOL', bdintg = 'EM'
 logical :: lhdia = .false., lvdw = .false., lanc_reorth = .false., keep_reuse_noise = .false.
+logical :: lrevive = .false., read_seed = .false., write_seed = .false., write_traj = .false.
 integer :: excluded_atoms = 0, lanc_mxitr = 0, nts_mobsam = 1, precision = 0
-integer(c_long_long) :: nts_sim = 0, nts_log = 0, nts_samp = 0
-character(len=512) :: fn_cfg = 'config.cfg', fn_out = 'final.xyz'
-! configuration (atmcfg_t)
-integer :: imcon = 0, natom_types = 1, num_atoms = 0, num_bonds = 0, nbond_types = 0, nvdw = 0
-real(rp) :: box(3) = 0
-integer, allocatable :: atoms(:), bonds(:,:), bond_styles(:), vdw_styles(:)
-real(rp), allocatable :: coordinates(:,:), bond_params(:,:), vdw_params(:,:)
-integer :: flow_style = 0
-real(rp) :: flow_rate = 0
+integer :: stats_binsize(2) = 0
+integer(c_int64_t) :: nts_sim = 0, nts_log = 0, nts_samp = 0, nts_dump = 0
+character(len=512) :: fn_cfg = 'config.cfg', fn_out = 'final.xyz', fn_revive = 'revive.bin'
+character(len=512) :: fn_stats = 'stats.txt', fn_traj = 'traj.bin'
+! configuration (atmcfg_t) and files
+type(cfg_t) :: cfg
+type(traj_t) :: traj
+integer :: fu_stats
+logical :: lvdw_stats, limud, has_rng, lexists
 ! run
 type(c_ptr) :: ctx
 type(bd_opts_t) :: opts
 type(bd_runinfo_t) :: info
 integer(c_int) :: rc
-integer(c_long_long) :: nts, chunk, used
-integer :: i, iarg
+integer(c_int64_t) :: nts, nts_beg, nxt, used, seed, offset
+integer(c_int), allocatable :: bonds(:,:), bond_styles(:), vdw_styles(:), mol(:,:)
+real(rp), allocatable :: bond_params(:,:), vdw_params(:,:)
+real(rp) :: box(3), st(30)
+integer :: i, iarg, fu, nvdw, nbt
 integer(8) :: t0, t1, rate
-real(rp) :: rgsq
 
 fn_control = 'control.txt'; job_tag = ''
 do iarg = 1, command_argument_count()
@@ -48,7 +64,18 @@ do iarg = 1, command_argument_count()
 end do
 
 call read_control(trim(fn_control))
-call read_cfg(trim(fn_cfg))
+
+! ---- configuration: a new run reads the .cfg, a revived run the dump
+has_rng = .false.
+if (lrevive) then
+    call read_dump(trim(fn_revive)//trim(job_tag), nts_beg, cfg, has_rng, seed, offset)
+    print '(3a)', 'read revive file `', trim(fn_revive)//trim(job_tag), '`'
+else
+    call read_config(trim(fn_cfg)//trim(job_tag), cfg)
+    nts_beg = 0
+end if
+call check_supported()
+
 select case (trim(bdintg))
 case ('EM'); opts%integrator = BD_EM
 case ('PC'); opts%integrator = BD_PC
@@ -56,7 +83,6 @@ case default
     print '(a)', 'bdrun: bdintg must be EM or PC on the device'
     stop 2
 end select
-
 opts%lhdia = merge(1, 0, lhdia)
 select case (trim(mob_fctr))
 case ('LANC'); opts%mob_fctr = BD_LANC
@@ -70,43 +96,113 @@ opts%tim_stp = tim_stp
 opts%precision = precision
 opts%keep_reuse_noise = merge(1, 0, keep_reuse_noise)
 opts%lanc_reorth = merge(1, 0, lanc_reorth)
-opts%flow_style = flow_style
-opts%flow_rate = flow_rate
+opts%flow_style = cfg%flow_style
+if (cfg%flow_style /= 0) opts%flow_rate = cfg%flow_params(1)
 
-ctx = bd_ctx_create(int(num_atoms, c_int), opts)
+! ---- topology in the device's arrays
+nbt = max(cfg%num_bond_types, 1)
+allocate(bonds(3, max(cfg%num_bonds, 1)), bond_styles(nbt), bond_params(mxparam, nbt))
+bonds = 0; bond_styles = 0; bond_params = 0
+if (cfg%num_bonds > 0) bonds(:, 1:cfg%num_bonds) = cfg%bonds
+if (cfg%num_bond_types > 0) then
+    bond_styles(1:cfg%num_bond_types) = cfg%bond_styles
+    bond_params(:, 1:cfg%num_bond_types) = cfg%bond_params
+end if
+call vdw_table()
+limud = (cfg%num_tethers == 0) .and. (cfg%imcon == 0)
+box = [cfg%basis(1,1), cfg%basis(2,2), cfg%basis(3,3)]
+
+ctx = bd_ctx_create(int(cfg%num_atoms, c_int), opts)
 if (.not. c_associated(ctx)) call fail(-1)
-rc = bd_set_box(ctx, int(imcon, c_int), box); if (rc /= BD_OK) call fail(rc)
-rc = bd_set_topology(ctx, int(num_bonds, c_int), bonds, int(nbond_types, c_int), bond_styles, bond_params, &
-        int(natom_types, c_int), atoms, int(nvdw, c_int), vdw_styles, vdw_params, merge(1, 0, lvdw), &
-        int(excluded_atoms, c_int), pt_code(mth_ptgen), rcutoff, tskin)
+rc = bd_set_box(ctx, int(cfg%imcon, c_int), box); if (rc /= BD_OK) call fail(rc)
+rc = bd_set_topology(ctx, int(cfg%num_bonds, c_int), bonds, int(cfg%num_bond_types, c_int), bond_styles, &
+        bond_params, int(max(cfg%num_atom_types, 1), c_int), cfg%atoms, int(nvdw, c_int), vdw_styles, &
+        vdw_params, merge(1, 0, lvdw), int(excluded_atoms, c_int), pt_code(mth_ptgen), rcutoff, tskin)
 if (rc /= BD_OK) call fail(rc)
-rc = bd_set_coordinates(ctx, coordinates); if (rc /= BD_OK) call fail(rc)
-rc = bd_set_rng(ctx, int(seed_from(job_tag), c_int64_t), 0_c_int64_t); if (rc /= BD_OK) call fail(rc)
+rc = bd_set_coordinates(ctx, cfg%coordinates); if (rc /= BD_OK) call fail(rc)
+if (lrevive) then
+    ! mid-run coordinates: no second centring (bd_resume_run)
+    rc = bd_resume_run(ctx, cfg%molecule_com); if (rc /= BD_OK) call fail(rc)
+end if
 
+! ---- random numbers (setup_m.f90:99-105)
+if (.not. has_rng) then
+    offset = 0
+    if (read_seed) then
+        open(newunit=fu, file='random_seed.txt'//trim(job_tag), action='read', status='old')
+        read(fu, *) seed
+        close(fu)
+    else
+        seed = seed_from(job_tag)
+    end if
+end if
+if (write_seed) then
+    open(newunit=fu, file='random_seed.txt'//trim(job_tag), action='write', status='replace')
+    write(fu, *) seed
+    close(fu)
+end if
+rc = bd_set_rng(ctx, seed, offset); if (rc /= BD_OK) call fail(rc)
+
+! ---- statistics (stats_init) and trajectory (setup_m.f90:62, 69-97)
+lvdw_stats = lvdw .and. (cfg%num_vdw_types > 0)
+call stats_open(trim(fn_stats)//trim(job_tag), lrevive, cfg, lvdw_stats, fu_stats)
+allocate(mol(2, max(cfg%num_molecules, 1)))
+mol = 0
+if (cfg%num_molecules > 0) mol(:, 1:cfg%num_molecules) = cfg%molecules(2:3, :)
+rc = bd_stats_enable(ctx, int(stats_binsize(1), c_int), int(stats_binsize(2), c_int), &
+        int(merge(cfg%num_molecules, 0, cfg%imcon /= 0), c_int), mol)
+if (rc /= BD_OK) call fail(rc)
+if (write_traj) then
+    inquire(file=trim(fn_traj)//trim(job_tag), exist=lexists)
+    if (lrevive .and. lexists) then
+        call traj_open(trim(fn_traj)//trim(job_tag), traj)
+    else
+        call traj_create(trim(fn_traj)//trim(job_tag), cfg%num_atoms, traj)
+    end if
+end if
+
+! ---- bds_run (bd_solver_m.f90:128-212)
 call system_clock(t0, rate)
-nts = 0
+nts = nts_beg
+rc = bd_run(ctx, 0_c_int, c_null_ptr, 0_c_long_long, used)     ! centring of an isolated molecule
+if (rc /= BD_OK) call fail(rc)
 print '(a,i0)', 'nts = ', nts
-if (nts_log <= 0) nts_log = max(nts_sim, 1_c_long_long)
-do while (nts < nts_sim)
-    chunk = min(nts_log, nts_sim - nts)
-    rc = bd_run(ctx, int(chunk, c_int), c_null_ptr, 0_c_long_long, used)
-    if (rc /= BD_OK .and. rc /= BD_WARN_NOTCONV) call fail(rc)
-    nts = nts + chunk
-    rc = bd_rgsq(ctx, rgsq)
-    print '(a,i0,a,es14.6)', 'nts = ', nts, '  rgsq = ', rgsq
+call dump(nts)
+do
+    if (nts > nts_sim) exit
+    nxt = next_event(nts)
+    rc = bd_run(ctx, int(nxt - nts, c_int), c_null_ptr, 0_c_long_long, used)
+    if (rc /= BD_OK .and. rc /= BD_WARN_NOTCONV) then
+        ! bds_run returns without the final dump (bd_solver_m.f90:171)
+        print '(a,i0)', 'bdrun: failed in the steps after nts = ', nts
+        call fail(rc)
+    end if
+    nts = nxt
+    if (every(nts, nts_log)) print '(a,i0)', 'nts = ', nts
+    if (every(nts, nts_dump)) call dump(nts)
+    if (every(nts, nts_samp)) then
+        rc = bd_stats_get(ctx, st); if (rc /= BD_OK) call fail(rc)
+        call stats_write_line(fu_stats, nts, cfg, lvdw_stats, st)
+        if (write_traj) then
+            rc = bd_get_coordinates(ctx, cfg%coordinates); if (rc /= BD_OK) call fail(rc)
+            call traj_append(traj, nts, cfg%coordinates)
+        end if
+    end if
 end do
+call dump(nts)
 call system_clock(t1)
 rc = bd_get_runinfo(ctx, info)
 print '(a,es12.4)', 'execution time(s) ', real(t1 - t0, rp) / real(rate, rp)
 print '(a,i0,a,i0)', 'mobility updates ', info%n_mob_updates, '  Lanczos iterations ', info%total_lanczos_iters
-rc = bd_get_coordinates(ctx, coordinates)
-open(newunit=i, file=trim(fn_out)//trim(job_tag), status='replace', action='write')
-write(i, '(i0)') num_atoms
-write(i, '(a,i0)') 'nts = ', nts
-do iarg = 1, num_atoms
-    write(i, '(a,3es24.16)') 'M ', coordinates(:, iarg)
+open(newunit=fu, file=trim(fn_out)//trim(job_tag), status='replace', action='write')
+write(fu, '(i0)') cfg%num_atoms
+write(fu, '(a,i0)') 'nts = ', nts
+do i = 1, cfg%num_atoms
+    write(fu, '(a,3es24.16)') 'M ', cfg%coordinates(:, i)
 end do
-close(i)
+close(fu)
+close(fu_stats)
+if (write_traj) call traj_close(traj)
 call bd_ctx_destroy(ctx)
 
 contains
@@ -115,6 +211,56 @@ subroutine fail(code)
     integer(c_int), intent(in) :: code
     print '(a,i0,2a)', 'bdrun: error ', code, ': ', bd_error_message()
     stop 1
+end subroutine
+
+logical function every(n, k)
+    integer(c_int64_t), intent(in) :: n, k
+    every = .false.
+    if (k > 0) every = (mod(n, k) == 0)
+end function
+
+!> The first nts after n at which the loop logs, dumps or samples, or at
+!> which it ends (nts_sim + 1); at most a million steps per bd_run call.
+function next_event(n) result(t)
+    integer(c_int64_t), intent(in) :: n
+    integer(c_int64_t) :: t
+    t = min(nts_sim + 1, n + 1000000)
+    if (nts_log > 0) t = min(t, (n / nts_log + 1) * nts_log)
+    if (nts_dump > 0) t = min(t, (n / nts_dump + 1) * nts_dump)
+    if (nts_samp > 0) t = min(t, (n / nts_samp + 1) * nts_samp)
+end function
+
+!> write_dump at nts of the device state: coordinates, molecule_com, and
+!> the generator's position.
+subroutine dump(n)
+    integer(c_int64_t), intent(in) :: n
+    integer(c_int64_t) :: sd, off
+    rc = bd_get_coordinates(ctx, cfg%coordinates); if (rc /= BD_OK) call fail(rc)
+    rc = bd_get_runinfo(ctx, info); if (rc /= BD_OK) call fail(rc)
+    if (limud) cfg%molecule_com = info%molecule_com
+    rc = bd_get_rng(ctx, sd, off); if (rc /= BD_OK) call fail(rc)
+    call write_dump(trim(fn_revive)//trim(job_tag), n, cfg, sd, off)
+end subroutine
+
+!> Parts of atmcfg_t the device step does not evaluate.
+subroutine check_supported()
+    if (cfg%num_angles > 0 .or. cfg%num_dihedrals > 0 .or. cfg%num_tethers > 0 .or. &
+            cfg%num_externals > 0 .or. cfg%num_branches > 0) then
+        print '(a)', 'bdrun: angles, dihedrals, tethers, external fields and branches are not evaluated on the device'
+        stop 2
+    end if
+end subroutine
+
+!> vdw_styles/params in the .cfg's row order: row typ serves the atom-type
+!> pair that interaction_m.f90:182-186 maps to typ.
+subroutine vdw_table()
+    nvdw = cfg%num_vdw_types
+    allocate(vdw_styles(max(nvdw, 1)), vdw_params(mxparam, max(nvdw, 1)))
+    vdw_styles = 0; vdw_params = 0
+    if (nvdw > 0) then
+        vdw_styles(1:nvdw) = cfg%vdw_styles
+        vdw_params(:, 1:nvdw) = cfg%vdw_params
+    end if
 end subroutine
 
 function pt_code(m) result(c)
@@ -165,12 +311,21 @@ subroutine read_control(fn)
         case ('lanc_mxitr'); read(val, *) lanc_mxitr
         case ('lanc_tol'); read(val, *) lanc_tol
         case ('bdintg'); read(val, *) bdintg
+        case ('stats_binsize'); read(val, *) stats_binsize
         case ('tim_stp'); read(val, *) tim_stp
-        case ('nts_sim'); read(val, *) d; nts_sim = int(d, c_long_long)
+        case ('nts_sim'); read(val, *) d; nts_sim = int(d, c_int64_t)
         case ('nts_mobsam'); read(val, *) d; nts_mobsam = int(d)
-        case ('nts_log'); read(val, *) d; nts_log = int(d, c_long_long)
-        case ('nts_samp'); read(val, *) d; nts_samp = int(d, c_long_long)
+        case ('nts_dump'); read(val, *) d; nts_dump = int(d, c_int64_t)
+        case ('nts_samp'); read(val, *) d; nts_samp = int(d, c_int64_t)
+        case ('nts_log'); read(val, *) d; nts_log = int(d, c_int64_t)
+        case ('lrevive'); read(val, *) lrevive
+        case ('read_seed'); read(val, *) read_seed
+        case ('write_seed'); read(val, *) write_seed
+        case ('write_traj'); read(val, *) write_traj
         case ('fn_cfg'); fn_cfg = trim(val)
+        case ('fn_revive'); fn_revive = trim(val)
+        case ('fn_stats'); fn_stats = trim(val)
+        case ('fn_traj'); fn_traj = trim(val)
         case ('fn_out'); fn_out = trim(val)
         case ('precision'); read(val, *) precision
         case ('lanc_reorth'); read(val, *) lanc_reorth
@@ -178,76 +333,6 @@ subroutine read_control(fn)
         end select
     end do
     close(fu)
-end subroutine
-
-!> .cfg reader for the sections the BD step uses (config_io_m.f90:247-544).
-subroutine read_cfg(fn)
-    character(len=*), intent(in) :: fn
-    character(len=512) :: line, word
-    integer :: fu, ios, k, n, npar, p, t1, t2
-    real(rp) :: mat(3,3)
-    character(len=32) :: name
-    open(newunit=fu, file=fn, action='read', status='old')
-    do
-        read(fu, '(a)', iostat=ios) line
-        if (ios /= 0) exit
-        p = index(line, '#'); if (p > 0) line = line(:p-1)
-        line = adjustl(line)
-        if (len_trim(line) == 0) cycle
-        read(line, *) word
-        select case (trim(word))
-        case ('SIMBOX')
-            read(fu, *) mat(:, 1); read(fu, *) mat(:, 2); read(fu, *) mat(:, 3)
-            box = [mat(1,1), mat(2,2), mat(3,3)]
-        case ('IMCON')
-            read(line, *) word, imcon
-        case ('ATOM_TYPES')
-            read(line, *) word, natom_types
-            do k = 1, natom_types
-                read(fu, *) name
-            end do
-        case ('ATOMS')
-            read(line, *) word, num_atoms
-            allocate(atoms(num_atoms), coordinates(3, num_atoms))
-            do k = 1, num_atoms
-                read(fu, *) atoms(k), mat(1,1), coordinates(:, k)
-            end do
-        case ('BOND_TYPES')
-            read(line, *) word, nbond_types
-            allocate(bond_styles(nbond_types), bond_params(mxparam, nbond_types))
-            bond_params = 0
-            do k = 1, nbond_types
-                read(fu, '(a)') line
-                read(line, *) bond_styles(k), npar
-                if (npar > 0) read(line, *) t1, t2, bond_params(1:npar, k)
-            end do
-        case ('BONDS')
-            read(line, *) word, num_bonds
-            allocate(bonds(3, num_bonds))
-            do k = 1, num_bonds
-                read(fu, *) bonds(:, k)
-            end do
-        case ('VDW')
-            read(line, *) word, nvdw
-            allocate(vdw_styles(nvdw), vdw_params(mxparam, nvdw))
-            vdw_params = 0
-            do k = 1, nvdw
-                read(fu, '(a)') line
-                read(line, *) t1, t2, vdw_styles(k), npar
-                if (npar > 0) read(line, *) t1, t2, n, n, vdw_params(1:npar, k)
-            end do
-        case ('FLOW_FIELD')
-            read(fu, '(a)') line
-            read(line, *) flow_style
-            if (flow_style > 0) read(line, *) flow_style, npar, flow_rate
-        end select
-    end do
-    close(fu)
-    if (.not. allocated(bonds)) allocate(bonds(3, 1), bond_styles(1), bond_params(mxparam, 1))
-    if (.not. allocated(vdw_styles)) allocate(vdw_styles(1), vdw_params(mxparam, 1))
-    if (num_bonds == 0) then
-        bonds = 0; bond_styles = 0; bond_params = 0
-    end if
 end subroutine
 
 end program bdrun

@@ -124,6 +124,40 @@ interface
         integer(c_int64_t), value :: seed, offset
         integer(c_int) :: rc
     end function
+    function bd_get_rng(ctx, seed, offset) bind(C, name="bd_get_rng") result(rc)
+        import :: c_ptr, c_int, c_int64_t
+        type(c_ptr), value :: ctx
+        integer(c_int64_t), intent(out) :: seed, offset
+        integer(c_int) :: rc
+    end function
+    function bd_resume_run(ctx, com) bind(C, name="bd_resume_run") result(rc)
+        import :: c_ptr, c_int, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(in) :: com(3)
+        integer(c_int) :: rc
+    end function
+    function bd_stats_enable(ctx, binsize, binsize_stress, nmolecules, molecules) &
+            bind(C, name="bd_stats_enable") result(rc)
+        import :: c_ptr, c_int
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: binsize, binsize_stress, nmolecules
+        integer(c_int), intent(in) :: molecules(*)
+        integer(c_int) :: rc
+    end function
+    function bd_stats_get(ctx, out) bind(C, name="bd_stats_get") result(rc)
+        import :: c_ptr, c_int, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(out) :: out(30)
+        integer(c_int) :: rc
+    end function
+    function bd_mob_symv(ctx, x, s, y) bind(C, name="bd_mob_symv") result(rc)
+        import :: c_ptr, c_int, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(in) :: x(*)
+        integer(c_int), value :: s
+        real(c_double), intent(out) :: y(*)
+        integer(c_int) :: rc
+    end function
     function bd_get_runinfo(ctx, info) bind(C, name="bd_get_runinfo") result(rc)
         import :: c_ptr, c_int, bd_runinfo_t
         type(c_ptr), value :: ctx

@@ -178,11 +178,23 @@ int bd_pair_table(bd_ctx *ctx, int *pairs2, int cap, int *npairs, int *rebuilt);
  * PBC wrap.  xi is the injected Gaussian stream consumed exactly as
  * brn_calc_dw would (n*s values per mobility update with HI, n per step
  * without); *used receives the number of values consumed.  With xi ==
- * NULL the device Philox generator (seeded by bd_set_rng) supplies it. */
+ * NULL the device Philox generator (seeded by bd_set_rng) supplies it.
+ * The first call after bd_set_coordinates or bd_reset_run also moves an
+ * isolated molecule's centre of mass to the origin before its first step
+ * (bd_solver_m.f90:161); later calls continue the same loop, so a run
+ * split into several calls takes the same steps as one call (nsteps 0
+ * does only that centring). */
 int bd_run(bd_ctx *ctx, int nsteps, const double *xi, long long nxi,
            long long *used);
-/* Seeds the device Gaussian generator (Philox-4x32-10 + Box-Muller). */
+/* Seeds the device Gaussian generator (Philox-4x32-10 + Box-Muller).  The
+ * generator's whole state is (seed, offset): offset counts the Gaussian
+ * values drawn (rounded up to even per draw), so bd_get_rng after a run and
+ * bd_set_rng with the same pair in a new context continue the same stream.
+ * Replaces init_stream / save_seed (random_m.f90:57-86, 118-129), whose
+ * MT19937 stream is not saved with the revive file (save_stream,
+ * random_m.f90:135-147, is never called). */
 int bd_set_rng(bd_ctx *ctx, uint64_t seed, uint64_t offset);
+int bd_get_rng(bd_ctx *ctx, uint64_t *seed, uint64_t *offset);
 
 /* Run statistics: nts, molecule_com[3] (accumulated COM shift),
  * Lanczos iterations of the last update, total iterations, mobility
@@ -203,6 +215,39 @@ int bd_reset_run(bd_ctx *ctx);
 
 /* Rg^2 of the device coordinates (compute_ic0, stats_m.f90:441-446). */
 int bd_rgsq(bd_ctx *ctx, double *rgsq);
+
+/* Continues a run from a revive file (read_dump, config_io_m.f90:15-172):
+ * sets atc%molecule_com, the accumulated centre-of-mass shift of an
+ * isolated molecule, and marks the coordinates just set as mid-run, so the
+ * next bd_run does not centre them again.  The reference's revived bds_run
+ * does centre again (bd_solver_m.f90:161), which only moves coordinates
+ * that a step left centred by rounding; skipping it lets a revived run,
+ * with bd_set_rng restoring the generator, repeat the uninterrupted run
+ * bit for bit. */
+int bd_resume_run(bd_ctx *ctx, const double *com3);
+
+/* Run statistics on the device (stats_init / stats_accumulate,
+ * stats_m.f90:53-105, 299-361).  After bd_stats_enable every step of bd_run
+ * (and bd_run_record) accumulates, on the device, the running means of
+ * stats_accumulate over bins of binsize steps (energies, rgsq, reedsq, span)
+ * and binsize_stress steps (stress), from the energies and stress of the
+ * step's force evaluation and the structure of the end-of-step coordinates:
+ * compute_ic0 (one molecule, imcon 0) or compute_ic1 (imcon 1, molecules
+ * (2, nmolecules) = (atom count, first atom 1-based), i.e. rows 2 and 3 of
+ * atc%molecules).  Enabling again restarts the accumulators (stats_init
+ * after a revive).  bd_stats_get returns, in the order stats_write prints
+ * them (stats_m.f90:235-290):
+ *   out[0..2]   bndlen, bndlen_min, bndlen_max of the last step
+ *   out[3..9]   energy accumulators: bond, angle, dihedral, vdw, tether,
+ *               external, total
+ *   out[10..18] stress accumulator, 3 x 3 column-major
+ *   out[19..20] rgsq, reedsq accumulators
+ *   out[21..23] span accumulator
+ *   out[24..26] reev of the last step
+ *   out[27..29] molecule_com */
+int bd_stats_enable(bd_ctx *ctx, int binsize, int binsize_stress,
+                    int nmolecules, const int *molecules);
+int bd_stats_get(bd_ctx *ctx, double *out30);
 
 /* Synchronises the context's stream (timing helper). */
 int bd_sync(bd_ctx *ctx);
