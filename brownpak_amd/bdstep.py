@@ -59,6 +59,7 @@ EXPORTS = (
     "bd_batch_get_coordinates", "bd_batch_set_rng", "bd_batch_run", "bd_batch_info", "bd_batch_prof",
     "bd_batch_tile_shape", "bd_batch_set_tile_shape", "bd_plan_layout", "bd_rpy_apply",
     "bd_rpy_apply_noalloc", "bd_get_rng", "bd_resume_run", "bd_stats_enable", "bd_stats_get",
+    "bd_set_angles", "bd_set_tethers", "bd_set_externals", "bd_set_charges", "bd_energies",
 )
 
 # bd_stats_get's output, in stats_write's column order (stats_m.f90:235-290)
@@ -147,6 +148,11 @@ def load_library(path: str = LIB_PATH):
     L.bd_resume_run.argtypes = [vp, DP]
     L.bd_stats_enable.argtypes = [vp, C.c_int, C.c_int, C.c_int, IP]
     L.bd_stats_get.argtypes = [vp, DP]
+    L.bd_set_angles.argtypes = [vp, C.c_int, IP, C.c_int, IP, DP]
+    L.bd_set_tethers.argtypes = [vp, C.c_int, IP, DP, C.c_int, IP, DP]
+    L.bd_set_externals.argtypes = [vp, C.c_int, IP, DP]
+    L.bd_set_charges.argtypes = [vp, DP]
+    L.bd_energies.argtypes = [vp, DP]
     L.bd_get_runinfo.argtypes = [vp, C.POINTER(BdRuninfo)]
     L.bd_reset_run.argtypes = [vp]
     L.bd_rgsq.argtypes = [vp, DP]
@@ -270,6 +276,41 @@ class BdStep:
             int(natom_types), _ip(self._atypes), len(vdw_styles), _ip(self._vstyles),
             _dp(self._vparams), int(bool(lvdw)), int(excluded_atoms), mth, float(rcutoff),
             float(tskin)))
+
+    def set_angles(self, angles, styles, params):
+        """angles: rows [type, i-1, i, i+1] 1-based; params (ntypes, <= 12)."""
+        a = _i32(np.reshape(angles, (-1, 4))) if len(angles) else np.zeros((1, 4), np.int32)
+        st = _i32(styles) if len(styles) else np.zeros(1, np.int32)
+        p = _params(params, len(styles))
+        self._chk(self.L.bd_set_angles(self.h, len(angles), _ip(a), len(styles), _ip(st), _dp(p)))
+
+    def set_tethers(self, tethers, points, styles, params):
+        """tethers: rows [type, atom] 1-based; points (n, 3)."""
+        t = _i32(np.reshape(tethers, (-1, 2))) if len(tethers) else np.zeros((1, 2), np.int32)
+        pts = _f64(np.reshape(points, (-1, 3))) if len(tethers) else np.zeros((1, 3))
+        st = _i32(styles) if len(styles) else np.zeros(1, np.int32)
+        p = _params(params, len(styles))
+        self._chk(self.L.bd_set_tethers(self.h, len(tethers), _ip(t), _dp(pts), len(styles), _ip(st), _dp(p)))
+
+    def set_externals(self, styles, params):
+        st = _i32(styles) if len(styles) else np.zeros(1, np.int32)
+        p = _params(params, len(styles))
+        self._chk(self.L.bd_set_externals(self.h, len(styles), _ip(st), _dp(p)))
+
+    def set_charges(self, charge):
+        if charge is None:
+            self._chk(self.L.bd_set_charges(self.h, None))
+        else:
+            q = _f64(charge)
+            assert q.size == self.nbeads
+            self._chk(self.L.bd_set_charges(self.h, _dp(q)))
+
+    def energies(self):
+        """Energies of the last evaluation: dict in stats order."""
+        e = np.zeros(7)
+        self._chk(self.L.bd_energies(self.h, _dp(e)))
+        return dict(zip(("energy_bond", "energy_angle", "energy_dihedral", "energy_vdw", "energy_tether",
+                         "energy_external", "energy_tot"), e))
 
     def set_coordinates(self, crd):
         crd = _f64(crd).reshape(self.nbeads, 3)

@@ -214,7 +214,7 @@ int bd_run_record(bd_ctx *h, int nsteps, const double *xi, long long nxi, long l
     DevBuf rec;
     BD_TRY(rec.alloc((size_t)c.npad * std::max(nsteps, 1)));
     BD_HIP(hipMemsetAsync(&c.st->steps_done, 0, sizeof(int), c.stream));
-    if (c.imcon == 0)
+    if (c.limud())
         BD_TRY(launch_to_origin(c, nullptr));
     long long pos = 0;
     int rc = BD_OK;
@@ -242,7 +242,7 @@ int bd_run_replay(bd_ctx *h, int nsteps, const double *disp)
     BD_HIP(hipMemcpy2D(dd.p, (size_t)c.npad * sizeof(double), disp, (size_t)c.n * sizeof(double),
                        (size_t)c.n * sizeof(double), nsteps, hipMemcpyHostToDevice));
     BD_HIP(hipMemsetAsync(&c.st->steps_done, 0, sizeof(int), c.stream));
-    if (c.imcon == 0)
+    if (c.limud())
         BD_TRY(launch_to_origin(c, nullptr));
     int rc = replay_steps(c, dd.p, 0, nsteps, nullptr);
     int rc2 = check_status(c, false);
@@ -258,6 +258,11 @@ int bd_adjoint_rgsq(bd_ctx *h, int nsteps, const double *disp, int ckpt, int tie
     if (!c.have_topology) { set_error("topology not set"); return BD_ERR_STATE; }
     if (c.imcon != 0) { set_error("the Rg^2 adjoint is for an isolated chain (imcon = 0)"); return BD_ERR_ARG; }
     if (c.opt.integrator != BD_EM) { set_error("the Rg^2 adjoint differentiates the EM step"); return BD_ERR_ARG; }
+    if (c.vdw_general || c.nangles || c.ntethers || c.nexternals) {
+        set_error("the Rg^2 adjoint differentiates bonds and LJ pairs only (no angles, tethers, external fields "
+                  "or vdw styles 2-6)");
+        return BD_ERR_ARG;
+    }
     const int s = c.opt.lhdia ? c.s : 1;
     if (ckpt <= 0 || ckpt % s != 0) { set_error("ckpt must be a positive multiple of nts_mobsam"); return BD_ERR_ARG; }
     if (c.cntr_mobsam != 0) { set_error("the adjoint needs a run that starts a mobility block"); return BD_ERR_STATE; }

@@ -373,6 +373,8 @@ int check_status(Ctx &c, bool clear)
     if (e == 1) { set_error("bond length too large (overstretched bond)"); return BD_ERR_BOND; }
     if (e == 2) { set_error("factorisation failed"); return BD_ERR_FACTOR; }
     if (e == 4) { set_error("negative coordinate in cell-list mode (cell_list_m.f90:193-196)"); return BD_ERR_CELL; }
+    if (e == 5) { set_error("rigid tether violated (ia_tether_m.f90:110-128)"); return BD_ERR_BOND; }
+    if (e == 6) { set_error("external wall crossed (ia_external_m.f90:93-101)"); return BD_ERR_BOND; }
     return BD_OK;
 }
 
@@ -451,7 +453,7 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
     }
     // ---- calc_drift (bd_solver_m.f90:407-454): drift = dt (U(r) + D F(r))
     // (an EM step of an isolated molecule leaves the drift to its tail kernel)
-    const bool fused_tail = c.opt.integrator != BD_PC && c.imcon == 0;
+    const bool fused_tail = c.opt.integrator != BD_PC && c.limud();
     const double *vdrift = nullptr;
     auto drift_at_crd = [&](bool launch, bool energies) -> int {
         BD_TRY(compute_forces(c, energies, false));
@@ -495,7 +497,7 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
         hipLaunchKernelGGL(update_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, c.drift, dcol, nb, c.imcon, c.box[0],
                            c.box[1], c.box[2], c.st);
         // ---- bds_run tail (bd_solver_m.f90:176-186)
-        if (c.imcon == 0)
+        if (c.limud())
             hipLaunchKernelGGL(to_origin_kernel, dim3(1), dim3(1024), 0, sm, c.crd, nb, c.scal + 16, c.st);
     }
     // stats_accumulate (bd_solver_m.f90:189) on the end-of-step coordinates
@@ -743,6 +745,48 @@ int bd_forces(bd_ctx *h, double *frc, double *energies, double *stress9, double 
     return BD_OK;
 }
 
+int bd_energies(bd_ctx *h, double *energies7)
+{
+    if (!h || !energies7) { set_error("null argument"); return BD_ERR_ARG; }
+    Ctx &c = h->c;
+    double e[32];
+    BD_HIP(hipStreamSynchronize(c.stream));
+    BD_HIP(hipMemcpy(e, c.scal, sizeof(e), hipMemcpyDeviceToHost));
+    const double out[7] = {e[0], e[24], e[25], e[1], e[26], e[27], e[2]};
+    for (int k = 0; k < 7; k++) energies7[k] = out[k];
+    return BD_OK;
+}
+
+int bd_set_angles(bd_ctx *h, int nangles, const int *angles, int nangle_types, const int *angle_styles,
+                  const double *angle_params)
+{
+    if (!h) { set_error("null context"); return BD_ERR_ARG; }
+    BD_TRY(set_angles(h->c, nangles, angles, nangle_types, angle_styles, angle_params));
+    return BD_OK;
+}
+
+int bd_set_tethers(bd_ctx *h, int ntethers, const int *tethers, const double *tether_points, int ntether_types,
+                   const int *tether_styles, const double *tether_params)
+{
+    if (!h) { set_error("null context"); return BD_ERR_ARG; }
+    BD_TRY(set_tethers(h->c, ntethers, tethers, tether_points, ntether_types, tether_styles, tether_params));
+    return BD_OK;
+}
+
+int bd_set_externals(bd_ctx *h, int nexternals, const int *external_styles, const double *external_params)
+{
+    if (!h) { set_error("null context"); return BD_ERR_ARG; }
+    BD_TRY(set_externals(h->c, nexternals, external_styles, external_params));
+    return BD_OK;
+}
+
+int bd_set_charges(bd_ctx *h, const double *charge)
+{
+    if (!h) { set_error("null context"); return BD_ERR_ARG; }
+    BD_TRY(set_charges(h->c, charge));
+    return BD_OK;
+}
+
 int bd_pair_table(bd_ctx *h, int *pairs2, int cap, int *npairs, int *rebuilt)
 {
     if (!h || !npairs) { set_error("null argument"); return BD_ERR_ARG; }
@@ -792,7 +836,7 @@ int bd_run(bd_ctx *h, int nsteps, const double *xi, long long nxi, long long *us
     long long pos = 0;
     const long long nts0 = c.info.nts;
     BD_HIP(hipMemsetAsync(&c.st->steps_done, 0, sizeof(int), c.stream));
-    if (c.imcon == 0 && c.origin_pending) {
+    if (c.limud() && c.origin_pending) {
         // isolated molecule: COM to the origin before the loop (bd_solver_m.f90:161),
         // once per run: later calls continue the same bds_run loop
         hipLaunchKernelGGL(to_origin_kernel, dim3(1), dim3(1024), 0, c.stream, c.crd, c.nb, (double *)nullptr, c.st);

@@ -30,8 +30,8 @@ __device__ inline bool excluded(const int *ex_ptr, const int *ex_col, int i, int
 
 // ------------------------------------------------------------ energy sums
 // acc layout: 0 e_bond, 1 e_vdw, 2..10 stress(3x3 col-major), 11 bndlen sum,
-// 12 bndlen min, 13 bndlen max
-constexpr int NACC = 14;
+// 12 bndlen min, 13 bndlen max, 14 e_angle, 15 e_tether, 16 e_external
+constexpr int NACC = 17;
 
 __device__ void block_acc_store(double (&a)[NACC], double *out)
 {
@@ -61,6 +61,25 @@ __device__ void block_acc_store(double (&a)[NACC], double *out)
 // butterfly order (a handful of terms per lane instead of a chain of
 // dependent loads per bead).  Energies/stress/bond lengths are counted once
 // per term: bonds from their first bead, pairs from the lower index.
+// Angles, tethers, external fields and charges (bd_set_angles, bd_set_tethers,
+// bd_set_externals, bd_set_charges): per-bead CSR lists so that every bead's
+// wavefront gathers its own terms.
+struct ExtraArg {
+    int nangles, ntethers;            // 0: the term's lists are not set (null)
+    const int *ang_ptr, *ang_idx;     // bead -> 4 * angle + role (0: i-1, 1: i, 2: i+1)
+    const int *ang_atoms, *ang_type;  // (3, nangles) 0-based beads; type 0-based
+    const int *ang_styles;
+    const double *ang_params;
+    const int *teth_ptr, *teth_idx;   // bead -> tether
+    const int *teth_type, *teth_styles;
+    const double *teth_pts, *teth_params;
+    int nexternals;
+    const int *ext_styles;
+    const double *ext_params;
+    const double *charge;             // null: all charges 0
+};
+
+template <bool EXTRA, bool GEN>
 __global__ __launch_bounds__(FB) void force_gather_kernel(
     const double *__restrict__ crd, int nb, BoxArg bx, const int *__restrict__ bond_ptr,
     const int *__restrict__ bond_idx, const int *__restrict__ bond_ij, const int *__restrict__ bond_type,
@@ -68,8 +87,8 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
     const int *__restrict__ nbr_cnt, const int *__restrict__ nbr_idx, int nbr_cap,
     const double *__restrict__ crd_ref, double ref_rc2, const int *__restrict__ ex_ptr,
     const int *__restrict__ ex_col, const int *__restrict__ atype, int natype, const int *__restrict__ ptype, const int *__restrict__ vstyles,
-    const double *__restrict__ vparams, double *__restrict__ frc, int want_energy, double *__restrict__ epart,
-    DevStatus *st)
+    const double *__restrict__ vparams, ExtraArg xa, double *__restrict__ frc, int want_energy,
+    double *__restrict__ epart, DevStatus *st)
 {
     const int lane = threadIdx.x % 64;
     const int i = blockIdx.x * FW + threadIdx.x / 64;
@@ -121,12 +140,18 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
                     j = nbr_idx[(long)k * nb + i];
                 }
                 int t = ptype[ati * natype + atype[j]];
-                if (vstyles[t] != 1) continue;
+                const int sty = vstyles[t];
+                if (GEN ? sty == 0 : sty != 1) continue;
                 double x, y, z;
                 pair_vec(crd, i, j, bx, x, y, z);
                 double r = sqrt(x * x + y * y + z * z);
                 double en, f;
-                lj_force(r, vparams + t * MXPARAM, en, f);
+                if (GEN) {
+                    const double qq = xa.charge ? xa.charge[i] * xa.charge[j] : 0.0;
+                    vdw_force(r, qq, sty, vparams + t * MXPARAM, en, f);
+                } else {
+                    lj_force(r, vparams + t * MXPARAM, en, f);
+                }
                 double gx = f * x / r, gy = f * y / r, gz = f * z / r;
                 fx += gx; fy += gy; fz += gz;
                 if (want_energy && i < j) {
@@ -134,6 +159,63 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
                     double rv[3] = {x, y, z}, fv[3] = {gx, gy, gz};
                     for (int bb = 0; bb < 3; bb++)
                         for (int a = 0; a < 3; a++) acc[2 + a + 3 * bb] -= rv[a] * fv[bb];
+                }
+            }
+        }
+        if (EXTRA) {
+            // ia_add_angle_forces (interaction_m.f90:276-325): the bead's share
+            // of each of its angles; energy and stress once, at the centre bead
+            const int ka0 = xa.nangles > 0 ? xa.ang_ptr[i] : 0, ka1 = xa.nangles > 0 ? xa.ang_ptr[i + 1] : 0;
+            for (int k = ka0 + lane; k < ka1; k += 64) {
+                const int code = xa.ang_idx[k], a = code >> 2, role = code & 3;
+                const int b0 = xa.ang_atoms[3 * a], b1 = xa.ang_atoms[3 * a + 1], b2 = xa.ang_atoms[3 * a + 2];
+                double q1[3], q2[3];
+                pair_vec(crd, b0, b1, bx, q1[0], q1[1], q1[2]);
+                pair_vec(crd, b1, b2, bx, q2[0], q2[1], q2[2]);
+                const int t = xa.ang_type[a];
+                double en, f0[3], f2[3];
+                angle_force(q1, q2, xa.ang_styles[t], xa.ang_params + t * MXPARAM, en, f0, f2);
+                if (role == 0) { fx += f0[0]; fy += f0[1]; fz += f0[2]; }
+                if (role == 2) { fx += f2[0]; fy += f2[1]; fz += f2[2]; }
+                if (role == 1) {
+                    fx -= f0[0] + f2[0]; fy -= f0[1] + f2[1]; fz -= f0[2] + f2[2];
+                    if (want_energy) {
+                        acc[14] += en;
+                        for (int bb = 0; bb < 3; bb++)
+                            for (int aa = 0; aa < 3; aa++)
+                                acc[2 + aa + 3 * bb] += -q1[aa] * f0[bb] + q2[aa] * f2[bb];
+                    }
+                }
+            }
+            // ia_add_tether_forces (interaction_m.f90:375-418), no imaging
+            const int kt0 = xa.ntethers > 0 ? xa.teth_ptr[i] : 0, kt1 = xa.ntethers > 0 ? xa.teth_ptr[i + 1] : 0;
+            for (int k = kt0 + lane; k < kt1; k += 64) {
+                const int tt = xa.teth_idx[k], t = xa.teth_type[tt];
+                double q[3];
+                for (int d = 0; d < 3; d++) q[d] = crd[3 * i + d] - xa.teth_pts[3 * tt + d];
+                const double qm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+                double en, f;
+                if (tether_force(qm, xa.teth_styles[t], xa.teth_params + t * MXPARAM, en, f))
+                    atomicCAS(&st->ierr, 0, 5);
+                const double fj[3] = {-f * q[0] / qm, -f * q[1] / qm, -f * q[2] / qm};
+                fx += fj[0]; fy += fj[1]; fz += fj[2];
+                if (want_energy) {
+                    acc[15] += en;
+                    for (int bb = 0; bb < 3; bb++)
+                        for (int aa = 0; aa < 3; aa++) acc[2 + aa + 3 * bb] += q[aa] * fj[bb];
+                }
+            }
+            // ia_add_external_forces (ia_external_m.f90:61-107)
+            if (lane == 0) {
+                for (int e = 0; e < xa.nexternals; e++) {
+                    const double *p = xa.ext_params + e * MXPARAM;
+                    if (xa.ext_styles[e] == 1 && (int)p[0] - 1 == i) {
+                        fx += p[1];
+                        if (want_energy) acc[16] += -p[1] * (crd[3 * i] - crd[0]);
+                    } else if (xa.ext_styles[e] == 2) {
+                        const double x = crd[3 * i + (int)p[0] - 1];
+                        if (p[2] > 0.0 ? x < p[1] : x > p[1]) atomicCAS(&st->ierr, 0, 6);
+                    }
                 }
             }
         }
@@ -173,11 +255,17 @@ __global__ void energy_final_kernel(const double *__restrict__ epart, int nblk, 
     for (int q = 0; q < NACC; q++) a[q] = sa[q];
     out[0] = a[0];
     out[1] = a[1];
-    out[2] = a[0] + a[1];
+    // energy_tot in the reference's order: bond, angle, dihedral (0), vdw,
+    // tether, external (interaction_m.f90:141-142)
+    out[2] = a[0] + a[14] + 0.0 + a[1] + a[15] + a[16];
     for (int k = 0; k < 9; k++) out[3 + k] = imcon ? a[2 + k] / vol : a[2 + k];
     out[12] = nbonds > 0 ? a[11] / nbonds : 0.0;
     out[13] = nbonds > 0 ? a[12] : 0.0;
     out[14] = a[13];
+    out[24] = a[14];
+    out[25] = 0.0;
+    out[26] = a[15];
+    out[27] = a[16];
 }
 
 // ------------------------------------------------------------ pair lists
@@ -685,6 +773,40 @@ static int upload(T **dst, const std::vector<T> &v)
     return BD_OK;
 }
 
+// The derived slots of ia_vdw_setup's setters (ia_vdw_m.f90:85-151, 185-206,
+// 270-306, 356-389): cut-off energies (and LJ's cut-off force) per style.
+static void vdw_derived(int sty, double *p)
+{
+    auto lj_at = [](double eps, double sigma, double rc) {
+        const double sir = sigma / rc, sir2 = sir * sir, sir6 = sir2 * sir2 * sir2, sir12 = sir6 * sir6;
+        return std::make_pair(4 * eps * (sir12 - sir6), -24 * eps * (2 * sir12 - sir6) / rc);
+    };
+    switch (sty) {
+    case 1: {  // vdw_lj_set
+        auto v = lj_at(p[0], p[1], p[2]);
+        p[3] = v.first;
+        p[4] = v.second;
+        break;
+    }
+    case 2: {  // vdw_gaussian_set
+        const double ex = std::exp(-p[1] * (p[2] * p[2]));
+        p[3] = p[0] * ex;
+        p[4] = -2.0 * p[0] * p[1] * p[2] * ex;
+        break;
+    }
+    case 4:  // vdw_lj_coul_debye_set
+        p[6] = lj_at(p[0], p[1], p[2]).first;
+        p[7] = p[3] > 0.0 ? p[4] * std::exp(-p[5] * p[3]) / p[3] : 0.0;
+        break;
+    case 5:  // vdw_lj_coul_set
+        p[5] = lj_at(p[0], p[1], p[2]).first;
+        p[6] = p[3] > 0.0 ? p[4] / p[3] : 0.0;
+        break;
+    default:
+        break;
+    }
+}
+
 int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const int *bond_styles,
                  const double *bond_params, int natom_types, const int *atom_types, int nvdw_types,
                  const int *vdw_styles, const double *vdw_params)
@@ -741,6 +863,7 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const in
             ptype[(a - 1) * nat + (b - 1)] = typ - 1;
         }
     std::vector<int> vstyle(std::max(nvdw_types, 1), 0);
+    bool general = false;
     std::vector<double> vpar((size_t)std::max(nvdw_types, 1) * MXPARAM, 0.0);
     if (c.lvdw && nvdw_types > 0 && nvdw_types < nat * (nat + 1) / 2) {
         set_error("VDW section must list all nat*(nat+1)/2 type pairs");
@@ -748,17 +871,13 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const in
     }
     for (int t = 0; t < nvdw_types; t++) {
         vstyle[t] = vdw_styles[t];
-        if (vstyle[t] != 0 && vstyle[t] != 1) {
-            set_error("vdw style " + std::to_string(vstyle[t]) + " is not supported (only 1: LJ)");
+        if (vstyle[t] < 0 || vstyle[t] > 6) {
+            set_error("vdw style " + std::to_string(vstyle[t]) + " is not one of 0..6 (ia_vdw_m.f90:21-47)");
             return BD_ERR_ARG;
         }
         for (int k = 0; k < MXPARAM; k++) vpar[t * MXPARAM + k] = vdw_params[t * MXPARAM + k];
-        if (vstyle[t] == 1) {  // vdw_lj_set, ia_vdw_m.f90:107-151
-            double *p = &vpar[t * MXPARAM];
-            double sir = p[1] / p[2], sir2 = sir * sir, sir6 = sir2 * sir2 * sir2, sir12 = sir6 * sir6;
-            p[3] = 4 * p[0] * (sir12 - sir6);
-            p[4] = -24 * p[0] * (2 * sir12 - sir6) / p[2];
-        }
+        vdw_derived(vstyle[t], &vpar[t * MXPARAM]);
+        if (vstyle[t] >= 2) general = true;
     }
     std::vector<int> eptr;
     std::vector<int> ecol = host_exclusions(nb, bi, bj, c.excluded_atoms, eptr);
@@ -771,6 +890,7 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const in
     c.nbond_types = nbond_types;
     c.natom_types = nat;
     c.nvdw_types = nvdw_types;
+    c.vdw_general = general;
     BD_TRY(upload(&c.bond_ij, bij));
     BD_TRY(upload(&c.bond_type, btype));
     BD_TRY(upload(&c.bead_bond_ptr, bptr));
@@ -854,7 +974,14 @@ void forces_free(Ctx &c)
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
-    double **dp[] = {&c.bond_params, &c.vdw_params, &c.crd_save, &c.red};
+    int **ip2[] = {&c.ang_ptr, &c.ang_idx, &c.ang_atoms, &c.ang_type, &c.ang_styles, &c.teth_ptr, &c.teth_idx,
+                   &c.teth_type, &c.teth_styles, &c.ext_styles};
+    for (auto p : ip2) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    double **dp[] = {&c.bond_params, &c.vdw_params, &c.crd_save, &c.red, &c.ang_params, &c.teth_pts,
+                     &c.teth_params, &c.ext_params, &c.charge};
     for (auto p : dp) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
@@ -968,10 +1095,16 @@ int compute_forces(Ctx &c, bool want_energy, bool sync)
         set_error("internal: force partial buffer smaller than the launch");
         return BD_ERR_STATE;
     }
-    hipLaunchKernelGGL(force_gather_kernel, dim3(std::max(nblk, 1)), dim3(FB), 0, c.stream, c.crd, nb, bx,
-                       c.bead_bond_ptr, c.bead_bond_idx, c.bond_ij, c.bond_type, c.bond_styles, c.bond_params,
-                       pairs ? 1 : 0, c.nbr_cnt, c.nbr_idx, c.pair_cap, ver ? c.crd_save : c.crd, rl * rl, c.ex_ptr,
-                       c.ex_col, c.atom_types, c.natom_types, c.scan_tmp, c.vdw_styles, c.vdw_params, c.frc,
+    ExtraArg xa{c.nangles, c.ntethers, c.ang_ptr, c.ang_idx, c.ang_atoms, c.ang_type, c.ang_styles, c.ang_params, c.teth_ptr, c.teth_idx,
+                c.teth_type, c.teth_styles, c.teth_pts, c.teth_params, c.nexternals, c.ext_styles, c.ext_params,
+                c.charge};
+    const bool extra = c.nangles > 0 || c.ntethers > 0 || c.nexternals > 0;
+    auto kern = extra ? (c.vdw_general ? force_gather_kernel<true, true> : force_gather_kernel<true, false>)
+                      : (c.vdw_general ? force_gather_kernel<false, true> : force_gather_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3(std::max(nblk, 1)), dim3(FB), 0, c.stream, c.crd, nb, bx, c.bead_bond_ptr,
+                       c.bead_bond_idx, c.bond_ij, c.bond_type, c.bond_styles, c.bond_params, pairs ? 1 : 0,
+                       c.nbr_cnt, c.nbr_idx, c.pair_cap, ver ? c.crd_save : c.crd, rl * rl, c.ex_ptr, c.ex_col,
+                       c.atom_types, c.natom_types, c.scan_tmp, c.vdw_styles, c.vdw_params, xa, c.frc,
                        want_energy ? 1 : 0, c.red, c.st);
     BD_HIP(hipGetLastError());
     if (want_energy) {
@@ -1022,6 +1155,131 @@ int set_theta(Ctx &c, double eps, double sigma, double kb, int tie_rc)
     }
     c.verlet_first = true;
     return BD_OK;
+}
+
+// ---------------------------------------------- angles, tethers, externals
+
+// Parameters of a bonded-term type table (12 x ntypes), styles in lo..hi.
+static int check_types(const char *what, int ntypes, const int *styles, const double *params, int lo, int hi)
+{
+    if (ntypes > 0 && (!styles || !params)) {
+        set_error(std::string(what) + ": null style or parameter table");
+        return BD_ERR_ARG;
+    }
+    for (int t = 0; t < ntypes; t++)
+        if (styles[t] < lo || styles[t] > hi) {
+            set_error(std::string(what) + " style " + std::to_string(styles[t]) + " is not one of " +
+                      std::to_string(lo) + ".." + std::to_string(hi));
+            return BD_ERR_ARG;
+        }
+    return BD_OK;
+}
+
+int set_angles(Ctx &c, int nangles, const int *angles, int ntypes, const int *styles, const double *params)
+{
+    if (nangles < 0 || ntypes < 0 || (nangles > 0 && !angles)) { set_error("bad angle arguments"); return BD_ERR_ARG; }
+    BD_TRY(check_types("angle", ntypes, styles, params, 0, 1));
+    const int nb = c.nb;
+    std::vector<int> atoms(3 * (size_t)nangles), type(nangles);
+    std::vector<std::vector<int>> per(nb);
+    for (int a = 0; a < nangles; a++) {
+        const int t = angles[4 * a] - 1;
+        if (t < 0 || t >= ntypes) { set_error("angle " + std::to_string(a + 1) + ": type out of range"); return BD_ERR_ARG; }
+        type[a] = t;
+        for (int r = 0; r < 3; r++) {
+            const int b = angles[4 * a + 1 + r] - 1;
+            if (b < 0 || b >= nb) { set_error("angle " + std::to_string(a + 1) + ": bead out of range"); return BD_ERR_ARG; }
+            atoms[3 * a + r] = b;
+            per[b].push_back(4 * a + r);
+        }
+    }
+    std::vector<int> ptr(nb + 1, 0), idx;
+    for (int i = 0; i < nb; i++) {
+        idx.insert(idx.end(), per[i].begin(), per[i].end());
+        ptr[i + 1] = (int)idx.size();
+    }
+    std::vector<int> sty(styles, styles + ntypes);
+    std::vector<double> par(params, params + (size_t)ntypes * MXPARAM);   // ang_cos_set: k = params(1)
+    BD_TRY(upload(&c.ang_ptr, ptr));
+    BD_TRY(upload(&c.ang_idx, idx));
+    BD_TRY(upload(&c.ang_atoms, atoms));
+    BD_TRY(upload(&c.ang_type, type));
+    BD_TRY(upload(&c.ang_styles, sty));
+    BD_TRY(upload(&c.ang_params, par));
+    c.nangles = nangles;
+    return BD_OK;
+}
+
+int set_tethers(Ctx &c, int ntethers, const int *tethers, const double *points, int ntypes, const int *styles,
+                const double *params)
+{
+    if (ntethers < 0 || ntypes < 0 || (ntethers > 0 && (!tethers || !points))) {
+        set_error("bad tether arguments");
+        return BD_ERR_ARG;
+    }
+    BD_TRY(check_types("tether", ntypes, styles, params, 0, 2));
+    const int nb = c.nb;
+    std::vector<int> type(ntethers);
+    std::vector<double> pts(points, points + 3 * (size_t)ntethers);
+    std::vector<std::vector<int>> per(nb);
+    for (int k = 0; k < ntethers; k++) {
+        const int t = tethers[2 * k] - 1, b = tethers[2 * k + 1] - 1;
+        if (t < 0 || t >= ntypes || b < 0 || b >= nb) {
+            set_error("tether " + std::to_string(k + 1) + ": type or bead out of range");
+            return BD_ERR_ARG;
+        }
+        type[k] = t;
+        per[b].push_back(k);
+    }
+    std::vector<int> ptr(nb + 1, 0), idx;
+    for (int i = 0; i < nb; i++) {
+        idx.insert(idx.end(), per[i].begin(), per[i].end());
+        ptr[i + 1] = (int)idx.size();
+    }
+    std::vector<int> sty(styles, styles + ntypes);
+    std::vector<double> par(params, params + (size_t)ntypes * MXPARAM);
+    BD_TRY(upload(&c.teth_ptr, ptr));
+    BD_TRY(upload(&c.teth_idx, idx));
+    BD_TRY(upload(&c.teth_type, type));
+    BD_TRY(upload(&c.teth_styles, sty));
+    BD_TRY(upload(&c.teth_params, par));
+    BD_TRY(upload(&c.teth_pts, pts));
+    c.ntethers = ntethers;
+    return BD_OK;
+}
+
+int set_externals(Ctx &c, int nexternals, const int *styles, const double *params)
+{
+    if (nexternals < 0) { set_error("bad external-field arguments"); return BD_ERR_ARG; }
+    BD_TRY(check_types("external field", nexternals, styles, params, 0, 2));
+    for (int e = 0; e < nexternals; e++) {
+        const double *p = params + (size_t)e * MXPARAM;
+        if (styles[e] == 1 && ((int)p[0] < 1 || (int)p[0] > c.nb)) {
+            set_error("external field " + std::to_string(e + 1) + ": atom index out of range");
+            return BD_ERR_ARG;
+        }
+        if (styles[e] == 2 && ((int)p[0] < 1 || (int)p[0] > 3)) {
+            set_error("external field " + std::to_string(e + 1) + ": component must be 1..3");
+            return BD_ERR_ARG;
+        }
+    }
+    std::vector<int> sty(styles, styles + nexternals);
+    std::vector<double> par(params, params + (size_t)nexternals * MXPARAM);
+    BD_TRY(upload(&c.ext_styles, sty));
+    BD_TRY(upload(&c.ext_params, par));
+    c.nexternals = nexternals;
+    return BD_OK;
+}
+
+int set_charges(Ctx &c, const double *charge)
+{
+    if (!charge) {
+        if (c.charge) (void)hipFree(c.charge);
+        c.charge = nullptr;
+        return BD_OK;
+    }
+    std::vector<double> q(charge, charge + c.nb);
+    return upload(&c.charge, q);
 }
 
 int pair_table_host(Ctx &c, std::vector<int> &pairs)

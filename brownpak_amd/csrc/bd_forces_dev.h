@@ -92,4 +92,109 @@ __device__ inline void lj_force(double r, const double *p, double &enrg, double 
     }
 }
 
+// ia_get_vdw_force (ia_vdw_m.f90:51-81) for every style; qq = qi*qj.
+// Styles: 1 LJ (force-shifted, 155-180), 2 Gaussian (210-229), 3 cosine
+// (249-266), 4 LJ + screened Coulomb (310-352), 5 LJ + Coulomb (391-432),
+// 6 DPD (454-470).  The derived slots are filled by vdw_derived.
+__device__ inline void vdw_force(double r, double qq, int sty, const double *p, double &enrg, double &frc)
+{
+    enrg = 0.0;
+    frc = 0.0;
+    switch (sty) {
+    case 1:
+        lj_force(r, p, enrg, frc);
+        return;
+    case 2:
+        if (r < p[2]) {
+            const double ex = exp(-p[1] * r * r);
+            enrg = p[0] * ex - p[3] - (r - p[2]) * p[4];
+            frc = -2 * p[0] * p[1] * r * ex - p[4];
+        }
+        return;
+    case 3: {
+        const double pf = M_PI / p[1], pr = pf * r;
+        if (r < p[1]) {
+            enrg = p[0] * (1.0 + cos(pr));
+            frc = -p[0] * pf * sin(pr);
+        }
+        return;
+    }
+    case 4:
+    case 5: {
+        // 4: eps sigma rcut rcut_coul C kappa | pot_rcut pot_rcut_coul (7, 8)
+        // 5: eps sigma rcut rcut_coul C       | pot_rcut pot_rcut_coul (6, 7)
+        const double rcc = p[3], kap = sty == 4 ? p[5] : 0.0;
+        const double prc = sty == 4 ? p[6] : p[5], prcc = sty == 4 ? p[7] : p[6];
+        const double ekr = sty == 4 ? p[4] * qq * exp(-kap * r) / r : p[4] * qq / r;
+        const double fc = sty == 4 ? -ekr * (1 + kap * r) / r : -ekr / r;
+        if (rcc > 0.0) {
+            if (r < rcc) {
+                enrg = ekr - qq * prcc;
+                frc = fc;
+            }
+        } else {
+            enrg = ekr;
+            frc = fc;
+        }
+        if (r < p[2]) {
+            const double sir = p[1] / r, sir2 = sir * sir, sir6 = sir2 * sir2 * sir2, sir12 = sir6 * sir6;
+            enrg = enrg + 4 * p[0] * (sir12 - sir6) - prc;
+            frc = frc - 24 * p[0] * (2 * sir12 - sir6) / r;
+        }
+        return;
+    }
+    case 6: {
+        const double ir = 1.0 - r / p[1];
+        if (r < p[1]) {
+            enrg = 0.5 * p[0] * p[1] * ir * ir;
+            frc = -p[0] * ir;
+        }
+        return;
+    }
+    default:
+        return;
+    }
+}
+
+// ia_get_angle_force (ia_angle_m.f90:46-79), style 1: k (1 - cos theta),
+// theta the angle between q1 = r_i - r_(i-1) and q2 = r_(i+1) - r_i.
+// Returns the forces on i-1 (f0) and i+1 (f2); the force on i is -(f0 + f2).
+__device__ inline void angle_force(const double q1[3], const double q2[3], int sty, const double *p, double &enrg,
+                                   double f0[3], double f2[3])
+{
+    enrg = 0.0;
+    for (int d = 0; d < 3; d++) f0[d] = f2[d] = 0.0;
+    if (sty != 1) return;
+    const double m1 = sqrt(q1[0] * q1[0] + q1[1] * q1[1] + q1[2] * q1[2]);
+    const double m2 = sqrt(q2[0] * q2[0] + q2[1] * q2[1] + q2[2] * q2[2]);
+    double h1[3], h2[3];
+    for (int d = 0; d < 3; d++) {
+        h1[d] = q1[d] / m1;
+        h2[d] = q2[d] / m2;
+    }
+    double ct = h1[0] * h2[0] + h1[1] * h2[1] + h1[2] * h2[2];
+    ct = fmin(fmax(ct, -1.0), 1.0);
+    const double k = p[0];
+    enrg = k * (1.0 - ct);
+    for (int d = 0; d < 3; d++) {
+        f0[d] = k * (-h2[d] + ct * h1[d]) / m1;
+        f2[d] = k * (h1[d] - ct * h2[d]) / m2;
+    }
+}
+
+// ia_get_tether_force (ia_tether_m.f90:53-80): style 1 rigid (no force; an
+// error once |r - r0| >= eps, 110-128), style 2 harmonic (146-158).
+// Returns 1 on a rigid-tether violation.
+__device__ inline int tether_force(double r, int sty, const double *p, double &enrg, double &frc)
+{
+    enrg = 0.0;
+    frc = 0.0;
+    if (sty == 1) return fabs(r - p[0]) < p[1] ? 0 : 1;
+    if (sty == 2) {
+        enrg = 0.5 * p[0] * (r - p[1]) * (r - p[1]);
+        frc = p[0] * (r - p[1]);
+    }
+    return 0;
+}
+
 }  // namespace bd

@@ -48,7 +48,8 @@ void set_error(const std::string &msg);
 
 // Device-side status words (one struct in device memory per context).
 struct DevStatus {
-    int ierr;          // 1: bond overstretched; 2: factorisation failed; 4: cell-list coordinate < 0
+    int ierr;          // 1: bond overstretched; 2: factorisation failed; 4: cell-list coordinate < 0;
+                       // 5: rigid tether violated; 6: external wall crossed
     int converged_j;   // Lanczos: first j with err <= tol (0 = not yet)
     int last_j;        // last Lanczos iteration executed
     int rebuild;       // Verlet list rebuild flag for the current step
@@ -208,6 +209,21 @@ struct Ctx {
     int ex_total = 0;
     std::vector<double> h_bpar, h_vpar;       // host copies (12 x ntypes)
     std::vector<int> h_bsty, h_vsty;
+    bool vdw_general = false;      // a vdw style other than 0 and 1 is set
+    // angles, tethers, external fields, charges (bd_set_angles, ...)
+    int nangles = 0, ntethers = 0, nexternals = 0;
+    int *ang_ptr = nullptr, *ang_idx = nullptr;   // CSR bead -> 4 * angle + role
+    int *ang_atoms = nullptr, *ang_type = nullptr, *ang_styles = nullptr;
+    double *ang_params = nullptr;
+    int *teth_ptr = nullptr, *teth_idx = nullptr; // CSR bead -> tether
+    int *teth_type = nullptr, *teth_styles = nullptr;
+    double *teth_pts = nullptr, *teth_params = nullptr;
+    int *ext_styles = nullptr;
+    double *ext_params = nullptr;
+    double *charge = nullptr;
+    // an isolated molecule without tethers: bds_run keeps its centre of mass
+    // at the origin (limud, bd_solver_m.f90:153-158)
+    bool limud() const { return imcon == 0 && ntethers == 0; }
 
     // pair lists
     int pair_cap = 0;            // capacity of the full list
@@ -300,6 +316,11 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types,
                  const int *atom_types, int nvdw_types, const int *vdw_styles,
                  const double *vdw_params);
 void forces_free(Ctx &c);
+int set_angles(Ctx &c, int nangles, const int *angles, int ntypes, const int *styles, const double *params);
+int set_tethers(Ctx &c, int ntethers, const int *tethers, const double *points, int ntypes, const int *styles,
+                const double *params);
+int set_externals(Ctx &c, int nexternals, const int *styles, const double *params);
+int set_charges(Ctx &c, const double *charge);
 // Computes forces into c.frc at c.crd.  Energies/stress reductions are
 // computed when want_energy is set (written to c.scal[0..14]).
 int compute_forces(Ctx &c, bool want_energy, bool sync);

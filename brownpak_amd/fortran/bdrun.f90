@@ -119,6 +119,7 @@ rc = bd_set_topology(ctx, int(cfg%num_bonds, c_int), bonds, int(cfg%num_bond_typ
         bond_params, int(max(cfg%num_atom_types, 1), c_int), cfg%atoms, int(nvdw, c_int), vdw_styles, &
         vdw_params, merge(1, 0, lvdw), int(excluded_atoms, c_int), pt_code(mth_ptgen), rcutoff, tskin)
 if (rc /= BD_OK) call fail(rc)
+call set_terms()
 rc = bd_set_coordinates(ctx, cfg%coordinates); if (rc /= BD_OK) call fail(rc)
 if (lrevive) then
     ! mid-run coordinates: no second centring (bd_resume_run)
@@ -242,12 +243,35 @@ subroutine dump(n)
     call write_dump(trim(fn_revive)//trim(job_tag), n, cfg, sd, off)
 end subroutine
 
-!> Parts of atmcfg_t the device step does not evaluate.
+!> Parts of atmcfg_t the device step does not cover: the statistics of
+!> branched molecules (compute_ic0's backbone and side-chain terms).
+!> Dihedrals add no force (ia_dihedral_m.f90 has no styles).
 subroutine check_supported()
-    if (cfg%num_angles > 0 .or. cfg%num_dihedrals > 0 .or. cfg%num_tethers > 0 .or. &
-            cfg%num_externals > 0 .or. cfg%num_branches > 0) then
-        print '(a)', 'bdrun: angles, dihedrals, tethers, external fields and branches are not evaluated on the device'
+    if (cfg%num_branches > 0) then
+        print '(a)', 'bdrun: branched molecules (BRANCHES) are not supported on the device'
         stop 2
+    end if
+end subroutine
+
+!> Charges, angles, tethers and external fields (ia_calc_forces,
+!> interaction_m.f90:84-147).
+subroutine set_terms()
+    if (cfg%num_atoms > 0) then
+        rc = bd_set_charges(ctx, cfg%charge); if (rc /= BD_OK) call fail(rc)
+    end if
+    if (cfg%num_angles > 0) then
+        rc = bd_set_angles(ctx, int(cfg%num_angles, c_int), cfg%angles, int(cfg%num_angle_types, c_int), &
+                cfg%angle_styles, cfg%angle_params)
+        if (rc /= BD_OK) call fail(rc)
+    end if
+    if (cfg%num_tethers > 0) then
+        rc = bd_set_tethers(ctx, int(cfg%num_tethers, c_int), cfg%tethers, cfg%tether_points, &
+                int(cfg%num_tether_types, c_int), cfg%tether_styles, cfg%tether_params)
+        if (rc /= BD_OK) call fail(rc)
+    end if
+    if (cfg%num_externals > 0) then
+        rc = bd_set_externals(ctx, int(cfg%num_externals, c_int), cfg%external_styles, cfg%external_params)
+        if (rc /= BD_OK) call fail(rc)
     end if
 end subroutine
 

@@ -158,6 +158,39 @@ interface
         real(c_double), intent(out) :: y(*)
         integer(c_int) :: rc
     end function
+    function bd_set_angles(ctx, nangles, angles, nangle_types, angle_styles, angle_params) &
+            bind(C, name="bd_set_angles") result(rc)
+        import :: c_ptr, c_int, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: nangles, nangle_types
+        integer(c_int), intent(in) :: angles(*), angle_styles(*)
+        real(c_double), intent(in) :: angle_params(*)
+        integer(c_int) :: rc
+    end function
+    function bd_set_tethers(ctx, ntethers, tethers, tether_points, ntether_types, tether_styles, &
+            tether_params) bind(C, name="bd_set_tethers") result(rc)
+        import :: c_ptr, c_int, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: ntethers, ntether_types
+        integer(c_int), intent(in) :: tethers(*), tether_styles(*)
+        real(c_double), intent(in) :: tether_points(*), tether_params(*)
+        integer(c_int) :: rc
+    end function
+    function bd_set_externals(ctx, nexternals, external_styles, external_params) &
+            bind(C, name="bd_set_externals") result(rc)
+        import :: c_ptr, c_int, c_double
+        type(c_ptr), value :: ctx
+        integer(c_int), value :: nexternals
+        integer(c_int), intent(in) :: external_styles(*)
+        real(c_double), intent(in) :: external_params(*)
+        integer(c_int) :: rc
+    end function
+    function bd_set_charges(ctx, charge) bind(C, name="bd_set_charges") result(rc)
+        import :: c_ptr, c_int, c_double
+        type(c_ptr), value :: ctx
+        real(c_double), intent(in) :: charge(*)
+        integer(c_int) :: rc
+    end function
     function bd_get_runinfo(ctx, info) bind(C, name="bd_get_runinfo") result(rc)
         import :: c_ptr, c_int, bd_runinfo_t
         type(c_ptr), value :: ctx

@@ -28,7 +28,9 @@
  * (use one context per host thread).  No exception crosses the ABI.
  *
  * Return codes mirror the reference's ierr:
- *   BD_OK 0, BD_ERR_BOND 1 (bond overstretched, ia_bond_m.f90:213-217),
+ *   BD_OK 0, BD_ERR_BOND 1 (ia_calc_forces' ierr: bond overstretched,
+ *   ia_bond_m.f90:213-217; rigid tether violated, ia_tether_m.f90:110-128;
+ *   external wall crossed, ia_external_m.f90:93-101),
  *   BD_ERR_FACTOR 2 (Cholesky/eigen failure, brown_m.f90:540-544),
  *   BD_WARN_NOTCONV 3 (Lanczos not converged; results valid, bd_solver_m.f90:
  *   393-397), BD_ERR_ARG -1, BD_ERR_HIP -2, BD_ERR_STATE -3, BD_ERR_CELL -4
@@ -127,14 +129,45 @@ int bd_set_box(bd_ctx *ctx, int imcon, const double box[3]);
 
 /* ia_setup + pt_init (interaction_m.f90:30-71, pairtab_m.fpp:36-98):
  * topology, exclusions (connectivity_m.f90:43-317) and potential
- * parameters.  Bond styles 1-4 and vdw style 1 are supported; other vdw
- * styles are rejected with BD_ERR_ARG. */
+ * parameters.  Bond styles 1-4 (ia_bond_m.f90) and vdw styles 1-6
+ * (ia_vdw_m.f90: LJ, Gaussian, cosine, LJ + screened Coulomb, LJ +
+ * Coulomb, DPD); styles 4 and 5 take the charges of bd_set_charges. */
 int bd_set_topology(bd_ctx *ctx, int nbonds, const int *bonds, int nbond_types,
                     const int *bond_styles, const double *bond_params,
                     int natom_types, const int *atom_types, int nvdw_types,
                     const int *vdw_styles, const double *vdw_params, int lvdw,
                     int excluded_atoms, int mth_ptgen, double rcutoff,
                     double tskin);
+
+/* The remaining terms of ia_calc_forces (interaction_m.f90:84-147), set
+ * after bd_set_topology; each call replaces the previous set (0 removes it).
+ *   bd_set_angles: angles (4, nangles) [type, i-1, i, i+1] 1-based, styles
+ *     0 (none) or 1 (k (1 - cos theta), ia_angle_m.f90:46-79), params
+ *     (12, ntypes) with k = params(1) (ia_add_angle_forces, 276-325).
+ *   bd_set_tethers: tethers (2, ntethers) [type, atom], tether_points
+ *     (3, ntethers), styles 1 (rigid: an error once |r - r0| >= eps) or 2
+ *     (harmonic k, r0), ia_tether_m.f90:53-158; not imaged (375-418).  A
+ *     tethered system is not re-centred after each step (limud,
+ *     bd_solver_m.f90:153-158).
+ *   bd_set_externals: styles 1 (constant x force params(2) on atom
+ *     params(1)) or 2 (wall: an error once component params(1) of any atom
+ *     passes params(2) on the side of sign params(3)),
+ *     ia_external_m.f90:61-107.
+ *   bd_set_charges: charge (N) for vdw styles 4 and 5 (NULL: all 0).
+ * The dihedral styles of the reference are all empty (ia_dihedral_m.f90),
+ * so dihedrals add nothing and need no call.
+ * bd_energies returns the energies of the last evaluation that computed
+ * them (bd_forces, or a step with statistics on) in stats order: bond,
+ * angle, dihedral, vdw, tether, external, total. */
+int bd_set_angles(bd_ctx *ctx, int nangles, const int *angles, int nangle_types,
+                  const int *angle_styles, const double *angle_params);
+int bd_set_tethers(bd_ctx *ctx, int ntethers, const int *tethers,
+                   const double *tether_points, int ntether_types,
+                   const int *tether_styles, const double *tether_params);
+int bd_set_externals(bd_ctx *ctx, int nexternals, const int *external_styles,
+                     const double *external_params);
+int bd_set_charges(bd_ctx *ctx, const double *charge);
+int bd_energies(bd_ctx *ctx, double *energies7);
 
 /* Host <-> device coordinates, (3, N) FP64. */
 int bd_set_coordinates(bd_ctx *ctx, const double *crd);

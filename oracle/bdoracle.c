@@ -702,6 +702,7 @@ double orc_rgsq(int nb, const double *crd)
 /* ===================================================================== */
 
 #define MATH_SXRT2 1.122462048309373
+#define MATH_PI 3.1415926535897931 /* constants_m.f90:31 */
 
 /* ia_bond_setup / ia_vdw_setup: derived parameter slots (ia_bond_m.f90:158-
  * 190, 227-265; ia_vdw_m.f90:107-151). */
@@ -715,13 +716,27 @@ void orc_setup_params(orc_system *sys)
             p[5] = MATH_SXRT2 * p[3];
         }
     }
+    /* ia_vdw_setup, ia_vdw_m.f90:21-47: the setters' derived slots */
     for (int t = 0; t < sys->nvdw_types; t++) {
         double *p = sys->vdw_params + (size_t)t * ORC_MXPARAM;
-        if (sys->vdw_styles[t] == 1) {
+        int sty = sys->vdw_styles[t];
+        if (sty == 1 || sty == 4 || sty == 5) { /* vdw_lj_set 85-151, 270-306, 356-389 */
             double sir = p[1] / p[2];
             double sir2 = sir * sir, sir6 = sir2 * sir2 * sir2, sir12 = sir6 * sir6;
-            p[3] = 4 * p[0] * (sir12 - sir6);
-            p[4] = -24 * p[0] * (2 * sir12 - sir6) / p[2];
+            double pot_rcut = 4 * p[0] * (sir12 - sir6);
+            if (sty == 1) {
+                p[3] = pot_rcut;
+                p[4] = -24 * p[0] * (2 * sir12 - sir6) / p[2];
+            } else if (sty == 4) {
+                p[6] = pot_rcut;
+                p[7] = (p[3] > 0.0) ? p[4] * exp(-p[5] * p[3]) / p[3] : 0.0;
+            } else {
+                p[5] = pot_rcut;
+                p[6] = (p[3] > 0.0) ? p[4] / p[3] : 0.0;
+            }
+        } else if (sty == 2) { /* vdw_gaussian_set 185-206 */
+            p[3] = p[0] * exp(-p[1] * (p[2] * p[2]));
+            p[4] = -2.0 * p[0] * p[1] * p[2] * exp(-p[1] * (p[2] * p[2]));
         }
     }
 }
@@ -772,19 +787,122 @@ static int bond_force(double r, int sty, const double *p, double *enrg, double *
     }
 }
 
-/* vdw_lj, ia_vdw_m.f90:155-180 (style 1, truncated and force-shifted). */
-static void vdw_force(double r, int sty, const double *p, double *enrg, double *frc)
+/* ia_get_vdw_force, ia_vdw_m.f90:51-81, every style; qiqj = qi*qj. */
+static void vdw_force(double r, double qiqj, int sty, const double *p, double *enrg, double *frc)
 {
     *enrg = 0.0;
     *frc = 0.0;
-    if (sty != 1) return;
-    double eps = p[0], sigma = p[1], rcut = p[2], vrc = p[3], dvrc = p[4];
-    if (r < rcut) {
-        double sir = sigma / r, sir2 = sir * sir, sir6 = sir2 * sir2 * sir2;
-        double sir12 = sir6 * sir6;
-        *enrg = 4 * eps * (sir12 - sir6) - vrc - (r - rcut) * dvrc;
-        *frc = -24 * eps * (2 * sir12 - sir6) / r - dvrc;
+    switch (sty) {
+    case 1: { /* vdw_lj, 155-180 */
+        double eps = p[0], sigma = p[1], rcut = p[2], vrc = p[3], dvrc = p[4];
+        if (r < rcut) {
+            double sir = sigma / r, sir2 = sir * sir, sir6 = sir2 * sir2 * sir2;
+            double sir12 = sir6 * sir6;
+            *enrg = 4 * eps * (sir12 - sir6) - vrc - (r - rcut) * dvrc;
+            *frc = -24 * eps * (2 * sir12 - sir6) / r - dvrc;
+        }
+        return;
     }
+    case 2: { /* vdw_gaussian, 210-229 */
+        double A = p[0], B = p[1], rcut = p[2];
+        if (r < rcut) {
+            double exrs = exp(-B * r * r);
+            *enrg = A * exrs - p[3] - (r - rcut) * p[4];
+            *frc = -2 * A * B * r * exrs - p[4];
+        }
+        return;
+    }
+    case 3: { /* vdw_cosine, 249-266 */
+        double A = p[0], rcut = p[1];
+        double pf = MATH_PI / rcut, pr = pf * r;
+        if (r < rcut) {
+            *enrg = A * (1.0 + cos(pr));
+            *frc = -A * pf * sin(pr);
+        }
+        return;
+    }
+    case 4:   /* vdw_lj_coul_debye, 310-352 */
+    case 5: { /* vdw_lj_coul, 391-432 */
+        double eps = p[0], sigma = p[1], rcut = p[2], rcut_coul = p[3], C = p[4];
+        double kappa = (sty == 4) ? p[5] : 0.0;
+        double pot_rcut = (sty == 4) ? p[6] : p[5];
+        double pot_rcut_coul = (sty == 4) ? p[7] : p[6];
+        double ekr, fc;
+        if (sty == 4) {
+            ekr = C * qiqj * exp(-kappa * r) / r;
+            fc = -ekr * (1 + kappa * r) / r;
+        } else {
+            ekr = C * qiqj / r;
+            fc = -ekr / r;
+        }
+        if (rcut_coul > 0.0) {
+            if (r < rcut_coul) {
+                *enrg = ekr - qiqj * pot_rcut_coul;
+                *frc = fc;
+            }
+        } else {
+            *enrg = ekr;
+            *frc = fc;
+        }
+        if (r < rcut) {
+            double sir = sigma / r, sir2 = sir * sir, sir6 = sir2 * sir2 * sir2;
+            double sir12 = sir6 * sir6;
+            *enrg = *enrg + 4 * eps * (sir12 - sir6) - pot_rcut;
+            *frc = *frc - 24 * eps * (2 * sir12 - sir6) / r;
+        }
+        return;
+    }
+    case 6: { /* vdw_dpd, 454-470 */
+        double A = p[0], rcut = p[1], ir = 1.0 - (r / rcut);
+        if (r < rcut) {
+            *enrg = 0.5 * A * rcut * ir * ir;
+            *frc = -A * ir;
+        }
+        return;
+    }
+    default:
+        return;
+    }
+}
+
+/* ia_get_angle_force, ia_angle_m.f90:46-79 (style 1). */
+static void angle_force(const double *q1, const double *q2, int sty, const double *p, double *enrg,
+                        double *fim1, double *fi, double *fip1)
+{
+    *enrg = 0.0;
+    for (int d = 0; d < 3; d++) fim1[d] = fi[d] = fip1[d] = 0.0;
+    double q1mag = sqrt(q1[0] * q1[0] + q1[1] * q1[1] + q1[2] * q1[2]);
+    double q2mag = sqrt(q2[0] * q2[0] + q2[1] * q2[1] + q2[2] * q2[2]);
+    double q1hat[3], q2hat[3];
+    for (int d = 0; d < 3; d++) {
+        q1hat[d] = q1[d] / q1mag;
+        q2hat[d] = q2[d] / q2mag;
+    }
+    double ctheta = q1hat[0] * q2hat[0] + q1hat[1] * q2hat[1] + q1hat[2] * q2hat[2];
+    if (ctheta > 1.0) ctheta = 1.0;
+    if (ctheta < -1.0) ctheta = -1.0;
+    if (sty != 1) return;
+    double kang = p[0];
+    *enrg = kang * (1.0 - ctheta);
+    for (int d = 0; d < 3; d++) {
+        fim1[d] = kang * (-q2hat[d] + ctheta * q1hat[d]) / q1mag;
+        fip1[d] = kang * (q1hat[d] - ctheta * q2hat[d]) / q2mag;
+        fi[d] = -(fim1[d] + fip1[d]);
+    }
+}
+
+/* ia_get_tether_force, ia_tether_m.f90:53-80, teth_rigid 110-128, teth_harm
+ * 146-158.  Returns ierr. */
+static int tether_force(double r, int sty, const double *p, double *enrg, double *frc)
+{
+    *enrg = 0.0;
+    *frc = 0.0;
+    if (sty == 1) return (fabs(r - p[0]) < p[1]) ? 0 : 1;
+    if (sty == 2) {
+        *enrg = 0.5 * p[0] * (r - p[1]) * (r - p[1]);
+        *frc = p[0] * (r - p[1]);
+    }
+    return 0;
 }
 
 static void add_stress(double *st, const double *rij, const double *fi)
@@ -829,6 +947,47 @@ int orc_forces(const orc_system *sys, const double *crd, const int *pt_ptr,
         }
         en->bndlen /= sys->nbonds;
     }
+    /* ia_add_angle_forces, interaction_m.f90:276-325 */
+    for (int a = 0; a < sys->nangles; a++) {
+        int typ = sys->angles[4 * a] - 1;
+        int im1 = sys->angles[4 * a + 1] - 1, i = sys->angles[4 * a + 2] - 1, ip1 = sys->angles[4 * a + 3] - 1;
+        double q1[3], q2[3], fim1[3], fi[3], fip1[3], e;
+        for (int d = 0; d < 3; d++) {
+            q1[d] = crd[3 * i + d] - crd[3 * im1 + d];
+            q2[d] = crd[3 * ip1 + d] - crd[3 * i + d];
+        }
+        if (sys->imcon) {
+            orc_get_image(sys->box, q1);
+            orc_get_image(sys->box, q2);
+        }
+        angle_force(q1, q2, sys->angle_styles[typ], sys->angle_params + (size_t)typ * ORC_MXPARAM, &e, fim1, fi,
+                    fip1);
+        en->energy_angle += e;
+        for (int d = 0; d < 3; d++) {
+            frc[3 * im1 + d] += fim1[d];
+            frc[3 * i + d] += fi[d];
+            frc[3 * ip1 + d] += fip1[d];
+        }
+        for (int b = 0; b < 3; b++)
+            for (int c = 0; c < 3; c++)
+                en->stress[c + 3 * b] = en->stress[c + 3 * b] - q1[c] * fim1[b] + q2[c] * fip1[b];
+    }
+    /* ia_add_tether_forces, interaction_m.f90:375-418 (no imaging) */
+    for (int k = 0; k < sys->ntethers; k++) {
+        int typ = sys->tethers[2 * k] - 1, ia = sys->tethers[2 * k + 1] - 1;
+        double q[3];
+        for (int d = 0; d < 3; d++) q[d] = crd[3 * ia + d] - sys->tether_points[3 * k + d];
+        double qmag = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+        double e, f;
+        if (tether_force(qmag, sys->tether_styles[typ], sys->tether_params + (size_t)typ * ORC_MXPARAM, &e, &f))
+            return 1;
+        double fj[3];
+        for (int d = 0; d < 3; d++) fj[d] = -f * q[d] / qmag;
+        en->energy_tether += e;
+        for (int d = 0; d < 3; d++) frc[3 * ia + d] += fj[d];
+        for (int b = 0; b < 3; b++)
+            for (int c = 0; c < 3; c++) en->stress[c + 3 * b] += q[c] * fj[b];
+    }
     if (sys->lvdw && sys->nvdw_types > 0 && pt_ptr) {
         const int nt = sys->natom_types;
         for (int i = 0; i < nb; i++) {
@@ -844,7 +1003,8 @@ int orc_forces(const orc_system *sys, const double *crd, const int *pt_ptr,
                 if (sys->imcon) orc_get_image(sys->box, rij);
                 double r = sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2]);
                 double e, f;
-                vdw_force(r, sys->vdw_styles[typ], sys->vdw_params + (size_t)typ * ORC_MXPARAM, &e, &f);
+                double qq = sys->charge ? sys->charge[i] * sys->charge[j] : 0.0;
+                vdw_force(r, qq, sys->vdw_styles[typ], sys->vdw_params + (size_t)typ * ORC_MXPARAM, &e, &f);
                 en->energy_vdw += e;
                 double fi[3];
                 for (int d = 0; d < 3; d++) fi[d] = f * rij[d] / r;
@@ -856,7 +1016,23 @@ int orc_forces(const orc_system *sys, const double *crd, const int *pt_ptr,
             }
         }
     }
-    en->energy_tot = en->energy_bond + en->energy_vdw;
+    /* ia_add_external_forces, ia_external_m.f90:61-107 */
+    for (int e = 0; e < sys->nexternals; e++) {
+        const double *p = sys->external_params + (size_t)e * ORC_MXPARAM;
+        if (sys->external_styles[e] == 1) {
+            int ia = (int)p[0] - 1;
+            en->energy_external += -p[1] * (crd[3 * ia] - crd[0]);
+            frc[3 * ia] += p[1];
+        } else if (sys->external_styles[e] == 2) {
+            int m = (int)p[0] - 1;
+            for (int i = 0; i < nb; i++) {
+                double x = crd[3 * i + m];
+                if (p[2] > 0.0 ? x < p[1] : x > p[1]) return 1;
+            }
+        }
+    }
+    en->energy_tot = en->energy_bond + en->energy_angle + en->energy_dihedral + en->energy_vdw
+                     + en->energy_tether + en->energy_external;
     if (sys->imcon) {
         double vol = sys->box[0] * sys->box[1] * sys->box[2];
         for (int k = 0; k < 9; k++) en->stress[k] /= vol;
@@ -1189,7 +1365,7 @@ long long orc_bd_run_ex(const orc_system *sys, const orc_bdopts *opt, double *cr
     int *pt_col = NULL;
     orc_verlet vs = {1, NULL, 0.0, 0.0};
     int cntr = 0;
-    const int limud = (sys->imcon == 0);
+    const int limud = (sys->imcon == 0) && (sys->ntethers == 0);
     orc_energies en;
     long long ret = 0;
 

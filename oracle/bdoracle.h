@@ -71,12 +71,29 @@ typedef struct {
     int excluded_atoms;   /* 0..3 rings excluded                    */
     int mth_ptgen;        /* 0 DIR, 1 VER, 2 CL                     */
     double rcutoff, tskin;
+    /* the remaining ia_calc_forces terms (interaction_m.f90:84-147) */
+    const double *charge; /* (nbeads) or NULL (all 0)               */
+    int nangles;
+    const int *angles;    /* (4, nangles) [type, i-1, i, i+1]        */
+    int nangle_types;
+    const int *angle_styles;
+    const double *angle_params;   /* (12, nangle_types)             */
+    int ntethers;
+    const int *tethers;   /* (2, ntethers) [type, atom]              */
+    const double *tether_points;  /* (3, ntethers)                  */
+    int ntether_types;
+    const int *tether_styles;
+    const double *tether_params;  /* (12, ntether_types)            */
+    int nexternals;
+    const int *external_styles;
+    const double *external_params; /* (12, nexternals)              */
 } orc_system;
 
 typedef struct {
     double energy_bond, energy_vdw, energy_tot;
     double stress[9];     /* 3x3 column-major */
     double bndlen, bndlen_min, bndlen_max;
+    double energy_angle, energy_dihedral, energy_tether, energy_external;
 } orc_energies;
 
 void orc_setup_params(orc_system *sys);

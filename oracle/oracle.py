@@ -34,6 +34,12 @@ class OrcSystem(C.Structure):
         ("nvdw_types", C.c_int), ("vdw_styles", IP), ("vdw_params", DP),
         ("lvdw", C.c_int), ("excluded_atoms", C.c_int), ("mth_ptgen", C.c_int),
         ("rcutoff", C.c_double), ("tskin", C.c_double),
+        ("charge", DP),
+        ("nangles", C.c_int), ("angles", IP), ("nangle_types", C.c_int), ("angle_styles", IP),
+        ("angle_params", DP),
+        ("ntethers", C.c_int), ("tethers", IP), ("tether_points", DP), ("ntether_types", C.c_int),
+        ("tether_styles", IP), ("tether_params", DP),
+        ("nexternals", C.c_int), ("external_styles", IP), ("external_params", DP),
     ]
 
 
@@ -42,6 +48,8 @@ class OrcEnergies(C.Structure):
         ("energy_bond", C.c_double), ("energy_vdw", C.c_double), ("energy_tot", C.c_double),
         ("stress", C.c_double * 9),
         ("bndlen", C.c_double), ("bndlen_min", C.c_double), ("bndlen_max", C.c_double),
+        ("energy_angle", C.c_double), ("energy_dihedral", C.c_double), ("energy_tether", C.c_double),
+        ("energy_external", C.c_double),
     ]
 
 
@@ -206,7 +214,9 @@ class System:
     def __init__(self, nbeads, bonds, bond_styles, bond_params, atom_types=None,
                  vdw_styles=(), vdw_params=None, natom_types=1, lvdw=True,
                  excluded_atoms=0, mth_ptgen=0, rcutoff=0.0, tskin=0.0,
-                 imcon=0, box=(0.0, 0.0, 0.0)):
+                 imcon=0, box=(0.0, 0.0, 0.0), charge=None, angles=None, angle_styles=(),
+                 angle_params=None, tethers=None, tether_points=None, tether_styles=(),
+                 tether_params=None, external_styles=(), external_params=None):
         self.bonds = _i32(np.asarray(bonds).reshape(-1, 3)) if len(bonds) else np.zeros((0, 3), np.int32)
         self.bond_styles = _i32(bond_styles) if len(bond_styles) else np.zeros(1, np.int32)
         bp = np.zeros((max(len(bond_styles), 1), MXPARAM))
@@ -240,6 +250,40 @@ class System:
         s.mth_ptgen = mth_ptgen
         s.rcutoff = rcutoff
         s.tskin = tskin
+        # angles (n, 4), tethers (n, 2) + points (n, 3), externals: rows of
+        # the reference's column-major arrays; parameters (ntypes, <= 12)
+        def table(styles, params):
+            st = _i32(styles) if len(styles) else np.zeros(1, np.int32)
+            p = np.zeros((max(len(styles), 1), MXPARAM))
+            if params is not None and len(styles):
+                pi = np.atleast_2d(np.asarray(params, dtype=np.float64))
+                p[:, :pi.shape[1]] = pi
+            return st, p
+        self.charge = None if charge is None else _f64(charge)
+        s.charge = _dp(self.charge) if self.charge is not None else None
+        self.angles = _i32(np.asarray(angles).reshape(-1, 4)) if angles is not None and len(angles) \
+            else np.zeros((1, 4), np.int32)
+        self.angle_styles, self.angle_params = table(angle_styles, angle_params)
+        s.nangles = 0 if angles is None else len(angles)
+        s.angles = _ip(self.angles)
+        s.nangle_types = len(angle_styles)
+        s.angle_styles = _ip(self.angle_styles)
+        s.angle_params = _dp(self.angle_params)
+        self.tethers = _i32(np.asarray(tethers).reshape(-1, 2)) if tethers is not None and len(tethers) \
+            else np.zeros((1, 2), np.int32)
+        self.tether_points = _f64(np.asarray(tether_points).reshape(-1, 3)) if tethers is not None and len(tethers) \
+            else np.zeros((1, 3))
+        self.tether_styles, self.tether_params = table(tether_styles, tether_params)
+        s.ntethers = 0 if tethers is None else len(tethers)
+        s.tethers = _ip(self.tethers)
+        s.tether_points = _dp(self.tether_points)
+        s.ntether_types = len(tether_styles)
+        s.tether_styles = _ip(self.tether_styles)
+        s.tether_params = _dp(self.tether_params)
+        self.external_styles, self.external_params = table(external_styles, external_params)
+        s.nexternals = len(external_styles)
+        s.external_styles = _ip(self.external_styles)
+        s.external_params = _dp(self.external_params)
         self.s = s
         lib().orc_setup_params(C.byref(s))
 
