@@ -1335,6 +1335,34 @@ static int drift_eval(const orc_system *sys, const orc_bdopts *opt, const double
     return 0;
 }
 
+/* se_fval (bd_solver_m.f90:312-332): f = x - drift(x) - crd0 - diffusion. */
+typedef struct {
+    const orc_system *sys;
+    const orc_bdopts *opt;
+    const double *mob;
+    const int *ex_ptr, *ex_col;
+    orc_verlet *vs;
+    int *pt_ptr, *pt_col;
+    int ptcap;
+    double *frc, *drift;
+    orc_energies *en;
+    const double *crd0, *dcol;
+    int err;
+} se_ctx;
+
+static int se_fval(void *p, int n, const double *x, double *f)
+{
+    se_ctx *c = (se_ctx *)p;
+    int e = drift_eval(c->sys, c->opt, c->mob, c->ex_ptr, c->ex_col, c->vs, c->pt_ptr, c->pt_col, c->ptcap, x,
+                       c->frc, c->drift, c->en);
+    if (e) {
+        c->err = e;
+        return 1;
+    }
+    for (int i = 0; i < n; i++) f[i] = x[i] - c->drift[i] - c->crd0[i] - c->dcol[i];
+    return 0;
+}
+
 long long orc_bd_run(const orc_system *sys, const orc_bdopts *opt, double *crd,
                      int nsteps, const double *xi, long long nxi, orc_runinfo *info)
 {
@@ -1426,6 +1454,34 @@ long long orc_bd_run_ex(const orc_system *sys, const orc_bdopts *opt, double *cr
                 for (size_t k = 0; k < (size_t)n * s; k++) diff[k] = sqrt_two_dt * diff[k];
             }
             cntr++;
+        }
+        /* ---- integrate_se (bd_solver_m.f90:242-308): solve
+         * x - drift(x) - crd0 - diffusion = 0 by NITSOL from x = crd0, with
+         * input(1) = se_nlmxitr, input(4) = se_kdmax, input(7) = 1 ---- */
+        if (opt->integrator == 2) {
+            const double *dcol = disp ? disp + (size_t)step * n : diff + (size_t)(opt->lhdia ? cntr - 1 : 0) * n;
+            double *crd0 = (double *)malloc(sizeof(double) * n);
+            memcpy(crd0, crd, sizeof(double) * n);
+            se_ctx sc = {sys, opt, mob, ex_ptr, ex_col, &vs, pt_ptr, pt_col, ptcap, frc, drift, &en, crd0, dcol, 0};
+            int input[10] = {opt->se_nlmxitr, 0, 0, opt->se_kdmax, 0, 0, 1, 0, 0, 0};
+            int nit[6];
+            int iterm = orc_nitsol(n, crd, se_fval, &sc, opt->se_ftol, opt->se_stptol, input, nit);
+            free(crd0);
+            info->se_nfe += nit[0];
+            info->se_nli += nit[3];
+            info->se_nni += nit[4];
+            info->se_nbt += nit[5];
+            info->se_iterm = iterm;
+            if (iterm != 0) { ret = -13; goto done; }
+            if (opt->lhdia && cntr == s) cntr = 0;
+            info->nts++;
+            if (limud) {
+                double com[3];
+                orc_to_origin(nb, crd, com);
+                for (int d = 0; d < 3; d++) info->molecule_com[d] += com[d];
+            }
+            if (sys->imcon) orc_wrap_all(sys->box, nb, crd);
+            continue;
         }
         /* ---- calc_drift ---- */
         {

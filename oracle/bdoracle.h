@@ -147,7 +147,9 @@ typedef struct {
     int lanc_reorth;      /* 1: FP64 full reorthogonalisation (not in the reference) */
     int integrator;       /* 0 EM (integrate_em); 1 predictor-corrector (Heun drift,
                              mobility and Brownian term held over the step; not in
-                             the reference) */
+                             the reference); 2 SE (integrate_se, NITSOL) */
+    int se_nlmxitr, se_kdmax;     /* control keys se_nlmxitr, se_kdmax        */
+    double se_ftol, se_stptol;    /* control key se_tol(1:2)                  */
 } orc_bdopts;
 
 typedef struct {
@@ -158,6 +160,8 @@ typedef struct {
     int ierr;
     int total_lanczos_iters;
     int n_mob_updates;
+    /* semi-implicit steps: NITSOL totals and the last termination (iterm) */
+    int se_nfe, se_nli, se_nni, se_nbt, se_iterm;
 } orc_runinfo;
 
 /* Runs nsteps EM steps in place on crd.  xi is the injected Gaussian stream:
@@ -166,6 +170,15 @@ typedef struct {
 long long orc_bd_run(const orc_system *sys, const orc_bdopts *opt, double *crd,
                      int nsteps, const double *xi, long long nxi,
                      orc_runinfo *info);
+/* NITSOL v0.3 (src/nitsol) restated in bdnitsol.c: GMRES, finite-difference
+ * J*v, backtracking.  fn returns 0 or nonzero on failure.  input/info as
+ * nitsol's; returns iterm. */
+typedef int (*orc_fn)(void *ctx, int n, const double *x, double *f);
+int orc_nitsol(int n, double *x, orc_fn fn, void *ctx, double ftol, double stptol, const int *input, int *info);
+/* LAPACK dlaic1 restated from its definition (see bdnitsol.c). */
+void orc_dlaic1(int job, int j, const double *x, double sest, const double *w, double gamma, double *sestpr,
+                double *s, double *c);
+
 long long orc_bd_run_ex(const orc_system *sys, const orc_bdopts *opt, double *crd,
                         int nsteps, const double *xi, long long nxi,
                         const double *disp, orc_runinfo *info);

@@ -59,6 +59,7 @@ class OrcBdopts(C.Structure):
         ("lanc_mxitr", C.c_int), ("lanc_tol", C.c_double), ("tim_stp", C.c_double),
         ("keep_reuse_noise", C.c_int), ("flow_style", C.c_int), ("flow_rate", C.c_double),
         ("lanc_reorth", C.c_int), ("integrator", C.c_int),
+        ("se_nlmxitr", C.c_int), ("se_kdmax", C.c_int), ("se_ftol", C.c_double), ("se_stptol", C.c_double),
     ]
 
 
@@ -67,7 +68,12 @@ class OrcRuninfo(C.Structure):
         ("nts", C.c_longlong), ("molecule_com", C.c_double * 3),
         ("last_lanczos_iters", C.c_int), ("n_not_converged", C.c_int), ("ierr", C.c_int),
         ("total_lanczos_iters", C.c_int), ("n_mob_updates", C.c_int),
+        ("se_nfe", C.c_int), ("se_nli", C.c_int), ("se_nni", C.c_int), ("se_nbt", C.c_int),
+        ("se_iterm", C.c_int),
     ]
+
+
+NIT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double))
 
 
 class OrcVerlet(C.Structure):
@@ -117,6 +123,9 @@ def lib():
         L.orc_bd_run.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcBdopts), DP, C.c_int, DP,
                                  C.c_longlong, C.POINTER(OrcRuninfo)]
         L.orc_bd_run.restype = C.c_longlong
+        L.orc_nitsol.argtypes = [C.c_int, DP, NIT_FN, C.c_void_p, C.c_double, C.c_double, IP, IP]
+        L.orc_nitsol.restype = C.c_int
+        L.orc_dlaic1.argtypes = [C.c_int, C.c_int, DP, C.c_double, DP, C.c_double, DP, DP, DP]
         L.orc_bd_run_ex.argtypes = [C.POINTER(OrcSystem), C.POINTER(OrcBdopts), DP, C.c_int, DP,
                                     C.c_longlong, DP, C.POINTER(OrcRuninfo)]
         L.orc_bd_run_ex.restype = C.c_longlong
@@ -160,7 +169,7 @@ def symv_lower(mob_colmajor_logical, x, alpha=1.0, beta=0.0, y=None):
 
 # ------------------------------------------------------------ Brownian B.xi
 CHOL, LANC, CHOL_LT = 0, 1, 2
-EM, PC = 0, 1          # integrator: integrate_em, predictor-corrector
+EM, PC, SE = 0, 1, 2   # integrator: integrate_em, predictor-corrector, integrate_se
 
 
 def brn_calc_bdw(method, mob, xi, lanc_mxitr=50, lanc_tol=1e-6, reorth=False):
@@ -380,17 +389,51 @@ def rgsq(crd):
 
 def bd_run(sysobj, crd, nsteps, xi, lhdia=True, mob_fctr=CHOL, nts_mobsam=1,
            lanc_mxitr=50, lanc_tol=1e-6, tim_stp=1e-3, keep_reuse_noise=False,
-           flow_style=0, flow_rate=0.0, lanc_reorth=False, integrator=EM):
+           flow_style=0, flow_rate=0.0, lanc_reorth=False, integrator=EM,
+           se_nlmxitr=0, se_kdmax=0, se_tol=(1e-10, 1e-12)):
     """bds_run for nsteps with the injected Gaussian stream xi; integrator
-    EM (integrate_em) or PC (predictor-corrector, not in the reference)."""
+    EM (integrate_em), PC (predictor-corrector, not in the reference) or SE
+    (integrate_se with NITSOL; se_* are the control keys)."""
     crd = _f64(crd).copy()
     xi = _f64(np.ravel(xi))
     opt = OrcBdopts(int(lhdia), mob_fctr, nts_mobsam, lanc_mxitr, lanc_tol, tim_stp,
-                    int(keep_reuse_noise), flow_style, flow_rate, int(bool(lanc_reorth)), integrator)
+                    int(keep_reuse_noise), flow_style, flow_rate, int(bool(lanc_reorth)), integrator,
+                    se_nlmxitr, se_kdmax, se_tol[0], se_tol[1])
     info = OrcRuninfo()
     used = lib().orc_bd_run(C.byref(sysobj.s), C.byref(opt), _dp(crd), nsteps, _dp(xi),
                             xi.size, C.byref(info))
     return crd, used, info
+
+
+def nitsol(fn, x0, ftol, stptol, input10=None):
+    """NITSOL (restated, oracle/bdnitsol.c) on a Python function f(x) -> F.
+    Returns (x, iterm, info[6] = nfe, njve, nrpre, nli, nni, nbt)."""
+    x = _f64(x0).copy()
+    n = x.size
+
+    def cb(ctx, nn, xp, fp):
+        xv = np.ctypeslib.as_array(xp, shape=(nn,)).copy()
+        try:
+            fv = np.asarray(fn(xv), dtype=np.float64)
+        except Exception:
+            return 1
+        np.ctypeslib.as_array(fp, shape=(nn,))[:] = fv
+        return 0
+
+    cfn = NIT_FN(cb)
+    inp = np.zeros(10, np.int32) if input10 is None else _i32(input10)
+    info = np.zeros(6, np.int32)
+    iterm = lib().orc_nitsol(n, _dp(x), cfn, None, ftol, stptol, _ip(inp), _ip(info))
+    return x, iterm, info
+
+
+def dlaic1(job, x, sest, w, gamma):
+    """The dlaic1 restatement: (sestpr, s, c)."""
+    x, w = _f64(x), _f64(w)
+    out = np.zeros(3)
+    o = [C.c_double() for _ in range(3)]
+    lib().orc_dlaic1(job, x.size, _dp(x), sest, _dp(w), gamma, C.byref(o[0]), C.byref(o[1]), C.byref(o[2]))
+    return o[0].value, o[1].value, o[2].value
 
 
 def bd_run_replay(sysobj, crd, disp, lhdia=True, nts_mobsam=1, tim_stp=1e-3):
