@@ -38,7 +38,7 @@ CHOL, LANC, CHOL_LT = 0, 1, 2
 FP64, FP32 = 0, 1
 PT = {"DIR": 0, "VER": 1, "CL": 2}
 MOB_FCTR = {"CHOL": CHOL, "LANC": LANC, "CHOL_LT": CHOL_LT}
-INTEGRATOR = {"EM": 0, "PC": 1}
+INTEGRATOR = {"EM": 0, "PC": 1, "SE": 2}
 MOB_STORAGE = {"auto": 0, "tiles": 1, "matfree": 2}
 MXPARAM = 12
 # work-unit shapes (tile rows, tile columns) of the mobility product, in the
@@ -60,6 +60,7 @@ EXPORTS = (
     "bd_batch_tile_shape", "bd_batch_set_tile_shape", "bd_plan_layout", "bd_rpy_apply",
     "bd_rpy_apply_noalloc", "bd_get_rng", "bd_resume_run", "bd_stats_enable", "bd_stats_get",
     "bd_set_angles", "bd_set_tethers", "bd_set_externals", "bd_set_charges", "bd_energies",
+    "bd_se_info",
 )
 
 # bd_stats_get's output, in stats_write's column order (stats_m.f90:235-290)
@@ -77,7 +78,8 @@ class BdOpts(C.Structure):
         ("lanc_mxitr", C.c_int), ("lanc_tol", C.c_double), ("tim_stp", C.c_double),
         ("precision", C.c_int), ("keep_reuse_noise", C.c_int), ("flow_style", C.c_int),
         ("flow_rate", C.c_double), ("lanc_reorth", C.c_int), ("integrator", C.c_int),
-        ("mob_storage", C.c_int),
+        ("mob_storage", C.c_int), ("se_nlmxitr", C.c_int), ("se_kdmax", C.c_int),
+        ("se_ftol", C.c_double), ("se_stptol", C.c_double),
     ]
 
 
@@ -153,6 +155,7 @@ def load_library(path: str = LIB_PATH):
     L.bd_set_externals.argtypes = [vp, C.c_int, IP, DP]
     L.bd_set_charges.argtypes = [vp, DP]
     L.bd_energies.argtypes = [vp, DP]
+    L.bd_se_info.argtypes = [vp, IP, C.POINTER(C.c_longlong)]
     L.bd_get_runinfo.argtypes = [vp, C.POINTER(BdRuninfo)]
     L.bd_reset_run.argtypes = [vp]
     L.bd_rgsq.argtypes = [vp, DP]
@@ -214,7 +217,8 @@ class BdStep:
 
     def __init__(self, nbeads, *, lhdia=True, mob_fctr="LANC", nts_mobsam=1, lanc_mxitr=50,
                  lanc_tol=1e-6, tim_stp=1e-3, precision="fp64", keep_reuse_noise=False,
-                 flow_style=0, flow_rate=0.0, lanc_reorth=False, integrator="EM", mob_storage="auto"):
+                 flow_style=0, flow_rate=0.0, lanc_reorth=False, integrator="EM", mob_storage="auto",
+                 se_nlmxitr=0, se_kdmax=0, se_tol=(1e-10, 1e-12)):
         self.L = load_library()
         self.nbeads = int(nbeads)
         self.n = 3 * self.nbeads
@@ -225,7 +229,8 @@ class BdStep:
                            float(tim_stp), prec, int(bool(keep_reuse_noise)), int(flow_style),
                            float(flow_rate), int(bool(lanc_reorth)),
                            INTEGRATOR[integrator] if isinstance(integrator, str) else int(integrator),
-                           MOB_STORAGE[mob_storage] if isinstance(mob_storage, str) else int(mob_storage))
+                           MOB_STORAGE[mob_storage] if isinstance(mob_storage, str) else int(mob_storage),
+                           int(se_nlmxitr), int(se_kdmax), float(se_tol[0]), float(se_tol[1]))
         self.h = self.L.bd_ctx_create(self.nbeads, C.byref(self.opts))
         if not self.h:
             raise BdError(BD_ERR_ARG, self.L.bd_last_error().decode())
@@ -304,6 +309,14 @@ class BdStep:
             q = _f64(charge)
             assert q.size == self.nbeads
             self._chk(self.L.bd_set_charges(self.h, _dp(q)))
+
+    def se_info(self):
+        """NITSOL counts (nfe, njve, nrpre, nli, nni, nbt) of the last
+        semi-implicit step and of the run."""
+        last = np.zeros(6, np.int32)
+        tot = np.zeros(6, np.int64)
+        self._chk(self.L.bd_se_info(self.h, _ip(last), tot.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return last, tot
 
     def energies(self):
         """Energies of the last evaluation: dict in stats order."""

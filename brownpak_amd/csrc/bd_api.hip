@@ -104,6 +104,14 @@ __global__ void drift_kernel(const double *__restrict__ v, const double *__restr
     for (int d = 0; d < 3; d++) drift[3 * i + d] = dt * (v[3 * i + d] + f[d]);
 }
 
+int launch_drift(Ctx &c, const double *v)
+{
+    hipLaunchKernelGGL(drift_kernel, dim3((c.nb + 255) / 256), dim3(256), 0, c.stream, v, c.crd, c.nb,
+                       c.opt.flow_style, c.opt.flow_rate, c.opt.tim_stp, c.drift, c.st);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
 // r <- r + drift + diff (diff may be null: zero Brownian term); PBC wrap.
 __global__ void update_kernel(double *__restrict__ crd, const double *__restrict__ drift, const double *__restrict__ diff,
                               int nb, int imcon, double Lx, double Ly, double Lz, DevStatus *st)
@@ -451,6 +459,25 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
         }
         c.cntr_mobsam++;
     }
+    // ---- integrate_se (bd_solver_m.f90:242-308): the new coordinates solve
+    // x - drift(x) - crd0 - w = 0 with this step's D and w
+    if (c.opt.integrator == BD_SE) {
+        const double *w = nullptr;
+        if (disp_dev) w = disp_dev;
+        else if (noise) w = c.diff + (size_t)(hi ? c.cntr_mobsam - 1 : 0) * c.npad;
+        if (rec_dev) {
+            if (w) BD_HIP(hipMemcpyAsync(rec_dev, w, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, sm));
+            else BD_HIP(hipMemsetAsync(rec_dev, 0, (size_t)n * sizeof(double), sm));
+        }
+        BD_TRY(se_step(c, w));
+        // the bds_run tail (bd_solver_m.f90:176-186); se_step wrapped a periodic box
+        if (c.limud())
+            hipLaunchKernelGGL(to_origin_kernel, dim3(1), dim3(1024), 0, sm, c.crd, nb, c.scal + 16, c.st);
+        if (c.stats_on) BD_TRY(stats_step(c));
+        if (hi && c.cntr_mobsam == s) c.cntr_mobsam = 0;
+        BD_HIP(hipGetLastError());
+        return BD_OK;
+    }
     // ---- calc_drift (bd_solver_m.f90:407-454): drift = dt (U(r) + D F(r))
     // (an EM step of an isolated molecule leaves the drift to its tail kernel)
     const bool fused_tail = c.opt.integrator != BD_PC && c.limud();
@@ -524,7 +551,14 @@ bd_ctx *bd_ctx_create(int nbeads, const bd_opts *opts)
     }
     if (opts->mob_fctr < BD_CHOL || opts->mob_fctr > BD_CHOL_LT) { set_error("unknown mob_fctr"); return nullptr; }
     if (opts->precision != BD_FP64 && opts->precision != BD_FP32) { set_error("unknown precision"); return nullptr; }
-    if (opts->integrator != BD_EM && opts->integrator != BD_PC) { set_error("unknown integrator"); return nullptr; }
+    if (opts->integrator != BD_EM && opts->integrator != BD_PC && opts->integrator != BD_SE) {
+        set_error("unknown integrator");
+        return nullptr;
+    }
+    if (opts->integrator == BD_SE && (opts->se_nlmxitr < 0 || opts->se_kdmax < 0 || opts->se_kdmax > 60)) {
+        set_error("se_nlmxitr must be >= 0 and se_kdmax in 0..60");
+        return nullptr;
+    }
     if (opts->mob_storage < BD_MOB_AUTO || opts->mob_storage > BD_MOB_MATFREE) {
         set_error("unknown mob_storage");
         return nullptr;
@@ -562,6 +596,7 @@ bd_ctx *bd_ctx_create(int nbeads, const bd_opts *opts)
         set_error("device allocation failed");
         return fail(0);
     }
+    if (opts->integrator == BD_SE && se_alloc(c) != BD_OK) return fail(0);
     if (hipMalloc(&c.st, sizeof(DevStatus)) != hipSuccess || hipMemset(c.st, 0, sizeof(DevStatus)) != hipSuccess ||
         hipHostMalloc(&c.st_host, 2 * sizeof(DevStatus)) != hipSuccess) {
         set_error("status allocation failed");
@@ -589,6 +624,7 @@ void bd_ctx_destroy(bd_ctx *h)
     mob_free(c);
     mf_free(c);
     stats_free(c);
+    se_free(c);
     lanczos_free(c);
     forces_free(c);
     double **dp[] = {&c.crd, &c.frc, &c.drift, &c.diff, &c.xi_dev, &c.scal, &c.pc_r0, &c.pc_d0};

@@ -235,6 +235,12 @@ struct Ctx {
     // bds_run's pre-loop centring (bd_solver_m.f90:161) is still due: set by
     // bd_set_coordinates and bd_reset_run, cleared by the first bd_run
     bool origin_pending = true;
+    // semi-implicit step (bd_se.hip): Krylov basis and work vectors, reduction
+    // scalars (device and pinned host), NITSOL counts of the last step and
+    // the run (nfe, njve, nrpre, nli, nni, nbt)
+    double *se_buf = nullptr, *se_sc = nullptr, *se_hsc = nullptr;
+    int se_last[6] = {0, 0, 0, 0, 0, 0};
+    long long se_total[6] = {0, 0, 0, 0, 0, 0};
     int *cell_cnt = nullptr, *cell_ptr = nullptr, *cell_beads = nullptr, *bead_cell = nullptr;
     int ncell[3] = {0, 0, 0};
     double cl_cs[3] = {0, 0, 0};   // CL cell sides L / nc
@@ -328,6 +334,12 @@ int grow_pairs(Ctx &c);
 int adj_force(Ctx &c, const double *u, double *out, double *tpart, int tie_rc);
 int set_theta(Ctx &c, double eps, double sigma, double kb, int tie_rc);
 int pair_table_host(Ctx &c, std::vector<int> &pairs);
+
+// --- semi-implicit step (bd_se.hip)
+int se_alloc(Ctx &c);
+void se_free(Ctx &c);
+int se_step(Ctx &c, const double *w);
+int launch_drift(Ctx &c, const double *v);
 
 // --- statistics (bd_stats.hip)
 int stats_step(Ctx &c);

@@ -11,7 +11,8 @@
 !> and a trajectory frame every nts_samp, and a final dump.  integrate_em /
 !> calc_diffusion / calc_drift run on the GPU through libbdstep (bdstep_c);
 !> the driver calls bd_run for the steps between two of those events.
-!> bdintg EM and PC (predictor-corrector) are available on the device.
+!> bdintg EM, SE (semi-implicit, NITSOL keys se_nlmxitr, se_kdmax, se_tol)
+!> and PC (predictor-corrector) are available on the device.
 !>
 !> The Gaussian stream is the device Philox generator.  Its seed comes from
 !> random_seed.txt<job_tag> with read_seed T (init_stream,
@@ -35,7 +36,8 @@ character(len=8) :: mth_ptgen = 'DIR', mob_fctr = 'CHOL', bdintg = 'EM'
 logical :: lhdia = .false., lvdw = .false., lanc_reorth = .false., keep_reuse_noise = .false.
 logical :: lrevive = .false., read_seed = .false., write_seed = .false., write_traj = .false.
 integer :: excluded_atoms = 0, lanc_mxitr = 0, nts_mobsam = 1, precision = 0
-integer :: stats_binsize(2) = 0
+integer :: stats_binsize(2) = 0, se_nlmxitr = 0, se_kdmax = 0
+real(rp) :: se_tol(2) = 0
 integer(c_int64_t) :: nts_sim = 0, nts_log = 0, nts_samp = 0, nts_dump = 0
 character(len=512) :: fn_cfg = 'config.cfg', fn_out = 'final.xyz', fn_revive = 'revive.bin'
 character(len=512) :: fn_stats = 'stats.txt', fn_traj = 'traj.bin'
@@ -79,8 +81,9 @@ call check_supported()
 select case (trim(bdintg))
 case ('EM'); opts%integrator = BD_EM
 case ('PC'); opts%integrator = BD_PC
+case ('SE'); opts%integrator = BD_SE
 case default
-    print '(a)', 'bdrun: bdintg must be EM or PC on the device'
+    print '(a)', 'bdrun: bdintg must be EM, SE or PC'
     stop 2
 end select
 opts%lhdia = merge(1, 0, lhdia)
@@ -96,6 +99,10 @@ opts%tim_stp = tim_stp
 opts%precision = precision
 opts%keep_reuse_noise = merge(1, 0, keep_reuse_noise)
 opts%lanc_reorth = merge(1, 0, lanc_reorth)
+opts%se_nlmxitr = se_nlmxitr
+opts%se_kdmax = se_kdmax
+opts%se_ftol = se_tol(1)
+opts%se_stptol = se_tol(2)
 opts%flow_style = cfg%flow_style
 if (cfg%flow_style /= 0) opts%flow_rate = cfg%flow_params(1)
 
@@ -336,6 +343,9 @@ subroutine read_control(fn)
         case ('lanc_tol'); read(val, *) lanc_tol
         case ('bdintg'); read(val, *) bdintg
         case ('stats_binsize'); read(val, *) stats_binsize
+        case ('se_nlmxitr'); read(val, *) se_nlmxitr
+        case ('se_kdmax'); read(val, *) se_kdmax
+        case ('se_tol'); read(val, *) se_tol
         case ('tim_stp'); read(val, *) tim_stp
         case ('nts_sim'); read(val, *) d; nts_sim = int(d, c_int64_t)
         case ('nts_mobsam'); read(val, *) d; nts_mobsam = int(d)

@@ -10,11 +10,11 @@ use iso_c_binding
 implicit none
 
 integer(c_int), parameter :: BD_OK = 0, BD_ERR_BOND = 1, BD_ERR_FACTOR = 2, &
-    BD_WARN_NOTCONV = 3
+    BD_WARN_NOTCONV = 3, BD_ERR_SOLVE = 4
 integer(c_int), parameter :: BD_CHOL = 0, BD_LANC = 1, BD_CHOL_LT = 2
 integer(c_int), parameter :: BD_FP64 = 0, BD_FP32 = 1
 integer(c_int), parameter :: BD_PT_DIR = 0, BD_PT_VER = 1, BD_PT_CL = 2
-integer(c_int), parameter :: BD_EM = 0, BD_PC = 1
+integer(c_int), parameter :: BD_EM = 0, BD_PC = 1, BD_SE = 2
 integer(c_int), parameter :: BD_MOB_AUTO = 0, BD_MOB_TILES = 1, BD_MOB_MATFREE = 2
 
 type, bind(C) :: bd_opts_t
@@ -31,6 +31,10 @@ type, bind(C) :: bd_opts_t
     integer(c_int) :: lanc_reorth = 0
     integer(c_int) :: integrator = 0
     integer(c_int) :: mob_storage = BD_MOB_AUTO
+    integer(c_int) :: se_nlmxitr = 0
+    integer(c_int) :: se_kdmax = 0
+    real(c_double) :: se_ftol = 0.0_c_double
+    real(c_double) :: se_stptol = 0.0_c_double
 end type
 
 type, bind(C) :: bd_runinfo_t

@@ -35,7 +35,9 @@
  *   BD_WARN_NOTCONV 3 (Lanczos not converged; results valid, bd_solver_m.f90:
  *   393-397), BD_ERR_ARG -1, BD_ERR_HIP -2, BD_ERR_STATE -3, BD_ERR_CELL -4
  *   (negative coordinate in cell-list mode, cell_list_m.f90:193-196),
- *   BD_ERR_CAPACITY -5.  bd_last_error() returns a message for the calling
+ *   BD_ERR_CAPACITY -5, BD_ERR_SOLVE 4 (the semi-implicit step's nonlinear
+ *   solve failed: integrate_se's nitsol iterm 1-6, bd_solver_m.f90:277-299).
+ *   bd_last_error() returns a message for the calling
  *   thread's last failure.
  */
 #ifndef BDSTEP_H
@@ -54,6 +56,7 @@ enum {
     BD_ERR_BOND = 1,
     BD_ERR_FACTOR = 2,
     BD_WARN_NOTCONV = 3,
+    BD_ERR_SOLVE = 4,
     BD_ERR_ARG = -1,
     BD_ERR_HIP = -2,
     BD_ERR_STATE = -3,
@@ -80,8 +83,14 @@ enum {
  * displacement w of the step held fixed:
  *   r* = r + d(r) + w,   r' = r + (d(r) + d(r*)) / 2 + w,
  * d(r) = dt (U(r) + D F(r)) -- one more force and D.F evaluation per step.
- * The reference's semi-implicit 'SE' (NITSOL) is not provided. */
-enum { BD_EM = 0, BD_PC = 1 };
+ * BD_SE is integrate_se (bd_solver_m.f90:242-332): r' solves
+ *   r' - d(r') - r - w = 0
+ * (D of the step held fixed) by NITSOL's inexact Newton-GMRES method as
+ * integrate_se configures it: finite-difference J*v, GMRES restarted every
+ * se_kdmax vectors with the residual re-evaluated at restarts, at most
+ * se_nlmxitr Newton steps, stopping at |F| <= se_ftol or a step shorter
+ * than se_stptol (control keys se_nlmxitr, se_kdmax, se_tol). */
+enum { BD_EM = 0, BD_PC = 1, BD_SE = 2 };
 
 /* mob_storage: how the mobility products D.x (dsymv/dsymm('L'),
  * bd_solver_m.f90:446, brown_m.f90:232, 386) get D.  BD_MOB_TILES stores the
@@ -111,9 +120,14 @@ typedef struct {
     int lanc_reorth;      /* 1: full FP64 reorthogonalisation of every new
                              Krylov block (not in the reference, whose Lanczos
                              loses orthogonality; see brown_m.f90:402-435)     */
-    int integrator;       /* BD_EM (integrate_em, bd_solver_m.f90:216-238) or
-                             BD_PC: predictor-corrector, not in the reference  */
+    int integrator;       /* BD_EM (integrate_em, bd_solver_m.f90:216-238),
+                             BD_SE (integrate_se, 242-332) or BD_PC:
+                             predictor-corrector, not in the reference         */
     int mob_storage;      /* BD_MOB_AUTO / BD_MOB_TILES / BD_MOB_MATFREE      */
+    int se_nlmxitr;       /* BD_SE: Newton steps (0: NITSOL's 200)            */
+    int se_kdmax;         /* BD_SE: Krylov vectors per cycle (0: 20)          */
+    double se_ftol;       /* BD_SE: se_tol(1), stop at |F| <= se_ftol          */
+    double se_stptol;     /* BD_SE: se_tol(2), stop at |step| <= se_stptol     */
 } bd_opts;
 
 /* Context.  Replaces bds_init/brn_init (bd_solver_m.f90:50-104,
@@ -281,6 +295,11 @@ int bd_resume_run(bd_ctx *ctx, const double *com3);
 int bd_stats_enable(bd_ctx *ctx, int binsize, int binsize_stress,
                     int nmolecules, const int *molecules);
 int bd_stats_get(bd_ctx *ctx, double *out30);
+
+/* NITSOL counts of the last semi-implicit step and of all of them: nfe,
+ * njve, nrpre (always 0), nli, nni, nbt (nitsol.f's info(1:6)).  Either
+ * pointer may be NULL. */
+int bd_se_info(bd_ctx *ctx, int *last6, long long *total6);
 
 /* Synchronises the context's stream (timing helper). */
 int bd_sync(bd_ctx *ctx);
