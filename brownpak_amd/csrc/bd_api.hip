@@ -676,7 +676,6 @@ int bd_set_coordinates(bd_ctx *h, const double *crd)
     Ctx &c = h->c;
     BD_HIP(hipMemcpyAsync(c.crd, crd, (size_t)c.n * sizeof(double), hipMemcpyHostToDevice, c.stream));
     BD_HIP(hipStreamSynchronize(c.stream));
-    c.mob_valid = false;
     c.origin_pending = true;
     return BD_OK;
 }

@@ -40,7 +40,8 @@ integer :: stats_binsize(2) = 0, se_nlmxitr = 0, se_kdmax = 0
 real(rp) :: se_tol(2) = 0
 integer(c_int64_t) :: nts_sim = 0, nts_log = 0, nts_samp = 0, nts_dump = 0
 character(len=512) :: fn_cfg = 'config.cfg', fn_out = 'final.xyz', fn_revive = 'revive.bin'
-character(len=512) :: fn_stats = 'stats.txt', fn_traj = 'traj.bin'
+character(len=512) :: fn_stats = 'stats.txt', fn_traj = 'traj.bin', fn_xi = ''
+character(len=8) :: device_path = 'run'
 ! configuration (atmcfg_t) and files
 type(cfg_t) :: cfg
 type(traj_t) :: traj
@@ -55,7 +56,9 @@ integer(c_int64_t) :: nts, nts_beg, nxt, used, seed, offset
 integer(c_int), allocatable :: bonds(:,:), bond_styles(:), vdw_styles(:), mol(:,:)
 real(rp), allocatable :: bond_params(:,:), vdw_params(:,:)
 real(rp) :: box(3), st(30)
-integer :: i, iarg, fu, nvdw, nbt
+integer :: i, iarg, fu, nvdw, nbt, cntr = 0
+real(rp), allocatable, target :: xi(:)
+integer(c_int64_t) :: xi_pos = 0
 integer(8) :: t0, t1, rate
 
 fn_control = 'control.txt'; job_tag = ''
@@ -151,6 +154,13 @@ if (write_seed) then
 end if
 rc = bd_set_rng(ctx, seed, offset); if (rc /= BD_OK) call fail(rc)
 
+! ---- an injected Gaussian stream in place of the device generator
+if (len_trim(fn_xi) > 0) call read_xi(trim(fn_xi))
+if (trim(device_path) == 'percall' .and. (.not. allocated(xi) .or. nts_samp > 0 .or. opts%integrator /= BD_EM)) then
+    print '(a)', 'bdrun: device_path percall needs fn_xi, bdintg EM and nts_samp 0'
+    stop 2
+end if
+
 ! ---- statistics (stats_init) and trajectory (setup_m.f90:62, 69-97)
 lvdw_stats = lvdw .and. (cfg%num_vdw_types > 0)
 call stats_open(trim(fn_stats)//trim(job_tag), lrevive, cfg, lvdw_stats, fu_stats)
@@ -179,7 +189,18 @@ call dump(nts)
 do
     if (nts > nts_sim) exit
     nxt = next_event(nts)
-    rc = bd_run(ctx, int(nxt - nts, c_int), c_null_ptr, 0_c_long_long, used)
+    if (trim(device_path) == 'percall') then
+        call steps_percall(nxt - nts)
+    else if (allocated(xi)) then
+        if (xi_pos >= size(xi, kind=c_int64_t)) then
+            print '(a)', 'bdrun: the fn_xi stream is exhausted'
+            stop 1
+        end if
+        rc = bd_run(ctx, int(nxt - nts, c_int), c_loc(xi(xi_pos + 1)), size(xi, kind=c_int64_t) - xi_pos, used)
+        xi_pos = xi_pos + used
+    else
+        rc = bd_run(ctx, int(nxt - nts, c_int), c_null_ptr, 0_c_long_long, used)
+    end if
     if (rc /= BD_OK .and. rc /= BD_WARN_NOTCONV) then
         ! bds_run returns without the final dump (bd_solver_m.f90:171)
         print '(a,i0)', 'bdrun: failed in the steps after nts = ', nts
@@ -245,9 +266,107 @@ subroutine dump(n)
     integer(c_int64_t) :: sd, off
     rc = bd_get_coordinates(ctx, cfg%coordinates); if (rc /= BD_OK) call fail(rc)
     rc = bd_get_runinfo(ctx, info); if (rc /= BD_OK) call fail(rc)
-    if (limud) cfg%molecule_com = info%molecule_com
+    if (limud .and. trim(device_path) /= 'percall') cfg%molecule_com = info%molecule_com
     rc = bd_get_rng(ctx, sd, off); if (rc /= BD_OK) call fail(rc)
     call write_dump(trim(fn_revive)//trim(job_tag), n, cfg, sd, off)
+end subroutine
+
+!> The injected Gaussian stream: raw float64 values.
+subroutine read_xi(fn)
+    character(len=*), intent(in) :: fn
+    integer :: fx
+    integer(c_int64_t) :: nbytes
+    open(newunit=fx, file=fn, access='stream', form='unformatted', action='read', status='old')
+    inquire(unit=fx, size=nbytes)
+    allocate(xi(nbytes / 8))
+    read(fx) xi
+    close(fx)
+end subroutine
+
+!> k steps of integrate_em and the bds_run tail (bd_solver_m.f90:167-238,
+!> 355-454) composed on the host from the entry points that replace its
+!> call sites: bd_rpy_assemble (calc_rpy_tensor), bd_brownian
+!> (brn_calc_bdw), bd_forces (ia_calc_forces) and bd_mob_symv (the dsymv of
+!> calc_drift); the flow field, the update, the centre-of-mass removal and
+!> the wrap are done here, as the reference does them.
+subroutine steps_percall(k)
+    integer(c_int64_t), intent(in) :: k
+    real(rp), allocatable, save :: crd(:,:), frc(:,:), drift(:), dif(:,:), bdw(:,:), tmp(:)
+    real(rp) :: en3(3), st9(9), bl3(3), com(3), sq2dt
+    integer(c_int) :: lconv, iters
+    integer :: n, s, col, ia, d
+    integer(c_int64_t) :: it
+    n = 3 * cfg%num_atoms
+    s = merge(max(nts_mobsam, 1), 1, lhdia)
+    sq2dt = sqrt(2.0_rp * tim_stp)
+    if (.not. allocated(crd)) then
+        allocate(crd(3, cfg%num_atoms), frc(3, cfg%num_atoms), drift(n), dif(n, s), bdw(n, s), tmp(n))
+        dif = 0
+    end if
+    rc = bd_get_coordinates(ctx, crd); if (rc /= BD_OK) call fail(rc)
+    do it = 1, k
+        ! calc_diffusion
+        if (.not. lhdia) then
+            dif(:, 1) = sq2dt * xi(xi_pos + 1:xi_pos + n)
+            xi_pos = xi_pos + n
+            col = 1
+        else
+            if (cntr == 0) then
+                rc = bd_set_coordinates(ctx, crd); if (rc /= BD_OK) call fail(rc)
+                rc = bd_rpy_assemble(ctx); if (rc /= BD_OK) call fail(rc)
+                rc = bd_brownian(ctx, xi(xi_pos + 1:), int(s, c_int), opts%mob_fctr, bdw, lconv, iters)
+                if (rc /= BD_OK) call fail(rc)
+                xi_pos = xi_pos + int(n, c_int64_t) * s
+                dif = sq2dt * bdw
+            else if (.not. keep_reuse_noise) then
+                dif = 0
+            end if
+            cntr = cntr + 1
+            col = cntr
+        end if
+        ! calc_drift
+        rc = bd_set_coordinates(ctx, crd); if (rc /= BD_OK) call fail(rc)
+        rc = bd_forces(ctx, frc, en3, st9, bl3); if (rc /= BD_OK) call fail(rc)
+        drift = 0
+        do ia = 1, cfg%num_atoms
+            select case (cfg%flow_style)
+            case (1)
+                drift(3*ia-2) = opts%flow_rate * crd(2, ia)
+            case (2)
+                drift(3*ia-2) = opts%flow_rate * crd(1, ia)
+                drift(3*ia-1) = -opts%flow_rate * crd(2, ia)
+            case (3)
+                drift(3*ia-2) = opts%flow_rate * crd(1, ia)
+                drift(3*ia-1) = -0.5_rp * opts%flow_rate * crd(2, ia)
+                drift(3*ia) = -0.5_rp * opts%flow_rate * crd(3, ia)
+            end select
+        end do
+        if (lhdia) then
+            rc = bd_mob_symv(ctx, frc, 1_c_int, tmp); if (rc /= BD_OK) call fail(rc)
+            drift = tmp + drift
+        else
+            drift = drift + reshape(frc, [n])
+        end if
+        drift = tim_stp * drift
+        ! integrate_em
+        crd = crd + reshape(drift, [3, cfg%num_atoms]) + reshape(dif(:, col), [3, cfg%num_atoms])
+        if (lhdia .and. cntr == s) cntr = 0
+        ! the bds_run tail: centre of mass (to_origin) or wrap
+        if (limud) then
+            com = sum(crd, 2) / cfg%num_atoms
+            do ia = 1, cfg%num_atoms
+                crd(:, ia) = crd(:, ia) - com
+            end do
+            cfg%molecule_com = cfg%molecule_com + com
+        else if (cfg%imcon /= 0) then
+            do ia = 1, cfg%num_atoms
+                do d = 1, 3
+                    crd(d, ia) = crd(d, ia) - box(d) * floor(crd(d, ia) / box(d))
+                end do
+            end do
+        end if
+    end do
+    rc = bd_set_coordinates(ctx, crd); if (rc /= BD_OK) call fail(rc)
 end subroutine
 
 !> Parts of atmcfg_t the device step does not cover: the statistics of
@@ -316,7 +435,10 @@ end function
 
 !> Whitespace key-value control file; unknown keys are ignored
 !> (control_m.f90:92-150).  Extra device keys: precision, lanc_reorth,
-!> keep_reuse_noise, fn_out.
+!> keep_reuse_noise, fn_out, fn_xi (a raw float64 file: the Gaussian stream
+!> brn_calc_dw would draw, in place of the device generator) and
+!> device_path (run: bd_run; percall: integrate_em composed on the host from
+!> the per-call-site entries, see steps_percall).
 subroutine read_control(fn)
     character(len=*), intent(in) :: fn
     character(len=512) :: line, key, val
@@ -361,6 +483,8 @@ subroutine read_control(fn)
         case ('fn_stats'); fn_stats = trim(val)
         case ('fn_traj'); fn_traj = trim(val)
         case ('fn_out'); fn_out = trim(val)
+        case ('fn_xi'); fn_xi = trim(val)
+        case ('device_path'); read(val, *) device_path
         case ('precision'); read(val, *) precision
         case ('lanc_reorth'); read(val, *) lanc_reorth
         case ('keep_reuse_noise'); read(val, *) keep_reuse_noise

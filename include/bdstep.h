@@ -194,7 +194,9 @@ int bd_rpy_assemble(bd_ctx *ctx);
 int bd_get_mobility(bd_ctx *ctx, double *mob);
 
 /* dsymv/dsymm('L') products (bd_solver_m.f90:446, brown_m.f90:386):
- * y(n,s) := D x(n,s) with the assembled D. */
+ * y(n,s) := D x(n,s) with the last assembled D.  New coordinates do not
+ * replace D until the next bd_rpy_assemble, as the reference keeps mob over
+ * nts_mobsam steps. */
 int bd_mob_symv(bd_ctx *ctx, const double *x, int s, double *y);
 
 /* brn_calc_dw + brn_calc_bdw (brown_m.f90:147-181, 558-573) on the

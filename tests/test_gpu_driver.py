@@ -93,3 +93,32 @@ def test_driver_files_and_revive(tmp_path):
     assert (da / "revive.bin_r").read_bytes() == dump
     nts, _, _, atc, com, rng = S.read_dump(dump)
     assert nts == 8 and rng == g.get_rng()
+
+
+@pytest.mark.parametrize("mode", ["lanc_s4", "chol", "free"])
+def test_driver_percall_composition_matches_bd_run(tmp_path, mode):
+    """integrate_em composed in Fortran from the per-call-site entries
+    (bd_rpy_assemble, bd_brownian, bd_forces, bd_mob_symv; update, centring
+    and flow on the host) takes the same steps as bd_run on the same
+    injected Gaussian stream."""
+    m = fene_wca_chain(100, 12)
+    # Lanczos with reorthogonalisation: without it the converged B.xi moves by
+    # ~1e-7 under a rounding-level change of the coordinates (the centre of
+    # mass is summed in another order on the host)
+    ctl = {"lanc_s4": CONTROL + "lanc_reorth T\n",
+           "chol": CONTROL.replace("mob_fctr LANC", "mob_fctr CHOL").replace("nts_mobsam 4", "nts_mobsam 1"),
+           "free": CONTROL.replace("lhdia T", "lhdia F")}[mode]
+    ctl += "nts_sim 11\nnts_log 4\nfn_cfg chain.cfg\nfn_xi xi.bin\nfn_out final.xyz\n"
+    xi = np.random.default_rng(21).standard_normal(4000)
+    out = {}
+    for path in ("run", "percall"):
+        d = tmp_path / path
+        d.mkdir()
+        write_cfg(d / "chain.cfg_p", m)
+        xi.tofile(d / "xi.bin")
+        (d / "control.txt").write_text(ctl + f"device_path {path}\n")
+        _drive(d, "_p")
+        lines = (d / "final.xyz_p").read_text().splitlines()
+        out[path] = np.array([[float(x) for x in ln.split()[1:4]] for ln in lines[2:]])
+    print(f"  {mode}: rel {rel(out['percall'], out['run']):.2e}")
+    assert rel(out["percall"], out["run"]) < 1e-12
