@@ -166,6 +166,36 @@ def cpu_baseline(budget_s=20.0):
                        f"block Lanczos s=4 tol 1e-6, FP64, oracle/bdoracle.c at 1 thread, {dt:.1f} s")
 
 
+def cpu_model(lanczos_iters_per_update):
+    """A bandwidth model of the reference's own CPU path on this host: its
+    step streams the stored FP64 mobility mob(n,n) (dsymm/dsymv('L') read the
+    lower triangle, n(n+1)/2 doubles) once per Lanczos product and once for
+    the drift, and rewrites it every s steps (calc_rpy_tensor), so with m
+    Lanczos iterations per update a step moves about
+    T (1 + (m+1)/s) + 8 n^2 / s bytes, T = 8 n(n+1)/2.  The host's memory
+    rate is measured here by a multi-threaded 1 GiB copy.  The model is an
+    upper bound for the MKL-based reference (which does not build here): it
+    ignores the block Lanczos eigensolves and the force loop."""
+    import torch
+    nbytes = 2 ** 30
+    a = torch.ones(nbytes // 8, dtype=torch.float64)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    reps = 4
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    dt = time.perf_counter() - t0
+    bw = 2.0 * nbytes * reps / dt
+    n = 3 * N_BEADS
+    T = 8.0 * n * (n + 1) / 2
+    m = lanczos_iters_per_update
+    per_step = T * (1.0 + (m + 1.0) / S_BLOCK) + 8.0 * n * n / S_BLOCK
+    return dict(value=round(bw / per_step, 1), unit="steps/s", host_copy_gbs=round(bw / 1e9, 1),
+                threads=torch.get_num_threads(), bytes_per_step=round(per_step),
+                basis="host memory rate / bytes the reference's dense-mobility step streams")
+
+
 ENSEMBLE_CHAINS = 512
 
 
@@ -210,6 +240,76 @@ def time_ensemble(steps, warmup, precision="fp32"):
                 not_converged=info2["n_not_converged"])
 
 
+def time_c1(steps=400, warmup=20, cpu_steps=400):
+    """The plain-method configuration: one chain of N = 100 beads, FENE/WCA
+    (k = 30, Q0 = 1.5 sigma), RPY with Cholesky noise (the reference's U.xi,
+    a new factorisation every step), FP64, dt 1e-3, unbounded domain;
+    device Philox noise.  Also the oracle on the host (one thread) over
+    cpu_steps steps with injected noise."""
+    from brownpak_amd import BdStep
+    from brownpak_amd.configs import chain_bonds, fene_wca_params, random_walk_chain
+    nb = 100
+    crd = random_walk_chain(nb, 100)
+    bond, vdw = fene_wca_params()
+    g = BdStep(nb, lhdia=True, mob_fctr="CHOL", nts_mobsam=1, tim_stp=1e-3)
+    g.set_box(0, (0.0, 0.0, 0.0))
+    g.ia_setup(chain_bonds(nb), [2], [bond], vdw_styles=[1], vdw_params=[vdw], excluded_atoms=0,
+               mth_ptgen="DIR", rcutoff=vdw[2])
+    g.set_coordinates(crd)
+    g.set_rng(1)
+    g.bds_run(warmup)
+    t0 = time.perf_counter()
+    g.bds_run(steps)
+    g.sync()
+    dt = time.perf_counter() - t0
+    out = dict(steps_per_s=round(steps / dt, 1), ms_per_step=round(1e3 * dt / steps, 4), nbeads=nb,
+               noise="Cholesky (U.xi)", dtype="f64")
+    g.close()
+    try:
+        from oracle import oracle as O
+        O.lib()
+        sysobj = O.System(nb, chain_bonds(nb), [2], [bond], vdw_styles=[1], vdw_params=[vdw], lvdw=True,
+                          excluded_atoms=0, mth_ptgen=0, rcutoff=vdw[2])
+        xi = np.random.default_rng(5).standard_normal(3 * nb * cpu_steps)
+        t0 = time.perf_counter()
+        O.bd_run(sysobj, crd, cpu_steps, xi, lhdia=True, mob_fctr=O.CHOL, tim_stp=1e-3)
+        cdt = time.perf_counter() - t0
+        out["cpu_port_steps_per_s"] = round(cpu_steps / cdt, 1)
+        out["cpu_port"] = f"oracle/bdoracle.c, 1 thread, {cpu_steps} steps"
+    except Exception as e:  # the oracle is the checker; its absence leaves the GPU figure
+        out["cpu_port"] = f"not run: {e}"
+    return out
+
+
+def time_c4(steps=200, warmup=20):
+    """The semi-dilute free-draining solution: 200 chains x 100 beads in a
+    periodic cubic box at 0.3 beads / sigma^3, harmonic bonds (k = 100,
+    r0 = sigma), full LJ (rc = 2.5 sigma) over a Verlet list with a 0.3
+    sigma skin built from cells, FP64, dt 1e-3, device noise."""
+    from brownpak_amd import BdStep
+    from brownpak_amd.configs import chain_bonds, periodic_melt
+    nch, nper = 200, 100
+    crd, L = periodic_melt(nch, nper, 0.3, 31)
+    nb = nch * nper
+    bonds = np.concatenate([chain_bonds(nper, offset=c * nper) for c in range(nch)])
+    g = BdStep(nb, lhdia=False, tim_stp=1e-3)
+    g.set_box(1, (L, L, L))
+    g.ia_setup(bonds, [1], [[25.0, 2.0]], vdw_styles=[1], vdw_params=[[1.0, 2.0, 5.0]], excluded_atoms=1,
+               mth_ptgen="VER", rcutoff=5.0, tskin=0.6)
+    g.set_coordinates(crd)
+    g.set_rng(2)
+    g.bds_run(warmup)
+    r0 = g.runinfo()["n_pair_rebuilds"]
+    t0 = time.perf_counter()
+    g.bds_run(steps)
+    g.sync()
+    dt = time.perf_counter() - t0
+    r1 = g.runinfo()["n_pair_rebuilds"]
+    g.close()
+    return dict(steps_per_s=round(steps / dt, 1), ms_per_step=round(1e3 * dt / steps, 4), nbeads=nb,
+                box=round(L, 3), pair_rebuilds=int(r1 - r0), dtype="f64", pair_table="VER (cells)")
+
+
 def measured_traffic(kind, prec="fp32"):
     """HBM bytes per launch of the FP32 Lanczos product kernel (kind "tiles":
     symm_tiles_kernel<float, 4>; "mf": mf_product_kernel<float, 4, 1>) from
@@ -252,6 +352,8 @@ def main():
                     help="end with a 256 MiB device copy (counter calibration of profiled runs)")
     ap.add_argument("--ensemble-steps", type=int, default=20,
                     help="steps of the 512-chain ensemble reported beside the main line (0: skip)")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the N=100 Cholesky and 20k-bead periodic figures")
     ap.add_argument("--workload", choices=["chain", "ensemble"], default="chain",
                     help="ensemble: time only the 512-chain ensemble and print its object")
     args = ap.parse_args()
@@ -282,6 +384,10 @@ def main():
     if args.ensemble_steps > 0:
         ens = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK)
         print(f"# ensemble: {ens['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
+    other = None
+    if not args.no_other_configs:
+        other = {"c1_chain100_cholesky": time_c1(), "c4_melt20k_periodic_verlet": time_c4()}
+        print(f"# other: {json.dumps(other)}", file=sys.stderr, flush=True)
     acc = None if args.no_error_check else accuracy_states(args.states, best)
     # the headline: the fastest form whose one-step displacement stays within
     # 1e-6 of FP64 (FP64 itself by definition; the mixed form when its
@@ -380,9 +486,17 @@ def main():
         "roofline_tiles": roof_tiles_entry,
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "other_configs": other,
     }
     if cpu:
         out["vs_cpu_fp64"] = round(fp64["steps_per_s"] / cpu["value"], 1)
+        try:
+            cm = cpu_model(hrun["lanczos_iters"] / max(hrun["mob_updates"], 1))
+            out["cpu_model"] = cm
+            out["vs_cpu_model_fp64"] = round(fp64["steps_per_s"] / cm["value"], 1)
+        except Exception as e:
+            out["cpu_model"] = f"not measured: {e}"
+
     print(json.dumps(out), flush=True)
     if args.calibration_copy:
         calibration_copy()

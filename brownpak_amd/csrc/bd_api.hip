@@ -387,12 +387,12 @@ int check_status(Ctx &c, bool clear)
 }
 
 // B.xi for the block in c.xi_dev -> c.diff (scaled later by the caller).
-static int brownian_dev(Ctx &c, int method, int s, int *lconv, int *iters)
+static int brownian_dev(Ctx &c, int method, int s, int *lconv, int *iters, bool sync = true)
 {
     if (method == BD_LANC) return brownian_lanczos(c, c.xi_dev, c.diff, s, lconv, iters);
     *lconv = 1;
     *iters = 0;
-    return brownian_cholesky(c, c.xi_dev, c.diff, s, method);
+    return brownian_cholesky(c, c.xi_dev, c.diff, s, method, sync);
 }
 
 // One EM step (integrate_em + the bds_run loop tail).  xi_host: next block of
@@ -443,7 +443,9 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
                 }
                 int lconv = 1, iters = 0;
                 int method = c.opt.mob_fctr;
-                int rc = brownian_dev(c, method, s, &lconv, &iters);
+                // a Cholesky failure stops the step's later kernels (ierr) and
+                // bd_run reports it; no host round trip per step
+                int rc = brownian_dev(c, method, s, &lconv, &iters, false);
                 if (rc != BD_OK) return rc;
                 if (method == BD_LANC) {
                     c.info.last_lanczos_iters = iters;

@@ -314,7 +314,7 @@ int lanczos_alloc(Ctx &c);
 void lanczos_free(Ctx &c);
 // xi (npad x s, device) -> out (npad x s, device); returns ierr-style code.
 int brownian_lanczos(Ctx &c, const double *xi, double *out, int s, int *lconv, int *iters);
-int brownian_cholesky(Ctx &c, const double *xi, double *out, int s, int method);
+int brownian_cholesky(Ctx &c, const double *xi, double *out, int s, int method, bool sync = true);
 
 // --- forces (bd_forces.hip)
 int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types,

@@ -1574,7 +1574,9 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, int ld, int 
     }
 }
 
-// Panel: rows k0+nb..n-1 of block column k0 := A21 L11^{-T}.
+// Panel: rows k0+nb..n-1 of block column k0 := A21 L11^{-T}.  The row is
+// solved in registers: the loops run over the full CNB with compile-time
+// indices (columns j >= nb are skipped uniformly).
 __global__ __launch_bounds__(256) void potrf_panel_kernel(double *A, int ld, int n, int k0, int nb, DevStatus *st)
 {
     if (st->ierr) return;
@@ -1587,13 +1589,267 @@ __global__ __launch_bounds__(256) void potrf_panel_kernel(double *A, int ld, int
     int r = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     double x[CNB];
-    for (int j = 0; j < nb; j++) x[j] = A[(long)(k0 + j) * ld + r];
-    for (int j = 0; j < nb; j++) {
-        double t = x[j];
-        for (int k = 0; k < j; k++) t -= x[k] * L[j][k];
-        x[j] = t / L[j][j];
+#pragma unroll
+    for (int j = 0; j < CNB; j++) x[j] = j < nb ? A[(long)(k0 + j) * ld + r] : 0.0;
+#pragma unroll
+    for (int j = 0; j < CNB; j++) {
+        if (j < nb) {
+            double t = x[j];
+#pragma unroll
+            for (int k = 0; k < j; k++) t -= x[k] * L[j][k];
+            x[j] = t / L[j][j];
+        }
     }
-    for (int j = 0; j < nb; j++) A[(long)(k0 + j) * ld + r] = x[j];
+#pragma unroll
+    for (int j = 0; j < CNB; j++)
+        if (j < nb) A[(long)(k0 + j) * ld + r] = x[j];
+}
+
+// The whole factorisation and B.xi in one workgroup of 8 wavefronts, for
+// n <= CS_MAX, right-looking over 32-column panels with a one-panel
+// lookahead.  For panel p (columns k0..k0+nb-1):
+//  A. one thread per row below the diagonal block solves that row against
+//     the block (L21 = A21 L11^-T) and stores it transposed in LDS,
+//     PT[j][t] = L(k0+nb+t, k0+j), and in A;
+//  B. wavefront 0 applies the panel to the next diagonal block itself and
+//     factors it, while wavefronts 1-7 form the panel's share of out and
+//     update the rest of the trailing lower triangle.
+// The diagonal block's factor (a chain of 32 dependent pivots on one
+// wavefront) is thus off the critical path except for the first panel.  It
+// runs in registers, lane i holding row i; each finished column of L goes to
+// DS[j][q] = L(k0+j+q, k0+j) and returns to every lane from there, and a
+// short last block is padded with the identity, which leaves its factor
+// unchanged.  out = L^T xi (the reference's U.xi, trans) takes column k0+j of
+// L whole, so rows k0..k0+nb-1 of out are final after panel p; out = L xi
+// adds the panel's columns into every row below k0.  The trailing update
+// works in 32 x 32 tiles, one per wavefront, each lane a 4 x 4 block whose two
+// row pairs and two column pairs come from PT as 16-byte reads (8 lanes cover
+// 128 contiguous bytes, the rest broadcast), accumulated onto A's values
+// loaded up front.  Only the last panel can be short (nb < 32), and it has no
+// rows below, so A and the trailing update always run with nb = 32.  Two
+// workgroup barriers per panel.  A non-positive pivot sets ierr = 2.
+constexpr int CS_MAX = 480;
+constexpr int CS_T = 512;
+constexpr int CS_PT = CS_MAX - CNB;   // trailing rows per panel, at most
+constexpr int CS_XS = 2;              // columns of xi staged in LDS
+
+// orders a wavefront's LDS stores before its later loads by other lanes
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// lane l's value of v (l wave-uniform), through the scalar unit
+__device__ inline double read_lane(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// One pivot j of the diagonal block.  Lane i holds row i from column j on,
+// x[q] = A(i, j + q); entries from LIVE on are zero.  The next pivot is lane
+// j+1's x[1] - L(j+1, j)^2, all its own values, so it is formed (dn) before
+// the column makes its round trip through LDS; x[0] after the update is the
+// same number.
+template <int LIVE>
+__device__ inline void diag_pivot(double (&x)[CNB], double &dn, int j, int lane, int nb, double *Acol,
+                                  double (*DS)[64], bool &bad, double &inv)
+{
+    const double djj = read_lane(dn, j);
+    if (!(djj > 0.0)) bad = true;
+    const double rp = rsqrt(fmax(djj, 1e-300));
+    const double cj = lane == j ? djj * rp : x[0] * rp;   // column j of L on lanes >= j
+    dn = x[1] - cj * cj;
+    inv = lane == j ? rp : inv;
+    if (lane >= j) DS[j][lane - j] = cj;
+    if (lane >= j && lane < nb) Acol[lane] = cj;
+    wave_sync();
+    double2 cv[LIVE / 2];
+#pragma unroll
+    for (int m = 0; m < LIVE / 2; m++) cv[m] = *reinterpret_cast<const double2 *>(&DS[j][2 * m]);
+#pragma unroll
+    for (int q = 1; q < LIVE; q++) x[q - 1] = x[q] - cj * ((q & 1) ? cv[q / 2].y : cv[q / 2].x);
+    x[LIVE - 1] = 0.0;
+}
+
+// Factor the diagonal block at (kb, kb) of order nbk on one wavefront, after
+// subtracting the panel in PT from it when UPD.  L goes to A's lower part and
+// to DS, the pivots' reciprocals to Dinv; a non-positive pivot sets *fail.
+template <bool UPD>
+__device__ inline void diag_block(double *A, int ld, int kb, int nbk, int lane, double (*DS)[64], double *Dinv,
+                                  const double (*PT)[CS_PT], int *fail)
+{
+    double x[CNB];
+#pragma unroll
+    for (int j = 0; j < CNB; j++)
+        x[j] = (j <= lane && lane < nbk) ? A[(long)(kb + j) * ld + kb + lane] : (j == lane ? 1.0 : 0.0);
+    if (UPD) {
+#pragma unroll 2
+        for (int k = 0; k < CNB; k++) {
+            const double pi = lane < nbk ? PT[k][lane] : 0.0;
+            double2 pv[CNB / 2];
+#pragma unroll
+            for (int m = 0; m < CNB / 2; m++) pv[m] = *reinterpret_cast<const double2 *>(&PT[k][2 * m]);
+#pragma unroll
+            for (int q = 0; q < CNB; q++) x[q] -= pi * ((q & 1) ? pv[q / 2].y : pv[q / 2].x);
+        }
+    }
+    bool bad = false;
+    double inv = 0.0, dn = x[0];
+    double *Ak = A + (long)kb * ld + kb;
+#pragma unroll 1
+    for (int j = 0; j < 8; j++) diag_pivot<32>(x, dn, j, lane, nbk, Ak + (long)j * ld, DS, bad, inv);
+#pragma unroll 1
+    for (int j = 8; j < 16; j++) diag_pivot<24>(x, dn, j, lane, nbk, Ak + (long)j * ld, DS, bad, inv);
+#pragma unroll 1
+    for (int j = 16; j < 24; j++) diag_pivot<16>(x, dn, j, lane, nbk, Ak + (long)j * ld, DS, bad, inv);
+#pragma unroll 1
+    for (int j = 24; j < 32; j++) diag_pivot<8>(x, dn, j, lane, nbk, Ak + (long)j * ld, DS, bad, inv);
+    if (lane < CNB) Dinv[lane] = inv;
+    if (bad && lane == 0) *fail = 1;
+}
+
+// One column j of a panel row's solve x L11^T = a: x[q] = the row's entry in
+// column j + q, entries from LIVE on are zero.
+template <int LIVE>
+__device__ inline void panel_col(double (&x)[CNB], int j, int t, const double (*DS)[64], const double *Dinv,
+                                 double (*PT)[CS_PT], double *Aj)
+{
+    double2 dv[LIVE / 2];
+#pragma unroll
+    for (int m = 0; m < LIVE / 2; m++) dv[m] = *reinterpret_cast<const double2 *>(&DS[j][2 * m]);
+    const double xj = x[0] * Dinv[j];
+    PT[j][t] = xj;
+    *Aj = xj;
+#pragma unroll
+    for (int q = 1; q < LIVE; q++) x[q - 1] = x[q] - xj * ((q & 1) ? dv[q / 2].y : dv[q / 2].x);
+    x[LIVE - 1] = 0.0;
+}
+
+__global__ __launch_bounds__(CS_T) void chol_small_kernel(double *A, int ld, int n, const double *__restrict__ xi,
+                                                          double *__restrict__ out, int s, int ldx, int trans,
+                                                          DevStatus *st)
+{
+    __shared__ __align__(16) double PT[CNB][CS_PT];
+    __shared__ __align__(16) double DS[2][CNB][64];   // DS[p & 1][j][q] = L(k0+j+q, k0+j), zero for j+q >= nb
+    __shared__ double Dinv[CNB];
+    __shared__ double XS[CS_XS][CS_MAX];
+    __shared__ int fail;
+    constexpr int W = CS_T / 64, W1 = W - 1, JW = (CNB + W1 - 1) / W1;
+    const int t = threadIdx.x, lane = t % 64, w = t / 64;
+    if (st->ierr) return;
+    if (t == 0) fail = 0;
+    const int sx = min(s, CS_XS);
+    for (int e = t; e < n * sx; e += CS_T) XS[e / n][e % n] = xi[(long)(e / n) * ldx + e % n];
+    auto xv = [&](int c, int i) { return c < CS_XS ? XS[c][i] : xi[(long)c * ldx + i]; };
+    if (w == 0) diag_block<false>(A, ld, 0, min(CNB, n), lane, DS[0], Dinv, PT, &fail);
+    __syncthreads();
+    for (int k0 = 0, p = 0; k0 < n; k0 += CNB, p++) {
+        if (fail) {
+            if (t == 0) atomicCAS(&st->ierr, 0, 2);
+            return;
+        }
+        const int nb = min(CNB, n - k0), m2 = n - k0 - nb, base = k0 + nb;
+        const double(*D)[64] = DS[p & 1];
+        if (t < m2) {   // A.  nb == CNB; x[q] = the row's entry in column j + q
+            const int r = base + t;
+            double x[CNB];
+#pragma unroll
+            for (int j = 0; j < CNB; j++) x[j] = A[(long)(k0 + j) * ld + r];
+            double *Ar = A + (long)k0 * ld + r;
+#pragma unroll 2
+            for (int j = 0; j < 8; j++) panel_col<32>(x, j, t, D, Dinv, PT, Ar + (long)j * ld);
+#pragma unroll 2
+            for (int j = 8; j < 16; j++) panel_col<24>(x, j, t, D, Dinv, PT, Ar + (long)j * ld);
+#pragma unroll 2
+            for (int j = 16; j < 24; j++) panel_col<16>(x, j, t, D, Dinv, PT, Ar + (long)j * ld);
+#pragma unroll 2
+            for (int j = 24; j < 32; j++) panel_col<8>(x, j, t, D, Dinv, PT, Ar + (long)j * ld);
+        }
+        __syncthreads();
+        if (w == 0) {   // B on wavefront 0: the next diagonal block
+            if (m2 > 0) diag_block<true>(A, ld, base, min(CNB, m2), lane, DS[(p + 1) & 1], Dinv, PT, &fail);
+        } else {
+            const int w1 = w - 1, t1 = t - 64;
+            if (trans) {   // columns w1, w1 + W1, ... of L, whole, against xi
+                for (int c = 0; c < s; c++) {
+                    double acc[JW];
+#pragma unroll
+                    for (int u = 0; u < JW; u++) {
+                        const int j = w1 + W1 * u;
+                        acc[u] = (j < nb && lane >= j && lane < nb) ? D[j][lane - j] * xv(c, k0 + lane) : 0.0;
+                    }
+                    for (int i = lane; i < m2; i += 64) {   // m2 > 0 only when nb == CNB
+                        const double xr = xv(c, base + i);
+#pragma unroll
+                        for (int u = 0; u < JW; u++)
+                            if (w1 + W1 * u < CNB) acc[u] += PT[w1 + W1 * u][i] * xr;
+                    }
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+                        for (int u = 0; u < JW; u++) acc[u] += __shfl_xor(acc[u], off, 64);
+                    if (lane == 0)
+#pragma unroll
+                        for (int u = 0; u < JW; u++)
+                            if (w1 + W1 * u < nb) out[(long)c * ldx + k0 + w1 + W1 * u] = acc[u];
+                }
+            } else {   // the panel's columns into rows k0.. of L xi
+                for (int r = k0 + t1; r < n; r += CS_T - 64)
+                    for (int c = 0; c < s; c++) {
+                        double acc = 0.0;
+                        if (r < base)
+                            for (int j = 0; j <= r - k0; j++) acc += D[j][r - k0 - j] * xv(c, k0 + j);
+                        else
+                            for (int j = 0; j < nb; j++) acc += PT[j][r - base] * xv(c, k0 + j);
+                        double *o = out + (long)c * ldx + r;
+                        *o = k0 == 0 ? acc : *o + acc;
+                    }
+            }
+            // trailing update A22 -= L21 L21^T but its first diagonal tile
+            const int qt = (m2 + 31) / 32, ntile = qt * (qt + 1) / 2;
+            const int lr = 2 * (lane % 8), lc = 2 * (lane / 8);
+            for (int e = 1 + w1; e < ntile; e += W1) {
+                int tr = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+                while ((tr + 1) * (tr + 2) / 2 <= e) tr++;
+                while (tr * (tr + 1) / 2 > e) tr--;
+                const int tc = e - tr * (tr + 1) / 2;
+                const int R0 = 32 * tr + lr, C0 = 32 * tc + lc;
+                double acc[4][4];
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const int r = R0 + (a & 1) + 16 * (a >> 1), cc = C0 + (b & 1) + 16 * (b >> 1);
+                        acc[a][b] = (r < m2 && cc < m2 && r >= cc) ? A[(long)(base + cc) * ld + base + r] : 0.0;
+                    }
+#pragma unroll 8
+                for (int k = 0; k < CNB; k++) {
+                    const double2 u0 = *reinterpret_cast<const double2 *>(&PT[k][R0]);
+                    const double2 u1 = *reinterpret_cast<const double2 *>(&PT[k][R0 + 16]);
+                    const double2 v0 = *reinterpret_cast<const double2 *>(&PT[k][C0]);
+                    const double2 v1 = *reinterpret_cast<const double2 *>(&PT[k][C0 + 16]);
+                    const double u[4] = {u0.x, u0.y, u1.x, u1.y}, v[4] = {v0.x, v0.y, v1.x, v1.y};
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int b = 0; b < 4; b++) acc[a][b] -= u[a] * v[b];
+                }
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const int r = R0 + (a & 1) + 16 * (a >> 1), cc = C0 + (b & 1) + 16 * (b >> 1);
+                        if (r < m2 && cc < m2 && r >= cc) A[(long)(base + cc) * ld + base + r] = acc[a][b];
+                    }
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // Trailing update A22 -= L21 L21^T (lower part), 16x16 output tiles.
@@ -1634,7 +1890,7 @@ __global__ void trmv_kernel(const double *L, int ld, int n, const double *x, dou
     if (threadIdx.x == 0) out[(long)c * ldx + row] = t;
 }
 
-int brownian_cholesky(Ctx &c, const double *xi, double *out, int s, int method)
+int brownian_cholesky(Ctx &c, const double *xi, double *out, int s, int method, bool sync)
 {
     const int n = c.n;
     if (!c.chol) {
@@ -1644,6 +1900,23 @@ int brownian_cholesky(Ctx &c, const double *xi, double *out, int s, int method)
     BD_TRY(mob_unpack(c, c.chol, c.chol_ld));
     hipStream_t sm = c.stream;
     const int ld = c.chol_ld;
+    if (n <= CS_MAX) {
+        hipLaunchKernelGGL(chol_small_kernel, dim3(1), dim3(CS_T), 0, sm, c.chol, ld, n, xi, out, s, c.npad,
+                           method == BD_CHOL ? 1 : 0, c.st);
+        BD_HIP(hipGetLastError());
+        if (!sync) return BD_OK;   // a failure stops the step's later kernels (ierr)
+        BD_HIP(hipMemcpyAsync(c.st_host, c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, sm));
+        BD_HIP(hipStreamSynchronize(sm));
+        if (c.st_host->ierr == 1) {
+            set_error("bond length too large (overstretched bond)");
+            return BD_ERR_BOND;
+        }
+        if (c.st_host->ierr) {
+            set_error("Cholesky factorisation failed: mobility not positive definite");
+            return BD_ERR_FACTOR;
+        }
+        return BD_OK;
+    }
     for (int k0 = 0; k0 < n; k0 += CNB) {
         int nb = std::min(CNB, n - k0);
         hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, sm, c.chol, ld, k0, nb, c.st);
@@ -1659,6 +1932,7 @@ int brownian_cholesky(Ctx &c, const double *xi, double *out, int s, int method)
     hipLaunchKernelGGL(trmv_kernel, dim3(n, s), dim3(64), 0, sm, c.chol, ld, n, xi, out, s, c.npad,
                        method == BD_CHOL ? 1 : 0);
     BD_HIP(hipGetLastError());
+    if (!sync) return BD_OK;
     BD_HIP(hipMemcpyAsync(c.st_host, c.st, sizeof(DevStatus), hipMemcpyDeviceToHost, sm));
     BD_HIP(hipStreamSynchronize(sm));
     if (c.st_host->ierr == 1) {

@@ -90,6 +90,43 @@ def test_cholesky_bdw(oracle, chain100, method):
     assert ierr == 0 and rel(out, ref) < TOL
 
 
+@pytest.mark.parametrize("nb", [160, 161, 400])
+def test_cholesky_bdw_sizes(oracle, nb):
+    """Both factorisations: one workgroup up to n = 480 (N = 160), 32-column
+    panels over many workgroups above."""
+    m = fene_wca_chain(nb, 8)
+    n = 3 * nb
+    xi = np.random.default_rng(12).standard_normal((n, 1))
+    for method in ("CHOL", "CHOL_LT"):
+        g = gpu_system(m, mob_fctr=method, nts_mobsam=1)
+        g.calc_rpy_tensor()
+        out, _, _ = g.brn_calc_bdw(xi, method=method)
+        ref, _, _, ierr = oracle.brn_calc_bdw(oracle.CHOL if method == "CHOL" else oracle.CHOL_LT,
+                                              oracle.rpy(m["crd"]), xi)
+        assert ierr == 0 and rel(out, ref) < TOL, (nb, method)
+
+
+@pytest.mark.parametrize("nb,k", [(10, 3), (100, 50), (170, 120)])
+def test_cholesky_reports_a_failed_pivot(oracle, nb, k):
+    """Two coincident beads give RPY blocks with no direction (0/0), so a
+    pivot is not positive: the oracle's factorisation fails and so does the
+    device's, in the first panel (N = 10), in a later panel of the
+    one-workgroup factorisation (N = 100) and of the panelled one (N = 170)."""
+    from brownpak_amd import BdError
+    m = fene_wca_chain(nb, 8)
+    m["crd"] = m["crd"].copy()
+    m["crd"][k + 1] = m["crd"][k]
+    xi = np.random.default_rng(13).standard_normal((3 * nb, 1))
+    _, _, _, ierr = oracle.brn_calc_bdw(oracle.CHOL, oracle.rpy(m["crd"]), xi)
+    assert ierr != 0
+    for method in ("CHOL", "CHOL_LT"):
+        g = gpu_system(m, mob_fctr=method, nts_mobsam=1)
+        g.calc_rpy_tensor()
+        with pytest.raises(BdError) as ei:
+            g.brn_calc_bdw(xi, method=method)
+        assert ei.value.code == 2 and "positive definite" in str(ei.value)
+
+
 # ------------------------------------------------------------- Lanczos
 @pytest.mark.parametrize("storage", STORAGES)
 @pytest.mark.parametrize("nb,s,mx", [(100, 1, 12), (100, 4, 7), (100, 3, 7), (2000, 4, 7)])
