@@ -265,6 +265,8 @@ def time_c1(steps=400, warmup=20, cpu_steps=400):
     out = dict(steps_per_s=round(steps / dt, 1), ms_per_step=round(1e3 * dt / steps, 4), nbeads=nb,
                noise="Cholesky (U.xi)", dtype="f64")
     g.close()
+    if cpu_steps <= 0:
+        return out
     try:
         from oracle import oracle as O
         O.lib()
@@ -354,8 +356,9 @@ def main():
                     help="steps of the 512-chain ensemble reported beside the main line (0: skip)")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the N=100 Cholesky and 20k-bead periodic figures")
-    ap.add_argument("--workload", choices=["chain", "ensemble"], default="chain",
-                    help="ensemble: time only the 512-chain ensemble and print its object")
+    ap.add_argument("--workload", choices=["chain", "ensemble", "c1", "c4"], default="chain",
+                    help="ensemble / c1 (N=100 Cholesky) / c4 (20k-bead periodic Verlet): time only that "
+                         "workload over --steps steps and print its object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
@@ -369,6 +372,13 @@ def main():
     if args.workload == "ensemble":
         ens = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK)
         print(json.dumps({"ensemble": ens}), flush=True)
+        return
+    if args.workload == "c1":
+        print(json.dumps({"c1_chain100_cholesky": time_c1(steps, warmup, 0 if args.no_cpu_baseline else 400)}),
+              flush=True)
+        return
+    if args.workload == "c4":
+        print(json.dumps({"c4_melt20k_periodic_verlet": time_c4(steps, warmup)}), flush=True)
         return
     runs = {}
     for prec in ("fp32", "fp64"):

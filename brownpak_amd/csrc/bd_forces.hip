@@ -85,7 +85,7 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
     const int *__restrict__ bond_idx, const int *__restrict__ bond_ij, const int *__restrict__ bond_type,
     const int *__restrict__ bstyles, const double *__restrict__ bparams, int use_pairs,
     const int *__restrict__ nbr_cnt, const int *__restrict__ nbr_idx, int nbr_cap,
-    const double *__restrict__ crd_ref, double ref_rc2, const int *__restrict__ ex_ptr,
+    PairScan ps, const int *__restrict__ ex_ptr,
     const int *__restrict__ ex_col, const int *__restrict__ atype, int natype, const int *__restrict__ ptype, const int *__restrict__ vstyles,
     const double *__restrict__ vparams, ExtraArg xa, double *__restrict__ frc, int want_energy,
     double *__restrict__ epart, DevStatus *st)
@@ -129,13 +129,12 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
             const bool full = cnt > nbr_cap;  // list overflowed: scan all beads
             const int kend = full ? nb : cnt;
             for (int k = lane; k < kend; k += 64) {
-                int j;
+                int j, mult = 1;
                 if (full) {
                     j = k;
                     if (j == i) continue;
-                    double x, y, z;
-                    pair_vec(crd_ref, i, j, bx, x, y, z);
-                    if (!(rsq3(x, y, z) < ref_rc2) || excluded(ex_ptr, ex_col, i, j)) continue;
+                    mult = scan_mult(ps, bx, i, j);
+                    if (mult == 0 || excluded(ex_ptr, ex_col, i, j)) continue;
                 } else {
                     j = nbr_idx[(long)k * nb + i];
                 }
@@ -153,6 +152,9 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
                     lj_force(r, vparams + t * MXPARAM, en, f);
                 }
                 double gx = f * x / r, gy = f * y / r, gz = f * z / r;
+                if (mult != 1) {
+                    gx *= mult; gy *= mult; gz *= mult; en *= mult;
+                }
                 fx += gx; fy += gy; fz += gz;
                 if (want_energy && i < j) {
                     acc[1] += en;
@@ -631,7 +633,7 @@ __global__ __launch_bounds__(FB) void adj_force_kernel(
     const int *__restrict__ bond_idx, const int *__restrict__ bond_ij, const int *__restrict__ bond_type,
     const int *__restrict__ bstyles, const double *__restrict__ bparams, int use_pairs,
     const int *__restrict__ nbr_cnt, const int *__restrict__ nbr_idx, int nbr_cap,
-    const double *__restrict__ crd_ref, double ref_rc2, const int *__restrict__ ex_ptr,
+    PairScan ps, const int *__restrict__ ex_ptr,
     const int *__restrict__ ex_col, const int *__restrict__ atype, int natype, const int *__restrict__ ptype,
     const int *__restrict__ vstyles, const double *__restrict__ vparams, int tie_rc, const double *__restrict__ u,
     double *__restrict__ out, double *__restrict__ tpart)
@@ -667,13 +669,12 @@ __global__ __launch_bounds__(FB) void adj_force_kernel(
             const bool full = cnt > nbr_cap;
             const int kend = full ? nb : cnt;
             for (int k = 0; k < kend; k++) {
-                int j;
+                int j, mult = 1;
                 if (full) {
                     j = k;
                     if (j == i) continue;
-                    double x, y, z;
-                    pair_vec(crd_ref, i, j, bx, x, y, z);
-                    if (!(rsq3(x, y, z) < ref_rc2) || excluded(ex_ptr, ex_col, i, j)) continue;
+                    mult = scan_mult(ps, bx, i, j);
+                    if (mult == 0 || excluded(ex_ptr, ex_col, i, j)) continue;
                 } else {
                     j = nbr_idx[(long)k * nb + i];
                 }
@@ -686,12 +687,14 @@ __global__ __launch_bounds__(FB) void adj_force_kernel(
                 double en, f;
                 lj_force(r, p, en, f);
                 double df = lj_dforce(r, p);
+                f *= mult;   // a pair the list would hold mult times
+                df *= mult;
                 double du[3] = {u[3 * j] - ui[0], u[3 * j + 1] - ui[1], u[3 * j + 2] - ui[2]};
                 jg_apply(x, y, z, r, f, df, du, ox, oy, oz);
                 if (ty == 0 && r < p[2]) {
                     double eps = p[0], sig = p[1];
-                    double proj = (x * ui[0] + y * ui[1] + z * ui[2]) / r;
-                    t[0] += (f / eps) * proj;
+                    double proj = mult * (x * ui[0] + y * ui[1] + z * ui[2]) / r;
+                    t[0] += (f / mult / eps) * proj;
                     double sr = sig / r, s6 = sr * sr * sr * sr * sr * sr, s12 = s6 * s6;
                     double dsig = -144.0 * eps * (4 * s12 - s6) / (sig * r);
                     if (!tie_rc) {
@@ -1081,6 +1084,13 @@ int grow_pairs(Ctx &c)
     return BD_OK;
 }
 
+static PairScan pair_scan(const Ctx &c)
+{
+    const bool ver = (c.mth_ptgen == BD_PT_VER), cl = (c.mth_ptgen == BD_PT_CL);
+    const double rl = ver ? c.rcutoff + c.tskin : c.rcutoff;
+    return PairScan{ver ? c.crd_save : c.crd, rl * rl, cl ? c.bead_cell : nullptr, {c.ncell[0], c.ncell[1], c.ncell[2]}};
+}
+
 int compute_forces(Ctx &c, bool want_energy, bool sync)
 {
     if (!c.have_topology) { set_error("topology not set"); return BD_ERR_STATE; }
@@ -1103,7 +1113,7 @@ int compute_forces(Ctx &c, bool want_energy, bool sync)
                       : (c.vdw_general ? force_gather_kernel<false, true> : force_gather_kernel<false, false>);
     hipLaunchKernelGGL(kern, dim3(std::max(nblk, 1)), dim3(FB), 0, c.stream, c.crd, nb, bx, c.bead_bond_ptr,
                        c.bead_bond_idx, c.bond_ij, c.bond_type, c.bond_styles, c.bond_params, pairs ? 1 : 0,
-                       c.nbr_cnt, c.nbr_idx, c.pair_cap, ver ? c.crd_save : c.crd, rl * rl, c.ex_ptr, c.ex_col,
+                       c.nbr_cnt, c.nbr_idx, c.pair_cap, pair_scan(c), c.ex_ptr, c.ex_col,
                        c.atom_types, c.natom_types, c.scan_tmp, c.vdw_styles, c.vdw_params, xa, c.frc,
                        want_energy ? 1 : 0, c.red, c.st);
     BD_HIP(hipGetLastError());
@@ -1128,7 +1138,7 @@ int adj_force(Ctx &c, const double *u, double *out, double *tpart, int tie_rc)
     int nblk = std::max((nb + FB - 1) / FB, 1);
     hipLaunchKernelGGL(adj_force_kernel, dim3(nblk), dim3(FB), 0, c.stream, c.crd, nb, bx, c.bead_bond_ptr,
                        c.bead_bond_idx, c.bond_ij, c.bond_type, c.bond_styles, c.bond_params, pairs ? 1 : 0,
-                       c.nbr_cnt, c.nbr_idx, c.pair_cap, ver ? c.crd_save : c.crd, rl * rl, c.ex_ptr, c.ex_col,
+                       c.nbr_cnt, c.nbr_idx, c.pair_cap, pair_scan(c), c.ex_ptr, c.ex_col,
                        c.atom_types, c.natom_types, c.scan_tmp, c.vdw_styles, c.vdw_params, tie_rc, u, out, tpart);
     BD_HIP(hipGetLastError());
     return BD_OK;

@@ -35,6 +35,39 @@ __device__ inline void pair_vec(const double *crd, int i, int j, const BoxArg &b
     }
 }
 
+// How the force kernels find the partners of a bead whose neighbour list
+// overflowed: every bead j is scanned and taken as many times as the list
+// would hold it.  DIR and VER: once when within rc2 at the list's reference
+// coordinates.  CL (bead_cell set): once per cell of the bead's 27-cell
+// stencil that holds j, with no distance test, as the cell-list table of
+// link_cell_m.f90 has none; with two cells along an axis the neighbour cell
+// is reached from both sides and j counts twice along it.
+struct PairScan {
+    const double *crd_ref;
+    double rc2;
+    const int *bead_cell;
+    int nc[3];
+};
+
+__device__ inline int scan_mult(const PairScan &ps, const BoxArg &bx, int i, int j)
+{
+    if (ps.bead_cell) {
+        int ci = ps.bead_cell[i], cj = ps.bead_cell[j], m = 1;
+        for (int d = 0; d < 3; d++) {
+            const int nc = ps.nc[d], a = ci % nc, b = cj % nc;
+            ci /= nc;
+            cj /= nc;
+            int md = 0;
+            for (int dd = -1; dd <= 1; dd++) md += ((a + dd + nc) % nc == b) ? 1 : 0;
+            m *= md;
+        }
+        return m;
+    }
+    double x, y, z;
+    pair_vec(ps.crd_ref, i, j, bx, x, y, z);
+    return rsq3(x, y, z) < ps.rc2 ? 1 : 0;
+}
+
 // ia_get_bond_force (ia_bond_m.f90:85-362); returns 1 on overstretch.
 __device__ inline int bond_force(double r, int sty, const double *p, double &enrg, double &frc)
 {

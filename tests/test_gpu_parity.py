@@ -451,6 +451,40 @@ def test_cell_list_pairs_and_forces(oracle, case):
     assert rel(g.get_coordinates(), ref) < TOL
 
 
+@pytest.mark.parametrize("case", ["melt_7cells", "box_2cells"])
+def test_cell_list_overflow_steps_match_oracle(oracle, case):
+    """A fresh context stepped at once: its neighbour list starts at 64
+    entries per bead, fewer than these tables hold, and the step loop does not
+    stop to grow it, so every bead takes the all-bead fallback, which counts
+    each partner as often as the cell-list table would (twice along an axis
+    of two cells)."""
+    _say(f"case: cell list overflow {case}")
+    if case == "melt_7cells":
+        m = _melt(20, 100, 46)
+    else:
+        L = 11.0
+        crd = np.stack(np.meshgrid(*[np.arange(4) * 2.7 + 0.3] * 3, indexing="ij"), -1).reshape(-1, 3)[:60]
+        bonds = np.concatenate([chain_bonds(20, offset=c * 20) for c in range(3)])
+        m = dict(crd=crd, bonds=bonds, bond_styles=[1], bond_params=[[25.0, 2.0]], vdw_styles=[1],
+                 vdw_params=[[1.0, 2.0, 5.0]], natom_types=1, excluded_atoms=1, rcutoff=5.0, tskin=0.0,
+                 imcon=1, box=(L, L, L))
+    m = dict(m, mth_ptgen="CL", tskin=0.0)
+    nb = len(m["crd"])
+    so = oracle_system(oracle, m, "CL")
+    ptr, col = oracle.pairs(so, m["crd"], "CL", cap=nb * nb * 16)
+    count = np.zeros(nb, dtype=int)   # entries per bead, both directions (partners are 1-based)
+    for i in range(nb):
+        count[i] += ptr[i + 1] - ptr[i]
+        np.add.at(count, np.asarray(col[ptr[i]:ptr[i + 1]], dtype=int) - 1, 1)
+    assert count.max() > 64
+    nsteps = 3
+    xi = np.random.default_rng(49).standard_normal(3 * nb * nsteps)
+    g = gpu_system(m, lhdia=False)
+    g.bds_run(nsteps, xi)
+    ref, _, _ = oracle.bd_run(so, m["crd"], nsteps, xi, lhdia=False)
+    assert rel(g.get_coordinates(), ref) < TOL
+
+
 def test_cell_list_rejects_negative_coordinates_and_small_boxes():
     from brownpak_amd import BdError
     m = dict(_melt(2, 50, 49), mth_ptgen="CL", tskin=0.0)
