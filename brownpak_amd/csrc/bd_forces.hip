@@ -280,14 +280,19 @@ __device__ inline bool excluded(const int *ex_ptr, const int *ex_col, int i, int
 }
 
 // Verlet trigger (pairtab_m.fpp:211-222): rebuild iff first call or
-// 4 max|dr|^2 >= skin^2.  One block.
-__global__ __launch_bounds__(1024) void verlet_check_kernel(const double *__restrict__ crd,
-                                                            const double *__restrict__ save, int nb, BoxArg bx,
-                                                            double tskin_sq, int first, DevStatus *st)
+// 4 max|dr|^2 >= skin^2.  One thread per bead; each block takes the
+// maximum of its beads (fmax, so a NaN displacement is passed over as in a
+// serial maximum) and flags vmove when 4 max >= skin^2, which holds for some
+// block exactly when it holds for the maximum over all beads.  The last block
+// to finish (vdone ticket) sets rebuild and clears both counters.
+__global__ __launch_bounds__(256) void verlet_check_kernel(const double *__restrict__ crd,
+                                                           const double *__restrict__ save, int nb, BoxArg bx,
+                                                           double tskin_sq, int first, DevStatus *st)
 {
-    __shared__ double sh[16];
+    __shared__ double sh[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double mx = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    if (i < nb) {
         double dx = __dsub_rn(crd[3 * i], save[3 * i]);
         double dy = __dsub_rn(crd[3 * i + 1], save[3 * i + 1]);
         double dz = __dsub_rn(crd[3 * i + 2], save[3 * i + 2]);
@@ -300,7 +305,14 @@ __global__ __launch_bounds__(1024) void verlet_check_kernel(const double *__rest
     if (threadIdx.x == 0) {
         double m = 0.0;
         for (int w = 0; w < (int)(blockDim.x / 64); w++) m = fmax(m, sh[w]);
-        st->rebuild = (first || !(4.0 * m < tskin_sq)) ? 1 : 0;
+        if (!(4.0 * m < tskin_sq)) __hip_atomic_fetch_or(&st->vmove, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int done = __hip_atomic_fetch_add(&st->vdone, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (done == (int)gridDim.x - 1) {
+            const int mv = __hip_atomic_load(&st->vmove, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            st->rebuild = (first || mv) ? 1 : 0;
+            __hip_atomic_store(&st->vmove, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st->vdone, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -430,73 +442,108 @@ __global__ __launch_bounds__(1024) void cell_scan_kernel(int *__restrict__ cell_
 }
 
 __global__ void cell_fill_kernel(const int *__restrict__ bead_cell, int nb, const int *__restrict__ cell_ptr,
-                                 int *__restrict__ cell_fill, int *__restrict__ cell_beads, int check_flag,
+                                 int *__restrict__ taken, int *__restrict__ out, int check_flag,
                                  const DevStatus *st)
 {
     if (check_flag && !st->rebuild) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb) return;
     int c = bead_cell[i];
-    int pos = cell_ptr[c] + atomicAdd(&cell_fill[c], 1);
-    cell_beads[pos] = i;
+    int pos = cell_ptr[c] + atomicAdd(&taken[c], 1);
+    out[pos] = i;
 }
 
-// One thread per bead; candidates from the 27 surrounding cells.
-__global__ __launch_bounds__(256) void nbr_cell_kernel(const double *__restrict__ crd, int nb, BoxArg bx, double rc2,
-                                                       int ncx, int ncy, int ncz, const int *__restrict__ bead_cell,
-                                                       const int *__restrict__ cell_ptr, const int *__restrict__ cell_beads,
-                                                       const int *__restrict__ ex_ptr, const int *__restrict__ ex_col,
-                                                       int *__restrict__ nbr_cnt, int *__restrict__ nbr_idx, int cap,
-                                                       double *__restrict__ save, DevStatus *st)
+// One wavefront per bead (NC_WAVES per block); candidates from the 27
+// surrounding cells, taken in the order of a serial scan (cells dz, dy, dx
+// from -1 to 1, beads of a cell in ascending order) so the list is the same:
+// lanes 0-26 read the cells' bead ranges, a prefix sum over the lanes lays
+// the ranges end to end, each lane finds its candidate's cell by bisection
+// over those sums, and the candidates that pass are packed in order by a
+// ballot.
+constexpr int NC_WAVES = 4;
+__global__ __launch_bounds__(64 * NC_WAVES) void nbr_cell_kernel(
+    const double *__restrict__ crd, int nb, BoxArg bx, double rc2, int ncx, int ncy, int ncz,
+    const int *__restrict__ bead_cell, const int *__restrict__ cell_ptr, const int *__restrict__ cell_beads,
+    const int *__restrict__ ex_ptr, const int *__restrict__ ex_col, int *__restrict__ nbr_cnt,
+    int *__restrict__ nbr_idx, int cap, double *__restrict__ save, DevStatus *st)
 {
+    __shared__ int s_end[NC_WAVES][32], s_beg[NC_WAVES][32];
     if (!st->rebuild) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    int c = bead_cell[i];
-    int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    const int i = blockIdx.x * NC_WAVES + wv;
+    if (i >= nb) return;   // the whole wavefront
+    const int c = bead_cell[i];
+    const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+    int b0 = 0, len = 0;
+    if (lane < 27) {
+        const int dz = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dx = lane % 3 - 1;
+        const int jx = (cx + dx + ncx) % ncx, jy = (cy + dy + ncy) % ncy, jz = (cz + dz + ncz) % ncz;
+        const int jc = (jz * ncy + jy) * ncx + jx;
+        b0 = cell_ptr[jc];
+        len = cell_ptr[jc + 1] - b0;
+    }
+    int incl = len;   // inclusive prefix over lanes 0..31
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, 26, 64);
+    if (lane < 32) {
+        s_end[wv][lane] = incl;
+        s_beg[wv][lane] = b0 - (incl - len);   // cell_beads index = s_beg + candidate number
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int cnt = 0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                int jx = (cx + dx + ncx) % ncx, jy = (cy + dy + ncy) % ncy, jz = (cz + dz + ncz) % ncz;
-                int jc = (jz * ncy + jy) * ncx + jx;
-                for (int p = cell_ptr[jc]; p < cell_ptr[jc + 1]; p++) {
-                    int j = cell_beads[p];
-                    if (j == i) continue;
-                    double x, y, z;
-                    pair_vec(crd, i, j, bx, x, y, z);
-                    if (rsq3(x, y, z) < rc2 && !excluded(ex_ptr, ex_col, i, j)) {
-                        if (cnt < cap) nbr_idx[(long)cnt * nb + i] = j;
-                        cnt++;
-                    }
-                }
+    for (int k0 = 0; k0 < total; k0 += 64) {
+        const int k = k0 + lane;
+        int j = -1;
+        bool take = false;
+        if (k < total) {
+            int lo = 0, hi = 26;   // first cell whose end exceeds k
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_end[wv][mid] > k) hi = mid;
+                else lo = mid + 1;
             }
-    nbr_cnt[i] = cnt;
-    if (cnt > cap) atomicExch(&st->overflow, 1);
-    save[3 * i] = crd[3 * i];
-    save[3 * i + 1] = crd[3 * i + 1];
-    save[3 * i + 2] = crd[3 * i + 2];
+            j = cell_beads[s_beg[wv][lo] + k];
+            if (j != i) {
+                double x, y, z;
+                pair_vec(crd, i, j, bx, x, y, z);
+                take = rsq3(x, y, z) < rc2 && !excluded(ex_ptr, ex_col, i, j);
+            }
+        }
+        const unsigned long long m = __ballot(take);
+        if (take) {
+            const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (pos < cap) nbr_idx[(long)pos * nb + i] = j;
+        }
+        cnt += __popcll(m);
+    }
+    if (lane == 0) {
+        nbr_cnt[i] = cnt;
+        if (cnt > cap) atomicExch(&st->overflow, 1);
+    }
+    if (lane < 3) save[3 * i + lane] = crd[3 * i + lane];
 }
 
 // Beads of every cell in ascending index order (the fill above places them
-// in atomic order); one thread per cell, insertion sort of a few dozen.
-// Pair lists, and so the force sums, are then the same run to run.
-__global__ void cell_sort_kernel(const int *__restrict__ cell_ptr, int ncell, int *__restrict__ cell_beads,
-                                 int check_flag, const DevStatus *st)
+// in atomic order): bead i goes to its cell's start plus the number of the
+// cell's beads with a smaller index.  Pair lists, and so the force sums, are
+// then the same run to run.
+__global__ void cell_rank_kernel(const int *__restrict__ bead_cell, int nb, const int *__restrict__ cell_ptr,
+                                 const int *__restrict__ unsorted, int *__restrict__ cell_beads, int check_flag,
+                                 const DevStatus *st)
 {
     if (check_flag && !st->rebuild) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const int b = cell_ptr[c], e = cell_ptr[c + 1];
-    for (int p = b + 1; p < e; p++) {
-        const int v = cell_beads[p];
-        int q = p - 1;
-        while (q >= b && cell_beads[q] > v) {
-            cell_beads[q + 1] = cell_beads[q];
-            q--;
-        }
-        cell_beads[q + 1] = v;
-    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const int c = bead_cell[i], b = cell_ptr[c], e = cell_ptr[c + 1];
+    int rank = 0;
+    for (int p = b; p < e; p++) rank += unsorted[p] < i ? 1 : 0;
+    cell_beads[b + rank] = i;
 }
 
 // cl_build cell index (cell_list_m.f90:186-204): cells of side
@@ -945,6 +992,7 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const in
         if (c.cell_cnt) (void)hipFree(c.cell_cnt);
         if (c.cell_ptr) (void)hipFree(c.cell_ptr);
         if (!c.cell_beads) BD_HIP(hipMalloc(&c.cell_beads, (size_t)nb * sizeof(int)));
+        if (!c.cell_fill) BD_HIP(hipMalloc(&c.cell_fill, (size_t)nb * sizeof(int)));
         if (!c.bead_cell) BD_HIP(hipMalloc(&c.bead_cell, (size_t)nb * sizeof(int)));
         BD_HIP(hipMalloc(&c.cell_cnt, (size_t)c.ncell_tot * sizeof(int)));
         BD_HIP(hipMalloc(&c.cell_ptr, (size_t)(c.ncell_tot + 1) * sizeof(int)));
@@ -959,6 +1007,7 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const in
             if (c.cell_cnt) (void)hipFree(c.cell_cnt);
             if (c.cell_ptr) (void)hipFree(c.cell_ptr);
             if (!c.cell_beads) BD_HIP(hipMalloc(&c.cell_beads, (size_t)nb * sizeof(int)));
+            if (!c.cell_fill) BD_HIP(hipMalloc(&c.cell_fill, (size_t)nb * sizeof(int)));
             if (!c.bead_cell) BD_HIP(hipMalloc(&c.bead_cell, (size_t)nb * sizeof(int)));
             BD_HIP(hipMalloc(&c.cell_cnt, (size_t)c.ncell_tot * sizeof(int)));
             BD_HIP(hipMalloc(&c.cell_ptr, (size_t)(c.ncell_tot + 1) * sizeof(int)));
@@ -972,7 +1021,7 @@ void forces_free(Ctx &c)
 {
     int **ip[] = {&c.bond_ij, &c.bond_type, &c.bead_bond_ptr, &c.bead_bond_idx, &c.bond_styles, &c.atom_types,
                   &c.vdw_styles, &c.ex_ptr, &c.ex_col, &c.nbr_cnt, &c.nbr_ptr, &c.nbr_idx, &c.cell_cnt,
-                  &c.cell_ptr, &c.cell_beads, &c.bead_cell, &c.scan_tmp};
+                  &c.cell_ptr, &c.cell_beads, &c.cell_fill, &c.bead_cell, &c.scan_tmp};
     for (auto p : ip) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
@@ -1017,7 +1066,8 @@ static int build_pairs(Ctx &c, bool sync)
         // DIR builds on every call and nbr_n2_kernel reports that itself
         const bool n2_dir = !ver && c.mth_ptgen != BD_PT_CL;
         if (ver)
-            hipLaunchKernelGGL(verlet_check_kernel, dim3(1), dim3(1024), 0, sm, c.crd, c.crd_save, nb, bx,
+            hipLaunchKernelGGL(verlet_check_kernel, dim3(std::max((nb + 255) / 256, 1)), dim3(256), 0, sm, c.crd,
+                               c.crd_save, nb, bx,
                                c.tskin * c.tskin, c.verlet_first ? 1 : 0, c.st);
         else if (!n2_dir)
             hipLaunchKernelGGL(set_rebuild_kernel, dim3(1), dim3(1), 0, sm, c.st);  // CL: every call
@@ -1029,9 +1079,9 @@ static int build_pairs(Ctx &c, bool sync)
             hipLaunchKernelGGL(cell_scan_kernel, dim3(1), dim3(1024), 0, sm, c.cell_cnt, c.cell_ptr, c.ncell_tot, 0,
                                c.st);
             hipLaunchKernelGGL(cell_fill_kernel, dim3(nbk), dim3(256), 0, sm, c.bead_cell, nb, c.cell_ptr, c.cell_cnt,
+                               c.cell_fill, 0, c.st);
+            hipLaunchKernelGGL(cell_rank_kernel, dim3(nbk), dim3(256), 0, sm, c.bead_cell, nb, c.cell_ptr, c.cell_fill,
                                c.cell_beads, 0, c.st);
-            hipLaunchKernelGGL(cell_sort_kernel, dim3(nck), dim3(256), 0, sm, c.cell_ptr, c.ncell_tot, c.cell_beads, 0,
-                               c.st);
             hipLaunchKernelGGL(cl_nbr_kernel, dim3(nbk), dim3(256), 0, sm, nb, c.ncell[0], c.ncell[1], c.ncell[2],
                                c.bead_cell, c.cell_ptr, c.cell_beads, c.ex_ptr, c.ex_col, c.nbr_cnt, c.nbr_idx,
                                c.pair_cap, c.st);
@@ -1044,10 +1094,11 @@ static int build_pairs(Ctx &c, bool sync)
             hipLaunchKernelGGL(cell_scan_kernel, dim3(1), dim3(1024), 0, sm, c.cell_cnt, c.cell_ptr, c.ncell_tot, 1,
                                c.st);
             hipLaunchKernelGGL(cell_fill_kernel, dim3(nbk), dim3(256), 0, sm, c.bead_cell, nb, c.cell_ptr, c.cell_cnt,
+                               c.cell_fill, 1, c.st);
+            hipLaunchKernelGGL(cell_rank_kernel, dim3(nbk), dim3(256), 0, sm, c.bead_cell, nb, c.cell_ptr, c.cell_fill,
                                c.cell_beads, 1, c.st);
-            hipLaunchKernelGGL(cell_sort_kernel, dim3((c.ncell_tot + 255) / 256), dim3(256), 0, sm, c.cell_ptr,
-                               c.ncell_tot, c.cell_beads, 1, c.st);
-            hipLaunchKernelGGL(nbr_cell_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, nb, bx, rc2, c.ncell[0], c.ncell[1],
+            hipLaunchKernelGGL(nbr_cell_kernel, dim3((nb + NC_WAVES - 1) / NC_WAVES), dim3(64 * NC_WAVES), 0, sm,
+                               c.crd, nb, bx, rc2, c.ncell[0], c.ncell[1],
                                c.ncell[2], c.bead_cell, c.cell_ptr, c.cell_beads, c.ex_ptr, c.ex_col, c.nbr_cnt,
                                c.nbr_idx, c.pair_cap, c.crd_save, c.st);
         } else {

@@ -61,6 +61,8 @@ struct DevStatus {
     int nrebuild;      // Verlet list rebuilds
     int ycnt;          // blocks of the Lanczos combine kernel done (last one finalises)
     int fail_j;        // Lanczos: smallest Krylov block whose CholeskyQR failed (0: none)
+    int vmove;         // Verlet check: some block saw a bead move by skin / 2 (cleared by the last block)
+    int vdone;         // Verlet check: blocks done (the last one publishes rebuild)
     double err;        // last Lanczos relative change
     double lam_lo, lam_hi;  // spectral bounds of the last small problem
 };
@@ -242,6 +244,7 @@ struct Ctx {
     int se_last[6] = {0, 0, 0, 0, 0, 0};
     long long se_total[6] = {0, 0, 0, 0, 0, 0};
     int *cell_cnt = nullptr, *cell_ptr = nullptr, *cell_beads = nullptr, *bead_cell = nullptr;
+    int *cell_fill = nullptr;   // beads of each cell in arrival order, before ranking into cell_beads
     int ncell[3] = {0, 0, 0};
     double cl_cs[3] = {0, 0, 0};   // CL cell sides L / nc
     int ncell_tot = 0;
