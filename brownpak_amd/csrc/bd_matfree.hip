@@ -11,16 +11,18 @@
 // (bd_solver_m.f90:484-504 with a = 1; c2 multiplies r r^T, not r^ r^T).
 //
 // Work decomposition.  A wavefront owns 64 row beads (R x 64 with R rows per
-// lane) and walks a range of column beads j; all its lanes take the same j,
-// so the record of bead j -- coordinates (FP64) and its 3 x S block of X --
-// is read with scalar loads into SGPRs and feeds the vector instructions as
-// a scalar operand: no LDS and no vector memory traffic in the inner loop.
-// The column range is split over the 4 waves of a workgroup (summed in LDS,
-// fixed order) and over JG workgroups (partials summed by the next kernel in
-// a fixed order), so results repeat bit for bit from run to run.
+// lane); the NW waves of a workgroup share those rows and split the
+// workgroup's range of column beads, and JG workgroups split the column
+// range of a row block.  The workgroup first stages the records of its column
+// beads -- coordinates (FP64) and the 3 x S block of X -- in LDS with vector
+// loads all in flight at once, then every wave walks its share of them, all
+// lanes on the same bead j, so each record is an LDS broadcast read whose
+// latency the in-order LDS queue lets the loop hide.  The waves' sums are
+// added in LDS in wave order and the JG workgroup sums by the next kernel in
+// a fixed order, so results repeat bit for bit from run to run.
 //
-// Precision.  FP64: everything in FP64 (1/r from v_rsq_f64 refined by two
-// Newton steps).  FP32: the coordinate differences are formed in FP64 and
+// Precision.  FP64: everything in FP64 (1/r from v_rsq_f64 refined by one
+// third-order step).  FP32: the coordinate differences are formed in FP64 and
 // rounded (so near pairs keep their relative accuracy however far the chain
 // is from the origin), the block and its product in FP32, and the products
 // are summed in FP32 over 8 column beads, then in FP64.
@@ -36,20 +38,19 @@
 namespace bd {
 
 constexpr int MF_CH = 8;        // column beads per FP32 partial sum (and record padding)
-constexpr int MF_WAVES = 4;     // waves per workgroup (column-range split)
+constexpr int MF_WAVES = 8;     // waves per workgroup (column-range split)
 constexpr int MF_JG_MAX = 64;   // largest workgroup split of the column range
+constexpr int MF_TILE = 256;    // column beads staged in LDS at a time
 
-#define AS4 __attribute__((address_space(4)))
+int mf_rows_per_lane(const Ctx &) { return 1; }
 
-int mf_rows_per_lane(const Ctx &c) { return c.opt.precision == BD_FP32 ? 1 : 2; }
-
-// Workgroups along the column range: about 2048 waves in total, at least
-// one 8-bead chunk per wave.
+// Workgroups along the column range: about 4096 waves in total (sixteen
+// per CU), at least one 8-bead chunk per wave.
 static int mf_jg_for(int nb, int R)
 {
     const int nib = (nb + 64 * R - 1) / (64 * R);
     const int nch = (nb + MF_CH - 1) / MF_CH;
-    int jg = 512 / nib;
+    int jg = 4096 / (nib * MF_WAVES);
     jg = std::min(jg, nch / MF_WAVES);
     return std::max(1, std::min(jg, MF_JG_MAX));
 }
@@ -98,60 +99,47 @@ __device__ inline float mf_rsqrt<float>(float v)
 template <>
 __device__ inline double mf_rsqrt<double>(double v)
 {
-    double y = __builtin_amdgcn_rsq(v);
-    double e = fma(-v * y, y, 1.0);   // two Newton steps: y <- y + y (1 - v y^2) / 2
-    y = fma(0.5 * y, e, y);
-    e = fma(-v * y, y, 1.0);
-    return fma(0.5 * y, e, y);
+    // v_rsq_f64 (relative error ~6e-8) and one third-order correction:
+    // 1/sqrt(v) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + ...), e = 1 - v y^2,
+    // truncated after e^2 (~1e-22), about one ulp after rounding
+    const double y = __builtin_amdgcn_rsq(v);
+    const double e = fma(-(v * y), y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
 }
 
-// c1, c2 of the pair (see the header), including the overlap branch and
-// the diagonal block.
+// r^2 is offset by this much before 1/sqrt: a bead's pair with itself then
+// gives finite overlap coefficients (c1 = 1, the r r^T term multiplying a
+// zero vector) without a special case.  The offset changes r^2 only for
+// two beads closer than about 1e-7.
 template <typename T>
-__device__ inline void mf_coef(T r2, bool self, T &c1, T &c2)
-{
-    const T y = mf_rsqrt<T>(r2), y2 = y * y;
-    if (r2 >= (T)4) {
-        c1 = y * ((T)0.75 + (T)0.5 * y2);
-        c2 = y * y2 * ((T)0.75 - (T)1.5 * y2);
-    } else {
-        const T r = r2 * y;
-        c1 = (T)1 - (T)(9.0 / 32.0) * r;
-        c2 = (T)(3.0 / 32.0) * y;
-    }
-    if (self) {
-        c1 = (T)1;
-        c2 = (T)0;
-    }
-}
+__device__ constexpr T mf_r2_floor();
+template <>
+__device__ constexpr double mf_r2_floor<double>() { return 1e-30; }
+template <>
+__device__ constexpr float mf_r2_floor<float>() { return 1e-30f; }
 
+// LDS record of one column bead: its coordinates and its 3 x S block of X.
 template <typename T, int S>
-struct MfRec {
+struct alignas(16) MfRec {
     double x, y, z;
     T v[3][S];
 };
 
-template <typename T, int S>
-__device__ inline MfRec<T, S> mf_load(const double AS4 *crec, const T AS4 *xrec, int j)
-{
-    MfRec<T, S> r;
-    r.x = crec[4 * j];
-    r.y = crec[4 * j + 1];
-    r.z = crec[4 * j + 2];
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int k = 0; k < S; k++) r.v[a][k] = xrec[(3 * j + a) * S + k];
-    return r;
-}
-
 // part[g][i][a S + k] = sum over the column beads j of workgroup column g of
-// (D_ij X_j)(a, k), FP64.  grid (ceil(nb / (64 R)), JG), 256 threads.
+// (D_ij X_j)(a, k), FP64.  grid (ceil(nb / (64 R)), JG), 64 MF_WAVES threads.
 template <typename T, int S, int R>
-__global__ __launch_bounds__(256) void mf_product_kernel(const double AS4 *crec, const T AS4 *xrec, int nb, int nch,
-                                                         double *__restrict__ part, const DevStatus *stop)
+__global__ __launch_bounds__(64 * MF_WAVES) void mf_product_kernel(const double *__restrict__ crec,
+                                                                   const T *__restrict__ xrec, int nb, int nch,
+                                                                   double *__restrict__ part,
+                                                                   const DevStatus *stop)
 {
-    __shared__ double red[MF_WAVES][64][3 * S + 1];
+    using Rec = MfRec<T, S>;
+    constexpr int RED_BYTES = MF_WAVES * 64 * (3 * S + 1) * (int)sizeof(double);
+    constexpr int TILE_BYTES = MF_TILE * (int)sizeof(Rec);
+    // the staged records, and after the last tile the waves' sums
+    __shared__ __align__(16) unsigned char smem[RED_BYTES > TILE_BYTES ? RED_BYTES : TILE_BYTES];
+    Rec *tile = reinterpret_cast<Rec *>(smem);
+    double(*red)[64][3 * S + 1] = reinterpret_cast<double(*)[64][3 * S + 1]>(smem);
     __shared__ int s_done;
     if (threadIdx.x == 0) s_done = (stop && lz_done(stop)) ? 1 : 0;
     __syncthreads();
@@ -168,8 +156,6 @@ __global__ __launch_bounds__(256) void mf_product_kernel(const double AS4 *crec,
         yi[r] = crec[4 * ic + 1];
         zi[r] = crec[4 * ic + 2];
     }
-    const int nparts = (int)gridDim.y * MF_WAVES, p = (int)blockIdx.y * MF_WAVES + w;
-    const int cb = (int)((long)nch * p / nparts), ce = (int)((long)nch * (p + 1) / nparts);
     double acc[R][3][S];
 #pragma unroll
     for (int r = 0; r < R; r++)
@@ -177,21 +163,21 @@ __global__ __launch_bounds__(256) void mf_product_kernel(const double AS4 *crec,
         for (int a = 0; a < 3; a++)
 #pragma unroll
             for (int k = 0; k < S; k++) acc[r][a][k] = 0.0;
-    // the pair term (D_ij X_j)(a, k) added to acc[r][a][k]
-    auto pair = [&](const MfRec<T, S> &cur, int j, auto &acc_) __attribute__((always_inline)) {
+    // the pair term (D_ij X_j)(a, k) added to acc_[r][a][k]: both
+    // coefficient branches are formed and the one that applies selected (no
+    // branch in the loop); j == i takes the overlap branch at r = 0, where
+    // c1 = 1 and the r r^T term vanishes, which is the unit diagonal block
+    auto pair = [&](const Rec &cur, auto &acc_) __attribute__((always_inline)) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const T dx = (T)(cur.x - xi[r]), dy = (T)(cur.y - yi[r]), dz = (T)(cur.z - zi[r]);
             const T r2 = dx * dx + dy * dy + dz * dz;
-            T c1, c2;
-            if (__builtin_amdgcn_ballot_w64(r2 < (T)4) == 0) {
-                // every lane far from bead j: the r >= 2 branch only
-                const T y = mf_rsqrt<T>(r2), y2 = y * y;
-                c1 = y * ((T)0.75 + (T)0.5 * y2);
-                c2 = y * y2 * ((T)0.75 - (T)1.5 * y2);
-            } else {
-                mf_coef<T>(r2, j == ii[r], c1, c2);
-            }
+            const T y = mf_rsqrt<T>(r2 + mf_r2_floor<T>()), y2 = y * y;
+            const T c1f = y * ((T)0.75 + (T)0.5 * y2);
+            const T c2f = y * y2 * ((T)0.75 - (T)1.5 * y2);
+            const T c1n = (T)1 - (T)(9.0 / 32.0) * (r2 * y), c2n = (T)(3.0 / 32.0) * y;
+            const bool nr = r2 < (T)4;
+            const T c1 = nr ? c1n : c1f, c2 = nr ? c2n : c2f;
 #pragma unroll
             for (int k = 0; k < S; k++) {
                 const T t = c2 * (dx * cur.v[0][k] + dy * cur.v[1][k] + dz * cur.v[2][k]);
@@ -204,58 +190,51 @@ __global__ __launch_bounds__(256) void mf_product_kernel(const double AS4 *crec,
             }
         }
     };
-    // records small enough are double-buffered in SGPRs: the next one loads
-    // while the current one is used
-    constexpr bool PF = (6 + 3 * S * (int)sizeof(T) / 4) <= 36;
-    if constexpr (sizeof(T) == 8) {
-        // FP64: straight into the accumulators
-        const int jb = MF_CH * cb, je = MF_CH * ce;
-        if (jb < je) {
-            MfRec<T, S> cur = mf_load<T, S>(crec, xrec, jb);
-            for (int j = jb; j < je; j++) {
-                if constexpr (PF) {
-                    const MfRec<T, S> nxt = mf_load<T, S>(crec, xrec, j + 1 < je ? j + 1 : j);
-                    pair(cur, j, acc);
-                    cur = nxt;
-                } else {
-                    pair(mf_load<T, S>(crec, xrec, j), j, acc);
-                }
-            }
+    // the workgroup's column chunks, MF_TILE beads of them staged at a time
+    const int cb = (int)((long)nch * blockIdx.y / gridDim.y), ce = (int)((long)nch * (blockIdx.y + 1) / gridDim.y);
+    for (int t0 = cb; t0 < ce; t0 += MF_TILE / MF_CH) {
+        const int nct = min(ce - t0, MF_TILE / MF_CH), jb = t0 * MF_CH;
+        for (int e = threadIdx.x; e < nct * MF_CH; e += blockDim.x) {
+            const int j = jb + e;
+            Rec rc;
+            rc.x = crec[4 * j];
+            rc.y = crec[4 * j + 1];
+            rc.z = crec[4 * j + 2];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int k = 0; k < S; k++) rc.v[a][k] = xrec[(3 * j + a) * S + k];
+            tile[e] = rc;
         }
-    } else if (cb < ce) {
-        // FP32: partial sums over 8 column beads, then added in FP64
-        for (int ch = cb; ch < ce; ch++) {
-            T cacc[R][3][S];
+        __syncthreads();
+        // this wave's chunks of the tile
+        const int qb = nct * w / MF_WAVES, qe = nct * (w + 1) / MF_WAVES;
+        if constexpr (sizeof(T) == 8) {
+            // FP64: straight into the accumulators
+            for (int q = MF_CH * qb; q < MF_CH * qe; q++) pair(tile[q], acc);
+        } else {
+            // FP32: partial sums over 8 column beads, then added in FP64
+            for (int ch = qb; ch < qe; ch++) {
+                T cacc[R][3][S];
 #pragma unroll
-            for (int r = 0; r < R; r++)
+                for (int r = 0; r < R; r++)
 #pragma unroll
-                for (int a = 0; a < 3; a++)
+                    for (int a = 0; a < 3; a++)
 #pragma unroll
-                    for (int k = 0; k < S; k++) cacc[r][a][k] = (T)0;
-            if constexpr (PF) {
-                MfRec<T, S> cur = mf_load<T, S>(crec, xrec, MF_CH * ch);
+                        for (int k = 0; k < S; k++) cacc[r][a][k] = (T)0;
 #pragma unroll 2
-                for (int q = 0; q < MF_CH; q++) {
-                    const int j = MF_CH * ch + q;
-                    const MfRec<T, S> nxt = mf_load<T, S>(crec, xrec, q + 1 < MF_CH ? j + 1 : j);
-                    pair(cur, j, cacc);
-                    cur = nxt;
-                }
-            } else {
-                for (int q = 0; q < MF_CH; q++) {
-                    const int j = MF_CH * ch + q;
-                    pair(mf_load<T, S>(crec, xrec, j), j, cacc);
-                }
+                for (int q = 0; q < MF_CH; q++) pair(tile[MF_CH * ch + q], cacc);
+#pragma unroll
+                for (int r = 0; r < R; r++)
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int k = 0; k < S; k++) acc[r][a][k] += (double)cacc[r][a][k];
             }
-#pragma unroll
-            for (int r = 0; r < R; r++)
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-#pragma unroll
-                    for (int k = 0; k < S; k++) acc[r][a][k] += (double)cacc[r][a][k];
         }
+        __syncthreads();
     }
-    // the four waves' sums, added in wave order
+    // the waves' sums, added in wave order (the staging area reused)
 #pragma unroll
     for (int r = 0; r < R; r++) {
 #pragma unroll
@@ -517,14 +496,10 @@ static int mf_launch_t(Ctx &c, const DevStatus *stop)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     BD_TRY(prof_pair(c, S, &e0, &e1));
     if (e0) BD_HIP(hipEventRecord(e0, c.stream));
-    const double AS4 *crec = (const double AS4 *)c.mf_crd;
-    const T AS4 *xrec = (const T AS4 *)c.mf_x;
-    if (R == 1)
-        hipLaunchKernelGGL((mf_product_kernel<T, S, 1>), grid, dim3(256), 0, c.stream, crec, xrec, c.nb, nch,
-                           c.mf_part, stop);
-    else
-        hipLaunchKernelGGL((mf_product_kernel<T, S, 2>), grid, dim3(256), 0, c.stream, crec, xrec, c.nb, nch,
-                           c.mf_part, stop);
+    const double *crec = c.mf_crd;
+    const T *xrec = (const T *)c.mf_x;
+    hipLaunchKernelGGL((mf_product_kernel<T, S, 1>), grid, dim3(64 * MF_WAVES), 0, c.stream, crec, xrec, c.nb, nch,
+                       c.mf_part, stop);
     BD_HIP(hipGetLastError());
     if (e1) BD_HIP(hipEventRecord(e1, c.stream));
     return BD_OK;
@@ -612,7 +587,7 @@ constexpr long MF_RAW_PART = 524288;   // 4 MiB of partial sums
 
 static int mf_raw_jg(int nb, int s, int precision)
 {
-    const int jg = mf_jg_for(nb, precision == BD_FP32 ? 1 : 2);
+    const int jg = mf_jg_for(nb, 1);
     const long cap = MF_RAW_PART / ((long)nb * 3 * s);
     return (int)std::max(1L, std::min((long)jg, cap));
 }
@@ -628,14 +603,9 @@ template <typename T, int S>
 static void mf_raw_product(int nb, int nbp, int jg, const double *crec, const T *xrec, double *part,
                            hipStream_t st)
 {
-    const int R = sizeof(T) == 4 ? 1 : 2;
-    const dim3 grid((nb + 64 * R - 1) / (64 * R), jg);
-    if constexpr (sizeof(T) == 4)
-        hipLaunchKernelGGL((mf_product_kernel<T, S, 1>), grid, dim3(256), 0, st, (const double AS4 *)crec,
-                           (const T AS4 *)xrec, nb, nbp / MF_CH, part, (const DevStatus *)nullptr);
-    else
-        hipLaunchKernelGGL((mf_product_kernel<T, S, 2>), grid, dim3(256), 0, st, (const double AS4 *)crec,
-                           (const T AS4 *)xrec, nb, nbp / MF_CH, part, (const DevStatus *)nullptr);
+    const dim3 grid((nb + 63) / 64, jg);
+    hipLaunchKernelGGL((mf_product_kernel<T, S, 1>), grid, dim3(64 * MF_WAVES), 0, st, crec, xrec, nb, nbp / MF_CH,
+                       part, (const DevStatus *)nullptr);
 }
 
 template <typename T>
