@@ -313,11 +313,11 @@ def time_c4(steps=200, warmup=20):
 
 
 def measured_traffic(kind, prec="fp32"):
-    """HBM bytes per launch of the FP32 Lanczos product kernel (kind "tiles":
-    symm_tiles_kernel<float, 4>; "mf": mf_product_kernel<float, 4, 1>) from
-    the two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in
-    profiles/, already scaled by the copy-kernel calibration; None until
-    measured."""
+    """HBM bytes per launch of a Lanczos product kernel (kind "tiles":
+    symm_tiles_kernel<T, 4>; "mf": mf_product_kernel<T, 4, 1>; prec fp32 or
+    fp64) from the two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE)
+    summarised in profiles/, already scaled by the copy-kernel calibration;
+    None until measured."""
     name = {"tiles": "symm_tiles_n2000_s4_hbm_traffic.md", "mf": "mf_product_n2000_s4_hbm_traffic.md"}[kind]
     path = os.path.join(ROOT, "profiles", name)
     try:
@@ -386,8 +386,11 @@ def main():
             runs[prec, storage] = time_path(prec, steps, warmup, storage)
             print(f"# {prec} {storage}: {runs[prec, storage]['steps_per_s']:.1f} steps/s", file=sys.stderr,
                   flush=True)
-    # the library's automatic storage: FP64 matrix-free, FP32 (mixed) stored
-    # tiles (FP32 storage, FP64 arithmetic: the accurate mixed form)
+    # the library's automatic forms: FP64 the faster one (matrix-free at this
+    # size); mixed the stored FP32 tiles with FP64 arithmetic, whose D.X error
+    # (~2e-8, one rounding per entry) is a quarter of the FP32 matrix-free
+    # product's: with the latter every seeded state stops one Lanczos
+    # iteration away from FP64 (its rate is in steps_per_s_by_storage)
     best = {"fp32": "tiles", "fp64": max(("matfree", "tiles"), key=lambda st: runs["fp64", st]["steps_per_s"])}
     mixed, fp64 = runs["fp32", best["fp32"]], runs["fp64", best["fp64"]]
     ens = None
@@ -439,30 +442,36 @@ def main():
 
     rt32, rt64 = roof_tiles(runs["fp32", "tiles"], 4), roof_tiles(runs["fp64", "tiles"], 8)
     rm32, rm64 = roof_mf(runs["fp32", "matfree"]), roof_mf(runs["fp64", "matfree"])
-    if head == ("fp64", "matfree"):
-        roofline = {"bound": "mfma", "achieved": round(rm64["achieved"], 2), "peak": FP64_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(rm64["achieved"] / FP64_PEAK_TFLOPS, 4),
-                    "traffic": measured_traffic("mf", "fp64"),
-                    "kernel": "mf_product_kernel<double, 4, 2>",
-                    "note": "FLOP-bound on the FP64 vector ALUs (no MFMA in the product; FP64 vector and "
-                            "FP64 matrix peaks are both 78.6 TFLOP/s)",
-                    "avg_launch_us": round(rm64["avg_launch_us"], 2),
-                    "flops_per_working_launch": rm64["flops_per_working_launch"],
-                    "launches": rm64["launches"], "empty_launches": rm64["empty_launches"],
+    mf_note = ("FLOP-bound on the vector ALUs: the RPY blocks are formed in registers, so the product has no "
+               "stored operand for MFMA (FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s; FP32 157.3 "
+               "counts packed FP32, which this build does not emit)")
+    if head[1] == "matfree":
+        rm, pk = (rm64, FP64_PEAK_TFLOPS) if head[0] == "fp64" else (rm32, FP32_PEAK_TFLOPS)
+        roofline = {"bound": "mfma", "achieved": round(rm["achieved"], 2), "peak": pk,
+                    "unit": "TFLOP/s", "frac": round(rm["achieved"] / pk, 4),
+                    "traffic": measured_traffic("mf", head[0]),
+                    "kernel": "mf_product_kernel<%s, 4, 1>" % ("double" if head[0] == "fp64" else "float"),
+                    "note": mf_note,
+                    "avg_launch_us": round(rm["avg_launch_us"], 2),
+                    "flops_per_working_launch": rm["flops_per_working_launch"],
+                    "launches": rm["launches"], "empty_launches": rm["empty_launches"],
                     "fp32_kernel": "mf_product_kernel<float, 4, 1>",
                     "fp32_achieved": round(rm32["achieved"], 2), "fp32_peak": FP32_PEAK_TFLOPS,
-                    "fp32_avg_launch_us": round(rm32["avg_launch_us"], 2)}
+                    "fp32_avg_launch_us": round(rm32["avg_launch_us"], 2),
+                    "fp64_achieved": round(rm64["achieved"], 2), "fp64_avg_launch_us": round(rm64["avg_launch_us"], 2)}
     else:
-        roofline = {"bound": "hbm", "achieved": round(rt32["achieved"], 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(rt32["achieved"] / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic("tiles"), "kernel": "symm_tiles_kernel<float, 4>",
-                    "avg_launch_us": round(rt32["avg_launch_us"], 2)}
+        rt = rt64 if head[0] == "fp64" else rt32
+        roofline = {"bound": "hbm", "achieved": round(rt["achieved"], 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(rt["achieved"] / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic("tiles", head[0]),
+                    "kernel": "symm_tiles_kernel<%s, 4>" % ("double" if head[0] == "fp64" else "float"),
+                    "avg_launch_us": round(rt["avg_launch_us"], 2)}
     roof_tiles_entry = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                         "kernel": "symm_tiles_kernel<float, 4>", "achieved": round(rt32["achieved"], 1),
                         "frac": round(rt32["achieved"] / HBM_PEAK_GBS, 4),
                         "avg_launch_us": round(rt32["avg_launch_us"], 2),
                         "bytes_per_working_launch": round(rt32["bytes_per_working_launch"]),
-                        "traffic": measured_traffic("tiles"),
+                        "traffic": measured_traffic("tiles", "fp32"),
                         "fp64_achieved": round(rt64["achieved"], 1),
                         "fp64_avg_launch_us": round(rt64["avg_launch_us"], 2)}
     out = {

@@ -541,8 +541,8 @@ static int b_lanczos(Batch &b, int *iters_out, bool capture = false)
         if (b.prof_on) {
             if (b.prof_used + 2 > b.prof_ev.size()) {
                 hipEvent_t x, y;
-                BD_HIP(hipEventCreate(&x));
-                BD_HIP(hipEventCreate(&y));
+                BD_HIP(hipEventCreateWithFlags(&x, kTimingEvent));
+                BD_HIP(hipEventCreateWithFlags(&y, kTimingEvent));
                 b.prof_ev.push_back(x);
                 b.prof_ev.push_back(y);
             }

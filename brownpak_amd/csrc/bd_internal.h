@@ -24,6 +24,9 @@ namespace bd {
 constexpr int TILE = 64;
 constexpr int MXPARAM = 12;
 constexpr int MAX_S = 8;          // largest Lanczos block size supported
+// Events that only time kernels: no system-scope fence when they complete
+// (no cache writeback and invalidation around the timed launch).
+constexpr unsigned kTimingEvent = hipEventDisableSystemFence;
 constexpr int NQUAD_MAX = 160;    // quadrature nodes for H^{-1/2}
 constexpr int NBOUND = 48;        // Cholesky probes for the lower spectral bound
 constexpr int LZ_NEV = 8;         // events handing Lanczos iterations to the side stream

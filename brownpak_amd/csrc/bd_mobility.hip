@@ -974,8 +974,8 @@ int plan_tune(MobPlan &p, const std::vector<int> &nb, const std::vector<long> &r
         t = hit->second.second;
     } else {
         hipEvent_t e0, e1;
-        BD_HIP(hipEventCreate(&e0));
-        BD_HIP(hipEventCreate(&e1));
+        BD_HIP(hipEventCreateWithFlags(&e0, kTimingEvent));
+        BD_HIP(hipEventCreateWithFlags(&e1, kTimingEvent));
         for (int k = 0; k < BD_TILE_SHAPES; k++) {
             plan_free(p);
             BD_TRY(plan_build(p, nb, row_off, kTileShapes[k][0], kTileShapes[k][1]));
@@ -1017,8 +1017,8 @@ int prof_pair(Ctx &c, int w, hipEvent_t *e0, hipEvent_t *e1)
     if (c.prof_used + 2 > c.prof_ev.size()) prof_flush(c);
     if (c.prof_used + 2 > c.prof_ev.size()) {
         hipEvent_t a, b;
-        BD_HIP(hipEventCreate(&a));
-        BD_HIP(hipEventCreate(&b));
+        BD_HIP(hipEventCreateWithFlags(&a, kTimingEvent));
+        BD_HIP(hipEventCreateWithFlags(&b, kTimingEvent));
         c.prof_ev.push_back(a);
         c.prof_ev.push_back(b);
     }
