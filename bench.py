@@ -199,18 +199,21 @@ def cpu_model(lanczos_iters_per_update):
 ENSEMBLE_CHAINS = 512
 
 
-def time_ensemble(steps, warmup, precision="fp32"):
+def time_ensemble(steps, warmup, precision="fp32", storage="auto"):
     """The batched dilute ensemble: 512 chains with lengths drawn from
     U[50, 800] (seeded), FENE/WCA/RPY, block Lanczos s = 4, tol 1e-6, all
-    chains advanced by shared launches (bd_batch_run), device noise.
-    Returns steps/s (every chain advances one step per step) and the tile
-    product's kernel time, from a second pass with HIP events."""
+    chains advanced by shared launches (bd_batch_run), device noise; the
+    mobility products matrix-free (the batch's automatic form) or from the
+    stored tiles.  Returns steps/s (every chain advances one step per step)
+    and the Lanczos product's kernel time, from a second pass with HIP
+    events."""
     import torch
     from brownpak_amd import BdBatch
     from brownpak_amd.configs import fene_wca_params, overlap_free_chain
     rng = np.random.default_rng(3)
     lengths = [int(x) for x in rng.integers(50, 801, size=ENSEMBLE_CHAINS)]
-    b = BdBatch(lengths, nts_mobsam=S_BLOCK, lanc_mxitr=50, lanc_tol=1e-6, precision=precision)
+    b = BdBatch(lengths, nts_mobsam=S_BLOCK, lanc_mxitr=50, lanc_tol=1e-6, precision=precision,
+                mob_storage=storage)
     bond, vdw = fene_wca_params()
     b.set_model(2, bond, lvdw=True, vdw_params=vdw, rcutoff=vdw[2])
     b.set_coordinates([overlap_free_chain(nb, 10 + c) for c, nb in enumerate(lengths)])
@@ -229,15 +232,21 @@ def time_ensemble(steps, warmup, precision="fp32"):
     b.prof(0)
     info2 = b.info()
     n = [3 * nb for nb in lengths]
-    return dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, chains=ENSEMBLE_CHAINS,
-                beads=int(sum(lengths)), precision=precision,
-                lanczos_iters_per_chain_update=round(
-                    info["total_lanczos_iters"] / max(info["mob_updates"] * ENSEMBLE_CHAINS, 1), 2),
-                tile_product_avg_us=round(ms / max(launches, 1) * 1e3, 2),
-                tile_product_gbs=round(tile_bytes / (ms / max(launches, 1) * 1e-3) / 1e9, 1),
-                stored_tile_bytes=tile_bytes,
-                algorithmic_tile_bytes=float(sum(4.0 * k * (k + 1) / 2 for k in n)),
-                not_converged=info2["n_not_converged"])
+    avg_s = ms / max(launches, 1) * 1e-3
+    out = dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, chains=ENSEMBLE_CHAINS,
+               beads=int(sum(lengths)), precision=precision, mobility="tiles" if tile_bytes > 0 else "matfree",
+               lanczos_iters_per_chain_update=round(
+                   info["total_lanczos_iters"] / max(info["mob_updates"] * ENSEMBLE_CHAINS, 1), 2),
+               product_avg_us=round(avg_s * 1e6, 2), not_converged=info2["n_not_converged"])
+    if tile_bytes > 0:
+        out.update(tile_product_gbs=round(tile_bytes / avg_s / 1e9, 1), stored_tile_bytes=tile_bytes,
+                   algorithmic_tile_bytes=float(sum(4.0 * k * (k + 1) / 2 for k in n)))
+    else:
+        # every launch, converged chains idle: the algorithmic flops of all
+        # chains' ordered bead pairs (an upper bound of the work done)
+        fl = float(sum(nb * (nb - 1) for nb in lengths)) * MF_FLOPS_PAIR
+        out.update(product_tflops_all_chains=round(fl / avg_s / 1e12, 2), flops_all_chains=fl)
+    return out
 
 
 def time_c1(steps=400, warmup=20, cpu_steps=400):
@@ -371,7 +380,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.workload == "ensemble":
         ens = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK)
-        print(json.dumps({"ensemble": ens}), flush=True)
+        tiles = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK, storage="tiles")
+        print(json.dumps({"ensemble": ens, "ensemble_tiles": tiles}), flush=True)
         return
     if args.workload == "c1":
         print(json.dumps({"c1_chain100_cholesky": time_c1(steps, warmup, 0 if args.no_cpu_baseline else 400)}),
@@ -393,10 +403,12 @@ def main():
     # iteration away from FP64 (its rate is in steps_per_s_by_storage)
     best = {"fp32": "tiles", "fp64": max(("matfree", "tiles"), key=lambda st: runs["fp64", st]["steps_per_s"])}
     mixed, fp64 = runs["fp32", best["fp32"]], runs["fp64", best["fp64"]]
-    ens = None
+    ens = ens_tiles = None
     if args.ensemble_steps > 0:
         ens = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK)
-        print(f"# ensemble: {ens['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
+        ens_tiles = time_ensemble(S_BLOCK, S_BLOCK, storage="tiles")
+        print(f"# ensemble: {ens['steps_per_s']:.1f} steps/s (tiles {ens_tiles['steps_per_s']:.1f})",
+              file=sys.stderr, flush=True)
     other = None
     if not args.no_other_configs:
         other = {"c1_chain100_cholesky": time_c1(), "c4_melt20k_periodic_verlet": time_c4()}
@@ -505,6 +517,7 @@ def main():
         "roofline_tiles": roof_tiles_entry,
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "ensemble_tiles": ens_tiles,
         "other_configs": other,
     }
     if cpu:

@@ -496,7 +496,7 @@ class BdBatch:
     """
 
     def __init__(self, nbeads, *, nts_mobsam=1, lanc_mxitr=50, lanc_tol=1e-6, tim_stp=1e-3,
-                 precision="fp64", keep_reuse_noise=False):
+                 precision="fp64", keep_reuse_noise=False, mob_storage="auto"):
         self.L = load_library()
         self.nbeads = np.asarray(nbeads, dtype=np.int32).ravel().copy()
         self.nchains = int(self.nbeads.size)
@@ -504,6 +504,9 @@ class BdBatch:
         prec = {"fp64": FP64, "fp32": FP32}[precision] if isinstance(precision, str) else int(precision)
         self.opts = BdOpts(1, LANC, int(nts_mobsam), int(lanc_mxitr), float(lanc_tol), float(tim_stp), prec,
                            int(bool(keep_reuse_noise)), 0, 0.0, 0)
+        # "auto" and "matfree": the RPY blocks formed inside every product;
+        # "tiles": the stored lower triangles with the autotuned unit shape
+        self.opts.mob_storage = MOB_STORAGE[mob_storage] if isinstance(mob_storage, str) else int(mob_storage)
         self.h = self.L.bd_batch_create(self.nchains, _ip(self.nbeads), C.byref(self.opts))
         if not self.h:
             raise BdError(BD_ERR_ARG, self.L.bd_last_error().decode())

@@ -19,6 +19,7 @@
 #include "bd_forces_dev.h"
 #include "bd_internal.h"
 #include "bd_lanczos_dev.h"
+#include "bd_matfree_dev.h"
 
 namespace bd {
 
@@ -46,6 +47,14 @@ struct Batch {
     double tune_us[BD_TILE_SHAPES] = {};   // unit-shape timings of the autotuner
     float *mob32 = nullptr;
     double *mob64 = nullptr;
+    // matrix-free products (no stored tiles): the bead records of every
+    // chain (padded to MF_CH beads), each chain's first record, and the work
+    // list of (chain, 64-bead row block), largest chains first
+    bool mf = false;
+    double *mf_rec = nullptr;
+    int *mf_bo = nullptr;
+    int2 *mf_items = nullptr;
+    int mf_nitems = 0;
     BChainDev *chains = nullptr;
     int *bead_chain = nullptr;    // per bead (global): chain
     int *bead_local = nullptr;    // per bead: index inside its chain
@@ -507,6 +516,82 @@ __global__ void b_lz_init_kernel(int *active, int *run, int *failed, int *conv, 
     if (c == 0) counters[0] = C;
 }
 
+// Records of every chain for the matrix-free products: (x, y, z, 0) per
+// bead, padded to a multiple of MF_CH beads; one block per chain.
+__global__ void b_mf_rec_kernel(const double *__restrict__ crd, const BChainDev *__restrict__ ch,
+                                const int *__restrict__ bo, double *__restrict__ rec)
+{
+    const int c = blockIdx.x;
+    const int nb = ch[c].nb, nbp = (nb + MF_CH - 1) / MF_CH * MF_CH;
+    const double *cr = crd + ch[c].row;
+    double *r = rec + 4L * bo[c];
+    for (int j = threadIdx.x; j < nbp; j += blockDim.x) {
+        double x, y, z;
+        if (j < nb) {
+            x = cr[3 * j];
+            y = cr[3 * j + 1];
+            z = cr[3 * j + 2];
+        } else {
+            mf_far_point(j, nb, x, y, z);
+        }
+        r[4 * j] = x;
+        r[4 * j + 1] = y;
+        r[4 * j + 2] = z;
+        r[4 * j + 3] = 0.0;
+    }
+}
+
+// Y = D X for every running chain with D never stored (bd_matfree_dev.h):
+// one workgroup per (chain, 64-bead row block) over the chain's whole column
+// range, so every row is complete in its workgroup (no partial sums).  X and
+// Y are the padded vectors (ld NP); the chain's pad rows of Y are zero.
+template <typename T, int S>
+__global__ __launch_bounds__(64 * MF_WAVES) void b_mf_product_kernel(const double *__restrict__ rec,
+                                                                     const int *__restrict__ bo,
+                                                                     const int2 *__restrict__ items,
+                                                                     const BChainDev *__restrict__ ch,
+                                                                     const double *__restrict__ X,
+                                                                     double *__restrict__ Y, long NP,
+                                                                     const int *__restrict__ run)
+{
+    using Rec = MfRec<T, S>;
+    __shared__ __align__(16) unsigned char smem[mf_smem_bytes<T, S>()];
+    const int2 it = items[blockIdx.x];
+    const int c = it.x, rb = it.y;
+    if (run && !run[c]) return;
+    const int nb = ch[c].nb, npad = ch[c].npad, nch = (nb + MF_CH - 1) / MF_CH;
+    const double *crec = rec + 4L * bo[c];
+    const double *x = X + ch[c].row;
+    double *y = Y + ch[c].row;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ic = min(rb * 64 + lane, nb - 1);
+    const double xi = crec[4 * ic], yi = crec[4 * ic + 1], zi = crec[4 * ic + 2];
+    double acc[3][S];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int k = 0; k < S; k++) acc[a][k] = 0.0;
+    auto stage = [&](int, int j, Rec &rc) {
+        rc.x = crec[4 * j];
+        rc.y = crec[4 * j + 1];
+        rc.z = crec[4 * j + 2];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int k = 0; k < S; k++) rc.v[a][k] = j < nb ? (T)x[(long)k * NP + 3 * j + a] : (T)0;
+    };
+    mf_rows_times_cols<T, S>(xi, yi, zi, 0, nch, reinterpret_cast<Rec *>(smem), w, stage, acc);
+    mf_wave_sums<S>(acc, reinterpret_cast<double(*)[64][3 * S + 1]>(smem), w, lane, [&](int l, int col, double v) {
+        const int ib = rb * 64 + l;
+        if (ib < nb) y[(long)(col % S) * NP + 3 * ib + col / S] = v;
+    });
+    if (rb == (nb - 1) / 64) {
+        const int np = npad - 3 * nb;
+        for (int e = threadIdx.x; e < S * np; e += blockDim.x) y[(long)(e / np) * NP + 3 * nb + e % np] = 0.0;
+    }
+}
+
 // results of non-converged chains: y of the last iteration
 __global__ void b_finish_kernel(const double *Y, double *out, long NP, int s, int jlast, const int *conv,
                                 const int *rowblk)
@@ -520,6 +605,45 @@ __global__ void b_finish_kernel(const double *Y, double *out, long NP, int s, in
 }
 
 // ------------------------------------------------------------ host
+// y = D x for every chain (active: only chains with active[c] != 0), from
+// the stored tiles or matrix-free; HIP events e0/e1 around the launch.
+template <int S>
+static int b_product(Batch &b, const double *x, double *y, const int *active, hipEvent_t e0, hipEvent_t e1)
+{
+    const long NP = b.NP;
+    hipStream_t sm = b.stream;
+    if (!b.mf) {
+        if (b.mob32)
+            return plan_product<float, S>(b.plan, b.mob32, x, y, (int)NP, (int)NP, &b.st->ticket, nullptr, active, sm,
+                                          b.device, e0, e1);
+        return plan_product<double, S>(b.plan, b.mob64, x, y, (int)NP, (int)NP, &b.st->ticket, nullptr, active, sm,
+                                       b.device, e0, e1);
+    }
+    if (e0) BD_HIP(hipEventRecord(e0, sm));
+    if (b.opt.precision == BD_FP32)
+        hipLaunchKernelGGL((b_mf_product_kernel<float, S>), dim3(b.mf_nitems), dim3(64 * MF_WAVES), 0, sm, b.mf_rec,
+                           b.mf_bo, b.mf_items, b.chains, x, y, NP, active);
+    else
+        hipLaunchKernelGGL((b_mf_product_kernel<double, S>), dim3(b.mf_nitems), dim3(64 * MF_WAVES), 0, sm,
+                           b.mf_rec, b.mf_bo, b.mf_items, b.chains, x, y, NP, active);
+    BD_HIP(hipGetLastError());
+    if (e1) BD_HIP(hipEventRecord(e1, sm));
+    return BD_OK;
+}
+
+// The mobility at the current coordinates: the tiles, or the bead records
+// of the matrix-free products.
+static int b_mobility(Batch &b, hipStream_t sm)
+{
+    if (b.mf) {
+        hipLaunchKernelGGL(b_mf_rec_kernel, dim3(b.C), dim3(256), 0, sm, b.crd, b.chains, b.mf_bo, b.mf_rec);
+        BD_HIP(hipGetLastError());
+        return BD_OK;
+    }
+    if (b.mob32) return rpy_launch<float>(b.plan, b.crd, b.mob32, sm);
+    return rpy_launch<double>(b.plan, b.crd, b.mob64, sm);
+}
+
 // capture: the iterations run to dimk_max without looking at the host
 // (kernels of converged chains are no-ops), nothing waits on the host, and
 // the iteration counts are tallied on the device (counters[2], [3]).
@@ -550,12 +674,7 @@ static int b_lanczos(Batch &b, int *iters_out, bool capture = false)
             e1 = b.prof_ev[b.prof_used + 1];
             b.prof_used += 2;
         }
-        if (b.mob32)
-            BD_TRY((plan_product<float, S>(b.plan, b.mob32, Vj, b.W, (int)NP, (int)NP, &b.st->ticket, nullptr,
-                                           b.run, sm, b.device, e0, e1)));
-        else
-            BD_TRY((plan_product<double, S>(b.plan, b.mob64, Vj, b.W, (int)NP, (int)NP, &b.st->ticket, nullptr,
-                                            b.run, sm, b.device, e0, e1)));
+        BD_TRY(b_product<S>(b, Vj, b.W, b.run, e0, e1));
         hipLaunchKernelGGL((b_krylov_kernel<S>), dim3(C), dim3(BT), 0, sm, b.W, b.V, NP, dmax, j, b.chains, b.A,
                            b.B, b.run, b.failed);
         BD_HIP(hipGetLastError());
@@ -646,12 +765,7 @@ static int b_step_tail(Batch &b, int k, bool noise)
     hipLaunchKernelGGL(b_force_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.NB, b.bead_chain, b.bead_local,
                        b.bead_row, b.chains, b.bond_style, b.dev_bpar, b.lvdw, b.dev_vpar, b.rcutoff * b.rcutoff,
                        b.excl, b.frc, b.counters);
-    if (b.mob32)
-        BD_TRY((plan_product<float, 1>(b.plan, b.mob32, b.frc, b.dfr, (int)NP, (int)NP, &b.st->ticket, nullptr,
-                                       nullptr, sm, b.device, nullptr, nullptr)));
-    else
-        BD_TRY((plan_product<double, 1>(b.plan, b.mob64, b.frc, b.dfr, (int)NP, (int)NP, &b.st->ticket, nullptr,
-                                        nullptr, sm, b.device, nullptr, nullptr)));
+    BD_TRY(b_product<1>(b, b.frc, b.dfr, nullptr, nullptr, nullptr));
     const double *dcol = noise ? b.diff + (size_t)(k - 1) * NP : nullptr;
     hipLaunchKernelGGL(b_update_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.dfr, dcol, b.NB, b.bead_row,
                        b.opt.tim_stp, b.counters);
@@ -668,8 +782,7 @@ static int b_step(Batch &b, const double *xi_host, long long avail, long long *c
     *consumed = 0;
     bool noise = true;
     if (b.cntr == 0) {
-        if (b.mob32) BD_TRY(rpy_launch<float>(b.plan, b.crd, b.mob32, sm));
-        else BD_TRY(rpy_launch<double>(b.plan, b.crd, b.mob64, sm));
+        BD_TRY(b_mobility(b, sm));
         long long need = 0;
         for (int c = 0; c < b.C; c++) need += 3LL * b.nb[c] * s;
         if (xi_host) {
@@ -724,8 +837,7 @@ static int b_block_device(Batch &b)
     const int s = b.s;
     const long NP = b.NP;
     hipStream_t sm = b.stream;
-    if (b.mob32) BD_TRY(rpy_launch<float>(b.plan, b.crd, b.mob32, sm));
-    else BD_TRY(rpy_launch<double>(b.plan, b.crd, b.mob64, sm));
+    BD_TRY(b_mobility(b, sm));
     hipLaunchKernelGGL(b_gauss_kernel, dim3((unsigned)((NP * s + 255) / 256)), dim3(256), 0, sm, b.xi, NP, s,
                        b.plan.rowblk, b.chains, b.seed, b.dev_rng);
     hipLaunchKernelGGL(b_rng_advance_kernel, dim3(1), dim3(1), 0, sm, b.dev_rng, (uint64_t)NP * s);
@@ -824,7 +936,12 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
         hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking) != hipSuccess)
         return fail("stream creation failed");
     if (plan_build(b.plan, b.nb, b.row) != BD_OK) return fail("plan construction failed");
-    const size_t elems = (size_t)b.plan.ntiles * TILE * TILE;
+    if (opts->mob_storage != BD_MOB_AUTO && opts->mob_storage != BD_MOB_TILES && opts->mob_storage != BD_MOB_MATFREE)
+        return fail("mob_storage must be BD_MOB_AUTO, BD_MOB_TILES or BD_MOB_MATFREE");
+    // automatic: matrix-free, measured faster than the stored tiles on the
+    // 512-chain ensemble in both precisions, and without their n^2 storage
+    b.mf = opts->mob_storage != BD_MOB_TILES;
+    const size_t elems = b.mf ? 0 : (size_t)b.plan.ntiles * TILE * TILE;
     bool ok = true;
     auto dal = [&](double **p, size_t count) {
         if (ok) ok = hipMalloc(p, std::max<size_t>(count, 1) * sizeof(double)) == hipSuccess &&
@@ -834,8 +951,28 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
         if (ok) ok = hipMalloc(p, std::max<size_t>(count, 1) * sizeof(int)) == hipSuccess &&
                      hipMemset(*p, 0, std::max<size_t>(count, 1) * sizeof(int)) == hipSuccess;
     };
-    if (opts->precision == BD_FP32) ok = hipMalloc(&b.mob32, elems * sizeof(float)) == hipSuccess;
-    else ok = hipMalloc(&b.mob64, elems * sizeof(double)) == hipSuccess;
+    if (!b.mf) {
+        if (opts->precision == BD_FP32) ok = hipMalloc(&b.mob32, elems * sizeof(float)) == hipSuccess;
+        else ok = hipMalloc(&b.mob64, elems * sizeof(double)) == hipSuccess;
+    } else {
+        std::vector<int> bo(nchains), order(nchains);
+        long nrec = 0;
+        for (int c = 0; c < nchains; c++) {
+            bo[c] = (int)nrec;
+            nrec += (nbeads[c] + MF_CH - 1) / MF_CH * MF_CH;
+            order[c] = c;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return nbeads[u] > nbeads[v]; });
+        std::vector<int2> items;
+        for (int c : order)
+            for (int rb = 0; rb < (nbeads[c] + 63) / 64; rb++) items.push_back(make_int2(c, rb));
+        b.mf_nitems = (int)items.size();
+        ok = nrec < (1L << 30) && hipMalloc(&b.mf_rec, (size_t)nrec * 4 * sizeof(double)) == hipSuccess &&
+             hipMalloc(&b.mf_bo, nchains * sizeof(int)) == hipSuccess &&
+             hipMalloc(&b.mf_items, items.size() * sizeof(int2)) == hipSuccess &&
+             hipMemcpy(b.mf_bo, bo.data(), nchains * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(b.mf_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice) == hipSuccess;
+    }
     const int s = b.s, dm = b.dimk_max;
     dal(&b.crd, b.NP); dal(&b.frc, b.NP); dal(&b.dfr, b.NP); dal(&b.diff, (size_t)b.NP * s);
     dal(&b.xi, (size_t)b.NP * s); dal(&b.V, (size_t)b.NP * s * dm); dal(&b.W, (size_t)b.NP * s);
@@ -866,9 +1003,11 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
         if (hipEventCreateWithFlags(&b.chunk_ev[k], hipEventDisableTiming) != hipSuccess) return fail("event failed");
     if (hipEventCreateWithFlags(&b.join_ev, hipEventDisableTiming) != hipSuccess) return fail("event failed");
     // tile unit shape chosen by timing the Lanczos product on this plan
-    if (b.mob32) ok = hipMemset(b.mob32, 0, elems * sizeof(float)) == hipSuccess;
-    else ok = hipMemset(b.mob64, 0, elems * sizeof(double)) == hipSuccess;
-    if (!ok || batch_tune(b) != BD_OK) return fail("tile shape autotuning failed");
+    if (!b.mf) {
+        if (b.mob32) ok = hipMemset(b.mob32, 0, elems * sizeof(float)) == hipSuccess;
+        else ok = hipMemset(b.mob64, 0, elems * sizeof(double)) == hipSuccess;
+        if (!ok || batch_tune(b) != BD_OK) return fail("tile shape autotuning failed");
+    }
     return h;
 }
 
@@ -891,6 +1030,9 @@ void bd_batch_destroy(bd_batch *h)
     for (auto p : ip)
         if (*p) (void)hipFree(*p);
     if (b.mob32) (void)hipFree(b.mob32);
+    if (b.mf_rec) (void)hipFree(b.mf_rec);
+    if (b.mf_bo) (void)hipFree(b.mf_bo);
+    if (b.mf_items) (void)hipFree(b.mf_items);
     if (b.bead_row) (void)hipFree(b.bead_row);
     if (b.chains) (void)hipFree(b.chains);
     if (b.st) (void)hipFree(b.st);
@@ -1034,6 +1176,7 @@ int bd_batch_tile_shape(bd_batch *h, int *chr, int *wsc, double *times_us)
 {
     if (!h) { set_error("null batch"); return BD_ERR_ARG; }
     Batch &b = h->b;
+    if (b.mf) { set_error("the batch runs matrix-free products: it has no tile shape"); return BD_ERR_STATE; }
     if (chr) *chr = b.plan.chr;
     if (wsc) *wsc = b.plan.wsc;
     if (times_us)
@@ -1046,6 +1189,7 @@ int bd_batch_set_tile_shape(bd_batch *h, int chr, int wsc)
     if (!h) { set_error("null batch"); return BD_ERR_ARG; }
     if (!tile_shape_supported(chr, wsc)) { set_error("unsupported tile unit shape"); return BD_ERR_ARG; }
     Batch &b = h->b;
+    if (b.mf) { set_error("the batch runs matrix-free products: it has no tile shape"); return BD_ERR_STATE; }
     BD_HIP(hipStreamSynchronize(b.stream));
     b_graph_drop(b);
     plan_free(b.plan);
@@ -1078,7 +1222,7 @@ int bd_batch_prof(bd_batch *h, int on, double *total_ms, long long *launches, do
     b.prof_used = 0;
     if (total_ms) *total_ms = b.prof_ms;
     if (launches) *launches = b.prof_count;
-    if (tile_bytes) *tile_bytes = (double)b.plan.ntiles * TILE * TILE * (b.mob32 ? 4.0 : 8.0);
+    if (tile_bytes) *tile_bytes = b.mf ? 0.0 : (double)b.plan.ntiles * TILE * TILE * (b.mob32 ? 4.0 : 8.0);
     if (on >= 0) {
         b.prof_on = on != 0;
         b.prof_ms = 0.0;

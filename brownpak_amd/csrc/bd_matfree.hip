@@ -34,13 +34,11 @@
 
 #include "bd_internal.h"
 #include "bd_lanczos_dev.h"
+#include "bd_matfree_dev.h"
 
 namespace bd {
 
-constexpr int MF_CH = 8;        // column beads per FP32 partial sum (and record padding)
-constexpr int MF_WAVES = 8;     // waves per workgroup (column-range split)
 constexpr int MF_JG_MAX = 64;   // largest workgroup split of the column range
-constexpr int MF_TILE = 256;    // column beads staged in LDS at a time
 
 int mf_rows_per_lane(const Ctx &) { return 1; }
 
@@ -68,9 +66,7 @@ __global__ void mf_crd_kernel(const double *__restrict__ crd, int nb, int nbp, d
         y = crd[3 * j + 1];
         z = crd[3 * j + 2];
     } else {
-        x = 1.0e6 * (j - nb + 2);
-        y = 1.0e6;
-        z = 1.0e6;
+        mf_far_point(j, nb, x, y, z);
     }
     rec[4 * j] = x;
     rec[4 * j + 1] = y;
@@ -89,169 +85,45 @@ __global__ void mf_pack_kernel(const double *__restrict__ x, int ld, int n, int 
     xr[e] = r < n ? (T)x[(long)k * ld + r] : (T)0;
 }
 
-template <typename T>
-__device__ inline T mf_rsqrt(T v);
-template <>
-__device__ inline float mf_rsqrt<float>(float v)
-{
-    return __builtin_amdgcn_rsqf(v);
-}
-template <>
-__device__ inline double mf_rsqrt<double>(double v)
-{
-    // v_rsq_f64 (relative error ~6e-8) and one third-order correction:
-    // 1/sqrt(v) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + ...), e = 1 - v y^2,
-    // truncated after e^2 (~1e-22), about one ulp after rounding
-    const double y = __builtin_amdgcn_rsq(v);
-    const double e = fma(-(v * y), y, 1.0);
-    return fma(y * e, fma(0.375, e, 0.5), y);
-}
-
-// r^2 is offset by this much before 1/sqrt: a bead's pair with itself then
-// gives finite overlap coefficients (c1 = 1, the r r^T term multiplying a
-// zero vector) without a special case.  The offset changes r^2 only for
-// two beads closer than about 1e-7.
-template <typename T>
-__device__ constexpr T mf_r2_floor();
-template <>
-__device__ constexpr double mf_r2_floor<double>() { return 1e-30; }
-template <>
-__device__ constexpr float mf_r2_floor<float>() { return 1e-30f; }
-
-// LDS record of one column bead: its coordinates and its 3 x S block of X.
-template <typename T, int S>
-struct alignas(16) MfRec {
-    double x, y, z;
-    T v[3][S];
-};
-
 // part[g][i][a S + k] = sum over the column beads j of workgroup column g of
-// (D_ij X_j)(a, k), FP64.  grid (ceil(nb / (64 R)), JG), 64 MF_WAVES threads.
-template <typename T, int S, int R>
+// (D_ij X_j)(a, k), FP64.  grid (ceil(nb / 64), JG), 64 MF_WAVES threads.
+template <typename T, int S>
 __global__ __launch_bounds__(64 * MF_WAVES) void mf_product_kernel(const double *__restrict__ crec,
                                                                    const T *__restrict__ xrec, int nb, int nch,
                                                                    double *__restrict__ part,
                                                                    const DevStatus *stop)
 {
     using Rec = MfRec<T, S>;
-    constexpr int RED_BYTES = MF_WAVES * 64 * (3 * S + 1) * (int)sizeof(double);
-    constexpr int TILE_BYTES = MF_TILE * (int)sizeof(Rec);
-    // the staged records, and after the last tile the waves' sums
-    __shared__ __align__(16) unsigned char smem[RED_BYTES > TILE_BYTES ? RED_BYTES : TILE_BYTES];
-    Rec *tile = reinterpret_cast<Rec *>(smem);
-    double(*red)[64][3 * S + 1] = reinterpret_cast<double(*)[64][3 * S + 1]>(smem);
+    __shared__ __align__(16) unsigned char smem[mf_smem_bytes<T, S>()];
     __shared__ int s_done;
     if (threadIdx.x == 0) s_done = (stop && lz_done(stop)) ? 1 : 0;
     __syncthreads();
     if (s_done) return;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    double xi[R], yi[R], zi[R];
-    int ii[R];
+    const int ic = min((int)blockIdx.x * 64 + lane, nb - 1);
+    const double xi = crec[4 * ic], yi = crec[4 * ic + 1], zi = crec[4 * ic + 2];
+    double acc[3][S];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        ii[r] = (blockIdx.x * R + r) * 64 + lane;
-        const int ic = min(ii[r], nb - 1);
-        xi[r] = crec[4 * ic];
-        yi[r] = crec[4 * ic + 1];
-        zi[r] = crec[4 * ic + 2];
-    }
-    double acc[R][3][S];
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int k = 0; k < S; k++) acc[r][a][k] = 0.0;
-    // the pair term (D_ij X_j)(a, k) added to acc_[r][a][k]: both
-    // coefficient branches are formed and the one that applies selected (no
-    // branch in the loop); j == i takes the overlap branch at r = 0, where
-    // c1 = 1 and the r r^T term vanishes, which is the unit diagonal block
-    auto pair = [&](const Rec &cur, auto &acc_) __attribute__((always_inline)) {
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const T dx = (T)(cur.x - xi[r]), dy = (T)(cur.y - yi[r]), dz = (T)(cur.z - zi[r]);
-            const T r2 = dx * dx + dy * dy + dz * dz;
-            const T y = mf_rsqrt<T>(r2 + mf_r2_floor<T>()), y2 = y * y;
-            const T c1f = y * ((T)0.75 + (T)0.5 * y2);
-            const T c2f = y * y2 * ((T)0.75 - (T)1.5 * y2);
-            const T c1n = (T)1 - (T)(9.0 / 32.0) * (r2 * y), c2n = (T)(3.0 / 32.0) * y;
-            const bool nr = r2 < (T)4;
-            const T c1 = nr ? c1n : c1f, c2 = nr ? c2n : c2f;
-#pragma unroll
-            for (int k = 0; k < S; k++) {
-                const T t = c2 * (dx * cur.v[0][k] + dy * cur.v[1][k] + dz * cur.v[2][k]);
-                acc_[r][0][k] = fma(c1, cur.v[0][k], acc_[r][0][k]);
-                acc_[r][0][k] = fma(t, dx, acc_[r][0][k]);
-                acc_[r][1][k] = fma(c1, cur.v[1][k], acc_[r][1][k]);
-                acc_[r][1][k] = fma(t, dy, acc_[r][1][k]);
-                acc_[r][2][k] = fma(c1, cur.v[2][k], acc_[r][2][k]);
-                acc_[r][2][k] = fma(t, dz, acc_[r][2][k]);
-            }
-        }
-    };
-    // the workgroup's column chunks, MF_TILE beads of them staged at a time
+        for (int k = 0; k < S; k++) acc[a][k] = 0.0;
+    // the workgroup's column chunks
     const int cb = (int)((long)nch * blockIdx.y / gridDim.y), ce = (int)((long)nch * (blockIdx.y + 1) / gridDim.y);
-    for (int t0 = cb; t0 < ce; t0 += MF_TILE / MF_CH) {
-        const int nct = min(ce - t0, MF_TILE / MF_CH), jb = t0 * MF_CH;
-        for (int e = threadIdx.x; e < nct * MF_CH; e += blockDim.x) {
-            const int j = jb + e;
-            Rec rc;
-            rc.x = crec[4 * j];
-            rc.y = crec[4 * j + 1];
-            rc.z = crec[4 * j + 2];
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int k = 0; k < S; k++) rc.v[a][k] = xrec[(3 * j + a) * S + k];
-            tile[e] = rc;
-        }
-        __syncthreads();
-        // this wave's chunks of the tile
-        const int qb = nct * w / MF_WAVES, qe = nct * (w + 1) / MF_WAVES;
-        if constexpr (sizeof(T) == 8) {
-            // FP64: straight into the accumulators
-            for (int q = MF_CH * qb; q < MF_CH * qe; q++) pair(tile[q], acc);
-        } else {
-            // FP32: partial sums over 8 column beads, then added in FP64
-            for (int ch = qb; ch < qe; ch++) {
-                T cacc[R][3][S];
-#pragma unroll
-                for (int r = 0; r < R; r++)
-#pragma unroll
-                    for (int a = 0; a < 3; a++)
-#pragma unroll
-                        for (int k = 0; k < S; k++) cacc[r][a][k] = (T)0;
-#pragma unroll 2
-                for (int q = 0; q < MF_CH; q++) pair(tile[MF_CH * ch + q], cacc);
-#pragma unroll
-                for (int r = 0; r < R; r++)
-#pragma unroll
-                    for (int a = 0; a < 3; a++)
-#pragma unroll
-                        for (int k = 0; k < S; k++) acc[r][a][k] += (double)cacc[r][a][k];
-            }
-        }
-        __syncthreads();
-    }
-    // the waves' sums, added in wave order (the staging area reused)
-#pragma unroll
-    for (int r = 0; r < R; r++) {
+    auto stage = [&](int, int j, Rec &rc) {
+        rc.x = crec[4 * j];
+        rc.y = crec[4 * j + 1];
+        rc.z = crec[4 * j + 2];
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int k = 0; k < S; k++) red[w][lane][a * S + k] = acc[r][a][k];
-        __syncthreads();
-        for (int e = threadIdx.x; e < 64 * 3 * S; e += blockDim.x) {
-            const int l = e / (3 * S), col = e % (3 * S);
-            const int ib = (blockIdx.x * R + r) * 64 + l;
-            double v = red[0][l][col];
-#pragma unroll
-            for (int q = 1; q < MF_WAVES; q++) v += red[q][l][col];
-            if (ib < nb) part[((long)blockIdx.y * nb + ib) * (3 * S) + col] = v;
-        }
-        __syncthreads();
-    }
+            for (int k = 0; k < S; k++) rc.v[a][k] = xrec[(3 * j + a) * S + k];
+    };
+    mf_rows_times_cols<T, S>(xi, yi, zi, cb, ce, reinterpret_cast<Rec *>(smem), w, stage, acc);
+    mf_wave_sums<S>(acc, reinterpret_cast<double(*)[64][3 * S + 1]>(smem), w, lane, [&](int l, int col, double v) {
+        const int ib = (int)blockIdx.x * 64 + l;
+        if (ib < nb) part[((long)blockIdx.y * nb + ib) * (3 * S) + col] = v;
+    });
 }
 
 // y(rows, S) = sum over g of the partials (fixed order); rows >= n are zero.
@@ -498,7 +370,7 @@ static int mf_launch_t(Ctx &c, const DevStatus *stop)
     if (e0) BD_HIP(hipEventRecord(e0, c.stream));
     const double *crec = c.mf_crd;
     const T *xrec = (const T *)c.mf_x;
-    hipLaunchKernelGGL((mf_product_kernel<T, S, 1>), grid, dim3(64 * MF_WAVES), 0, c.stream, crec, xrec, c.nb, nch,
+    hipLaunchKernelGGL((mf_product_kernel<T, S>), grid, dim3(64 * MF_WAVES), 0, c.stream, crec, xrec, c.nb, nch,
                        c.mf_part, stop);
     BD_HIP(hipGetLastError());
     if (e1) BD_HIP(hipEventRecord(e1, c.stream));
@@ -604,7 +476,7 @@ static void mf_raw_product(int nb, int nbp, int jg, const double *crec, const T 
                            hipStream_t st)
 {
     const dim3 grid((nb + 63) / 64, jg);
-    hipLaunchKernelGGL((mf_product_kernel<T, S, 1>), grid, dim3(64 * MF_WAVES), 0, st, crec, xrec, nb, nbp / MF_CH,
+    hipLaunchKernelGGL((mf_product_kernel<T, S>), grid, dim3(64 * MF_WAVES), 0, st, crec, xrec, nb, nbp / MF_CH,
                        part, (const DevStatus *)nullptr);
 }
 

@@ -98,7 +98,7 @@ enum { BD_EM = 0, BD_PC = 1, BD_SE = 2 };
  * 64 x 64 tiles and streams it once per product; BD_MOB_MATFREE stores only
  * the coordinates and evaluates every 3 x 3 RPY block inside the product
  * (same formula, no n x n storage); BD_MOB_AUTO takes the one measured to be
- * faster for the bead count and precision. */
+ * faster for the bead count and precision (for a batch: matrix-free). */
 enum { BD_MOB_AUTO = 0, BD_MOB_TILES = 1, BD_MOB_MATFREE = 2 };
 
 /* mth_ptgen (pairtab_m.fpp:23-24) */
@@ -374,7 +374,14 @@ int bd_prof_get(bd_ctx *ctx, double *total_ms, long long *launches,
  *   bd_batch_info: steps taken, mobility updates, summed Lanczos iterations
  *     over chains and updates, and chain-updates that hit lanc_mxitr.
  *   bd_batch_prof: as bd_prof_enable + bd_prof_get for the batch (on < 0
- *     reads without changing the state).
+ *     reads without changing the state); tile_bytes is 0 for matrix-free
+ *     products.
+ *   Mobility products (opts->mob_storage): BD_MOB_MATFREE and BD_MOB_AUTO
+ *     evaluate every chain's RPY blocks inside the product (one workgroup
+ *     per chain and 64-bead row block, the chain's whole column range, so
+ *     each chain's arithmetic is independent of the others); BD_MOB_TILES
+ *     stores every chain's lower triangle in tiles, as below.  The tile
+ *     shape calls return BD_ERR_STATE for a matrix-free batch.
  *   Tile shapes: the mobility product works in units of chr x wsc tiles of
  *     64 x 64; bd_batch_create times the BD_TILE_SHAPES supported shapes
  *     (in this order: 2x4, 1x4, 4x4, 2x2, 2x5, 2x6, 2x8) on the batch's own

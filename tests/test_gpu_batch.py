@@ -48,15 +48,17 @@ def _split_xi(lengths, s, nupd, xi):
     return [np.concatenate(x) for x in per]
 
 
+@pytest.mark.parametrize("storage", ["matfree", "tiles"])
 @pytest.mark.parametrize("s,prec", [(1, "fp64"), (4, "fp64"), (3, "fp64"), (4, "fp32")])
-def test_batch_matches_oracle_fixed_iterations(oracle, s, prec):
-    lengths = [40, 17, 64, 23, 50]
+def test_batch_matches_oracle_fixed_iterations(oracle, s, prec, storage):
+    lengths = [40, 17, 64, 23, 50, 130]
     nsteps, mx = 2 * s, 5
     nupd = (nsteps + s - 1) // s
-    _say(f"case: batch fixed-iteration s={s} {prec} lengths={lengths}")
+    _say(f"case: batch fixed-iteration s={s} {prec} {storage} lengths={lengths}")
     tot = sum(3 * nb * s for nb in lengths) * nupd
     xi = np.random.default_rng(7 + s).standard_normal(tot)
-    b, chains = _batch(lengths, 300, nts_mobsam=s, lanc_mxitr=mx, lanc_tol=0.0, precision=prec)
+    b, chains = _batch(lengths, 300, nts_mobsam=s, lanc_mxitr=mx, lanc_tol=0.0, precision=prec,
+                       mob_storage=storage)
     used = b.run(nsteps, xi)
     assert used == tot
     out = b.get_coordinates()
@@ -108,17 +110,20 @@ def test_batch_converged_noise_accuracy(oracle):
     assert b.info()["n_not_converged"] == 0
 
 
-def test_batch_chain_independence_ensemble():
-    """Lengths U[50, 800] (seeded), FP32 tiles, s = 4, tol 1e-6, device RNG
+@pytest.mark.parametrize("storage", ["matfree", "tiles"])
+def test_batch_chain_independence_ensemble(storage):
+    """Lengths U[50, 800] (seeded), FP32, s = 4, tol 1e-6, device RNG
     replaced by an injected stream so single runs see the same numbers."""
     rng = np.random.default_rng(2024)
     lengths = [int(x) for x in rng.integers(50, 801, size=24)]
     s, nsteps = 4, 8
-    _say(f"case: batch independence {len(lengths)} chains, {sum(lengths)} beads")
+    _say(f"case: batch independence {storage} {len(lengths)} chains, {sum(lengths)} beads")
     tot = sum(3 * nb * s for nb in lengths) * (nsteps // s)
     xi = rng.standard_normal(tot)
-    b, chains = _batch(lengths, 900, nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32")
-    b.set_tile_shape(2, 4)   # the single-chain batches below use the same unit shape
+    b, chains = _batch(lengths, 900, nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32",
+                       mob_storage=storage)
+    if storage == "tiles":
+        b.set_tile_shape(2, 4)   # the single-chain batches below use the same unit shape
     b.run(nsteps, xi)
     out = b.get_coordinates()
     assert all(np.isfinite(o).all() for o in out)
@@ -126,8 +131,10 @@ def test_batch_chain_independence_ensemble():
         assert np.abs(o.mean(axis=0)).max() < 1e-9
     per = _split_xi(lengths, s, nsteps // s, xi)
     for c in (0, 7, len(lengths) - 1):
-        one, _ = _batch([lengths[c]], 900 + c, nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32")
-        one.set_tile_shape(2, 4)
+        one, _ = _batch([lengths[c]], 900 + c, nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32",
+                        mob_storage=storage)
+        if storage == "tiles":
+            one.set_tile_shape(2, 4)
         one.run(nsteps, per[c])
         assert np.array_equal(one.get_coordinates()[0], out[c]), c
 
@@ -141,7 +148,7 @@ def test_batch_tile_shapes_match_oracle(oracle, shape):
     _say(f"case: batch tile shape {shape}")
     tot = sum(3 * nb * s for nb in lengths)
     xi = np.random.default_rng(17).standard_normal(tot)
-    b, chains = _batch(lengths, 310, nts_mobsam=s, lanc_mxitr=mx, lanc_tol=0.0)
+    b, chains = _batch(lengths, 310, nts_mobsam=s, lanc_mxitr=mx, lanc_tol=0.0, mob_storage="tiles")
     b.set_tile_shape(*shape)
     assert b.tile_shape()[0] == shape
     b.run(nsteps, xi)
@@ -160,7 +167,8 @@ def test_batch_autotuner_keeps_the_fastest_shape():
     from brownpak_amd import BdError
     rng = np.random.default_rng(5)
     lengths = [int(x) for x in rng.integers(50, 801, size=32)]
-    b, _ = _batch(lengths, 60, nts_mobsam=4, lanc_mxitr=20, lanc_tol=1e-6, precision="fp32")
+    b, _ = _batch(lengths, 60, nts_mobsam=4, lanc_mxitr=20, lanc_tol=1e-6, precision="fp32",
+                  mob_storage="tiles")
     shape, t = b.tile_shape()
     from brownpak_amd.bdstep import TILE_SHAPES
     shapes = list(TILE_SHAPES)
@@ -168,24 +176,35 @@ def test_batch_autotuner_keeps_the_fastest_shape():
     assert shape in shapes and np.all(t > 0)
     assert shape == shapes[int(np.argmin(t))]
     # a second batch of the same lengths reuses the choice
-    b2, _ = _batch(lengths, 61, nts_mobsam=4, lanc_mxitr=20, lanc_tol=1e-6, precision="fp32")
+    b2, _ = _batch(lengths, 61, nts_mobsam=4, lanc_mxitr=20, lanc_tol=1e-6, precision="fp32",
+                   mob_storage="tiles")
     assert b2.tile_shape()[0] == shape
     with pytest.raises(BdError):
         b.set_tile_shape(3, 4)
+    # a matrix-free batch has no tile shape to report or set
+    m, _ = _batch(lengths[:4], 62, nts_mobsam=4, lanc_mxitr=20, lanc_tol=1e-6, precision="fp32",
+                  mob_storage="matfree")
+    with pytest.raises(BdError):
+        m.tile_shape()
+    with pytest.raises(BdError):
+        m.set_tile_shape(2, 4)
 
 
-def test_batch_graph_blocks_match_step_by_step():
+@pytest.mark.parametrize("storage", ["matfree", "tiles"])
+def test_batch_graph_blocks_match_step_by_step(storage):
     """Device noise: whole mobility blocks run as one HIP graph launch each;
     the same seed stepped launch by launch (profiling forces that path)
     gives bit-identical chains and the same Lanczos tallies, also across a
     partial block."""
     rng = np.random.default_rng(77)
     lengths = [int(x) for x in rng.integers(50, 401, size=16)]
-    _say(f"case: batch graph vs eager {len(lengths)} chains")
+    _say(f"case: batch graph vs eager {storage} {len(lengths)} chains")
     outs, infos = [], []
     for eager in (False, True):
-        b, _ = _batch(lengths, 700, nts_mobsam=4, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32")
-        b.set_tile_shape(2, 4)
+        b, _ = _batch(lengths, 700, nts_mobsam=4, lanc_mxitr=50, lanc_tol=1e-6, precision="fp32",
+                      mob_storage=storage)
+        if storage == "tiles":
+            b.set_tile_shape(2, 4)
         b.set_rng(99, 0)
         if eager:
             b.prof(1)
@@ -225,10 +244,11 @@ def test_batch_overstretch_reports_bond_error(path):
 C3_CHAINS = 512
 
 
-def test_batch_c3_full_size(oracle):
+@pytest.mark.parametrize("storage", ["matfree", "tiles"])
+def test_batch_c3_full_size(oracle, storage):
     """The dilute-ensemble configuration at its full size: 512 chains with
     lengths from U[50, 800] (seed 3), FENE/WCA/RPY, block Lanczos s = 4,
-    tol 1e-6, FP32 tiles.  (1) Device noise, whole mobility blocks as HIP
+    tol 1e-6, FP32 products (matrix-free or stored tiles).  (1) Device noise, whole mobility blocks as HIP
     graph launches, 8 steps: every chain finite, centre of mass at the
     origin, every solve converged.  (2) Injected xi, one mobility block:
     six sampled chains bit-identical to the same chain run alone, and three
@@ -237,9 +257,9 @@ def test_batch_c3_full_size(oracle):
     rng = np.random.default_rng(3)
     lengths = [int(x) for x in rng.integers(50, 801, size=C3_CHAINS)]
     s = 4
-    _say(f"case: batch C3 full size, {sum(lengths)} beads")
+    _say(f"case: batch C3 full size {storage}, {sum(lengths)} beads")
     b, chains = _batch(lengths, 1000, overlap_free=True, nts_mobsam=s, lanc_mxitr=50, lanc_tol=1e-6,
-                       precision="fp32")
+                       precision="fp32", mob_storage=storage)
     b.set_rng(11, 0)
     b.run(8)
     out = b.get_coordinates()
@@ -249,21 +269,23 @@ def test_batch_c3_full_size(oracle):
         assert np.isfinite(o).all()
         assert np.abs(o.mean(axis=0)).max() < 1e-9
     # (2) injected xi, chain independence and oracle parity at fixed iterations
-    shape = b.tile_shape()[0]
+    shape = b.tile_shape()[0] if storage == "tiles" else None
     tot = sum(3 * nb * s for nb in lengths)
     xi = rng.standard_normal(tot)
     per = _split_xi(lengths, s, 1, xi)
     for tol, mx in ((1e-6, 50), (0.0, 5)):
         bb, _ = _batch(lengths, 1000, overlap_free=True, nts_mobsam=s, lanc_mxitr=mx, lanc_tol=tol,
-                       precision="fp32")
-        bb.set_tile_shape(*shape)
+                       precision="fp32", mob_storage=storage)
+        if shape:
+            bb.set_tile_shape(*shape)
         bb.run(s, xi)
         res = bb.get_coordinates()
         if tol > 0:
             for c in (0, 101, 202, 303, 404, C3_CHAINS - 1):
                 one, _ = _batch([lengths[c]], 1000 + c, overlap_free=True, nts_mobsam=s, lanc_mxitr=mx,
-                                lanc_tol=tol, precision="fp32")
-                one.set_tile_shape(*shape)
+                                lanc_tol=tol, precision="fp32", mob_storage=storage)
+                if shape:
+                    one.set_tile_shape(*shape)
                 one.run(s, per[c])
                 assert np.array_equal(one.get_coordinates()[0], res[c]), c
         else:
