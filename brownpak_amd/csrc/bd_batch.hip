@@ -55,6 +55,9 @@ struct Batch {
     int *mf_bo = nullptr;
     int2 *mf_items = nullptr;
     int mf_nitems = 0;
+    // work list of the force kernel: (chain, BF_ROWS-bead block)
+    int2 *bf_items = nullptr;
+    int bf_nitems = 0;
     BChainDev *chains = nullptr;
     int *bead_chain = nullptr;    // per bead (global): chain
     int *bead_local = nullptr;    // per bead: index inside its chain
@@ -415,47 +418,72 @@ __global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const doub
 }
 
 // Forces of every bead: chain bonds (i-1, i+1) and the chain's LJ pairs
-// r^2 < rc^2 with |i - j| > excl (the 1..3-ring exclusions of a linear chain).
-__global__ __launch_bounds__(256) void b_force_kernel(const double *__restrict__ crd, int NB,
-                                                      const int *__restrict__ bead_chain,
-                                                      const int *__restrict__ bead_local, const long *__restrict__ bead_row,
-                                                      const BChainDev *__restrict__ ch, int bsty, const double *bp,
-                                                      int lvdw, const double *vp, double rc2, int excl,
-                                                      double *__restrict__ frc, int *counters)
+// r^2 < rc^2 with |i - j| > excl (the 1..3-ring exclusions of a linear
+// chain), in that order, partners j in increasing order.  One workgroup per
+// (chain, BF_ROWS-bead block) from items; the chain's coordinates are staged
+// in LDS BF_TILE beads at a time, so every lane of a wave reads the same
+// partner bead as an LDS broadcast.
+constexpr int BF_ROWS = 256;
+constexpr int BF_TILE = 1024;
+__global__ __launch_bounds__(BF_ROWS) void b_force_lds_kernel(const double *__restrict__ crd,
+                                                              const int2 *__restrict__ items,
+                                                              const BChainDev *__restrict__ ch, int bsty,
+                                                              const double *bp, int lvdw, const double *vp,
+                                                              double rc2, int excl, double *__restrict__ frc,
+                                                              int *counters)
 {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= NB) return;
-    const int c = bead_chain[g], i = bead_local[g], nbc = ch[c].nb;
+    __shared__ double sx[BF_TILE], sy[BF_TILE], sz[BF_TILE];
+    const int2 it = items[blockIdx.x];
+    const int c = it.x, nbc = ch[c].nb;
+    const int i = it.y * BF_ROWS + (int)threadIdx.x;
+    const bool mine = i < nbc;
     const double *cr = crd + ch[c].row;
+    const int ic = mine ? i : nbc - 1;
+    const double xi = cr[3 * ic], yi = cr[3 * ic + 1], zi = cr[3 * ic + 2];
     double f[3] = {0, 0, 0};
-    // bonds (i-1, i) then (i, i+1), the bond order of a linear chain
-    for (int side = -1; side <= 1; side += 2) {
-        int j = i + side;
-        if (j < 0 || j >= nbc) continue;
-        double x = __dsub_rn(cr[3 * j], cr[3 * i]), y = __dsub_rn(cr[3 * j + 1], cr[3 * i + 1]),
-               z = __dsub_rn(cr[3 * j + 2], cr[3 * i + 2]);
-        double r = sqrt(rsq3(x, y, z)), en, fr;
-        if (bond_force(r, bsty, bp, en, fr)) atomicCAS(&counters[1], 0, 1);
-        f[0] += fr * x / r;
-        f[1] += fr * y / r;
-        f[2] += fr * z / r;
-    }
-    if (lvdw) {
-        for (int j = 0; j < nbc; j++) {
-            if (j == i || abs(j - i) <= excl) continue;
-            double x = __dsub_rn(cr[3 * j], cr[3 * i]), y = __dsub_rn(cr[3 * j + 1], cr[3 * i + 1]),
-                   z = __dsub_rn(cr[3 * j + 2], cr[3 * i + 2]);
-            double r2 = rsq3(x, y, z);
-            if (!(r2 < rc2)) continue;
-            double r = sqrt(r2), en, fr;
-            lj_force(r, vp, en, fr);
+    if (mine) {
+        for (int side = -1; side <= 1; side += 2) {
+            int j = i + side;
+            if (j < 0 || j >= nbc) continue;
+            double x = __dsub_rn(cr[3 * j], xi), y = __dsub_rn(cr[3 * j + 1], yi), z = __dsub_rn(cr[3 * j + 2], zi);
+            double r = sqrt(rsq3(x, y, z)), en, fr;
+            if (bond_force(r, bsty, bp, en, fr)) atomicCAS(&counters[1], 0, 1);
             f[0] += fr * x / r;
             f[1] += fr * y / r;
             f[2] += fr * z / r;
         }
     }
-    long o = bead_row[g];
-    frc[o] = f[0]; frc[o + 1] = f[1]; frc[o + 2] = f[2];
+    if (lvdw) {
+        for (int j0 = 0; j0 < nbc; j0 += BF_TILE) {
+            const int nt = min(BF_TILE, nbc - j0);
+            __syncthreads();
+            for (int e = threadIdx.x; e < nt; e += BF_ROWS) {
+                sx[e] = cr[3 * (j0 + e)];
+                sy[e] = cr[3 * (j0 + e) + 1];
+                sz[e] = cr[3 * (j0 + e) + 2];
+            }
+            __syncthreads();
+            if (!mine) continue;
+            for (int q = 0; q < nt; q++) {
+                const int j = j0 + q;
+                if (j == i || abs(j - i) <= excl) continue;
+                double x = __dsub_rn(sx[q], xi), y = __dsub_rn(sy[q], yi), z = __dsub_rn(sz[q], zi);
+                double r2 = rsq3(x, y, z);
+                if (!(r2 < rc2)) continue;
+                double r = sqrt(r2), en, fr;
+                lj_force(r, vp, en, fr);
+                f[0] += fr * x / r;
+                f[1] += fr * y / r;
+                f[2] += fr * z / r;
+            }
+        }
+    }
+    if (mine) {
+        double *o = frc + ch[c].row + 3L * i;
+        o[0] = f[0];
+        o[1] = f[1];
+        o[2] = f[2];
+    }
 }
 
 // r += dt * (D F) + diff (diff may be null); one thread per bead.  Nothing
@@ -762,9 +790,9 @@ static int b_step_tail(Batch &b, int k, bool noise)
     const long NP = b.NP;
     hipStream_t sm = b.stream;
     const int nbk = (b.NB + 255) / 256;
-    hipLaunchKernelGGL(b_force_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.NB, b.bead_chain, b.bead_local,
-                       b.bead_row, b.chains, b.bond_style, b.dev_bpar, b.lvdw, b.dev_vpar, b.rcutoff * b.rcutoff,
-                       b.excl, b.frc, b.counters);
+    hipLaunchKernelGGL(b_force_lds_kernel, dim3(b.bf_nitems), dim3(BF_ROWS), 0, sm, b.crd, b.bf_items, b.chains,
+                       b.bond_style, b.dev_bpar, b.lvdw, b.dev_vpar, b.rcutoff * b.rcutoff, b.excl, b.frc,
+                       b.counters);
     BD_TRY(b_product<1>(b, b.frc, b.dfr, nullptr, nullptr, nullptr));
     const double *dcol = noise ? b.diff + (size_t)(k - 1) * NP : nullptr;
     hipLaunchKernelGGL(b_update_kernel, dim3(nbk), dim3(256), 0, sm, b.crd, b.dfr, dcol, b.NB, b.bead_row,
@@ -951,7 +979,16 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
         if (ok) ok = hipMalloc(p, std::max<size_t>(count, 1) * sizeof(int)) == hipSuccess &&
                      hipMemset(*p, 0, std::max<size_t>(count, 1) * sizeof(int)) == hipSuccess;
     };
-    if (!b.mf) {
+    {
+        std::vector<int2> fi;
+        for (int c = 0; c < nchains; c++)
+            for (int rb = 0; rb < (nbeads[c] + BF_ROWS - 1) / BF_ROWS; rb++) fi.push_back(make_int2(c, rb));
+        b.bf_nitems = (int)fi.size();
+        ok = hipMalloc(&b.bf_items, fi.size() * sizeof(int2)) == hipSuccess &&
+             hipMemcpy(b.bf_items, fi.data(), fi.size() * sizeof(int2), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!ok) {
+    } else if (!b.mf) {
         if (opts->precision == BD_FP32) ok = hipMalloc(&b.mob32, elems * sizeof(float)) == hipSuccess;
         else ok = hipMalloc(&b.mob64, elems * sizeof(double)) == hipSuccess;
     } else {
@@ -1033,6 +1070,7 @@ void bd_batch_destroy(bd_batch *h)
     if (b.mf_rec) (void)hipFree(b.mf_rec);
     if (b.mf_bo) (void)hipFree(b.mf_bo);
     if (b.mf_items) (void)hipFree(b.mf_items);
+    if (b.bf_items) (void)hipFree(b.bf_items);
     if (b.bead_row) (void)hipFree(b.bead_row);
     if (b.chains) (void)hipFree(b.chains);
     if (b.st) (void)hipFree(b.st);
