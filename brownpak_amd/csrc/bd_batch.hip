@@ -573,8 +573,9 @@ __global__ void b_mf_rec_kernel(const double *__restrict__ crd, const BChainDev 
 // one workgroup per (chain, 64-bead row block) over the chain's whole column
 // range, so every row is complete in its workgroup (no partial sums).  X and
 // Y are the padded vectors (ld NP); the chain's pad rows of Y are zero.
+constexpr int BMF_WAVES = 4;   // waves per workgroup: less LDS, so that the product shares CUs with the tests
 template <typename T, int S>
-__global__ __launch_bounds__(64 * MF_WAVES) void b_mf_product_kernel(const double *__restrict__ rec,
+__global__ __launch_bounds__(64 * BMF_WAVES) void b_mf_product_kernel(const double *__restrict__ rec,
                                                                      const int *__restrict__ bo,
                                                                      const int2 *__restrict__ items,
                                                                      const BChainDev *__restrict__ ch,
@@ -583,7 +584,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void b_mf_product_kernel(const doubl
                                                                      const int *__restrict__ run)
 {
     using Rec = MfRec<T, S>;
-    __shared__ __align__(16) unsigned char smem[mf_smem_bytes<T, S>()];
+    __shared__ __align__(16) unsigned char smem[mf_smem_bytes<T, S, BMF_WAVES>()];
     const int2 it = items[blockIdx.x];
     const int c = it.x, rb = it.y;
     if (run && !run[c]) return;
@@ -609,8 +610,9 @@ __global__ __launch_bounds__(64 * MF_WAVES) void b_mf_product_kernel(const doubl
 #pragma unroll
             for (int k = 0; k < S; k++) rc.v[a][k] = j < nb ? (T)x[(long)k * NP + 3 * j + a] : (T)0;
     };
-    mf_rows_times_cols<T, S>(xi, yi, zi, 0, nch, reinterpret_cast<Rec *>(smem), w, stage, acc);
-    mf_wave_sums<S>(acc, reinterpret_cast<double(*)[64][3 * S + 1]>(smem), w, lane, [&](int l, int col, double v) {
+    mf_rows_times_cols<T, S, BMF_WAVES>(xi, yi, zi, 0, nch, reinterpret_cast<Rec *>(smem), w, stage, acc);
+    mf_wave_sums<S, BMF_WAVES>(acc, reinterpret_cast<double(*)[64][3 * S + 1]>(smem), w, lane,
+                               [&](int l, int col, double v) {
         const int ib = rb * 64 + l;
         if (ib < nb) y[(long)(col % S) * NP + 3 * ib + col / S] = v;
     });
@@ -649,10 +651,10 @@ static int b_product(Batch &b, const double *x, double *y, const int *active, hi
     }
     if (e0) BD_HIP(hipEventRecord(e0, sm));
     if (b.opt.precision == BD_FP32)
-        hipLaunchKernelGGL((b_mf_product_kernel<float, S>), dim3(b.mf_nitems), dim3(64 * MF_WAVES), 0, sm, b.mf_rec,
+        hipLaunchKernelGGL((b_mf_product_kernel<float, S>), dim3(b.mf_nitems), dim3(64 * BMF_WAVES), 0, sm, b.mf_rec,
                            b.mf_bo, b.mf_items, b.chains, x, y, NP, active);
     else
-        hipLaunchKernelGGL((b_mf_product_kernel<double, S>), dim3(b.mf_nitems), dim3(64 * MF_WAVES), 0, sm,
+        hipLaunchKernelGGL((b_mf_product_kernel<double, S>), dim3(b.mf_nitems), dim3(64 * BMF_WAVES), 0, sm,
                            b.mf_rec, b.mf_bo, b.mf_items, b.chains, x, y, NP, active);
     BD_HIP(hipGetLastError());
     if (e1) BD_HIP(hipEventRecord(e1, sm));

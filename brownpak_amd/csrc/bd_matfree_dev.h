@@ -98,20 +98,21 @@ __device__ __forceinline__ void mf_pair(const MfRec<T, S> &cur, double xi, doubl
 
 // LDS a workgroup of the product needs: the staged records, and after the
 // last tile the waves' sums (the same bytes).
-template <typename T, int S>
+template <typename T, int S, int NW = MF_WAVES>
 constexpr int mf_smem_bytes()
 {
-    constexpr int red = MF_WAVES * 64 * (3 * S + 1) * (int)sizeof(double);
+    constexpr int red = NW * 64 * (3 * S + 1) * (int)sizeof(double);
     constexpr int til = MF_TILE * (int)sizeof(MfRec<T, S>);
     return red > til ? red : til;
 }
 
 // acc += sum over the column chunks [cb, ce) (MF_CH beads each) of the pair
-// terms of this lane's row bead (xi, yi, zi), by the whole workgroup: tile by
-// tile, stage(e, j, rec) fills record e of the tile for column bead j, then
-// wave w takes its share of the tile's chunks.  FP32 terms are summed over
-// each chunk in FP32, then added in FP64.  Every thread must call.
-template <typename T, int S, typename Stage>
+// terms of this lane's row bead (xi, yi, zi), by the whole workgroup of NW
+// waves: tile by tile, stage(e, j, rec) fills record e of the tile for
+// column bead j, then wave w takes its share of the tile's chunks.  FP32
+// terms are summed over each chunk in FP32, then added in FP64.  Every
+// thread must call.
+template <typename T, int S, int NW = MF_WAVES, typename Stage>
 __device__ __forceinline__ void mf_rows_times_cols(double xi, double yi, double zi, int cb, int ce, MfRec<T, S> *tile,
                                                    int w, Stage stage, double (&acc)[3][S])
 {
@@ -119,7 +120,7 @@ __device__ __forceinline__ void mf_rows_times_cols(double xi, double yi, double 
         const int nct = min(ce - t0, MF_TILE / MF_CH), jb = t0 * MF_CH;
         for (int e = threadIdx.x; e < nct * MF_CH; e += blockDim.x) stage(e, jb + e, tile[e]);
         __syncthreads();
-        const int qb = nct * w / MF_WAVES, qe = nct * (w + 1) / MF_WAVES;
+        const int qb = nct * w / NW, qe = nct * (w + 1) / NW;
         if constexpr (sizeof(T) == 8) {
             for (int q = MF_CH * qb; q < MF_CH * qe; q++) mf_pair<T, S>(tile[q], xi, yi, zi, acc);
         } else {
@@ -144,7 +145,7 @@ __device__ __forceinline__ void mf_rows_times_cols(double xi, double yi, double 
 // The waves' sums of acc, added in wave order through red (the staging
 // bytes, free after mf_rows_times_cols), handed to out(l, a S + k, value)
 // for every lane l of the row block.  Every thread must call.
-template <int S, typename Out>
+template <int S, int NW = MF_WAVES, typename Out>
 __device__ __forceinline__ void mf_wave_sums(const double (&acc)[3][S], double (*red)[64][3 * S + 1], int w, int lane,
                                              Out out)
 {
@@ -157,7 +158,7 @@ __device__ __forceinline__ void mf_wave_sums(const double (&acc)[3][S], double (
         const int l = e / (3 * S), col = e % (3 * S);
         double v = red[0][l][col];
 #pragma unroll
-        for (int q = 1; q < MF_WAVES; q++) v += red[q][l][col];
+        for (int q = 1; q < NW; q++) v += red[q][l][col];
         out(l, col, v);
     }
     __syncthreads();
