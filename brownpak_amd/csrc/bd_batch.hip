@@ -1017,7 +1017,7 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
     dal(&b.xi, (size_t)b.NP * s); dal(&b.V, (size_t)b.NP * s * dm); dal(&b.W, (size_t)b.NP * s);
     dal(&b.Y, (size_t)2 * b.NP * s);
     dal(&b.A, (size_t)nchains * dm * s * s); dal(&b.B, (size_t)nchains * dm * s * s);
-    b.scr_stride = (long)dm * (3 * s * s + s) * BNQP;
+    b.scr_stride = sp::scratch_doubles(s, dm, BNQP);
     dal(&b.R0, (size_t)nchains * s * s); dal(&b.scr, (size_t)nchains * b.scr_stride);
     dal(&b.Cc, (size_t)nchains * dm * s * s); dal(&b.dev_bpar, MXPARAM); dal(&b.dev_vpar, MXPARAM);
     ial(&b.active, nchains); ial(&b.run, nchains); ial(&b.failed, nchains); ial(&b.conv, nchains);

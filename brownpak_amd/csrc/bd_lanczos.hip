@@ -1185,7 +1185,7 @@ int lanczos_alloc(Ctx &c)
         return BD_ERR_ARG;
     }
     // small-problem scratch: per block of H, factors of 256 node shifts
-    BD_HIP(hipMalloc(&c.Cq, (size_t)c.dimk_max * (3 * s * s + s) * 256 * sizeof(double)));
+    BD_HIP(hipMalloc(&c.Cq, (size_t)sp::scratch_doubles(s, c.dimk_max, 256) * sizeof(double)));
     BD_HIP(hipMalloc(&c.Ccoef, (size_t)LG * ncol * s * sizeof(double)));
     BD_HIP(hipMalloc(&c.bound_ok, NBOUND * sizeof(int)));
     BD_HIP(hipMalloc(&c.lzpart, (size_t)(c.npad / TILE + 1) * 2 * MAX_S * MAX_S * sizeof(double)));
@@ -1298,9 +1298,8 @@ static int lanczos_t(Ctx &c, const double *xi, double *out, int *lconv, int *ite
     {
         // host-side capacity checks of the scratch every launch below indexes
         const size_t ncol = (size_t)S * dmax;
-        const size_t cq = (size_t)dmax * (3 * S * S + S) * 256;
-        const size_t need_part = grouped ? (size_t)LG * nwg * dmax * S * S
-                                         : (size_t)dmax * (3 * S * S + S) * 256;
+        const size_t cq = (size_t)sp::scratch_doubles(S, dmax, 256);
+        const size_t need_part = grouped ? (size_t)LG * nwg * dmax * S * S : cq;
         const size_t nblk_y = (size_t)(rows + YR - 1) / YR;
         if (need_part > cq || nblk_y * LG * 2 * S > (size_t)(c.npad / YR + 1) * LG * 2 * MAX_S ||
             (size_t)LG * dmax * S * S > (size_t)LG * ncol * S || (size_t)mob_rowblocks(c) > (size_t)(c.npad / TILE + 1) ||

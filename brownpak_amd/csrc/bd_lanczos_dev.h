@@ -278,19 +278,30 @@ __device__ void load_band(const double *Ablk, const double *Bblk, int js, double
 // blocks L_b (lower) and sub-diagonal blocks G_b = B_{b-1} L_{b-1}^{-T}
 // (upper).  One thread owns one shift; its blocks live in registers with
 // compile-time indices, and the blocks the backward sweep needs go to a
-// node-interleaved scratch array (coalesced over the threads of a block).
+// node-interleaved scratch array (coalesced over the threads of a block):
+// only the triangles of L_b and G_b that can be nonzero, and F_b; 1/diag(L_b)
+// is formed again from the stored diagonal (the same correctly rounded
+// division, so the same bits).
 namespace sp {
 
 template <int S>
-constexpr int NE = 3 * S * S + S;   // per block: L (S*S), 1/diag (S), G (S*S), F then w*Z (S*S)
+constexpr int NT = S * (S + 1) / 2;    // entries of a packed S x S triangle
+template <int S>
+constexpr int NE = 2 * NT<S> + S * S;  // per block: L (lower), G (upper), F (full)
 template <int S>
 constexpr int OFF_L = 0;
 template <int S>
-constexpr int OFF_I = S * S;
+constexpr int OFF_G = NT<S>;
 template <int S>
-constexpr int OFF_G = S * S + S;
-template <int S>
-constexpr int OFF_F = 2 * S * S + S;
+constexpr int OFF_F = 2 * NT<S>;
+
+// Doubles of small_problem's scratch for H_j of up to nbm blocks and nqp
+// node slots: the node-interleaved factor blocks, then one partial sum of Z
+// per wave of nodes.
+__host__ __device__ constexpr long scratch_doubles(int s, int nbm, int nqp)
+{
+    return (long)nbm * ((long)(s * (s + 1) + s * s) * nqp + (long)((nqp + 63) / 64) * s * s);
+}
 
 template <int S>
 __device__ inline void load_blk(const double *__restrict__ p, double (&a)[S][S])
@@ -326,7 +337,9 @@ __device__ inline bool chol(double (&a)[S][S], double (&inv)[S])
     return ok;
 }
 
-// G = B Lp^{-T} (B upper triangular, so is G)
+// G = B Lp^{-T}.  B is upper triangular (the R factor of CholeskyQR, whose
+// lower entries are exact zeros), so is G: its upper triangle is formed and
+// its lower triangle set to zero, the value the full recurrence gives there.
 template <int S>
 __device__ inline void gsolve(const double (&B)[S][S], const double (&Lp)[S][S], const double (&ip)[S],
                               double (&G)[S][S])
@@ -334,7 +347,11 @@ __device__ inline void gsolve(const double (&B)[S][S], const double (&Lp)[S][S],
 #pragma unroll
     for (int r = 0; r < S; r++)
 #pragma unroll
-        for (int c = 0; c < S; c++) {
+        for (int c = 0; c < r; c++) G[r][c] = 0.0;
+#pragma unroll
+    for (int r = 0; r < S; r++)
+#pragma unroll
+        for (int c = r; c < S; c++) {
             double t = B[r][c];
 #pragma unroll
             for (int k = 0; k < c; k++) t -= G[r][k] * Lp[c][k];
@@ -384,7 +401,10 @@ __device__ bool factor_ok(const double *__restrict__ A, const double *__restrict
     return true;
 }
 
-// Stores / loads of one block's S x S slot in the node-interleaved scratch.
+// Stores / loads of one block's slot in the node-interleaved scratch: the
+// full S x S block (column-major), the lower triangle with the diagonal, or
+// the upper triangle with the diagonal (packed by columns); get_up sets the
+// lower triangle to zero.
 template <int S>
 __device__ inline void put(double *__restrict__ scr, int nqp, int node, int slot, const double (&a)[S][S])
 {
@@ -401,14 +421,55 @@ __device__ inline void get(const double *__restrict__ scr, int nqp, int node, in
 #pragma unroll
         for (int r = 0; r < S; r++) a[r][c] = scr[(long)(slot + r + c * S) * nqp + node];
 }
+template <int S>
+__device__ inline void put_lo(double *__restrict__ scr, int nqp, int node, int slot, const double (&a)[S][S])
+{
+    int e = slot;
+#pragma unroll
+    for (int c = 0; c < S; c++)
+#pragma unroll
+        for (int r = c; r < S; r++, e++) scr[(long)e * nqp + node] = a[r][c];
+}
+template <int S>
+__device__ inline void get_lo(const double *__restrict__ scr, int nqp, int node, int slot, double (&a)[S][S])
+{
+    int e = slot;
+#pragma unroll
+    for (int c = 0; c < S; c++)
+#pragma unroll
+        for (int r = c; r < S; r++, e++) a[r][c] = scr[(long)e * nqp + node];
+}
+template <int S>
+__device__ inline void put_up(double *__restrict__ scr, int nqp, int node, int slot, const double (&a)[S][S])
+{
+    int e = slot;
+#pragma unroll
+    for (int c = 0; c < S; c++)
+#pragma unroll
+        for (int r = 0; r <= c; r++, e++) scr[(long)e * nqp + node] = a[r][c];
+}
+template <int S>
+__device__ inline void get_up(const double *__restrict__ scr, int nqp, int node, int slot, double (&a)[S][S])
+{
+    int e = slot;
+#pragma unroll
+    for (int c = 0; c < S; c++)
+#pragma unroll
+        for (int r = 0; r < S; r++)
+            if (r <= c) a[r][c] = scr[(long)(e++) * nqp + node];
+            else a[r][c] = 0.0;
+}
 
-// One quadrature node: solves (H + sigma I) Z = E1 R0 and leaves w Z in the
-// F slots of its scratch column.  Returns false if the factorisation fails.
-// PF: load the next block of the backward sweep one step ahead (worth it
-// when the scratch is global memory).
-template <int S, bool PF = true>
+// One quadrature node: solves (H + sigma I) Z = E1 R0 and hands each block
+// Z_b, from the last to the first, to sink(b, Z_b).  The factor blocks go
+// through the scratch column of slot node; wr = false leaves the scratch
+// unwritten (a lane that repeats another lane's node to keep its wave
+// whole).  Returns false if the factorisation fails.  PF: load the next
+// block of the backward sweep one step ahead (worth it when the scratch is
+// global memory).
+template <int S, bool PF = true, typename Sink>
 __device__ bool node_solve(const double *__restrict__ A, const double *__restrict__ B, const double *__restrict__ R0,
-                           int nb, double sigma, double w, double *__restrict__ scr, int nqp, int node)
+                           int nb, double sigma, double *__restrict__ scr, int nqp, int node, bool wr, Sink sink)
 {
     constexpr int E = NE<S>;
     double L[S][S], inv[S], F[S][S];
@@ -431,16 +492,16 @@ __device__ bool node_solve(const double *__restrict__ A, const double *__restric
     }
     // forward sweep: factor and F_b = -L_b^{-1} G_b F_{b-1}
     for (int b = 0;; b++) {
-        put<S>(scr, nqp, node, b * E + OFF_L<S>, L);
-#pragma unroll
-        for (int r = 0; r < S; r++) scr[(long)(b * E + OFF_I<S> + r) * nqp + node] = inv[r];
-        put<S>(scr, nqp, node, b * E + OFF_F<S>, F);
+        if (wr) {
+            put_lo<S>(scr, nqp, node, b * E + OFF_L<S>, L);
+            put<S>(scr, nqp, node, b * E + OFF_F<S>, F);
+        }
         if (b + 1 >= nb) break;
         double Bm[S][S], G[S][S], Ln[S][S], in[S];
         load_blk<S>(B + b * S * S, Bm);
         gsolve<S>(Bm, L, inv, G);
         ok = next_diag<S>(A + (b + 1) * S * S, sigma, G, Ln, in) && ok;
-        put<S>(scr, nqp, node, (b + 1) * E + OFF_G<S>, G);
+        if (wr) put_up<S>(scr, nqp, node, (b + 1) * E + OFF_G<S>, G);
         double Fn[S][S];
 #pragma unroll
         for (int c = 0; c < S; c++)
@@ -468,20 +529,18 @@ __device__ bool node_solve(const double *__restrict__ A, const double *__restric
 #pragma unroll
         for (int c = 0; c < S; c++) { Gn[r][c] = 0.0; Z[r][c] = 0.0; }
     for (int b = nb - 1; b >= 0; b--) {
-        double Gp[S][S], Lp[S][S], ip[S], Fp[S][S];
-        if (!PF && b < nb - 1) {   // block b now (G_{b+1}, L_b, 1/diag, F_b)
-            get<S>(scr, nqp, node, (b + 1) * E + OFF_G<S>, Gn);
-            get<S>(scr, nqp, node, b * E + OFF_L<S>, L);
+        double Gp[S][S], Lp[S][S], Fp[S][S];
+        if (!PF && b < nb - 1) {   // block b now (G_{b+1}, L_b, F_b)
+            get_up<S>(scr, nqp, node, (b + 1) * E + OFF_G<S>, Gn);
+            get_lo<S>(scr, nqp, node, b * E + OFF_L<S>, L);
             get<S>(scr, nqp, node, b * E + OFF_F<S>, F);
 #pragma unroll
-            for (int r = 0; r < S; r++) inv[r] = scr[(long)(b * E + OFF_I<S> + r) * nqp + node];
+            for (int r = 0; r < S; r++) inv[r] = 1.0 / L[r][r];
         }
         if (PF && b > 0) {  // prefetch block b-1 while block b is solved
-            get<S>(scr, nqp, node, b * E + OFF_G<S>, Gp);
-            get<S>(scr, nqp, node, (b - 1) * E + OFF_L<S>, Lp);
+            get_up<S>(scr, nqp, node, b * E + OFF_G<S>, Gp);
+            get_lo<S>(scr, nqp, node, (b - 1) * E + OFF_L<S>, Lp);
             get<S>(scr, nqp, node, (b - 1) * E + OFF_F<S>, Fp);
-#pragma unroll
-            for (int r = 0; r < S; r++) ip[r] = scr[(long)((b - 1) * E + OFF_I<S> + r) * nqp + node];
         }
         double Zn[S][S];
 #pragma unroll
@@ -495,23 +554,61 @@ __device__ bool node_solve(const double *__restrict__ A, const double *__restric
                 for (int k = ii + 1; k < S; k++) t -= L[k][ii] * Zn[k][c];
                 Zn[ii][c] = t * inv[ii];
             }
-        double wz[S][S];
 #pragma unroll
         for (int r = 0; r < S; r++)
 #pragma unroll
-            for (int c = 0; c < S; c++) { Z[r][c] = Zn[r][c]; wz[r][c] = w * Zn[r][c]; }
-        put<S>(scr, nqp, node, b * E + OFF_F<S>, wz);
+            for (int c = 0; c < S; c++) Z[r][c] = Zn[r][c];
+        sink(b, Zn);
         if (b == 0) break;
         if (PF) {
 #pragma unroll
             for (int r = 0; r < S; r++) {
-                inv[r] = ip[r];
 #pragma unroll
                 for (int c = 0; c < S; c++) { L[r][c] = Lp[r][c]; F[r][c] = Fp[r][c]; Gn[r][c] = Gp[r][c]; }
+                inv[r] = 1.0 / Lp[r][r];
             }
         }
     }
     return ok;
+}
+
+// Sum over the 64 lanes of a wave of each entry of the S x S block v, by
+// recursive halving: at the step with lane mask m a lane keeps half of its
+// entries (the upper half where lane & m) plus the partner's copies of
+// them; once one entry per lane is left, the remaining lane bits are summed
+// by butterfly.  Lane l returns the wave's total of entry
+// e = l >> (6 - log2 P) (entry index r + c S, P = S*S rounded up to a power
+// of two; e >= S*S are padding), the same value in all 64/P lanes of its
+// group.  The order of the additions is fixed.  Every lane must call.
+template <int S>
+constexpr int wave_sum_pad()
+{
+    int p = 1;
+    while (p < S * S) p *= 2;
+    return p;
+}
+template <int S>
+__device__ inline double wave_block_sum(const double (&v)[S][S], int lane, int &entry)
+{
+    constexpr int P = wave_sum_pad<S>();
+    constexpr int LOGP = P == 1 ? 0 : (P == 2 ? 1 : (P == 4 ? 2 : (P == 8 ? 3 : (P == 16 ? 4 : (P == 32 ? 5 : 6)))));
+    double x[P];
+#pragma unroll
+    for (int e = 0; e < P; e++) x[e] = e < S * S ? v[e % S][e / S] : 0.0;
+#pragma unroll
+    for (int st = 0; st < LOGP; st++) {
+        const int n = P >> st, m = 32 >> st;
+        const bool hi = (lane & m) != 0;
+#pragma unroll
+        for (int i = 0; i < n / 2; i++) {
+            const double send = hi ? x[i] : x[i + n / 2], keep = hi ? x[i + n / 2] : x[i];
+            x[i] = keep + __shfl_xor(send, m, 64);
+        }
+    }
+#pragma unroll
+    for (int m = 32 >> LOGP; m > 0; m >>= 1) x[0] += __shfl_xor(x[0], m, 64);
+    entry = lane >> (6 - LOGP);
+    return x[0];
 }
 
 // Gershgorin bound of H_j by the whole block (any blockDim multiple of 64).
@@ -548,7 +645,8 @@ struct SmallOut {
 //   H^{-1/2} X = (2/pi) int_0^inf (H + t^2 I)^{-1} X dt,  t = e^u,
 //   u = u0 + sinh(v), u0 = (ln m + ln M) / 4, nodes v = k h, |k| <= N,
 // one thread per node; Z = sum of w Z_k; C = H Z (so y = V C = V sqrt(H) E1 R0).
-// C is written to Cout (js x S, ld js).  scr holds nb * NE<S> * nqp doubles.
+// C is written to Cout (js x S, ld js).  scr holds scratch_doubles(S, nb, nqp)
+// doubles; nq <= nqp and nq <= blockDim.x (a multiple of 64).
 template <int S>
 __device__ SmallOut small_problem(const double *__restrict__ A, const double *__restrict__ B,
                                   const double *__restrict__ R0, int nb, int nbound, int dec_max,
@@ -580,21 +678,36 @@ __device__ SmallOut small_problem(const double *__restrict__ A, const double *__
     const int nq = 2 * N + 1;
     out.nq = nq;
     if (nq > nqp || nq > (int)blockDim.x) { out.err = 3; return out; }
-    const int node = threadIdx.x;
-    if (node < nq) {
+    // one node per thread, whole waves: the lanes of the last wave past the
+    // last node repeat it (no scratch writes) and add zeros to the wave sums
+    const int node = threadIdx.x, wv = node / 64, lane = node % 64, nw = (nq + 63) / 64;
+    double *wpart = scr + (long)nb * E * nqp;    // wave partial sums of Z, nw x (js S)
+    if (wv < nw) {
+        const int nd = min(node, nq - 1);
         const double u0 = 0.25 * (log(m) + log(M));
-        const double v = (node - N) * h;
+        const double v = (nd - N) * h;
         const double u = u0 + sinh(v);
-        const double w = (2.0 / M_PI) * h * cosh(v) * exp(u);
-        if (!node_solve<S>(A, B, R0, nb, exp(2.0 * u), w, scr, nqp, node)) s_fail = 1;
+        const double w = node < nq ? (2.0 / M_PI) * h * cosh(v) * exp(u) : 0.0;
+        constexpr int GRP = 64 / wave_sum_pad<S>();
+        auto sink = [&](int b, const double (&z)[S][S]) {
+            double wz[S][S];
+#pragma unroll
+            for (int r = 0; r < S; r++)
+#pragma unroll
+                for (int c = 0; c < S; c++) wz[r][c] = w * z[r][c];
+            int e;
+            const double t = wave_block_sum<S>(wz, lane, e);
+            if (lane % GRP == 0 && e < S * S) wpart[(long)wv * js * S + b * S * S + e] = t;
+        };
+        if (!node_solve<S>(A, B, R0, nb, exp(2.0 * u), scr, nqp, nd, node < nq, sink)) s_fail = 1;
     }
     __syncthreads();
     if (s_fail) { out.err = 2; return out; }
-    // Z = sum over nodes (fixed order), into LDS
+    // Z = sum of the wave partials (wave order), into LDS
     for (int e = threadIdx.x; e < js * S; e += blockDim.x) {
+        double t = 0.0;
+        for (int q = 0; q < nw; q++) t += wpart[(long)q * js * S + e];
         const int b = e / (S * S), rc = e % (S * S);     // rc = r + c S inside block b
-        const double *p = scr + (long)(b * E + OFF_F<S> + rc) * nqp;
-        const double t = sum_strided(p, 1, nq);
         const int r = rc % S, c = rc / S;
         s_Z[(b * S + r) + c * js] = t;
     }
@@ -713,7 +826,15 @@ __device__ bool small_problem_split(const double *__restrict__ Ag, const double 
             const double v = (node - N) * h;
             const double u = u0 + sinh(v);
             const double w = (2.0 / M_PI) * h * cosh(v) * exp(u);
-            if (!node_solve<S, false>(sA, sB, sR, nb, exp(2.0 * u), w, sF, NPB, lane)) s_fail = 1;
+            auto sink = [&](int b, const double (&z)[S][S]) {
+                double wz[S][S];
+#pragma unroll
+                for (int r = 0; r < S; r++)
+#pragma unroll
+                    for (int c = 0; c < S; c++) wz[r][c] = w * z[r][c];
+                put<S>(sF, NPB, lane, b * E + OFF_F<S>, wz);   // over F_b, read no more
+            };
+            if (!node_solve<S, false>(sA, sB, sR, nb, exp(2.0 * u), sF, NPB, lane, true, sink)) s_fail = 1;
         }
     };
     if (wv == 0) {
