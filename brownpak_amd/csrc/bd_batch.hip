@@ -71,6 +71,11 @@ struct Batch {
     // convergence test); run: its Krylov recurrence still advances (also
     // cleared on a breakdown); failed: iteration whose CholeskyQR broke down
     int *active = nullptr, *run = nullptr, *failed = nullptr, *conv = nullptr;
+    int *kb_hint = nullptr;       // per chain: probe index of the small problem's last iteration
+    int *ready = nullptr;         // per chain: last iteration whose small problem left C
+    int *ycnt = nullptr;          // per chain: row blocks of the y combination finished (0 between launches)
+    double *ypart = nullptr;      // per chain and row block: column sums of the convergence test
+    int yrb = 1;                  // row blocks of the largest chain
     int *counters = nullptr;      // [0] active chains, [1] error code, [2] iterations, [3] not converged
     int *host_counters = nullptr; // pinned, 2 slots x 2
     DevStatus *st = nullptr;
@@ -315,16 +320,14 @@ __device__ inline bool b_live(const int *active, const int *failed, int c, int j
 }
 
 // Per chain (one block): the small problem (sp::small_problem: bounds,
-// quadrature, C = sqrt(H_j) E1 R0), y_j = V C, the relative change against
-// y_{j-1} and the convergence test.
+// quadrature, C = sqrt(H_j) E1 R0 into Cbuf); ready[c] = j once C is there.
 template <int S>
 __global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const double *B, const double *R0, int dmax,
                                                      int j, double *scr, long scr_stride, double *Cbuf,
-                                                     const double *V, long NP, const BChainDev *ch, double *Y,
-                                                     int *active, int *run, const int *failed, int *conv, double tol,
+                                                     int *active, const int *failed, int *kb_hint, int *ready,
                                                      int *counters)
 {
-    const int c = blockIdx.x, js = j * S;
+    const int c = blockIdx.x;
     __shared__ int s_state;
     if (threadIdx.x == 0) s_state = !active[c] ? 0 : (b_live(active, failed, c, j) ? 1 : 2);
     __syncthreads();
@@ -341,35 +344,54 @@ __global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const doub
     const double *Ab = A + (long)c * dmax * S * S, *Bb = B + (long)c * dmax * S * S;
     double *Cc = Cbuf + (long)c * dmax * S * S;
     const sp::SmallOut o =
-        sp::small_problem<S>(Ab, Bb, R0 + (long)c * S * S, j, BNBOUND, BDEC, scr + c * scr_stride, BNQP, Cc);
-    if (o.err) {
-        if (threadIdx.x == 0) atomicCAS(&counters[1], 0, o.err);
-        return;
-    }
-    __syncthreads();
-    // C of this chain into shared memory (js S <= BJS S values)
-    __shared__ double sC[BJS * S];
-    for (int e = threadIdx.x; e < js * S; e += BT) sC[e] = Cc[e];
+        sp::small_problem<S>(Ab, Bb, R0 + (long)c * S * S, j, BNBOUND, BDEC, scr + c * scr_stride, BNQP, Cc, kb_hint + c);
+    if (threadIdx.x != 0) return;
+    if (o.err) atomicCAS(&counters[1], 0, o.err);
+    else ready[c] = j;
+}
+
+// y_j = V_{1:j} C of every chain whose small problem of iteration j is
+// done (ready[c] == j), one row per thread and YR rows per workgroup
+// (grid: chains x row blocks), the basis columns added in order; from j = 3
+// on, the convergence test: per column the sums over the chain's rows of
+// |y_j - y_{j-1}| and |y_{j-1}| (squares for s = 1), by each workgroup in a
+// fixed order, then by the chain's last workgroup to finish over the
+// workgroups in row-block order; the chain stops when max_k sum|d| /
+// max_k sum|y| (the 2-norm ratio for s = 1) is <= tol.
+constexpr int YR = 256;
+template <int S>
+__global__ __launch_bounds__(YR) void b_ycomb_kernel(const double *Cbuf, int dmax, int j, const double *V, long NP,
+                                                     const BChainDev *ch, double *Y, const int *ready, int *active,
+                                                     int *run, int *conv, double *ypart, int yrb, int *ycnt,
+                                                     double tol, int *counters)
+{
+    const int c = blockIdx.x, rb = blockIdx.y, js = j * S;
+    if (ready[c] != j) return;
+    const int rows = ch[c].npad, nrb = (rows + YR - 1) / YR;
+    if (rb >= nrb) return;
+    __shared__ double sC[BJS * S];      // js S <= BJS S values
+    const double *Cc = Cbuf + (long)c * dmax * S * S;
+    for (int e = threadIdx.x; e < js * S; e += YR) sC[e] = Cc[e];
     __syncthreads();
     const long r0 = ch[c].row;
-    const int rows = ch[c].npad;
+    const int r = rb * YR + threadIdx.x;
     double *y = Y + (long)(j % 2) * S * NP + r0;
     const double *yo = Y + (long)((j + 1) % 2) * S * NP + r0;
     const bool want = j > 2;
     double dn[S], on[S];
 #pragma unroll
     for (int k = 0; k < S; k++) { dn[k] = 0.0; on[k] = 0.0; }
-    for (int r = threadIdx.x; r < rows; r += BT) {
+    if (r < rows) {
         double acc[S];
 #pragma unroll
         for (int k = 0; k < S; k++) acc[k] = 0.0;
-        // sixteen basis columns in flight, added in column order
-        for (int m0 = 0; m0 < js; m0 += 16) {
-            double vm[16];
+        // thirty-two basis columns in flight, added in column order
+        for (int m0 = 0; m0 < js; m0 += 32) {
+            double vm[32];
 #pragma unroll
-            for (int q = 0; q < 16; q++) vm[q] = V[(long)min(m0 + q, js - 1) * NP + r0 + r];
+            for (int q = 0; q < 32; q++) vm[q] = V[(long)min(m0 + q, js - 1) * NP + r0 + r];
 #pragma unroll
-            for (int q = 0; q < 16; q++)
+            for (int q = 0; q < 32; q++)
                 if (m0 + q < js) {
 #pragma unroll
                     for (int k = 0; k < S; k++) acc[k] += vm[q] * sC[(m0 + q) + k * js];
@@ -379,15 +401,15 @@ __global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const doub
         for (int k = 0; k < S; k++) {
             y[(long)k * NP + r] = acc[k];
             if (want) {
-                double o = yo[(long)k * NP + r], d = acc[k] - o;
-                if (S == 1) { dn[k] += d * d; on[k] += o * o; }
-                else { dn[k] += fabs(d); on[k] += fabs(o); }
+                const double o = yo[(long)k * NP + r], d = acc[k] - o;
+                if (S == 1) { dn[k] = d * d; on[k] = o * o; }
+                else { dn[k] = fabs(d); on[k] = fabs(o); }
             }
         }
     }
     if (!want) return;
-    double tot[2 * S];
-    __shared__ double sh[BT / 64][2 * S];
+    __shared__ double sh[YR / 64][2 * S];
+    __shared__ int s_last;
     const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
 #pragma unroll
     for (int k = 0; k < 2 * S; k++) {
@@ -396,12 +418,23 @@ __global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const doub
         if (lane == 0) sh[wv][k] = v;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int k = 0; k < 2 * S; k++) {
-        double v = 0.0;
-        for (int w = 0; w < BT / 64; w++) v += sh[w][k];
-        tot[k] = v;
-    }
+    double *pc = ypart + (long)c * yrb * 2 * S;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 2 * S; k++) {
+            double v = 0.0;
+            for (int w = 0; w < YR / 64; w++) v += sh[w][k];
+            store_dev(pc + rb * 2 * S + k, v);
+        }
+    stores_complete();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = (atomicAdd(&ycnt[c], 1) == nrb - 1) ? 1 : 0;
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    ycnt[c] = 0;
+    double tot[2 * S];
+    for (int k = 0; k < 2 * S; k++) tot[k] = 0.0;
+    for (int q = 0; q < nrb; q++)
+        for (int k = 0; k < 2 * S; k++) tot[k] += load_dev(pc + q * 2 * S + k);
     double num = 0.0, den = 0.0;
     if (S == 1) {
         num = sqrt(tot[0]);
@@ -532,7 +565,8 @@ __global__ void b_scale_copy_kernel(double *dst, const double *src, long n, doub
     if (i < n) dst[i] = a * src[i];
 }
 
-__global__ void b_lz_init_kernel(int *active, int *run, int *failed, int *conv, int C, int *counters)
+__global__ void b_lz_init_kernel(int *active, int *run, int *failed, int *conv, int *kb_hint, int *ready, int C,
+                                 int *counters)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < C) {
@@ -540,6 +574,8 @@ __global__ void b_lz_init_kernel(int *active, int *run, int *failed, int *conv, 
         run[c] = 1;
         failed[c] = 0;
         conv[c] = 0;
+        kb_hint[c] = -1;
+        ready[c] = 0;
     }
     if (c == 0) counters[0] = C;
 }
@@ -683,8 +719,8 @@ static int b_lanczos(Batch &b, int *iters_out, bool capture = false)
     hipStream_t sm = b.stream, ss = b.side;
     const int C = b.C, dmax = b.dimk_max;
     const long NP = b.NP;
-    hipLaunchKernelGGL(b_lz_init_kernel, dim3((C + 255) / 256), dim3(256), 0, sm, b.active, b.run, b.failed, b.conv, C,
-                       b.counters);
+    hipLaunchKernelGGL(b_lz_init_kernel, dim3((C + 255) / 256), dim3(256), 0, sm, b.active, b.run, b.failed, b.conv,
+                       b.kb_hint, b.ready, C, b.counters);
     hipLaunchKernelGGL((b_qr0_kernel<S>), dim3(C), dim3(BT), 0, sm, b.xi, b.V, NP, b.chains, b.R0, b.counters);
     BD_HIP(hipGetLastError());
     int pending = -1, slot = 0, jlast = 0;
@@ -714,8 +750,10 @@ static int b_lanczos(Batch &b, int *iters_out, bool capture = false)
             BD_HIP(hipEventRecord(ev, sm));
             BD_HIP(hipStreamWaitEvent(ss, ev, 0));
             hipLaunchKernelGGL((b_small_kernel<S>), dim3(C), dim3(BT), 0, ss, b.A, b.B, b.R0, dmax, j, b.scr,
-                               b.scr_stride, b.Cc, b.V, NP, b.chains, b.Y, b.active, b.run, b.failed, b.conv,
-                               b.opt.lanc_tol, b.counters);
+                               b.scr_stride, b.Cc, b.active, b.failed, b.kb_hint, b.ready, b.counters);
+            hipLaunchKernelGGL((b_ycomb_kernel<S>), dim3(C, b.yrb), dim3(YR), 0, ss, b.Cc, dmax, j, b.V, NP, b.chains,
+                               b.Y, b.ready, b.active, b.run, b.conv, b.ypart, b.yrb, b.ycnt, b.opt.lanc_tol,
+                               b.counters);
             BD_HIP(hipGetLastError());
         }
         if (!capture && (j % 4 == 0 || j == dmax)) {
@@ -1021,6 +1059,13 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
     dal(&b.R0, (size_t)nchains * s * s); dal(&b.scr, (size_t)nchains * b.scr_stride);
     dal(&b.Cc, (size_t)nchains * dm * s * s); dal(&b.dev_bpar, MXPARAM); dal(&b.dev_vpar, MXPARAM);
     ial(&b.active, nchains); ial(&b.run, nchains); ial(&b.failed, nchains); ial(&b.conv, nchains);
+    ial(&b.kb_hint, nchains); ial(&b.ready, nchains); ial(&b.ycnt, nchains);
+    {
+        int mx = 0;
+        for (int c = 0; c < nchains; c++) mx = std::max(mx, ((3 * nbeads[c] + TILE - 1) / TILE) * TILE);
+        b.yrb = (mx + YR - 1) / YR;
+    }
+    dal(&b.ypart, (size_t)nchains * b.yrb * 2 * s);
     ial(&b.counters, 4); ial(&b.bead_chain, b.NB); ial(&b.bead_local, b.NB);
     dal(&b.crd_snap, b.NP);
     if (ok) ok = hipMalloc(&b.dev_rng, 2 * sizeof(uint64_t)) == hipSuccess &&
@@ -1062,10 +1107,11 @@ void bd_batch_destroy(bd_batch *h)
     if (b.dev_rng) (void)hipFree(b.dev_rng);
     plan_free(b.plan);
     double **dp[] = {&b.crd, &b.frc, &b.dfr, &b.diff, &b.xi, &b.V, &b.W, &b.Y, &b.A, &b.B, &b.R0, &b.scr,
-                     &b.Cc, &b.dev_bpar, &b.dev_vpar, &b.mob64};
+                     &b.Cc, &b.ypart, &b.dev_bpar, &b.dev_vpar, &b.mob64};
     for (auto p : dp)
         if (*p) (void)hipFree(*p);
-    int **ip[] = {&b.active, &b.run, &b.failed, &b.conv, &b.counters, &b.bead_chain, &b.bead_local};
+    int **ip[] = {&b.active, &b.run, &b.failed, &b.conv, &b.kb_hint, &b.ready, &b.ycnt, &b.counters, &b.bead_chain,
+                  &b.bead_local};
     for (auto p : ip)
         if (*p) (void)hipFree(*p);
     if (b.mob32) (void)hipFree(b.mob32);

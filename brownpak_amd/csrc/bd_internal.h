@@ -31,6 +31,19 @@ constexpr int NQUAD_MAX = 160;    // quadrature nodes for H^{-1/2}
 constexpr int NBOUND = 48;        // Cholesky probes for the lower spectral bound
 constexpr int LZ_NEV = 8;         // events handing Lanczos iterations to the side stream
 
+// 1/sqrt(v) in FP64 for normal v > 0: v_rsq_f64 (relative error ~6e-8) and
+// one third-order correction,
+//   1/sqrt(v) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + ...),  e = 1 - v y^2,
+// truncated after e^2 (~1e-22), so within about one ulp after rounding: four
+// dependent operations after the estimate, against a square root and a
+// division of about thirty.
+__device__ inline double rsqrt_f64(double v)
+{
+    const double y = __builtin_amdgcn_rsq(v);
+    const double e = fma(-(v * y), y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
+}
+
 void set_error(const std::string &msg);
 
 #define BD_HIP(call)                                                              \

@@ -279,9 +279,8 @@ __device__ void load_band(const double *Ablk, const double *Bblk, int js, double
 // (upper).  One thread owns one shift; its blocks live in registers with
 // compile-time indices, and the blocks the backward sweep needs go to a
 // node-interleaved scratch array (coalesced over the threads of a block):
-// only the triangles of L_b and G_b that can be nonzero, and F_b; 1/diag(L_b)
-// is formed again from the stored diagonal (the same correctly rounded
-// division, so the same bits).
+// only the triangles of L_b and G_b that can be nonzero, and F_b; the
+// diagonal slots of L_b hold 1/diag(L_b).
 namespace sp {
 
 template <int S>
@@ -312,7 +311,9 @@ __device__ inline void load_blk(const double *__restrict__ p, double (&a)[S][S])
         for (int r = 0; r < S; r++) a[r][c] = p[r + c * S];
 }
 
-// In-place Cholesky of the lower triangle of a; inv = 1 / diag(L).
+// In-place Cholesky of the lower triangle of a: the strictly lower part of
+// L below the diagonal, and inv = 1 / diag(L), which is also left on the
+// diagonal of a (diag(L) itself is never needed: every use divides by it).
 template <int S>
 __device__ inline bool chol(double (&a)[S][S], double (&inv)[S])
 {
@@ -323,8 +324,8 @@ __device__ inline bool chol(double (&a)[S][S], double (&inv)[S])
 #pragma unroll
         for (int k = 0; k < c; k++) d -= a[c][k] * a[c][k];
         ok = ok && (d > 0.0);
-        const double l = sqrt(fmax(d, 1e-300)), r = 1.0 / l;
-        a[c][c] = l;
+        const double r = rsqrt_f64(fmax(d, 1e-300));
+        a[c][c] = r;
         inv[c] = r;
 #pragma unroll
         for (int i = c + 1; i < S; i++) {
@@ -535,7 +536,7 @@ __device__ bool node_solve(const double *__restrict__ A, const double *__restric
             get_lo<S>(scr, nqp, node, b * E + OFF_L<S>, L);
             get<S>(scr, nqp, node, b * E + OFF_F<S>, F);
 #pragma unroll
-            for (int r = 0; r < S; r++) inv[r] = 1.0 / L[r][r];
+            for (int r = 0; r < S; r++) inv[r] = L[r][r];
         }
         if (PF && b > 0) {  // prefetch block b-1 while block b is solved
             get_up<S>(scr, nqp, node, b * E + OFF_G<S>, Gp);
@@ -565,7 +566,7 @@ __device__ bool node_solve(const double *__restrict__ A, const double *__restric
             for (int r = 0; r < S; r++) {
 #pragma unroll
                 for (int c = 0; c < S; c++) { L[r][c] = Lp[r][c]; F[r][c] = Fp[r][c]; Gn[r][c] = Gp[r][c]; }
-                inv[r] = 1.0 / Lp[r][r];
+                inv[r] = Lp[r][r];
             }
         }
     }
@@ -647,42 +648,58 @@ struct SmallOut {
 // one thread per node; Z = sum of w Z_k; C = H Z (so y = V C = V sqrt(H) E1 R0).
 // C is written to Cout (js x S, ld js).  scr holds scratch_doubles(S, nb, nqp)
 // doubles; nq <= nqp and nq <= blockDim.x (a multiple of 64).
+//
+// hint (optional, one int kept between the iterations of one solve, -1 at
+// its start): the probe index the previous iteration found.  With a hint the
+// last wave tests the probes while the other waves solve the nodes placed
+// from the hint plus one halving of margin (the smallest Ritz value only
+// decreases with j); a placement whose range [m, M] contains the spectrum
+// is as accurate as the tight one, so the nodes are solved again only when
+// the probes find the spectrum below the placed range.
 template <int S>
-__device__ SmallOut small_problem(const double *__restrict__ A, const double *__restrict__ B,
-                                  const double *__restrict__ R0, int nb, int nbound, int dec_max,
-                                  double *__restrict__ scr, int nqp, double *__restrict__ Cout)
+__device__ SmallOut small_problem(const double *__restrict__ Ag, const double *__restrict__ Bg,
+                                  const double *__restrict__ R0g, int nb, int nbound, int dec_max,
+                                  double *__restrict__ scr, int nqp, double *__restrict__ Cout,
+                                  int *hint = nullptr)
 {
     constexpr int E = NE<S>;
-    const int js = nb * S;
+    const int js = nb * S, nwv = blockDim.x / 64;
     __shared__ double s_red[16];
     __shared__ int s_ok[64];
     __shared__ int s_fail;
     __shared__ double s_Z[512 * S];      // js <= 512
+    // the blocks of H_j and R0 in LDS: the sequential block sweeps below
+    // then wait on LDS, not on global memory, at every block
+    __shared__ double sA[512 * S], sB[512 * S], sR[S * S];   // nb S^2 <= 512 S
     SmallOut out{0, 0, 0.0, 0.0};
+    const int hv = hint ? *hint : -1;    // read by every thread before the first barrier
+    for (int e = threadIdx.x; e < nb * S * S; e += blockDim.x) {
+        sA[e] = Ag[e];
+        sB[e] = Bg[e];
+    }
+    if ((int)threadIdx.x < S * S) sR[threadIdx.x] = R0g[threadIdx.x];
+    __syncthreads();
+    const double *A = sA, *B = sB, *R0 = sR;
     const double M = gersh_block<S>(A, B, js, s_red);
     out.lam_hi = M;
-    if (threadIdx.x == 0) s_fail = 0;
-    if ((int)threadIdx.x < nbound) s_ok[threadIdx.x] = factor_ok<S>(A, B, nb, -ldexp(M, -(int)(threadIdx.x + 1))) ? 1 : 0;
-    __syncthreads();
-    int kb = -1;
-    for (int b = 0; b < nbound; b++)
-        if (s_ok[b]) { kb = b; break; }
-    if (kb < 0) { out.err = 2; return out; }
-    const double m = ldexp(M, -(kb + 1));
-    out.lam_lo = m;
-    int dec = (int)ceil(log10(M / m));
-    if (dec > dec_max) { out.err = 3; return out; }
-    if (dec < 1) dec = 1;
-    const int N = c_qN[dec];
-    const double h = c_qh[dec];
-    const int nq = 2 * N + 1;
-    out.nq = nq;
-    if (nq > nqp || nq > (int)blockDim.x) { out.err = 3; return out; }
+    // node placement of probe index kb: (m, N, h); returns dec (> dec_max:
+    // beyond the table)
+    auto place = [&](int kb, double &m, int &N, double &h) {
+        m = ldexp(M, -(kb + 1));
+        int dec = (int)ceil(log10(M / m));
+        if (dec < 1) dec = 1;
+        if (dec > dec_max) return dec_max + 1;
+        N = c_qN[dec];
+        h = c_qh[dec];
+        return dec;
+    };
     // one node per thread, whole waves: the lanes of the last wave past the
     // last node repeat it (no scratch writes) and add zeros to the wave sums
-    const int node = threadIdx.x, wv = node / 64, lane = node % 64, nw = (nq + 63) / 64;
-    double *wpart = scr + (long)nb * E * nqp;    // wave partial sums of Z, nw x (js S)
-    if (wv < nw) {
+    const int node = threadIdx.x, wv = node / 64, lane = node % 64;
+    double *wpart = scr + (long)nb * E * nqp;    // wave partial sums of Z, (waves) x (js S)
+    auto solve_nodes = [&](double m, int N, double h) {
+        const int nq = 2 * N + 1, nw = (nq + 63) / 64;
+        if (wv >= nw) return;
         const int nd = min(node, nq - 1);
         const double u0 = 0.25 * (log(m) + log(M));
         const double v = (nd - N) * h;
@@ -700,8 +717,49 @@ __device__ SmallOut small_problem(const double *__restrict__ A, const double *__
             if (lane % GRP == 0 && e < S * S) wpart[(long)wv * js * S + b * S * S + e] = t;
         };
         if (!node_solve<S>(A, B, R0, nb, exp(2.0 * u), scr, nqp, nd, node < nq, sink)) s_fail = 1;
+    };
+    auto fits = [&](int N) { return 2 * N + 1 <= nqp && 2 * N + 1 <= (int)blockDim.x; };
+    // the guessed placement, solved beside the probes when it leaves the
+    // last wave free
+    int kb_guess = -1, N_g = 0;
+    double m_g = 0.0, h_g = 0.0;
+    if (hint && nbound <= 64 && nwv >= 2) {
+        kb_guess = hv >= 0 ? min(hv + 1, nbound - 1) : 0;
+        if (place(kb_guess, m_g, N_g, h_g) > dec_max || 2 * N_g + 1 > 64 * (nwv - 1) || !fits(N_g)) kb_guess = -1;
+    }
+    if (threadIdx.x == 0) s_fail = 0;
+    __syncthreads();
+    if (kb_guess >= 0) {
+        if (wv == nwv - 1) {
+            if (lane < nbound) s_ok[lane] = factor_ok<S>(A, B, nb, -ldexp(M, -(lane + 1))) ? 1 : 0;
+        } else {
+            solve_nodes(m_g, N_g, h_g);
+        }
+    } else if ((int)threadIdx.x < nbound) {
+        s_ok[threadIdx.x] = factor_ok<S>(A, B, nb, -ldexp(M, -(int)(threadIdx.x + 1))) ? 1 : 0;
     }
     __syncthreads();
+    int kb = -1;
+    for (int b = 0; b < nbound; b++)
+        if (s_ok[b]) { kb = b; break; }
+    if (kb < 0) { out.err = 2; return out; }
+    if (hint && threadIdx.x == 0) *hint = kb;
+    // the guess stands when its range reaches below the bound
+    const int use = (kb_guess >= 0 && kb <= kb_guess) ? kb_guess : kb;
+    double m, h;
+    int N;
+    if (place(use, m, N, h) > dec_max) { out.err = 3; return out; }
+    const int nq = 2 * N + 1;
+    out.lam_lo = m;
+    out.nq = nq;
+    if (!fits(N)) { out.err = 3; return out; }
+    if (use != kb_guess) {
+        if (threadIdx.x == 0) s_fail = 0;
+        __syncthreads();
+        solve_nodes(m, N, h);
+        __syncthreads();
+    }
+    const int nw = (nq + 63) / 64;
     if (s_fail) { out.err = 2; return out; }
     // Z = sum of the wave partials (wave order), into LDS
     for (int e = threadIdx.x; e < js * S; e += blockDim.x) {

@@ -34,12 +34,7 @@ __device__ inline float mf_rsqrt<float>(float v)
 template <>
 __device__ inline double mf_rsqrt<double>(double v)
 {
-    // v_rsq_f64 (relative error ~6e-8) and one third-order correction:
-    // 1/sqrt(v) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + ...), e = 1 - v y^2,
-    // truncated after e^2 (~1e-22), about one ulp after rounding
-    const double y = __builtin_amdgcn_rsq(v);
-    const double e = fma(-(v * y), y, 1.0);
-    return fma(y * e, fma(0.375, e, 0.5), y);
+    return rsqrt_f64(v);
 }
 
 // r^2 is offset by this much before 1/sqrt: a bead's pair with itself then
