@@ -363,6 +363,8 @@ def main():
                     help="end with a 256 MiB device copy (counter calibration of profiled runs)")
     ap.add_argument("--ensemble-steps", type=int, default=20,
                     help="steps of the 512-chain ensemble reported beside the main line (0: skip)")
+    ap.add_argument("--ensemble-storage", choices=["both", "matfree", "tiles"], default="both",
+                    help="with --workload ensemble: the mobility forms timed")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the N=100 Cholesky and 20k-bead periodic figures")
     ap.add_argument("--workload", choices=["chain", "ensemble", "c1", "c4"], default="chain",
@@ -379,9 +381,13 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.workload == "ensemble":
-        ens = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK)
-        tiles = time_ensemble(max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK), S_BLOCK, storage="tiles")
-        print(json.dumps({"ensemble": ens, "ensemble_tiles": tiles}), flush=True)
+        k = max(S_BLOCK, args.ensemble_steps // S_BLOCK * S_BLOCK)
+        res = {}
+        if args.ensemble_storage in ("both", "matfree"):
+            res["ensemble"] = time_ensemble(k, S_BLOCK)
+        if args.ensemble_storage in ("both", "tiles"):
+            res["ensemble_tiles"] = time_ensemble(k, S_BLOCK, storage="tiles")
+        print(json.dumps(res), flush=True)
         return
     if args.workload == "c1":
         print(json.dumps({"c1_chain100_cholesky": time_c1(steps, warmup, 0 if args.no_cpu_baseline else 400)}),
