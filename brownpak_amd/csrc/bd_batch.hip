@@ -1082,10 +1082,10 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
         hipMemcpy(b.bead_row, br.data(), b.NB * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
         return fail("upload failed");
     for (int k = 0; k < LZ_NEV; k++)
-        if (hipEventCreateWithFlags(&b.ev[k], hipEventDisableTiming) != hipSuccess) return fail("event failed");
+        if (hipEventCreateWithFlags(&b.ev[k], kStreamEvent) != hipSuccess) return fail("event failed");
     for (int k = 0; k < 2; k++)
         if (hipEventCreateWithFlags(&b.chunk_ev[k], hipEventDisableTiming) != hipSuccess) return fail("event failed");
-    if (hipEventCreateWithFlags(&b.join_ev, hipEventDisableTiming) != hipSuccess) return fail("event failed");
+    if (hipEventCreateWithFlags(&b.join_ev, kStreamEvent) != hipSuccess) return fail("event failed");
     // tile unit shape chosen by timing the Lanczos product on this plan
     if (!b.mf) {
         if (b.mob32) ok = hipMemset(b.mob32, 0, elems * sizeof(float)) == hipSuccess;

@@ -27,6 +27,11 @@ constexpr int MAX_S = 8;          // largest Lanczos block size supported
 // Events that only time kernels: no system-scope fence when they complete
 // (no cache writeback and invalidation around the timed launch).
 constexpr unsigned kTimingEvent = hipEventDisableSystemFence;
+// Events that hand work from one stream of the device to another: the
+// consumer is a kernel on the same GPU, so a device-scope release suffices
+// (a system-scope one writes the L2s back to memory, which the next kernel
+// on the producing stream then waits for).
+constexpr unsigned kStreamEvent = hipEventDisableTiming | hipEventDisableSystemFence;
 constexpr int NQUAD_MAX = 160;    // quadrature nodes for H^{-1/2}
 constexpr int NBOUND = 48;        // Cholesky probes for the lower spectral bound
 constexpr int LZ_NEV = 8;         // events handing Lanczos iterations to the side stream

@@ -1203,7 +1203,7 @@ int lanczos_alloc(Ctx &c)
         BD_HIP(hipMalloc(&c.Rcoef, (size_t)ncol * s * sizeof(double)));
     }
     BD_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
-    for (int k = 0; k < LZ_NEV; k++) BD_HIP(hipEventCreateWithFlags(&c.lz_ev[k], hipEventDisableTiming));
+    for (int k = 0; k < LZ_NEV; k++) BD_HIP(hipEventCreateWithFlags(&c.lz_ev[k], kStreamEvent));
     for (int k = 0; k < 2; k++) BD_HIP(hipEventCreateWithFlags(&c.lz_chunk_ev[k], hipEventDisableTiming));
     void *fl = nullptr;
     BD_HIP(hipHostMalloc(&fl, 64, hipHostMallocMapped | hipHostMallocCoherent));
