@@ -884,7 +884,12 @@ int bd_run(bd_ctx *h, int nsteps, const double *xi, long long nxi, long long *us
         long long cons = 0;
         rc = em_step(c, xi ? xi + pos : nullptr, xi ? nxi - pos : 0, &cons, nullptr, nullptr);
         pos += cons;
-        if (rc == BD_OK && c.opt.lhdia == 0 && (k % 64) == 63) rc = check_status(c, false);
+        if (rc == BD_OK && c.opt.lhdia == 0 && (k % 64) == 63) {
+            rc = check_status(c, false);
+            // an overflowed list is scanned in full by the forces until it is
+            // rebuilt with room: grow it here rather than at the end of the run
+            if (rc == BD_OK && c.st_host->overflow) rc = grow_pairs(c);
+        }
     }
     if (used) *used = pos;
     int rc2 = check_status(c, false);

@@ -22,11 +22,16 @@ namespace bd {
 
 constexpr int NBR_CAP0 = 64;        // initial neighbours per bead (full list)
 constexpr int FB = 256;             // threads per force block
-constexpr int FW = FB / 64;         // beads per force_gather_kernel block (a wavefront each)
+constexpr int FW = FB / 64;         // fewest beads per force_gather_kernel block (64 lanes each)
 constexpr int N2_WAVES = 8;         // beads per nbr_n2_kernel block (a wavefront each)
 constexpr int N2_CHUNK = 2048;      // coordinates staged in LDS per pass of nbr_n2_kernel
 
 __device__ inline bool excluded(const int *ex_ptr, const int *ex_col, int i, int j);
+
+// Slot of neighbour k of bead i in the list: bead-major, each bead's row of
+// cap entries contiguous (a group of lanes on one bead reads and writes
+// neighbouring words).
+__device__ __forceinline__ long nbr_slot(int i, int k, int cap) { return (long)i * cap + k; }
 
 // ------------------------------------------------------------ energy sums
 // acc layout: 0 e_bond, 1 e_vdw, 2..10 stress(3x3 col-major), 11 bndlen sum,
@@ -55,12 +60,15 @@ __device__ void block_acc_store(double (&a)[NACC], double *out)
 }
 
 // ------------------------------------------------------------ forces
-// Per-bead gather of bond and pair terms, one wavefront per bead: lane l
-// takes the bead's bonds l, l + 64, ... and then its neighbours l, l + 64,
-// ... (list order), and the wavefront adds the lanes' sums in a fixed
-// butterfly order (a handful of terms per lane instead of a chain of
-// dependent loads per bead).  Energies/stress/bond lengths are counted once
-// per term: bonds from their first bead, pairs from the lower index.
+// Per-bead gather of bond and pair terms, LPB lanes per bead (a group of
+// 8, 16 or 64 lanes of one wavefront, chosen from the neighbour list's
+// capacity): lane l of the group takes the bead's bonds l, l + LPB, ... and
+// then its neighbours l, l + LPB, ... (list order), and the group adds the
+// lanes' sums in a fixed butterfly order (a handful of terms per lane
+// instead of a chain of dependent loads per bead; a short list leaves few
+// lanes idle, and the beads of a wavefront read neighbouring words of the
+// list).  Energies/stress/bond lengths are counted once per term: bonds
+// from their first bead, pairs from the lower index.
 // Angles, tethers, external fields and charges (bd_set_angles, bd_set_tethers,
 // bd_set_externals, bd_set_charges): per-bead CSR lists so that every bead's
 // wavefront gathers its own terms.
@@ -79,7 +87,7 @@ struct ExtraArg {
     const double *charge;             // null: all charges 0
 };
 
-template <bool EXTRA, bool GEN>
+template <bool EXTRA, bool GEN, int LPB>
 __global__ __launch_bounds__(FB) void force_gather_kernel(
     const double *__restrict__ crd, int nb, BoxArg bx, const int *__restrict__ bond_ptr,
     const int *__restrict__ bond_idx, const int *__restrict__ bond_ij, const int *__restrict__ bond_type,
@@ -90,8 +98,8 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
     const double *__restrict__ vparams, ExtraArg xa, double *__restrict__ frc, int want_energy,
     double *__restrict__ epart, DevStatus *st)
 {
-    const int lane = threadIdx.x % 64;
-    const int i = blockIdx.x * FW + threadIdx.x / 64;
+    const int lane = threadIdx.x % LPB;
+    const int i = blockIdx.x * (FB / LPB) + threadIdx.x / LPB;
     double acc[NACC];
     for (int e = 0; e < NACC; e++) acc[e] = 0.0;
     acc[12] = 1.0e300;
@@ -102,7 +110,7 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
         const int kb0 = bond_ptr[i], kb1 = bond_ptr[i + 1];
         const int cnt = use_pairs ? nbr_cnt[i] : 0;
         const int ati = use_pairs ? atype[i] : 0;
-        for (int k = kb0 + lane; k < kb1; k += 64) {
+        for (int k = kb0 + lane; k < kb1; k += LPB) {
             int b = bond_idx[k];
             int bi = bond_ij[2 * b], bj = bond_ij[2 * b + 1];
             double x, y, z;
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
         if (use_pairs) {
             const bool full = cnt > nbr_cap;  // list overflowed: scan all beads
             const int kend = full ? nb : cnt;
-            for (int k = lane; k < kend; k += 64) {
+            for (int k = lane; k < kend; k += LPB) {
                 int j, mult = 1;
                 if (full) {
                     j = k;
@@ -136,7 +144,7 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
                     mult = scan_mult(ps, bx, i, j);
                     if (mult == 0 || excluded(ex_ptr, ex_col, i, j)) continue;
                 } else {
-                    j = nbr_idx[(long)k * nb + i];
+                    j = nbr_idx[nbr_slot(i, k, nbr_cap)];
                 }
                 int t = ptype[ati * natype + atype[j]];
                 const int sty = vstyles[t];
@@ -168,7 +176,7 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
             // ia_add_angle_forces (interaction_m.f90:276-325): the bead's share
             // of each of its angles; energy and stress once, at the centre bead
             const int ka0 = xa.nangles > 0 ? xa.ang_ptr[i] : 0, ka1 = xa.nangles > 0 ? xa.ang_ptr[i + 1] : 0;
-            for (int k = ka0 + lane; k < ka1; k += 64) {
+            for (int k = ka0 + lane; k < ka1; k += LPB) {
                 const int code = xa.ang_idx[k], a = code >> 2, role = code & 3;
                 const int b0 = xa.ang_atoms[3 * a], b1 = xa.ang_atoms[3 * a + 1], b2 = xa.ang_atoms[3 * a + 2];
                 double q1[3], q2[3];
@@ -191,7 +199,7 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
             }
             // ia_add_tether_forces (interaction_m.f90:375-418), no imaging
             const int kt0 = xa.ntethers > 0 ? xa.teth_ptr[i] : 0, kt1 = xa.ntethers > 0 ? xa.teth_ptr[i + 1] : 0;
-            for (int k = kt0 + lane; k < kt1; k += 64) {
+            for (int k = kt0 + lane; k < kt1; k += LPB) {
                 const int tt = xa.teth_idx[k], t = xa.teth_type[tt];
                 double q[3];
                 for (int d = 0; d < 3; d++) q[d] = crd[3 * i + d] - xa.teth_pts[3 * tt + d];
@@ -222,9 +230,10 @@ __global__ __launch_bounds__(FB) void force_gather_kernel(
             }
         }
     }
-    // the wavefront's sum (every lane ends with the same value: each step
-    // adds the same two operands on both lanes of a pair)
-    for (int off = 32; off > 0; off >>= 1) {
+    // the group's sum (every lane ends with the same value: each step adds
+    // the same two operands on both lanes of a pair; the partners stay
+    // inside the group)
+    for (int off = LPB / 2; off > 0; off >>= 1) {
         fx += __shfl_xor(fx, off, 64);
         fy += __shfl_xor(fy, off, 64);
         fz += __shfl_xor(fz, off, 64);
@@ -376,7 +385,7 @@ __global__ __launch_bounds__(64 * N2_WAVES) void nbr_n2_kernel(const double *__r
             unsigned long long m = __ballot(keep);
             if (keep) {
                 int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
-                if (pos < cap) nbr_idx[(long)pos * nb + i] = j;
+                if (pos < cap) nbr_idx[nbr_slot(i, pos, cap)] = j;
             }
             cnt += __popcll(m);
         }
@@ -453,80 +462,133 @@ __global__ void cell_fill_kernel(const int *__restrict__ bead_cell, int nb, cons
     out[pos] = i;
 }
 
-// One wavefront per bead (NC_WAVES per block); candidates from the 27
-// surrounding cells, taken in the order of a serial scan (cells dz, dy, dx
-// from -1 to 1, beads of a cell in ascending order) so the list is the same:
-// lanes 0-26 read the cells' bead ranges, a prefix sum over the lanes lays
-// the ranges end to end, each lane finds its candidate's cell by bisection
-// over those sums, and the candidates that pass are packed in order by a
-// ballot.
+// One workgroup of NC_WAVES waves per cell.  The candidates of the 27
+// surrounding cells are taken in the order of a serial scan (cells dz, dy,
+// dx from -1 to 1, beads of a cell in ascending order), so every list is
+// the one that scan gives: wave 0's lanes 0-26 read the cells' bead ranges
+// and a prefix sum lays them end to end; the candidates and their
+// coordinates are staged in LDS, NC_CAP at a time, once for all the beads
+// of the cell; each wave then takes beads of the cell in turn and tests a
+// staged chunk 64 candidates at a time, packing those that pass in order by
+// a ballot (a bead's running count stays in LDS between chunks).
 constexpr int NC_WAVES = 4;
+constexpr int NC_CAP = 512;      // candidates staged at a time
+constexpr int NC_BEADS = 256;    // beads of one cell counted at a time
+constexpr int NC_EX = 8;         // excluded partners of a bead held in registers
 __global__ __launch_bounds__(64 * NC_WAVES) void nbr_cell_kernel(
     const double *__restrict__ crd, int nb, BoxArg bx, double rc2, int ncx, int ncy, int ncz,
-    const int *__restrict__ bead_cell, const int *__restrict__ cell_ptr, const int *__restrict__ cell_beads,
-    const int *__restrict__ ex_ptr, const int *__restrict__ ex_col, int *__restrict__ nbr_cnt,
-    int *__restrict__ nbr_idx, int cap, double *__restrict__ save, DevStatus *st)
+    const int *__restrict__ cell_ptr, const int *__restrict__ cell_beads, const int *__restrict__ ex_ptr,
+    const int *__restrict__ ex_col, int *__restrict__ nbr_cnt, int *__restrict__ nbr_idx, int cap,
+    double *__restrict__ save, DevStatus *st)
 {
-    __shared__ int s_end[NC_WAVES][32], s_beg[NC_WAVES][32];
+    __shared__ int s_end[32], s_beg[32];
+    __shared__ int s_j[NC_CAP];
+    __shared__ double s_x[NC_CAP], s_y[NC_CAP], s_z[NC_CAP];
+    __shared__ int s_cnt[NC_BEADS];
     if (!st->rebuild) return;
-    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-    const int i = blockIdx.x * NC_WAVES + wv;
-    if (i >= nb) return;   // the whole wavefront
-    const int c = bead_cell[i];
+    const int t = threadIdx.x, lane = t % 64, wv = t / 64;
+    const int c = blockIdx.x;
     const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
-    int b0 = 0, len = 0;
-    if (lane < 27) {
-        const int dz = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dx = lane % 3 - 1;
-        const int jx = (cx + dx + ncx) % ncx, jy = (cy + dy + ncy) % ncy, jz = (cz + dz + ncz) % ncz;
-        const int jc = (jz * ncy + jy) * ncx + jx;
-        b0 = cell_ptr[jc];
-        len = cell_ptr[jc + 1] - b0;
-    }
-    int incl = len;   // inclusive prefix over lanes 0..31
+    if (wv == 0) {
+        int b0 = 0, len = 0;
+        if (lane < 27) {
+            const int dz = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dx = lane % 3 - 1;
+            const int jx = (cx + dx + ncx) % ncx, jy = (cy + dy + ncy) % ncy, jz = (cz + dz + ncz) % ncz;
+            const int jc = (jz * ncy + jy) * ncx + jx;
+            b0 = cell_ptr[jc];
+            len = cell_ptr[jc + 1] - b0;
+        }
+        int incl = len;   // inclusive prefix over lanes 0..31
 #pragma unroll
-    for (int off = 1; off < 32; off <<= 1) {
-        const int v = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += v;
+        for (int off = 1; off < 32; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        if (lane < 32) {
+            s_end[lane] = incl;
+            s_beg[lane] = b0 - (incl - len);   // cell_beads index = s_beg + candidate number
+        }
     }
-    const int total = __shfl(incl, 26, 64);
-    if (lane < 32) {
-        s_end[wv][lane] = incl;
-        s_beg[wv][lane] = b0 - (incl - len);   // cell_beads index = s_beg + candidate number
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    int cnt = 0;
-    for (int k0 = 0; k0 < total; k0 += 64) {
-        const int k = k0 + lane;
-        int j = -1;
-        bool take = false;
-        if (k < total) {
-            int lo = 0, hi = 26;   // first cell whose end exceeds k
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_end[wv][mid] > k) hi = mid;
-                else lo = mid + 1;
+    __syncthreads();
+    const int total = s_end[26];
+    const int m0 = cell_ptr[c], nm = cell_ptr[c + 1] - m0;   // the cell's own beads
+    for (int q0 = 0; q0 < nm; q0 += NC_BEADS) {
+        const int nq = min(NC_BEADS, nm - q0);
+        for (int q = t; q < nq; q += 64 * NC_WAVES) s_cnt[q] = 0;
+        for (int k0 = 0; k0 < total; k0 += NC_CAP) {
+            const int nk = min(NC_CAP, total - k0);
+            __syncthreads();   // the previous chunk's readers are done; s_cnt is set
+            for (int e = t; e < nk; e += 64 * NC_WAVES) {
+                const int k = k0 + e;
+                int lo = 0, hi = 26;   // first cell whose end exceeds k
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_end[mid] > k) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int j = cell_beads[s_beg[lo] + k];
+                s_j[e] = j;
+                s_x[e] = crd[3 * j];
+                s_y[e] = crd[3 * j + 1];
+                s_z[e] = crd[3 * j + 2];
             }
-            j = cell_beads[s_beg[wv][lo] + k];
-            if (j != i) {
-                double x, y, z;
-                pair_vec(crd, i, j, bx, x, y, z);
-                take = rsq3(x, y, z) < rc2 && !excluded(ex_ptr, ex_col, i, j);
+            __syncthreads();
+            for (int q = wv; q < nq; q += NC_WAVES) {
+                const int i = cell_beads[m0 + q0 + q];
+                const double xi = crd[3 * i], yi = crd[3 * i + 1], zi = crd[3 * i + 2];
+                // bead i's excluded partners: the first NC_EX of its row in
+                // registers, read once for the whole scan
+                const int eb = ex_ptr[i], ee = ex_ptr[i + 1];
+                int exl[NC_EX];
+#pragma unroll
+                for (int u = 0; u < NC_EX; u++) exl[u] = eb + u < ee ? ex_col[eb + u] : -1;
+                auto is_excluded = [&](int j) {
+                    bool r = false;
+#pragma unroll
+                    for (int u = 0; u < NC_EX; u++) r = r || exl[u] == j;
+                    for (int k = eb + NC_EX; k < ee && !r; k++) r = ex_col[k] == j;
+                    return r;
+                };
+                int cnt = s_cnt[q];
+                for (int e0 = 0; e0 < nk; e0 += 64) {
+                    const int e = e0 + lane;
+                    int j = -1;
+                    bool take = false;
+                    if (e < nk) {
+                        j = s_j[e];
+                        if (j != i) {
+                            // pair_vec on the staged coordinates (the same operations)
+                            double x = __dsub_rn(s_x[e], xi), y = __dsub_rn(s_y[e], yi), z = __dsub_rn(s_z[e], zi);
+                            if (bx.imcon) {
+                                x = img(x, bx.L[0]);
+                                y = img(y, bx.L[1]);
+                                z = img(z, bx.L[2]);
+                            }
+                            take = rsq3(x, y, z) < rc2 && !is_excluded(j);
+                        }
+                    }
+                    const unsigned long long m = __ballot(take);
+                    if (take) {
+                        const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+                        if (pos < cap) nbr_idx[nbr_slot(i, pos, cap)] = j;
+                    }
+                    cnt += __popcll(m);
+                }
+                if (lane == 0) s_cnt[q] = cnt;
             }
         }
-        const unsigned long long m = __ballot(take);
-        if (take) {
-            const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
-            if (pos < cap) nbr_idx[(long)pos * nb + i] = j;
+        __syncthreads();
+        for (int q = t; q < nq; q += 64 * NC_WAVES) {
+            const int i = cell_beads[m0 + q0 + q];
+            const int cnt = s_cnt[q];
+            nbr_cnt[i] = cnt;
+            if (cnt > cap) atomicExch(&st->overflow, 1);
+            save[3 * i] = crd[3 * i];
+            save[3 * i + 1] = crd[3 * i + 1];
+            save[3 * i + 2] = crd[3 * i + 2];
         }
-        cnt += __popcll(m);
+        __syncthreads();   // s_cnt is reused by the next bead batch
     }
-    if (lane == 0) {
-        nbr_cnt[i] = cnt;
-        if (cnt > cap) atomicExch(&st->overflow, 1);
-    }
-    if (lane < 3) save[3 * i + lane] = crd[3 * i + lane];
 }
 
 // Beads of every cell in ascending index order (the fill above places them
@@ -598,7 +660,7 @@ __global__ __launch_bounds__(256) void cl_nbr_kernel(int nb, int ncx, int ncy, i
             const int j = cell_beads[p];
             if (same && j == i) continue;
             if (excluded(ex_ptr, ex_col, i, j)) continue;
-            if (cnt < cap) nbr_idx[(long)cnt * nb + i] = j;
+            if (cnt < cap) nbr_idx[nbr_slot(i, cnt, cap)] = j;
             cnt++;
         }
     };
@@ -723,7 +785,7 @@ __global__ __launch_bounds__(FB) void adj_force_kernel(
                     mult = scan_mult(ps, bx, i, j);
                     if (mult == 0 || excluded(ex_ptr, ex_col, i, j)) continue;
                 } else {
-                    j = nbr_idx[(long)k * nb + i];
+                    j = nbr_idx[nbr_slot(i, k, nbr_cap)];
                 }
                 int ty = ptype[ati * natype + atype[j]];
                 if (vstyles[ty] != 1) continue;
@@ -1097,10 +1159,9 @@ static int build_pairs(Ctx &c, bool sync)
                                c.cell_fill, 1, c.st);
             hipLaunchKernelGGL(cell_rank_kernel, dim3(nbk), dim3(256), 0, sm, c.bead_cell, nb, c.cell_ptr, c.cell_fill,
                                c.cell_beads, 1, c.st);
-            hipLaunchKernelGGL(nbr_cell_kernel, dim3((nb + NC_WAVES - 1) / NC_WAVES), dim3(64 * NC_WAVES), 0, sm,
-                               c.crd, nb, bx, rc2, c.ncell[0], c.ncell[1],
-                               c.ncell[2], c.bead_cell, c.cell_ptr, c.cell_beads, c.ex_ptr, c.ex_col, c.nbr_cnt,
-                               c.nbr_idx, c.pair_cap, c.crd_save, c.st);
+            hipLaunchKernelGGL(nbr_cell_kernel, dim3(c.ncell_tot), dim3(64 * NC_WAVES), 0, sm, c.crd, nb, bx, rc2,
+                               c.ncell[0], c.ncell[1], c.ncell[2], c.cell_ptr, c.cell_beads, c.ex_ptr, c.ex_col,
+                               c.nbr_cnt, c.nbr_idx, c.pair_cap, c.crd_save, c.st);
         } else {
             hipLaunchKernelGGL(nbr_n2_kernel, dim3((nb + N2_WAVES - 1) / N2_WAVES), dim3(64 * N2_WAVES), 0, sm, c.crd,
                                nb, bx, rc2, c.ex_ptr,
@@ -1124,14 +1185,19 @@ static int build_pairs(Ctx &c, bool sync)
     return BD_ERR_CAPACITY;
 }
 
+// Doubles the list capacity.  The new list holds nothing, so the next build
+// is forced (a Verlet check would otherwise keep the old counts over it).
 int grow_pairs(Ctx &c)
 {
     int zero = 0;
     BD_HIP(hipStreamSynchronize(c.stream));
     BD_HIP(hipMemcpy(&c.st->overflow, &zero, sizeof(int), hipMemcpyHostToDevice));
     (void)hipFree(c.nbr_idx);
+    c.nbr_idx = nullptr;
     c.pair_cap *= 2;
     BD_HIP(hipMalloc(&c.nbr_idx, (size_t)c.pair_cap * c.nb * sizeof(int)));
+    c.verlet_first = true;
+    c.pairs_valid = false;
     return BD_OK;
 }
 
@@ -1151,7 +1217,11 @@ int compute_forces(Ctx &c, bool want_energy, bool sync)
     const bool ver = (c.mth_ptgen == BD_PT_VER);
     const double rl = ver ? c.rcutoff + c.tskin : c.rcutoff;
     BoxArg bx = box_arg(c);
-    int nblk = (nb + FW - 1) / FW;
+    // lanes per bead: a group about as wide as the list is long (the
+    // capacity bounds every bead's listed count; a bead whose list
+    // overflowed is scanned against all beads by the same group)
+    const int lpb = !pairs ? 8 : (c.pair_cap <= 32 ? 8 : (c.pair_cap <= 128 ? 16 : 64));
+    int nblk = (nb + FB / lpb - 1) / (FB / lpb);
     if ((long)std::max(nblk, 1) * NACC > c.red_cap) {
         set_error("internal: force partial buffer smaller than the launch");
         return BD_ERR_STATE;
@@ -1160,8 +1230,13 @@ int compute_forces(Ctx &c, bool want_energy, bool sync)
                 c.teth_type, c.teth_styles, c.teth_pts, c.teth_params, c.nexternals, c.ext_styles, c.ext_params,
                 c.charge};
     const bool extra = c.nangles > 0 || c.ntethers > 0 || c.nexternals > 0;
-    auto kern = extra ? (c.vdw_general ? force_gather_kernel<true, true> : force_gather_kernel<true, false>)
-                      : (c.vdw_general ? force_gather_kernel<false, true> : force_gather_kernel<false, false>);
+    auto pick = [&](auto lanes) {
+        constexpr int L = decltype(lanes)::value;
+        return extra ? (c.vdw_general ? force_gather_kernel<true, true, L> : force_gather_kernel<true, false, L>)
+                     : (c.vdw_general ? force_gather_kernel<false, true, L> : force_gather_kernel<false, false, L>);
+    };
+    auto kern = lpb == 8 ? pick(std::integral_constant<int, 8>{})
+                         : (lpb == 16 ? pick(std::integral_constant<int, 16>{}) : pick(std::integral_constant<int, 64>{}));
     hipLaunchKernelGGL(kern, dim3(std::max(nblk, 1)), dim3(FB), 0, c.stream, c.crd, nb, bx, c.bead_bond_ptr,
                        c.bead_bond_idx, c.bond_ij, c.bond_type, c.bond_styles, c.bond_params, pairs ? 1 : 0,
                        c.nbr_cnt, c.nbr_idx, c.pair_cap, pair_scan(c), c.ex_ptr, c.ex_col,
@@ -1356,7 +1431,7 @@ int pair_table_host(Ctx &c, std::vector<int> &pairs)
         if (cnt[i] > c.pair_cap) { set_error("pair list overflowed; call bd_forces first"); return BD_ERR_CAPACITY; }
         std::vector<int> row;
         for (int k = 0; k < cnt[i]; k++) {
-            int j = idx[(size_t)k * nb + i];
+            int j = idx[(size_t)i * c.pair_cap + k];
             if (j > i) row.push_back(j);
         }
         std::sort(row.begin(), row.end());
