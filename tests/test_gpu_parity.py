@@ -485,6 +485,34 @@ def test_cell_list_overflow_steps_match_oracle(oracle, case):
     assert rel(g.get_coordinates(), ref) < TOL
 
 
+def test_verlet_list_growth_forces_rebuild(oracle):
+    """Verlet lists of a fresh context hold 64 entries per bead, fewer than
+    this cutoff (rc = 8, skin 0.6) gives; the first run steps over the
+    all-bead fallback and grows the list when it ends.  The grown list holds
+    nothing yet, so the second run (small kicks: no displacement trigger)
+    must rebuild before its forces.  Both runs together against the oracle's
+    six steps."""
+    _say("case: verlet list growth")
+    m = _melt(8, 100, 53)
+    m = dict(m, vdw_params=[[1.0, 2.0, 8.0]], rcutoff=8.0)
+    crd0 = m["crd"].copy()
+    nb = len(crd0)
+    L = m["box"][0]
+    d = crd0[:, None, :] - crd0[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = (d * d).sum(-1)
+    near = (r2 < (8.0 + 0.6) ** 2).sum(1) - 1
+    assert near.max() > 64
+    xi1 = np.random.default_rng(54).standard_normal(3 * nb * 3)
+    xi2 = np.random.default_rng(55).standard_normal(3 * nb * 3) * 1e-3
+    g = gpu_system(m, lhdia=False, tim_stp=1e-3)
+    so = oracle_system(oracle, m, "VER")
+    g.bds_run(3, xi1)
+    g.bds_run(3, xi2)
+    ref, _, _ = oracle.bd_run(so, crd0, 6, np.concatenate([xi1, xi2]), lhdia=False)
+    assert rel(g.get_coordinates(), ref) < TOL
+
+
 def test_cell_list_rejects_negative_coordinates_and_small_boxes():
     from brownpak_amd import BdError
     m = dict(_melt(2, 50, 49), mth_ptgen="CL", tskin=0.0)
