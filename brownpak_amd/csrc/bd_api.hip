@@ -39,7 +39,7 @@ __device__ inline void philox4x32_10(uint32_t (&c)[4], uint32_t k0, uint32_t k1)
 
 // out[col * ld + row] for row < rows, col < cols, stream position
 // offset + (col * rows + row) (column-major order, as brn_calc_dw fills).
-__global__ void gauss_kernel(double *out, int ld, int rows, int cols, uint64_t seed, uint64_t offset)
+__global__ void gauss_kernel(double *out, int ld, int rows, int cols, uint64_t seed, uint64_t offset, double scale)
 {
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long total = (long)rows * cols;
@@ -59,18 +59,21 @@ __global__ void gauss_kernel(double *out, int ld, int rows, int cols, uint64_t s
         long e = pair0 + q;
         if (e < total) {
             int col = (int)(e / rows), row = (int)(e % rows);
-            out[(long)col * ld + row] = z[q];
+            out[(long)col * ld + row] = scale * z[q];
         }
     }
 }
 
-int launch_gauss(Ctx &c, double *out, long long count, uint64_t seed, uint64_t offset, int ld, int rows, int cols)
+// scale multiplies every value (the step's sqrt(2 dt) when the draws go
+// straight into the Brownian displacement; 1 leaves them as drawn).
+int launch_gauss(Ctx &c, double *out, long long count, uint64_t seed, uint64_t offset, int ld, int rows, int cols,
+                 double scale)
 {
     (void)count;
     long pairs = ((long)rows * cols + 1) / 2;
     if (pairs == 0) return BD_OK;
     hipLaunchKernelGGL(gauss_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, c.stream, out, ld, rows,
-                       cols, seed, offset);
+                       cols, seed, offset, scale);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }
@@ -123,6 +126,40 @@ __global__ void update_kernel(double *__restrict__ crd, const double *__restrict
     double L[3] = {Lx, Ly, Lz};
     for (int d = 0; d < 3; d++) {
         double r = crd[3 * i + d] + drift[3 * i + d];
+        if (diff) r = r + diff[3 * i + d];
+        if (imcon) r = r - L[d] * floor(r / L[d]);  // smbx_wrap_all, simbox_m.f90:128-153
+        crd[3 * i + d] = r;
+    }
+}
+
+// drift_kernel and update_kernel in one launch, bead by bead with the same
+// operations (an EM step without the isolated-molecule tail): the bead's
+// flow term reads its start-of-step coordinates before they are replaced.
+__global__ void drift_update_kernel(const double *__restrict__ v, double *__restrict__ crd, int nb, int flow_style,
+                                    double rate, double dt, double *__restrict__ drift,
+                                    const double *__restrict__ diff, int imcon, double Lx, double Ly, double Lz,
+                                    DevStatus *st)
+{
+    if (st->ierr) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    if (i == 0) st->steps_done += 1;
+    double f[3] = {0.0, 0.0, 0.0};
+    switch (flow_style) {  // bd_solver_m.f90:420-441
+    case 1: f[0] = rate * crd[3 * i + 1]; break;
+    case 2: f[0] = rate * crd[3 * i]; f[1] = -rate * crd[3 * i + 1]; break;
+    case 3:
+        f[0] = rate * crd[3 * i];
+        f[1] = -0.5 * rate * crd[3 * i + 1];
+        f[2] = -0.5 * rate * crd[3 * i + 2];
+        break;
+    default: break;
+    }
+    const double L[3] = {Lx, Ly, Lz};
+    for (int d = 0; d < 3; d++) {
+        const double dr = dt * (v[3 * i + d] + f[d]);
+        drift[3 * i + d] = dr;
+        double r = crd[3 * i + d] + dr;
         if (diff) r = r + diff[3 * i + d];
         if (imcon) r = r - L[d] * floor(r / L[d]);  // smbx_wrap_all, simbox_m.f90:128-153
         crd[3 * i + d] = r;
@@ -419,14 +456,14 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
             if (avail < n) { set_error("injected xi stream exhausted"); return BD_ERR_ARG; }
             BD_TRY(upload_xi(c, xi_host, 1));
             *consumed = n;
-        } else {
-            BD_TRY(launch_gauss(c, c.xi_dev, n, c.rng_seed, c.rng_offset, c.npad, n, 1));
-            c.rng_offset += (uint64_t)n + (n & 1);
-        }
-        if (!disp_dev) {
             BD_HIP(hipMemcpyAsync(c.diff, c.xi_dev, (size_t)c.npad * sizeof(double), hipMemcpyDeviceToDevice, sm));
             hipLaunchKernelGGL(scale_kernel, dim3((n + 255) / 256), dim3(256), 0, sm, c.diff, (long)n, sqrt_two_dt,
                                c.st);
+        } else {
+            // sqrt(2 dt) xi drawn straight into the displacement (the same
+            // product as the scaling of an injected stream)
+            BD_TRY(launch_gauss(c, c.diff, n, c.rng_seed, c.rng_offset, c.npad, n, 1, sqrt_two_dt));
+            c.rng_offset += (uint64_t)n + (n & 1);
         }
     } else {
         if (c.cntr_mobsam == 0) {
@@ -483,6 +520,9 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
     // ---- calc_drift (bd_solver_m.f90:407-454): drift = dt (U(r) + D F(r))
     // (an EM step of an isolated molecule leaves the drift to its tail kernel)
     const bool fused_tail = c.opt.integrator != BD_PC && c.limud();
+    // otherwise an EM step (no isolated-molecule tail then) updates in the
+    // drift's launch
+    const bool fused_update = c.opt.integrator == BD_EM && !fused_tail;
     const double *vdrift = nullptr;
     auto drift_at_crd = [&](bool launch, bool energies) -> int {
         BD_TRY(compute_forces(c, energies, false));
@@ -499,7 +539,7 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
     };
     // the statistics take the energies of this evaluation (calc_drift at the
     // start-of-step coordinates, bd_solver_m.f90:407-454)
-    BD_TRY(drift_at_crd(!fused_tail, c.stats_on));
+    BD_TRY(drift_at_crd(!fused_tail && !fused_update, c.stats_on));
     // Brownian displacement w of this step
     const double *dcol = nullptr;
     if (disp_dev) dcol = disp_dev;
@@ -522,6 +562,10 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
         // drift, update and the bds_run tail (bd_solver_m.f90:176-186) in one launch
         hipLaunchKernelGGL(em_tail_kernel, dim3(1), dim3(1024), 0, sm, c.crd, vdrift, dcol, nb, c.opt.flow_style,
                            c.opt.flow_rate, c.opt.tim_stp, c.drift, c.scal + 16, c.st);
+    } else if (fused_update) {
+        hipLaunchKernelGGL(drift_update_kernel, dim3(nbk), dim3(256), 0, sm, vdrift, c.crd, nb, c.opt.flow_style,
+                           c.opt.flow_rate, c.opt.tim_stp, c.drift, dcol, c.imcon, c.box[0], c.box[1], c.box[2],
+                           c.st);
     } else {
         hipLaunchKernelGGL(update_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, c.drift, dcol, nb, c.imcon, c.box[0],
                            c.box[1], c.box[2], c.st);

@@ -293,7 +293,8 @@ __device__ inline bool excluded(const int *ex_ptr, const int *ex_col, int i, int
 // maximum of its beads (fmax, so a NaN displacement is passed over as in a
 // serial maximum) and flags vmove when 4 max >= skin^2, which holds for some
 // block exactly when it holds for the maximum over all beads.  The last block
-// to finish (vdone ticket) sets rebuild and clears both counters.
+// to finish (vdone ticket) sets rebuild, counts it (nrebuild) and clears
+// both counters.
 __global__ __launch_bounds__(256) void verlet_check_kernel(const double *__restrict__ crd,
                                                            const double *__restrict__ save, int nb, BoxArg bx,
                                                            double tskin_sq, int first, DevStatus *st)
@@ -319,6 +320,7 @@ __global__ __launch_bounds__(256) void verlet_check_kernel(const double *__restr
         if (done == (int)gridDim.x - 1) {
             const int mv = __hip_atomic_load(&st->vmove, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             st->rebuild = (first || mv) ? 1 : 0;
+            if (first || mv) st->nrebuild += 1;
             __hip_atomic_store(&st->vmove, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&st->vdone, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -470,7 +472,9 @@ __global__ void cell_fill_kernel(const int *__restrict__ bead_cell, int nb, cons
 // coordinates are staged in LDS, NC_CAP at a time, once for all the beads
 // of the cell; each wave then takes beads of the cell in turn and tests a
 // staged chunk 64 candidates at a time, packing those that pass in order by
-// a ballot (a bead's running count stays in LDS between chunks).
+// a ballot (a bead's running count stays in LDS between chunks).  Each
+// workgroup also clears its cell's count, which the next build's
+// assignment then finds at zero.
 constexpr int NC_WAVES = 4;
 constexpr int NC_CAP = 512;      // candidates staged at a time
 constexpr int NC_BEADS = 256;    // beads of one cell counted at a time
@@ -479,7 +483,7 @@ __global__ __launch_bounds__(64 * NC_WAVES) void nbr_cell_kernel(
     const double *__restrict__ crd, int nb, BoxArg bx, double rc2, int ncx, int ncy, int ncz,
     const int *__restrict__ cell_ptr, const int *__restrict__ cell_beads, const int *__restrict__ ex_ptr,
     const int *__restrict__ ex_col, int *__restrict__ nbr_cnt, int *__restrict__ nbr_idx, int cap,
-    double *__restrict__ save, DevStatus *st)
+    double *__restrict__ save, int *__restrict__ cell_cnt, DevStatus *st)
 {
     __shared__ int s_end[32], s_beg[32];
     __shared__ int s_j[NC_CAP];
@@ -512,6 +516,7 @@ __global__ __launch_bounds__(64 * NC_WAVES) void nbr_cell_kernel(
     __syncthreads();
     const int total = s_end[26];
     const int m0 = cell_ptr[c], nm = cell_ptr[c + 1] - m0;   // the cell's own beads
+    if (t == 0) cell_cnt[c] = 0;   // the fill count of this build: zero again for the next one
     for (int q0 = 0; q0 < nm; q0 += NC_BEADS) {
         const int nq = min(NC_BEADS, nm - q0);
         for (int q = t; q < nq; q += 64 * NC_WAVES) s_cnt[q] = 0;
@@ -681,11 +686,6 @@ __global__ void zero_int_kernel(int *p, int n, int check_flag, const DevStatus *
     if (check_flag && !st->rebuild) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
-}
-
-__global__ void count_rebuild_kernel(DevStatus *st, int *counter)
-{
-    if (st->rebuild) *counter += 1;
 }
 
 // ------------------------------------------------------------ adjoint
@@ -1072,6 +1072,7 @@ int forces_setup(Ctx &c, int nbonds, const int *bonds, int nbond_types, const in
             if (!c.cell_fill) BD_HIP(hipMalloc(&c.cell_fill, (size_t)nb * sizeof(int)));
             if (!c.bead_cell) BD_HIP(hipMalloc(&c.bead_cell, (size_t)nb * sizeof(int)));
             BD_HIP(hipMalloc(&c.cell_cnt, (size_t)c.ncell_tot * sizeof(int)));
+            BD_HIP(hipMemset(c.cell_cnt, 0, (size_t)c.ncell_tot * sizeof(int)));
             BD_HIP(hipMalloc(&c.cell_ptr, (size_t)(c.ncell_tot + 1) * sizeof(int)));
         }
     }
@@ -1149,8 +1150,8 @@ static int build_pairs(Ctx &c, bool sync)
                                c.pair_cap, c.st);
         } else if (ver && c.ncell_tot > 0) {
             int nbk = (nb + 255) / 256;
-            hipLaunchKernelGGL(zero_int_kernel, dim3((c.ncell_tot + 255) / 256), dim3(256), 0, sm, c.cell_cnt,
-                               c.ncell_tot, 1, c.st);
+            // cell_cnt is zero here: set so at allocation, and each cell's
+            // workgroup of nbr_cell_kernel clears its count after a build
             hipLaunchKernelGGL(cell_assign_kernel, dim3(nbk), dim3(256), 0, sm, c.crd, nb, bx, c.ncell[0], c.ncell[1],
                                c.ncell[2], c.bead_cell, c.cell_cnt, 1, c.st);
             hipLaunchKernelGGL(cell_scan_kernel, dim3(1), dim3(1024), 0, sm, c.cell_cnt, c.cell_ptr, c.ncell_tot, 1,
@@ -1161,14 +1162,13 @@ static int build_pairs(Ctx &c, bool sync)
                                c.cell_beads, 1, c.st);
             hipLaunchKernelGGL(nbr_cell_kernel, dim3(c.ncell_tot), dim3(64 * NC_WAVES), 0, sm, c.crd, nb, bx, rc2,
                                c.ncell[0], c.ncell[1], c.ncell[2], c.cell_ptr, c.cell_beads, c.ex_ptr, c.ex_col,
-                               c.nbr_cnt, c.nbr_idx, c.pair_cap, c.crd_save, c.st);
+                               c.nbr_cnt, c.nbr_idx, c.pair_cap, c.crd_save, c.cell_cnt, c.st);
         } else {
             hipLaunchKernelGGL(nbr_n2_kernel, dim3((nb + N2_WAVES - 1) / N2_WAVES), dim3(64 * N2_WAVES), 0, sm, c.crd,
                                nb, bx, rc2, c.ex_ptr,
                                c.ex_col, c.nbr_cnt, c.nbr_idx, c.pair_cap, n2_dir ? 0 : 1, ver ? c.crd_save : nullptr,
                                c.st);
         }
-        if (ver) hipLaunchKernelGGL(count_rebuild_kernel, dim3(1), dim3(1), 0, sm, c.st, &c.st->nrebuild);
         BD_HIP(hipGetLastError());
         c.verlet_first = false;
         c.pairs_valid = true;

@@ -375,7 +375,8 @@ int em_step(Ctx &c, const double *xi_host, long long avail, long long *consumed,
 int check_status(Ctx &c, bool clear);
 int rpy_assemble_at(Ctx &c, const double *crd);
 int launch_to_origin(Ctx &c, double *com_acc);
-int launch_gauss(Ctx &c, double *out, long long count, uint64_t seed, uint64_t offset, int ld, int rows, int cols);
+int launch_gauss(Ctx &c, double *out, long long count, uint64_t seed, uint64_t offset, int ld, int rows, int cols,
+                 double scale = 1.0);
 
 }  // namespace bd
 
