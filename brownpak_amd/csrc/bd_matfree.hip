@@ -158,17 +158,71 @@ __global__ __launch_bounds__(256) void mf_reduce_lz_kernel(const double *__restr
                                                            double *__restrict__ W, double *__restrict__ lzpart,
                                                            DevStatus *st)
 {
-    constexpr int NO = 2 * S * S;
+    constexpr int NO = 2 * S * S, E = S * S;
+    constexpr int EPT = (TILE * S + 255) / 256;   // elements of the block per thread
+    constexpr int STR = 256 / E;                  // stripes over the Gram partial index
+    constexpr int GPT = (GBLK + STR - 1) / STR;   // Gram partials per stripe, at most
     __shared__ double sdx[TILE][S + 1], sx[TILE][S + 1], sw[TILE][S + 1], sv[TILE][S + 1];
-    __shared__ double sG[S * S], sR2[S * S], sB[S * S];
+    __shared__ double sG[S * S], sR2[S * S], sB[S * S], sgs[256];
     __shared__ int s_done, s_fail;
     const int tid = threadIdx.x, G = blockIdx.x;
+    const bool xpre = pre.gpart != nullptr;
+    // every independent global load of the block first, all in flight
+    // together (the status, this thread's stripe of the Gram partials, the
+    // rows of X and V_{j-1}, then the column-group partials), and only then
+    // the sums, in the same order as ever
     if (tid == 0) s_done = lz_done(st) ? 1 : 0;
+    double gl[GPT];
+    const int ge = tid % E, gs = tid / E;
+#pragma unroll
+    for (int q = 0; q < GPT; q++) {
+        const int b = gs + q * STR;
+        gl[q] = (xpre && b < pre.ng) ? pre.gpart[(long)b * E + ge] : 0.0;
+    }
+    double xv[EPT], vpv[EPT][S], dv[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+        const int e = tid + q * 256, r = e / S, k = e % S, row = G * TILE + r;
+        const bool in = e < TILE * S;
+        xv[q] = in ? X[(long)k * ld + row] : 0.0;
+#pragma unroll
+        for (int kp = 0; kp < S; kp++) vpv[q][kp] = (in && Vp) ? Vp[(long)kp * ld + row] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+        const int e = tid + q * 256, r = e / S, k = e % S, row = G * TILE + r;
+        double v = 0.0;
+        if (e < TILE * S && row < n) {
+            // all the partials of the element in flight at once (jg <= 64)
+            const long o = (long)(row / 3) * (3 * S) + (row % 3) * S + k, str = (long)nb * 3 * S;
+            v = sum_strided<false, 32>(part + o, str, jg);
+        }
+        dv[q] = v;
+    }
+    // the stripe sums of the Gram partials (partial index gs, gs + STR, ...)
+    double gsum = 0.0;
+#pragma unroll
+    for (int q = 0; q < GPT; q++)
+        if (gs + q * STR < pre.ng) gsum += gl[q];
+    sgs[tid] = gsum;
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+        const int e = tid + q * 256, r = e / S, k = e % S;
+        if (e < TILE * S) {
+            sdx[r][k] = dv[q];
+            sx[r][k] = xv[q];
+        }
+    }
     __syncthreads();
     if (s_done) return;
-    const bool xpre = pre.gpart != nullptr;
     if (xpre) {
-        sum_partials_n<S>(pre.gpart, pre.ng, sG);   // ends with a barrier
+        // the stripes in order, then R2 and B = R2 R1 (both upper triangular)
+        if (tid < E) {
+            double t = 0.0;
+            for (int q = 0; q < STR; q++) t += sgs[q * E + tid];
+            sG[tid] = t;
+        }
+        __syncthreads();
         if (tid == 0) {
             double R[S][S];
             const bool ok = small_chol<S>(sG, R);
@@ -176,7 +230,6 @@ __global__ __launch_bounds__(256) void mf_reduce_lz_kernel(const double *__restr
             for (int i = 0; i < S; i++)
                 for (int j = 0; j < S; j++) sR2[i + j * S] = R[i][j];
             if (ok) {
-                // B = R2 R1 (both upper triangular)
                 for (int i = 0; i < S; i++)
                     for (int j = 0; j < S; j++) {
                         double t = 0.0;
@@ -191,23 +244,14 @@ __global__ __launch_bounds__(256) void mf_reduce_lz_kernel(const double *__restr
             if (G == 0 && tid == 0) lz_factor_failed(st, pre.blk);
             return;
         }
-    } else if (Vp && tid < S * S) {
-        sB[tid] = Bp[tid];
+    } else if (Vp) {
+        if (tid < S * S) sB[tid] = Bp[tid];
+        __syncthreads();
     }
-    for (int e = tid; e < TILE * S; e += blockDim.x) {
-        const int r = e / S, k = e % S, row = G * TILE + r;
-        double v = 0.0;
-        if (row < n) {
-            // all the partials of the element in flight at once (jg <= 64)
-            const long o = (long)(row / 3) * (3 * S) + (row % 3) * S + k, str = (long)nb * 3 * S;
-            v = sum_strided<false, 32>(part + o, str, jg);
-        }
-        sdx[r][k] = v;
-        sx[r][k] = X[(long)k * ld + row];
-    }
-    __syncthreads();
-    for (int e = tid; e < TILE * S; e += blockDim.x) {
-        const int r = e / S, k = e % S, row = G * TILE + r;
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+        const int e = tid + q * 256, r = e / S, k = e % S, row = G * TILE + r;
+        if (e >= TILE * S) break;
         double w, vj;
         if (xpre) {
             // forward solves of the row against R2, columns 0..k
@@ -233,7 +277,7 @@ __global__ __launch_bounds__(256) void mf_reduce_lz_kernel(const double *__restr
         if (Vp) {
             double t = 0.0;
 #pragma unroll
-            for (int kp = 0; kp < S; kp++) t += Vp[(long)kp * ld + row] * sB[k + kp * S];
+            for (int kp = 0; kp < S; kp++) t += vpv[q][kp] * sB[k + kp * S];
             w = w - t;
         }
         W[(long)k * ld + row] = w;
