@@ -323,7 +323,7 @@ def time_c4(steps=200, warmup=20):
 
 def measured_traffic(kind, prec="fp32"):
     """HBM bytes per launch of a Lanczos product kernel (kind "tiles":
-    symm_tiles_kernel<T, 4>; "mf": mf_product_kernel<T, 4, 1>; prec fp32 or
+    symm_tiles_kernel<T, 4>; "mf": mf_product_kernel<T, 4>; prec fp32 or
     fp64) from the two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE)
     summarised in profiles/, already scaled by the copy-kernel calibration;
     None until measured."""
@@ -468,12 +468,12 @@ def main():
         roofline = {"bound": "mfma", "achieved": round(rm["achieved"], 2), "peak": pk,
                     "unit": "TFLOP/s", "frac": round(rm["achieved"] / pk, 4),
                     "traffic": measured_traffic("mf", head[0]),
-                    "kernel": "mf_product_kernel<%s, 4, 1>" % ("double" if head[0] == "fp64" else "float"),
+                    "kernel": "mf_product_kernel<%s, 4>" % ("double" if head[0] == "fp64" else "float"),
                     "note": mf_note,
                     "avg_launch_us": round(rm["avg_launch_us"], 2),
                     "flops_per_working_launch": rm["flops_per_working_launch"],
                     "launches": rm["launches"], "empty_launches": rm["empty_launches"],
-                    "fp32_kernel": "mf_product_kernel<float, 4, 1>",
+                    "fp32_kernel": "mf_product_kernel<float, 4>",
                     "fp32_achieved": round(rm32["achieved"], 2), "fp32_peak": FP32_PEAK_TFLOPS,
                     "fp32_avg_launch_us": round(rm32["avg_launch_us"], 2),
                     "fp64_achieved": round(rm64["achieved"], 2), "fp64_avg_launch_us": round(rm64["avg_launch_us"], 2)}
