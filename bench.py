@@ -138,11 +138,13 @@ def accuracy_states(nstates=10, storage=None):
     return summary
 
 
-def cpu_baseline(budget_s=20.0):
-    """The CPU oracle (C FP64 restatement of the reference algorithm, one
-    thread) on a bounded sample of the same workload: one mobility update
-    block of s = 4 steps (RPY, block Lanczos, 4 x forces + dsymv + update),
-    repeated while the budget lasts."""
+def cpu_baseline(budget_s=20.0, threads=1):
+    """The CPU oracle (C FP64 restatement of the reference algorithm) on a
+    bounded sample of the same workload: one mobility update block of s = 4
+    steps (RPY, block Lanczos, 4 x forces + dsymv + update), repeated while
+    the budget lasts.  threads = 1 is the reference's own configuration (it
+    links MKL's sequential layer); more runs the RPY assembly and the
+    products with OpenMP on that many host cores."""
     from oracle import oracle as O
     from brownpak_amd.configs import chain_bonds, fene_wca_params, random_walk_chain
     crd = random_walk_chain(N_BEADS, 2000)
@@ -152,18 +154,33 @@ def cpu_baseline(budget_s=20.0):
     n = 3 * N_BEADS
     rng = np.random.default_rng(7)
     steps = 0
-    t0 = time.perf_counter()
-    while True:
-        xi = rng.standard_normal(n * S_BLOCK)
-        crd, _, _ = O.bd_run(so, crd, S_BLOCK, xi, mob_fctr=O.LANC, nts_mobsam=S_BLOCK,
-                             lanc_mxitr=50, lanc_tol=1e-6)
-        steps += S_BLOCK
-        if time.perf_counter() - t0 > budget_s * 0.5:
-            break
-    dt = time.perf_counter() - t0
-    return dict(value=steps / dt, unit="steps/s", cores=1, kind="port",
+    O.set_threads(threads)
+    try:
+        t0 = time.perf_counter()
+        while True:
+            xi = rng.standard_normal(n * S_BLOCK)
+            crd, _, _ = O.bd_run(so, crd, S_BLOCK, xi, mob_fctr=O.LANC, nts_mobsam=S_BLOCK,
+                                 lanc_mxitr=50, lanc_tol=1e-6)
+            steps += S_BLOCK
+            if time.perf_counter() - t0 > budget_s * 0.5:
+                break
+        dt = time.perf_counter() - t0
+    finally:
+        O.set_threads(1)
+    what = "1 thread" if threads == 1 else f"{threads} threads (OpenMP RPY assembly and products)"
+    return dict(value=steps / dt, unit="steps/s", cores=threads, kind="port",
                 sample=f"{steps} EM steps ({steps // S_BLOCK} mobility updates) of N=2000 RPY + "
-                       f"block Lanczos s=4 tol 1e-6, FP64, oracle/bdoracle.c at 1 thread, {dt:.1f} s")
+                       f"block Lanczos s=4 tol 1e-6, FP64, oracle/bdoracle.c at {what}, {dt:.1f} s")
+
+
+def host_threads():
+    """Host cores this process may use: OMP_NUM_THREADS when set (16 on the
+    GPU box, its CPU share), else the CPU count, at most 16."""
+    try:
+        t = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        t = 0
+    return t if t > 0 else min(16, os.cpu_count() or 1)
 
 
 def cpu_model(lanczos_iters_per_update):
@@ -528,6 +545,11 @@ def main():
     }
     if cpu:
         out["vs_cpu_fp64"] = round(fp64["steps_per_s"] / cpu["value"], 1)
+        th = host_threads()
+        if th > 1:
+            cpu_all = cpu_baseline(threads=th)
+            out["cpu_baseline_all_cores"] = cpu_all
+            out["vs_cpu_all_cores_fp64"] = round(fp64["steps_per_s"] / cpu_all["value"], 1)
         try:
             cm = cpu_model(hrun["lanczos_iters"] / max(hrun["mob_updates"], 1))
             out["cpu_model"] = cm

@@ -26,6 +26,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -49,12 +50,20 @@ static double dot(int n, const double *x, const double *y)
 /* Mobility                                                               */
 /* ===================================================================== */
 
+/* Host threads for the two O(n^2) routines below (orc_rpy, orc_symv_lower).
+ * 1 (the default, and what the tests use) runs the serial code; more runs
+ * an OpenMP version for timing the restatement on all host cores: the same
+ * RPY entries, and a symv whose sums are added in another order. */
+static int g_threads = 1;
+void orc_set_threads(int t) { g_threads = t > 1 ? t : 1; }
+
 /* calc_rpy_tensor, bd_solver_m.f90:458-522.  RPY tensor for bead radius 1,
  * overlap branch for r < 2, unit diagonal.  Full n x n column-major. */
 void orc_rpy(int nb, const double *crd, double *mob)
 {
     const int n = 3 * nb;
     memset(mob, 0, sizeof(double) * (size_t)n * n);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads) if (g_threads > 1)
     for (int j = 1; j < nb; j++) {
         const double *rj = crd + 3 * j;
         for (int i = 0; i < j; i++) {
@@ -83,6 +92,7 @@ void orc_rpy(int nb, const double *crd, double *mob)
         }
     }
     /* strictly-upper -> strictly-lower copy, bd_solver_m.f90:511-516 */
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads) if (g_threads > 1)
     for (int j = 1; j < n; j++)
         for (int i = 0; i < j; i++) mob[IDX(j, i, n)] = mob[IDX(i, j, n)];
     for (int j = 0; j < n; j++) mob[IDX(j, j, n)] = 1.0;
@@ -90,9 +100,59 @@ void orc_rpy(int nb, const double *crd, double *mob)
 
 /* dsymv('L') as called at bd_solver_m.f90:446 and brown_m.f90:232:
  * y := alpha*A*x + beta*y reading only the lower triangle of A. */
+/* The smallest j in [0, n] whose columns 0..j-1 hold at least the fraction
+ * frac of the lower triangle (j (2n - j + 1) / 2 of n (n + 1) / 2 entries). */
+static int tri_split(int n, double frac)
+{
+    const double target = frac * 0.5 * (double)n * (n + 1.0);
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (0.5 * mid * (2.0 * n - mid + 1.0) >= target) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+static void symv_lower_threads(int n, double alpha, const double *a, const double *x, double beta, double *y)
+{
+    /* each thread takes a range of columns of about equal triangle area and
+     * accumulates its own copy of the row sums; the copies are then added */
+    const int nt = g_threads;
+    double *t = (double *)calloc((size_t)n * nt, sizeof(double));
+#pragma omp parallel num_threads(nt)
+    {
+        const int id = omp_get_thread_num(), m = omp_get_num_threads();
+        const int j0 = tri_split(n, (double)id / m), j1 = id == m - 1 ? n : tri_split(n, (double)(id + 1) / m);
+        double *tt = t + (size_t)id * n;
+        for (int j = j0; j < j1; j++) {
+            const double *col = a + (size_t)j * n;
+            const double xj = x[j];
+            double acc = col[j] * xj;
+            for (int i = j + 1; i < n; i++) {
+                tt[i] += col[i] * xj;
+                acc += col[i] * x[i];
+            }
+            tt[j] += acc;
+        }
+#pragma omp barrier
+#pragma omp for schedule(static)
+        for (int i = 0; i < n; i++) {
+            double v = 0.0;
+            for (int q = 0; q < m; q++) v += t[(size_t)q * n + i];
+            y[i] = alpha * v + (beta == 0.0 ? 0.0 : beta * y[i]);
+        }
+    }
+    free(t);
+}
+
 void orc_symv_lower(int n, double alpha, const double *a, const double *x,
                     double beta, double *y)
 {
+    if (g_threads > 1) {
+        symv_lower_threads(n, alpha, a, x, beta, y);
+        return;
+    }
     double *t = (double *)calloc((size_t)n, sizeof(double));
     for (int j = 0; j < n; j++) {
         const double *col = a + (size_t)j * n;

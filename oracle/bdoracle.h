@@ -25,6 +25,9 @@ extern "C" {
 
 /* ---- mobility -------------------------------------------------------- */
 void orc_rpy(int nbeads, const double *crd, double *mob);
+/* Host threads of orc_rpy and orc_symv_lower (1: the serial code the tests
+ * use; more: OpenMP, for timing on all host cores). */
+void orc_set_threads(int threads);
 void orc_symv_lower(int n, double alpha, const double *a, const double *x,
                     double beta, double *y);
 

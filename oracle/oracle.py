@@ -96,6 +96,7 @@ def lib():
         _lib = C.CDLL(_LIB_PATH)
         L = _lib
         L.orc_rpy.argtypes = [C.c_int, DP, DP]
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_symv_lower.argtypes = [C.c_int, C.c_double, DP, DP, C.c_double, DP]
         L.orc_chol_bdw.argtypes = [C.c_int, C.c_int, DP, DP]
         L.orc_chol_bdw.restype = C.c_int
@@ -149,6 +150,14 @@ def _i32(a):
 
 
 # ---------------------------------------------------------------- mobility
+def set_threads(threads):
+    """Host threads of the RPY assembly and the lower-triangle symv: 1 (the
+    default, used by the tests) is the serial code; more runs them with
+    OpenMP (the symv's sums then come in another order), for timing the
+    restatement on all host cores."""
+    lib().orc_set_threads(int(threads))
+
+
 def rpy(crd):
     """calc_rpy_tensor: returns the full n x n mobility (symmetric)."""
     crd = _f64(crd)

@@ -87,6 +87,25 @@ def test_symv_lower_uses_lower_triangle_only(oracle):
 
 
 # ------------------------------------------------------------ QR / eigen
+def test_threaded_rpy_and_symv_match_serial(oracle):
+    """The OpenMP paths (host timing on all cores only): the same RPY
+    entries bit for bit, and a symv equal up to the order of its sums; the
+    serial default is restored."""
+    crd = random_walk_chain(300, 11)
+    x = np.random.default_rng(12).standard_normal(900)
+    D1 = oracle.rpy(crd)
+    y1 = oracle.symv_lower(D1, x)
+    oracle.set_threads(4)
+    try:
+        D4 = oracle.rpy(crd)
+        y4 = oracle.symv_lower(D1, x)
+    finally:
+        oracle.set_threads(1)
+    assert np.array_equal(D1, D4)
+    assert rel(y4, y1) < 1e-14
+    assert np.array_equal(oracle.symv_lower(D1, x), y1)
+
+
 def test_geqp3_orgqr_match_lapack(oracle):
     import scipy.linalg.lapack as lp
     rng = np.random.default_rng(4)
