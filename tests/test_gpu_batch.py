@@ -77,11 +77,13 @@ def test_batch_matches_oracle_fixed_iterations(oracle, s, prec, storage):
     assert info["total_lanczos_iters"] == nupd * len(lengths) * (mx + 1)
 
 
-def test_batch_converged_noise_accuracy(oracle):
-    """One step; the displacement minus the exact drift is the Brownian term."""
+@pytest.mark.parametrize("s", [1, 2, 4])
+def test_batch_converged_noise_accuracy(oracle, s):
+    """One step; the displacement minus the exact drift is the Brownian term
+    (s = 1: the two-norm stopping test; s > 1: the column-sum one)."""
     lengths = [30, 55, 21, 80]
-    s, tol = 4, 1e-6
-    _say("case: batch converged noise")
+    tol = 1e-6
+    _say(f"case: batch converged noise s={s}")
     xi = np.random.default_rng(11).standard_normal(sum(3 * nb * s for nb in lengths))
     b, chains = _batch(lengths, 500, nts_mobsam=s, lanc_mxitr=60, lanc_tol=tol, tim_stp=0.5)
     # keep_reuse_noise off: steps 2..s of a block carry no noise; one step is enough

@@ -129,7 +129,8 @@ def test_cholesky_reports_a_failed_pivot(oracle, nb, k):
 
 # ------------------------------------------------------------- Lanczos
 @pytest.mark.parametrize("storage", STORAGES)
-@pytest.mark.parametrize("nb,s,mx", [(100, 1, 12), (100, 4, 7), (100, 3, 7), (2000, 4, 7)])
+@pytest.mark.parametrize("nb,s,mx", [(100, 1, 12), (100, 4, 7), (100, 3, 7), (100, 6, 5), (100, 8, 4),
+                                     (2000, 4, 7)])
 def test_lanczos_fixed_iterations(oracle, chain100, chain2000, nb, s, mx, storage):
     _say(f"case: lanczos fixed nb={nb} s={s} {storage}")
     m = chain100 if nb == 100 else chain2000
@@ -144,7 +145,7 @@ def test_lanczos_fixed_iterations(oracle, chain100, chain2000, nb, s, mx, storag
 
 
 @pytest.mark.parametrize("storage", STORAGES)
-@pytest.mark.parametrize("nb,s", [(100, 1), (100, 4), (2000, 4)])
+@pytest.mark.parametrize("nb,s", [(100, 1), (100, 4), (100, 8), (2000, 4)])
 def test_lanczos_reorth_converged(oracle, chain100, chain2000, nb, s, storage):
     _say(f"case: lanczos reorth nb={nb} s={s} {storage}")
     m = chain100 if nb == 100 else chain2000
