@@ -455,9 +455,12 @@ __global__ __launch_bounds__(YR) void b_ycomb_kernel(const double *Cbuf, int dma
 // chain), in that order, partners j in increasing order.  One workgroup per
 // (chain, BF_ROWS-bead block) from items; the chain's coordinates are staged
 // in LDS BF_TILE beads at a time, so every lane of a wave reads the same
-// partner bead as an LDS broadcast.
+// partner bead as an LDS broadcast; a chunk of BF_CH partners whose
+// bounding box lies beyond the cutoff from every bead of the wave is passed
+// over (it holds no pair in range, so the sums are the same).
 constexpr int BF_ROWS = 256;
 constexpr int BF_TILE = 1024;
+constexpr int BF_CH = 16;        // candidate beads per bounding box
 __global__ __launch_bounds__(BF_ROWS) void b_force_lds_kernel(const double *__restrict__ crd,
                                                               const int2 *__restrict__ items,
                                                               const BChainDev *__restrict__ ch, int bsty,
@@ -466,6 +469,7 @@ __global__ __launch_bounds__(BF_ROWS) void b_force_lds_kernel(const double *__re
                                                               int *counters)
 {
     __shared__ double sx[BF_TILE], sy[BF_TILE], sz[BF_TILE];
+    __shared__ double sbox[BF_TILE / BF_CH][6];   // per chunk of BF_CH staged beads: low x, y, z, high x, y, z
     const int2 it = items[blockIdx.x];
     const int c = it.x, nbc = ch[c].nb;
     const int i = it.y * BF_ROWS + (int)threadIdx.x;
@@ -496,18 +500,42 @@ __global__ __launch_bounds__(BF_ROWS) void b_force_lds_kernel(const double *__re
                 sz[e] = cr[3 * (j0 + e) + 2];
             }
             __syncthreads();
+            // the bounding box of each chunk of BF_CH staged beads
+            for (int b = threadIdx.x; b * BF_CH < nt; b += BF_ROWS) {
+                double lo[3] = {sx[b * BF_CH], sy[b * BF_CH], sz[b * BF_CH]}, hi[3] = {lo[0], lo[1], lo[2]};
+                for (int q = b * BF_CH + 1; q < min(nt, (b + 1) * BF_CH); q++) {
+                    lo[0] = fmin(lo[0], sx[q]); hi[0] = fmax(hi[0], sx[q]);
+                    lo[1] = fmin(lo[1], sy[q]); hi[1] = fmax(hi[1], sy[q]);
+                    lo[2] = fmin(lo[2], sz[q]); hi[2] = fmax(hi[2], sz[q]);
+                }
+                for (int a = 0; a < 3; a++) {
+                    sbox[b][a] = lo[a];
+                    sbox[b][3 + a] = hi[a];
+                }
+            }
+            __syncthreads();
             if (!mine) continue;
-            for (int q = 0; q < nt; q++) {
-                const int j = j0 + q;
-                if (j == i || abs(j - i) <= excl) continue;
-                double x = __dsub_rn(sx[q], xi), y = __dsub_rn(sy[q], yi), z = __dsub_rn(sz[q], zi);
-                double r2 = rsq3(x, y, z);
-                if (!(r2 < rc2)) continue;
-                double r = sqrt(r2), en, fr;
-                lj_force(r, vp, en, fr);
-                f[0] += fr * x / r;
-                f[1] += fr * y / r;
-                f[2] += fr * z / r;
+            for (int b = 0; b * BF_CH < nt; b++) {
+                // a chunk farther than the cutoff from every bead of the
+                // wave holds no pair in range (the squared distance to its
+                // box is a lower bound; the margin covers the rounding of
+                // both sums), and adds nothing: skipping it keeps the sums
+                const double d0 = fmax(0.0, fmax(sbox[b][0] - xi, xi - sbox[b][3]));
+                const double d1 = fmax(0.0, fmax(sbox[b][1] - yi, yi - sbox[b][4]));
+                const double d2 = fmax(0.0, fmax(sbox[b][2] - zi, zi - sbox[b][5]));
+                if (__all(d0 * d0 + d1 * d1 + d2 * d2 > rc2 * (1.0 + 1e-6))) continue;
+                for (int q = b * BF_CH; q < min(nt, (b + 1) * BF_CH); q++) {
+                    const int j = j0 + q;
+                    if (j == i || abs(j - i) <= excl) continue;
+                    double x = __dsub_rn(sx[q], xi), y = __dsub_rn(sy[q], yi), z = __dsub_rn(sz[q], zi);
+                    double r2 = rsq3(x, y, z);
+                    if (!(r2 < rc2)) continue;
+                    double r = sqrt(r2), en, fr;
+                    lj_force(r, vp, en, fr);
+                    f[0] += fr * x / r;
+                    f[1] += fr * y / r;
+                    f[2] += fr * z / r;
+                }
             }
         }
     }
