@@ -351,14 +351,16 @@ __global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const doub
 }
 
 // y_j = V_{1:j} C of every chain whose small problem of iteration j is
-// done (ready[c] == j), one row per thread and YR rows per workgroup
-// (grid: chains x row blocks), the basis columns added in order; from j = 3
-// on, the convergence test: per column the sums over the chain's rows of
-// |y_j - y_{j-1}| and |y_{j-1}| (squares for s = 1), by each workgroup in a
-// fixed order, then by the chain's last workgroup to finish over the
-// workgroups in row-block order; the chain stops when max_k sum|d| /
+// done (ready[c] == j), two adjacent rows per thread (16-byte loads; chain
+// rows start on 64-row boundaries and npad is a multiple of 64) and YRR rows
+// per workgroup (grid: chains x row blocks), the basis columns added in
+// order; from j = 3 on, the convergence test: per column the sums over the
+// chain's rows of |y_j - y_{j-1}| and |y_{j-1}| (squares for s = 1), by each
+// workgroup in a fixed order, then by the chain's last workgroup to finish
+// over the workgroups in row-block order; the chain stops when max_k sum|d| /
 // max_k sum|y| (the 2-norm ratio for s = 1) is <= tol.
-constexpr int YR = 256;
+constexpr int YR = 256;          // threads per workgroup
+constexpr int YRR = 2 * YR;      // rows per workgroup
 template <int S>
 __global__ __launch_bounds__(YR) void b_ycomb_kernel(const double *Cbuf, int dmax, int j, const double *V, long NP,
                                                      const BChainDev *ch, double *Y, const int *ready, int *active,
@@ -367,14 +369,14 @@ __global__ __launch_bounds__(YR) void b_ycomb_kernel(const double *Cbuf, int dma
 {
     const int c = blockIdx.x, rb = blockIdx.y, js = j * S;
     if (ready[c] != j) return;
-    const int rows = ch[c].npad, nrb = (rows + YR - 1) / YR;
+    const int rows = ch[c].npad, nrb = (rows + YRR - 1) / YRR;
     if (rb >= nrb) return;
     __shared__ double sC[BJS * S];      // js S <= BJS S values
     const double *Cc = Cbuf + (long)c * dmax * S * S;
     for (int e = threadIdx.x; e < js * S; e += YR) sC[e] = Cc[e];
     __syncthreads();
     const long r0 = ch[c].row;
-    const int r = rb * YR + threadIdx.x;
+    const int r = rb * YRR + 2 * threadIdx.x;   // rows r and r + 1 (rows is even)
     double *y = Y + (long)(j % 2) * S * NP + r0;
     const double *yo = Y + (long)((j + 1) % 2) * S * NP + r0;
     const bool want = j > 2;
@@ -382,28 +384,34 @@ __global__ __launch_bounds__(YR) void b_ycomb_kernel(const double *Cbuf, int dma
 #pragma unroll
     for (int k = 0; k < S; k++) { dn[k] = 0.0; on[k] = 0.0; }
     if (r < rows) {
-        double acc[S];
+        double2 acc[S];
 #pragma unroll
-        for (int k = 0; k < S; k++) acc[k] = 0.0;
-        // thirty-two basis columns in flight, added in column order
-        for (int m0 = 0; m0 < js; m0 += 32) {
-            double vm[32];
+        for (int k = 0; k < S; k++) acc[k] = make_double2(0.0, 0.0);
+        // sixteen basis columns in flight, added in column order
+        for (int m0 = 0; m0 < js; m0 += 16) {
+            double2 vm[16];
 #pragma unroll
-            for (int q = 0; q < 32; q++) vm[q] = V[(long)min(m0 + q, js - 1) * NP + r0 + r];
+            for (int q = 0; q < 16; q++)
+                vm[q] = *reinterpret_cast<const double2 *>(V + (long)min(m0 + q, js - 1) * NP + r0 + r);
 #pragma unroll
-            for (int q = 0; q < 32; q++)
+            for (int q = 0; q < 16; q++)
                 if (m0 + q < js) {
 #pragma unroll
-                    for (int k = 0; k < S; k++) acc[k] += vm[q] * sC[(m0 + q) + k * js];
+                    for (int k = 0; k < S; k++) {
+                        const double cq = sC[(m0 + q) + k * js];
+                        acc[k].x += vm[q].x * cq;
+                        acc[k].y += vm[q].y * cq;
+                    }
                 }
         }
 #pragma unroll
         for (int k = 0; k < S; k++) {
-            y[(long)k * NP + r] = acc[k];
+            *reinterpret_cast<double2 *>(y + (long)k * NP + r) = acc[k];
             if (want) {
-                const double o = yo[(long)k * NP + r], d = acc[k] - o;
-                if (S == 1) { dn[k] = d * d; on[k] = o * o; }
-                else { dn[k] = fabs(d); on[k] = fabs(o); }
+                const double2 o = *reinterpret_cast<const double2 *>(yo + (long)k * NP + r);
+                const double d0 = acc[k].x - o.x, d1 = acc[k].y - o.y;
+                if (S == 1) { dn[k] = d0 * d0 + d1 * d1; on[k] = o.x * o.x + o.y * o.y; }
+                else { dn[k] = fabs(d0) + fabs(d1); on[k] = fabs(o.x) + fabs(o.y); }
             }
         }
     }
@@ -1091,7 +1099,7 @@ bd_batch *bd_batch_create(int nchains, const int *nbeads, const bd_opts *opts)
     {
         int mx = 0;
         for (int c = 0; c < nchains; c++) mx = std::max(mx, ((3 * nbeads[c] + TILE - 1) / TILE) * TILE);
-        b.yrb = (mx + YR - 1) / YR;
+        b.yrb = (mx + YRR - 1) / YRR;
     }
     dal(&b.ypart, (size_t)nchains * b.yrb * 2 * s);
     ial(&b.counters, 4); ial(&b.bead_chain, b.NB); ial(&b.bead_local, b.NB);
