@@ -352,7 +352,8 @@ __global__ __launch_bounds__(BT) void b_small_kernel(const double *A, const doub
 
 // y_j = V_{1:j} C of every chain whose small problem of iteration j is
 // done (ready[c] == j), two adjacent rows per thread (16-byte loads; chain
-// rows start on 64-row boundaries and npad is a multiple of 64) and YRR rows
+// rows start on 64-row boundaries and npad is a multiple of 64; the basis is
+// loaded non-temporally, as nothing else reads its older blocks) and YRR rows
 // per workgroup (grid: chains x row blocks), the basis columns added in
 // order; from j = 3 on, the convergence test: per column the sums over the
 // chain's rows of |y_j - y_{j-1}| and |y_{j-1}| (squares for s = 1), by each
@@ -392,7 +393,12 @@ __global__ __launch_bounds__(YR) void b_ycomb_kernel(const double *Cbuf, int dma
             double2 vm[16];
 #pragma unroll
             for (int q = 0; q < 16; q++)
-                vm[q] = *reinterpret_cast<const double2 *>(V + (long)min(m0 + q, js - 1) * NP + r0 + r);
+            {
+                typedef double v2d __attribute__((ext_vector_type(2)));
+                const v2d t = __builtin_nontemporal_load(
+                    reinterpret_cast<const v2d *>(V + (long)min(m0 + q, js - 1) * NP + r0 + r));
+                vm[q] = make_double2(t.x, t.y);
+            }
 #pragma unroll
             for (int q = 0; q < 16; q++)
                 if (m0 + q < js) {
